@@ -1,0 +1,152 @@
+#!/usr/bin/env python3
+"""mdz benchmark: decompressed GB/s and fraction of the HBM roofline.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
+
+A step is one decode pass of the workload's hot path over its whole
+synthetic input, which is resident in HBM before the timed region starts.
+K steps are enqueued back-to-back on one stream between two HIP events; the
+device is synchronised on both sides.  Prints ONE JSON line.
+
+Algorithmic bytes (each byte touched once):
+  lane-transposed codecs: payload + 256*n (col_len) + 8*(n+1) (blk_off) + out
+  byte RLE:               payload + 8*(n+1) + 12 + out
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import statistics
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "decompressed GB/s + % HBM roofline per codec; batched vs plain; p99 latency"
+HBM_PEAK_GBS = 8000.0
+
+
+def _cpu_time(fn, min_reps=3, max_reps=7, budget_s=20.0):
+    ts = []
+    t_end = time.perf_counter() + budget_s
+    for i in range(max_reps):
+        t0 = time.perf_counter()
+        fn()
+        ts.append(time.perf_counter() - t0)
+        if i + 1 >= min_reps and time.perf_counter() > t_end:
+            break
+    return statistics.median(ts)
+
+
+# ------------------------------------------------------------- workloads --
+class RleV1Workload:
+    """RLE v1, plain method: 64 MiB u32 column, 128 KiB chunks, one chunk per
+    workgroup (one wave64), COMP_COL_LEN = line bytes."""
+    name = "rlev1_u32_64MiB_plain"
+    dtype = "u32"
+
+    def __init__(self, line=32, ring=16384, n_vals=1 << 24, seed=0x4D445A01):
+        from oracle import pyoracle as O
+        from compression_amd.lanes import DeviceLanes
+        import compression_amd as mdz
+        self.O, self.mdz = O, mdz
+        self.line, self.ring = line, ring
+        self.data = O.gen_rlev1_u32(n_vals, seed)
+        self.c = O.rlev1_encode(self.data, 131072, line)
+        c = self.c
+        self.lanes = DeviceLanes(c.payload, c.blk_off, c.col_len, c.n_chunks, 131072, 4, line)
+        self.dst = mdz.DeviceBuffer(c.raw_bytes)
+        self.status = mdz.DeviceBuffer(4 * c.n_chunks)
+        self.out_bytes = c.raw_bytes
+        n = c.n_chunks
+        self.alg_bytes = int(c.blk_off[-1]) + 256 * n + 8 * (n + 1) + c.raw_bytes
+        self.config = {"workload": self.name, "codec": "rle_v1", "elem": "u32",
+                       "out_bytes": c.raw_bytes, "payload_bytes": int(c.blk_off[-1]),
+                       "n_chunks": n, "chunk_bytes": 131072, "comp_col_len": line,
+                       "ring_words": ring, "ratio": round(int(c.blk_off[-1]) / c.raw_bytes, 4)}
+
+    def step(self, stream):
+        from compression_amd import rle_v1
+        rle_v1.decompress(self.lanes, self.dst, self.out_bytes, self.status, stream=stream,
+                          ring_words=self.ring)
+
+    def verify(self):
+        got = self.dst.download(self.out_bytes)
+        st = self.status.download(dtype=np.int32)
+        return bool((st == 0).all() and np.array_equal(got, self.data.view(np.uint8)))
+
+    def cpu(self, threads):
+        O, c = self.O, self.c
+        out = np.empty(c.raw_bytes, np.uint8)
+        t = _cpu_time(lambda: O.rlev1_decode(c, threads=threads))
+        return {"value": round(c.raw_bytes / t / 1e9, 3), "unit": "GB/s", "cores": threads,
+                "kind": "port",
+                "sample": f"whole workload ({c.raw_bytes >> 20} MiB out, {c.n_chunks} chunks), "
+                          f"oracle C decoder, median of >=3 runs"}
+
+
+WORKLOADS = {"rlev1": RleV1Workload}
+DEFAULT = "rlev1"
+
+
+def run(args):
+    import compression_amd as mdz
+    from oracle import pyoracle as O
+
+    if mdz.device_count() < 1:
+        raise SystemExit("bench.py needs a GPU")
+    wl = WORKLOADS[args.workload](**({"line": args.line} if args.line else {}))
+    stream = mdz.Stream()
+    for _ in range(args.warmup):
+        wl.step(stream.ptr)
+    stream.sync()
+    ok = wl.verify()
+    e0, e1 = mdz.Event(), mdz.Event()
+    mdz.lib().mdz_device_sync()
+    e0.record(stream)
+    for _ in range(args.steps):
+        wl.step(stream.ptr)
+    e1.record(stream)
+    e1.sync()
+    mdz.lib().mdz_device_sync()
+    ms = e0.elapsed_ms(e1) / args.steps
+    gbs = wl.out_bytes / (ms * 1e-3) / 1e9
+    ach = wl.alg_bytes / (ms * 1e-3) / 1e9
+    threads = max(1, min(O.host_cores(), 16))
+    line = {
+        "metric": METRIC, "value": round(gbs, 2), "unit": "GB/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 5),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": wl.dtype,
+        "data": "synthetic (seeded generator, encoded by the C oracle)", "config": wl.config,
+        "roofline": {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+                     "alg_bytes_per_launch": wl.alg_bytes},
+        "bit_exact": ok,
+    }
+    if not args.no_cpu:
+        line["cpu_baseline"] = wl.cpu(threads)
+    print(json.dumps(line), flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--workload", default=DEFAULT, choices=sorted(WORKLOADS))
+    ap.add_argument("--line", type=int, default=0, help="RLE v1 COMP_COL_LEN override")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "mdz decodes on one GPU; chunks shard across GPUs without "
+                                   "exchange, but only --gpus 1 is built and measured"}))
+        raise SystemExit(2)
+    run(args)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,187 @@
+"""ctypes loader for the mdz HIP library (compression_amd/libmdz.so).
+
+The library is the product: there is no CPU fallback.  If libmdz.so is
+missing this module raises at load time.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libmdz.so")
+
+vp = C.c_void_p
+u64 = C.c_uint64
+u32 = C.c_uint32
+i32 = C.c_int32
+
+
+class MdzError(RuntimeError):
+    def __init__(self, rc: int, what: str):
+        detail = ""
+        try:
+            detail = (_lib.mdz_last_error() or b"").decode() if _lib is not None else ""
+        except Exception:
+            pass
+        super().__init__(f"{what} failed with mdz error {rc}" + (f" [{detail}]" if detail else ""))
+        self.rc = rc
+
+
+class LanesT(C.Structure):
+    """mirror of mdz_lanes_t (include/mdz.h)"""
+    _fields_ = [("payload", vp), ("blk_off", vp), ("col_len", vp), ("col_map", vp),
+                ("n_chunks", u64), ("chunk_bytes", u64), ("elem_bytes", u32), ("lane_line", u32)]
+
+
+class LanesTuneT(C.Structure):
+    _fields_ = [("ring_words", u32), ("reserved", u32 * 7)]
+
+
+# name -> (restype, argtypes); every symbol declared in include/mdz.h
+SIGNATURES = {
+    "mdz_version": (C.c_char_p, []),
+    "mdz_last_error": (C.c_char_p, []),
+    "mdz_device_count": (C.c_int, [C.POINTER(C.c_int)]),
+    "mdz_dev_malloc": (C.c_int, [C.POINTER(vp), u64]),
+    "mdz_dev_free": (C.c_int, [vp]),
+    "mdz_memcpy_htod": (C.c_int, [vp, vp, u64, vp]),
+    "mdz_memcpy_dtoh": (C.c_int, [vp, vp, u64, vp]),
+    "mdz_memset_d8": (C.c_int, [vp, C.c_int, u64, vp]),
+    "mdz_stream_create": (C.c_int, [C.POINTER(vp)]),
+    "mdz_stream_destroy": (C.c_int, [vp]),
+    "mdz_stream_sync": (C.c_int, [vp]),
+    "mdz_device_sync": (C.c_int, []),
+    "mdz_event_create": (C.c_int, [C.POINTER(vp)]),
+    "mdz_event_destroy": (C.c_int, [vp]),
+    "mdz_event_record": (C.c_int, [vp, vp]),
+    "mdz_event_sync": (C.c_int, [vp]),
+    "mdz_event_elapsed_ms": (C.c_int, [vp, vp, C.POINTER(C.c_float)]),
+    "mdz_host_free": (None, [vp]),
+    "mdz_rlev1_decompress": (C.c_int, [C.POINTER(LanesT), vp, u64, vp, vp]),
+    "mdz_rlev1_decompress_tuned": (C.c_int, [C.POINTER(LanesT), vp, u64, vp,
+                                             C.POINTER(LanesTuneT), vp]),
+    "mdz_rlev1_decompress_host": (C.c_int, [vp, C.POINTER(vp), u64, C.POINTER(u64), C.c_int,
+                                            u64, u32, vp, vp, u64]),
+    "mdz_rle_v1_decompress_gpu": (C.c_int, [vp, C.POINTER(vp), u64, C.POINTER(u64), C.c_int,
+                                            u64, u32]),
+}
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is not built; run `python -m compression_amd.build` "
+                              "(there is no CPU fallback)")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(rc: int, what: str) -> None:
+    if rc != 0:
+        raise MdzError(rc, what)
+
+
+# ------------------------------------------------------------ plumbing ----
+class DeviceBuffer:
+    """Owned device allocation (hipMalloc through the C ABI)."""
+
+    def __init__(self, nbytes: int):
+        self.nbytes = int(nbytes)
+        p = vp()
+        check(lib().mdz_dev_malloc(C.byref(p), self.nbytes), "mdz_dev_malloc")
+        self.ptr = p.value
+
+    @classmethod
+    def from_array(cls, a: np.ndarray, pad: int = 0, stream=None) -> "DeviceBuffer":
+        a = np.ascontiguousarray(a)
+        b = cls(a.nbytes + pad)
+        if a.nbytes:
+            check(lib().mdz_memcpy_htod(b.ptr, a.ctypes.data, a.nbytes, stream), "mdz_memcpy_htod")
+        if stream is None:
+            check(lib().mdz_stream_sync(None), "mdz_stream_sync")
+        return b
+
+    def upload(self, a: np.ndarray, stream=None) -> None:
+        a = np.ascontiguousarray(a)
+        assert a.nbytes <= self.nbytes
+        check(lib().mdz_memcpy_htod(self.ptr, a.ctypes.data, a.nbytes, stream), "mdz_memcpy_htod")
+
+    def download(self, nbytes: int | None = None, dtype=np.uint8, stream=None) -> np.ndarray:
+        n = self.nbytes if nbytes is None else int(nbytes)
+        out = np.empty(n, np.uint8)
+        if n:
+            check(lib().mdz_memcpy_dtoh(out.ctypes.data, self.ptr, n, stream), "mdz_memcpy_dtoh")
+        check(lib().mdz_stream_sync(stream), "mdz_stream_sync")
+        return out.view(dtype)
+
+    def fill(self, value: int, stream=None) -> None:
+        check(lib().mdz_memset_d8(self.ptr, value, self.nbytes, stream), "mdz_memset_d8")
+
+    def free(self) -> None:
+        if getattr(self, "ptr", None):
+            lib().mdz_dev_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Stream:
+    def __init__(self):
+        p = vp()
+        check(lib().mdz_stream_create(C.byref(p)), "mdz_stream_create")
+        self.ptr = p.value
+
+    def sync(self) -> None:
+        check(lib().mdz_stream_sync(self.ptr), "mdz_stream_sync")
+
+    def __del__(self):
+        try:
+            lib().mdz_stream_destroy(self.ptr)
+        except Exception:
+            pass
+
+
+class Event:
+    def __init__(self):
+        p = vp()
+        check(lib().mdz_event_create(C.byref(p)), "mdz_event_create")
+        self.ptr = p.value
+
+    def record(self, stream=None) -> None:
+        check(lib().mdz_event_record(self.ptr, stream.ptr if isinstance(stream, Stream) else stream),
+              "mdz_event_record")
+
+    def sync(self) -> None:
+        check(lib().mdz_event_sync(self.ptr), "mdz_event_sync")
+
+    def elapsed_ms(self, end: "Event") -> float:
+        ms = C.c_float()
+        check(lib().mdz_event_elapsed_ms(self.ptr, end.ptr, C.byref(ms)), "mdz_event_elapsed_ms")
+        return float(ms.value)
+
+    def __del__(self):
+        try:
+            lib().mdz_event_destroy(self.ptr)
+        except Exception:
+            pass
+
+
+def device_count() -> int:
+    n = C.c_int(0)
+    rc = lib().mdz_device_count(C.byref(n))
+    return n.value if rc == 0 else 0

@@ -1,0 +1,183 @@
+// mdz C ABI: device plumbing, per-codec entry points and the host-pointer
+// wrappers that keep the reference's signature shapes.
+#include <hip/hip_runtime.h>
+
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <mutex>
+#include <unordered_map>
+
+#include "mdz_internal.h"
+
+namespace mdz {
+thread_local char g_last_error[256] = "";
+int record_hip(hipError_t e, const char* what) {
+    if (e == hipSuccess) return MDZ_OK;
+    snprintf(g_last_error, sizeof g_last_error, "%s: %s (%d)", what, hipGetErrorString(e), (int)e);
+    return MDZ_ERR_HIP;
+}
+
+hipError_t set_dynamic_lds(const void* kernel, size_t bytes) {
+    static std::mutex mu;
+    static std::unordered_map<const void*, size_t> done;
+    std::lock_guard<std::mutex> g(mu);
+    size_t& have = done[kernel];
+    if (have >= bytes) return hipSuccess;
+    hipError_t e = hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return e;
+    }
+    have = bytes;
+    return hipSuccess;
+}
+}  // namespace mdz
+
+namespace {
+
+inline int herr(hipError_t e) { return mdz::record_hip(e, "hip"); }
+inline hipStream_t S(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+// Scoped device buffer for the host wrappers.
+struct DevBuf {
+    void* p = nullptr;
+    ~DevBuf() { if (p) (void)hipFree(p); }
+    int alloc(uint64_t n) { return hipMalloc(&p, n ? n : 1) == hipSuccess ? MDZ_OK : MDZ_ERR_NOMEM; }
+};
+
+int read_file(const char* path, uint8_t** data, uint64_t* n) {
+    FILE* f = fopen(path, "rb");
+    if (!f) return MDZ_ERR_INVALID;
+    fseek(f, 0, SEEK_END);
+    long sz = ftell(f);
+    fseek(f, 0, SEEK_SET);
+    if (sz < 0) { fclose(f); return MDZ_ERR_INVALID; }
+    *data = (uint8_t*)malloc(sz ? (size_t)sz : 1);
+    if (!*data) { fclose(f); return MDZ_ERR_NOMEM; }
+    size_t got = fread(*data, 1, (size_t)sz, f);
+    fclose(f);
+    if (got != (size_t)sz) { free(*data); *data = nullptr; return MDZ_ERR_INVALID; }
+    *n = (uint64_t)sz;
+    return MDZ_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* mdz_version(void) { return "mdz 0.1.0 gfx950"; }
+const char* mdz_last_error(void) { return mdz::g_last_error; }
+
+int mdz_device_count(int* count) { return herr(hipGetDeviceCount(count)); }
+int mdz_dev_malloc(void** ptr, uint64_t bytes) {
+    return hipMalloc(ptr, bytes ? bytes : 1) == hipSuccess ? MDZ_OK : MDZ_ERR_NOMEM;
+}
+int mdz_dev_free(void* ptr) { return herr(hipFree(ptr)); }
+int mdz_memcpy_htod(void* dst, const void* src, uint64_t bytes, void* stream) {
+    return herr(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, S(stream)));
+}
+int mdz_memcpy_dtoh(void* dst, const void* src, uint64_t bytes, void* stream) {
+    return herr(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, S(stream)));
+}
+int mdz_memset_d8(void* dst, int value, uint64_t bytes, void* stream) {
+    return herr(hipMemsetAsync(dst, value, bytes, S(stream)));
+}
+int mdz_stream_create(void** stream) {
+    hipStream_t s;
+    int rc = herr(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    if (rc == MDZ_OK) *stream = s;
+    return rc;
+}
+int mdz_stream_destroy(void* stream) { return herr(hipStreamDestroy(S(stream))); }
+int mdz_stream_sync(void* stream) { return herr(hipStreamSynchronize(S(stream))); }
+int mdz_device_sync(void) { return herr(hipDeviceSynchronize()); }
+int mdz_event_create(void** event) {
+    hipEvent_t e;
+    int rc = herr(hipEventCreate(&e));
+    if (rc == MDZ_OK) *event = e;
+    return rc;
+}
+int mdz_event_destroy(void* event) { return herr(hipEventDestroy((hipEvent_t)event)); }
+int mdz_event_record(void* event, void* stream) { return herr(hipEventRecord((hipEvent_t)event, S(stream))); }
+int mdz_event_sync(void* event) { return herr(hipEventSynchronize((hipEvent_t)event)); }
+int mdz_event_elapsed_ms(void* start, void* stop, float* ms) {
+    return herr(hipEventElapsedTime(ms, (hipEvent_t)start, (hipEvent_t)stop));
+}
+void mdz_host_free(void* p) { free(p); }
+
+// ------------------------------------------------------------------ RLE v1
+int mdz_rlev1_decompress(const mdz_lanes_t* in, void* dst, uint64_t dst_bytes, int32_t* status,
+                         void* stream) {
+    return mdz::rlev1_decompress(in, dst, dst_bytes, status, nullptr, S(stream));
+}
+
+int mdz_rlev1_decompress_tuned(const mdz_lanes_t* in, void* dst, uint64_t dst_bytes,
+                               int32_t* status, const mdz_lanes_tune_t* tune, void* stream) {
+    return mdz::rlev1_decompress(in, dst, dst_bytes, status, tune, S(stream));
+}
+
+int mdz_rlev1_decompress_host(const uint8_t* in, uint8_t** out, uint64_t in_n_bytes,
+                              uint64_t* out_n_bytes, int comp_col_len, uint64_t chunk_size,
+                              uint32_t elem_bytes, const uint64_t* col_len,
+                              const uint64_t* blk_off, uint64_t n_chunks) {
+    if (!in || !out || !out_n_bytes || !col_len || !blk_off || in_n_bytes < 8 || comp_col_len <= 0)
+        return MDZ_ERR_INVALID;
+    uint64_t raw;
+    memcpy(&raw, in, 8);
+    const uint64_t pay_bytes = in_n_bytes - 8;
+    if (blk_off[n_chunks] > pay_bytes || raw > n_chunks * chunk_size) return MDZ_ERR_INVALID;
+    DevBuf d_pay, d_col, d_off, d_out, d_st;
+    int rc;
+    if ((rc = d_pay.alloc(pay_bytes)) || (rc = d_col.alloc(32 * n_chunks * 8)) ||
+        (rc = d_off.alloc((n_chunks + 1) * 8)) || (rc = d_out.alloc(raw)) ||
+        (rc = d_st.alloc(n_chunks * 4)))
+        return rc;
+    if (hipMemcpy(d_pay.p, in + 8, pay_bytes, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(d_col.p, col_len, 32 * n_chunks * 8, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(d_off.p, blk_off, (n_chunks + 1) * 8, hipMemcpyHostToDevice) != hipSuccess)
+        return MDZ_ERR_HIP;
+    mdz_lanes_t L = {d_pay.p, (const uint64_t*)d_off.p, (const uint64_t*)d_col.p, nullptr,
+                     n_chunks, chunk_size, elem_bytes, (uint32_t)comp_col_len};
+    rc = mdz::rlev1_decompress(&L, d_out.p, raw, (int32_t*)d_st.p, nullptr, nullptr);
+    if (rc) return rc;
+    if (hipDeviceSynchronize() != hipSuccess) return MDZ_ERR_HIP;
+    int32_t* st = (int32_t*)malloc(n_chunks * 4 + 4);
+    uint8_t* o = (uint8_t*)malloc(raw ? raw : 1);
+    if (!st || !o) { free(st); free(o); return MDZ_ERR_NOMEM; }
+    if (hipMemcpy(st, d_st.p, n_chunks * 4, hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(o, d_out.p, raw, hipMemcpyDeviceToHost) != hipSuccess) {
+        free(st); free(o);
+        return MDZ_ERR_HIP;
+    }
+    rc = MDZ_OK;
+    for (uint64_t c = 0; c < n_chunks; ++c)
+        if (st[c] != MDZ_ST_OK) { rc = MDZ_ERR_CORRUPT; break; }
+    free(st);
+    *out = o;
+    *out_n_bytes = raw;
+    return rc;
+}
+
+int mdz_rle_v1_decompress_gpu(const uint8_t* in, uint8_t** out, uint64_t in_n_bytes,
+                              uint64_t* out_n_bytes, int comp_col_len, uint64_t chunk_size,
+                              uint32_t elem_bytes) {
+    uint8_t *col = nullptr, *off = nullptr;
+    uint64_t ncol = 0, noff = 0;
+    int rc = read_file("./input_data/rle_v1_col_len.bin", &col, &ncol);
+    if (rc == MDZ_OK) rc = read_file("./input_data/rle_v1_blk_offset.bin", &off, &noff);
+    if (rc == MDZ_OK) {
+        uint64_t n = noff / 8 - 1;
+        if (noff < 16 || ncol < 32 * n * 8) rc = MDZ_ERR_INVALID;
+        else
+            rc = mdz_rlev1_decompress_host(in, out, in_n_bytes, out_n_bytes, comp_col_len,
+                                           chunk_size, elem_bytes, (const uint64_t*)col,
+                                           (const uint64_t*)off, n);
+    }
+    free(col);
+    free(off);
+    return rc;
+}
+
+}  // extern "C"

@@ -1,0 +1,238 @@
+// mdz device helpers for gfx950 (CDNA4, wave64).
+//
+// Building blocks shared by the lane-transposed decoders (RLE v1, RLE v2,
+// LZSS): the per-chunk lane-address table that locates lane t's r-th 4-byte
+// word inside the compacted round-robin interleave, the LDS staging ring, a
+// byte buffer fed from that ring, and a line emitter that turns a lane's
+// value stream into 16-byte (or narrower) global stores.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/mdz.h"
+
+#define MDZ_DEV __device__ __forceinline__
+
+namespace mdz {
+
+constexpr int kWave = 64;
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+constexpr int kLanes = 32;   // sub-streams per chunk, fixed by the container format
+
+MDZ_DEV uint32_t lane_id() { return __lane_id(); }
+
+MDZ_DEV uint32_t wave_min_u32(uint32_t v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        uint32_t w = __shfl_xor(v, o, kWave);
+        v = w < v ? w : v;
+    }
+    return v;
+}
+
+MDZ_DEV uint32_t wave_sum_u32(uint32_t v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+    return v;
+}
+
+MDZ_DEV uint32_t alignbyte(uint32_t hi, uint32_t lo, uint32_t k) {
+    return __builtin_amdgcn_alignbyte(hi, lo, k);
+}
+
+// ------------------------------------------------------------------------
+// Lane-address table.  In a chunk, round r stores word r of every lane that
+// has more than r words, in lane order.  Sort the 32 word counts W ascending
+// (ties by lane id): sW[0] <= ... <= sW[31].  Rounds [sW[k-1], sW[k]) form
+// segment k, in which exactly the lanes of sorted rank >= k are active
+// (mask M[k], 32-k of them).  S[k] = raw word index where segment k starts.
+// Lane t's word r, inside segment k, is at
+//     S[k] + (r - sW[k-1]) * (32-k) + popc(M[k] & lanes_below(t)).
+struct LaneTable {
+    uint32_t sW[kLanes];
+    uint32_t S[kLanes];
+    uint32_t M[kLanes];
+    uint32_t W[kLanes];
+    uint32_t rank[kLanes];
+    uint32_t total;     // sum of W: words the chunk's lanes occupy
+};
+
+// Builds the table; every lane of the wave calls it (lanes 32..63 mirror
+// 0..31).  `W` is this lane's word count.
+MDZ_DEV void lane_table_build(LaneTable& T, uint32_t W) {
+    const uint32_t l = lane_id(), t = l & 31;
+    if (l < 32) T.W[t] = W;
+    __syncthreads();
+    uint32_t rk = 0;
+#pragma unroll 8
+    for (uint32_t u = 0; u < 32; ++u) {
+        uint32_t Wu = T.W[u];
+        rk += (Wu < W) || (Wu == W && u < t);
+    }
+    if (l < 32) { T.sW[rk] = W; T.rank[t] = rk; }
+    __syncthreads();
+    if (l < 32) {
+        uint32_t m = 0;
+#pragma unroll 8
+        for (uint32_t u = 0; u < 32; ++u) m |= (T.rank[u] >= t ? 1u : 0u) << u;
+        uint32_t s = 0, prev = 0;
+        for (uint32_t j = 0; j < t; ++j) {
+            uint32_t w = T.sW[j];
+            s += (w - prev) * (32 - j);
+            prev = w;
+        }
+        T.M[t] = m;
+        T.S[t] = s;
+    }
+    uint32_t tot = wave_sum_u32(l < 32 ? W : 0);
+    if (l == 0) T.total = tot;
+    __syncthreads();
+}
+
+// Per-lane cursor over its own words.
+struct LaneCursor {
+    uint32_t r;        // next word to fetch
+    uint32_t W;        // word count
+    uint32_t k;        // current segment
+    uint32_t base, stride, seg_end;
+
+    MDZ_DEV void enter(const LaneTable& T, uint32_t t) {
+        stride = 32 - k;
+        uint32_t prevW = k ? T.sW[k - 1] : 0;
+        base = T.S[k] - prevW * stride + __popc(T.M[k] & ((1u << t) - 1u));
+        seg_end = T.sW[k];
+    }
+    MDZ_DEV void init(const LaneTable& T, uint32_t t, uint32_t words) {
+        r = 0; W = words; k = 0;
+        if (W) {
+            while (T.sW[k] == 0) ++k;
+            enter(T, t);
+        }
+    }
+    MDZ_DEV uint32_t addr() const { return base + r * stride; }
+    MDZ_DEV void advance(const LaneTable& T, uint32_t t) {
+        ++r;
+        if (r < W && r >= seg_end) {
+            do { ++k; } while (T.sW[k] <= r);
+            enter(T, t);
+        }
+    }
+};
+
+// ------------------------------------------------------------------------
+// Stage raw chunk words [a, b) (both multiples of 4) into the LDS ring of
+// `ring_words` (power of 2) with 16-byte loads; all 64 lanes participate.
+MDZ_DEV void ring_stage(uint32_t* ring, uint32_t ring_words, const u32x4* __restrict__ src,
+                        uint32_t a, uint32_t b) {
+    const uint32_t l = lane_id();
+    const uint32_t gmask = ring_words / 4 - 1;
+    u32x4* ring4 = reinterpret_cast<u32x4*>(ring);
+    uint32_t g = a / 4 + l, ge = b / 4;
+    for (; g + 3 * kWave < ge; g += 4 * kWave) {
+        u32x4 v0 = __builtin_nontemporal_load(src + g);
+        u32x4 v1 = __builtin_nontemporal_load(src + g + kWave);
+        u32x4 v2 = __builtin_nontemporal_load(src + g + 2 * kWave);
+        u32x4 v3 = __builtin_nontemporal_load(src + g + 3 * kWave);
+        ring4[g & gmask] = v0;
+        ring4[(g + kWave) & gmask] = v1;
+        ring4[(g + 2 * kWave) & gmask] = v2;
+        ring4[(g + 3 * kWave) & gmask] = v3;
+    }
+    for (; g < ge; g += kWave) ring4[g & gmask] = __builtin_nontemporal_load(src + g);
+}
+
+// ------------------------------------------------------------------------
+// Byte buffer: up to 8 (NB=8) or 16 (NB=16) stream bytes, little-endian,
+// byte 0 = next unread stream byte.
+template <int NB> struct ByteBuf;
+
+template <> struct ByteBuf<8> {
+    uint64_t v = 0;
+    uint32_t nb = 0;
+    MDZ_DEV void append(uint32_t w, uint32_t nbytes) { v |= (uint64_t)w << (8 * nb); nb += nbytes; }
+    MDZ_DEV uint64_t lo() const { return v; }
+    MDZ_DEV uint64_t hi() const { return 0; }
+    MDZ_DEV void consume(uint32_t n) {
+        v = n >= 8 ? 0 : v >> (8 * n);
+        nb -= n;
+    }
+};
+
+template <> struct ByteBuf<16> {
+    unsigned __int128 v = 0;
+    uint32_t nb = 0;
+    MDZ_DEV void append(uint32_t w, uint32_t nbytes) {
+        v |= (unsigned __int128)w << (8 * nb);
+        nb += nbytes;
+    }
+    MDZ_DEV uint64_t lo() const { return (uint64_t)v; }
+    MDZ_DEV uint64_t hi() const { return (uint64_t)(v >> 64); }
+    MDZ_DEV void consume(uint32_t n) {
+        v = n >= 16 ? 0 : v >> (8 * n);
+        nb -= n;
+    }
+};
+
+// ------------------------------------------------------------------------
+// Line emitter: lane t owns lines t, t+32, t+64, ... of `line` bytes in its
+// chunk's output.  Values are shifted into a 16-byte register window; every
+// SW bytes are written with one store (SW = 16, 8, 4, 2 or 1, dividing line).
+template <int EB, int SW> struct LineEmitter {
+    uint32_t a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+    uint32_t na = 0;          // bytes pending in the window
+    uint8_t* out;             // chunk output base
+    uint64_t opos;            // chunk byte offset of the pending group
+    uint64_t limit;           // bytes of this chunk that may be written
+    uint32_t lrem;            // bytes left in the current line
+    uint32_t line;
+
+    MDZ_DEV void init(uint8_t* chunk_out, uint64_t lim, uint32_t t, uint32_t line_bytes) {
+        out = chunk_out; limit = lim; line = line_bytes;
+        opos = (uint64_t)t * line_bytes; lrem = line_bytes; na = 0;
+    }
+    MDZ_DEV void push(uint64_t v) {
+        if constexpr (EB == 4) {
+            a0 = a1; a1 = a2; a2 = a3; a3 = (uint32_t)v;
+        } else if constexpr (EB == 8) {
+            a0 = a2; a1 = a3; a2 = (uint32_t)v; a3 = (uint32_t)(v >> 32);
+        } else if constexpr (EB == 2) {
+            a0 = alignbyte(a1, a0, 2); a1 = alignbyte(a2, a1, 2); a2 = alignbyte(a3, a2, 2);
+            a3 = (a3 >> 16) | ((uint32_t)v << 16);
+        } else {
+            a0 = alignbyte(a1, a0, 1); a1 = alignbyte(a2, a1, 1); a2 = alignbyte(a3, a2, 1);
+            a3 = (a3 >> 8) | ((uint32_t)v << 24);
+        }
+        na += EB;
+        if (na == SW) flush();
+    }
+    MDZ_DEV void flush() {
+        uint8_t* p = out + opos;
+        if (opos + SW <= limit) {
+            if constexpr (SW == 16) {
+                *reinterpret_cast<uint4*>(p) = make_uint4(a0, a1, a2, a3);
+            } else if constexpr (SW == 8) {
+                *reinterpret_cast<uint2*>(p) = make_uint2(a2, a3);
+            } else if constexpr (SW == 4) {
+                *reinterpret_cast<uint32_t*>(p) = a3;
+            } else if constexpr (SW == 2) {
+                *reinterpret_cast<uint16_t*>(p) = (uint16_t)(a3 >> 16);
+            } else {
+                *p = (uint8_t)(a3 >> 24);
+            }
+        } else if (opos < limit) {
+            // tail of a clipped output: bytewise
+            uint32_t w[4] = {a0, a1, a2, a3};
+            for (uint32_t i = 0; i < SW && opos + i < limit; ++i) {
+                uint32_t bi = 16 - SW + i;
+                p[i] = (uint8_t)(w[bi >> 2] >> (8 * (bi & 3)));
+            }
+        }
+        na = 0;
+        opos += SW;
+        lrem -= SW;
+        if (lrem == 0) { opos += (uint64_t)31 * line; lrem = line; }
+    }
+};
+
+}  // namespace mdz

@@ -1,0 +1,23 @@
+// Internal (C++) interfaces between the C-ABI layer (api.hip) and the
+// per-codec launchers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/mdz.h"
+
+namespace mdz {
+
+// Records a failed HIP call for mdz_last_error(); returns MDZ_OK/MDZ_ERR_HIP.
+int record_hip(hipError_t e, const char* what);
+
+// Raises a kernel's dynamic-LDS limit to at least `bytes` (gfx950 allows
+// 160 KiB per workgroup, static LDS included).  Thread-safe; each
+// kernel is raised once to the largest size requested so far.
+hipError_t set_dynamic_lds(const void* kernel, size_t bytes);
+
+int rlev1_decompress(const mdz_lanes_t* in, void* dst, uint64_t dst_bytes, int32_t* status,
+                     const mdz_lanes_tune_t* tune, hipStream_t s);
+
+}  // namespace mdz

@@ -1,0 +1,129 @@
+/*
+ * mdz -- MI355X-native (gfx950) decompression library: C ABI.
+ *
+ * Every decode entry point takes DEVICE pointers, is stream-ordered (it only
+ * enqueues work on `stream`, a hipStream_t passed as void*; NULL = default
+ * stream) and is reentrant.  It returns MDZ_OK or a negative MDZ_ERR_* API
+ * error; per-chunk (per-stream) decode status lands in a device int32 array
+ * using the MDZ_ST_* codes below, which follow the cuDF inflate status
+ * convention (include/cudf_deflate.cuh:440,527-574 in the reference).
+ *
+ * The *_host entry points keep the reference's host-pointer signature shape
+ * (callee allocates *out, caller frees it with mdz_host_free) so a reference
+ * driver works by swapping the call.  They copy in, decode, synchronise and
+ * copy out.
+ */
+#ifndef MDZ_H
+#define MDZ_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------ API errors */
+#define MDZ_OK               0
+#define MDZ_ERR_INVALID     -1   /* bad argument / container metadata */
+#define MDZ_ERR_HIP         -2   /* a HIP runtime call failed */
+#define MDZ_ERR_NOMEM       -3   /* device or host allocation failed */
+#define MDZ_ERR_UNSUPPORTED -4   /* parameter combination not built */
+#define MDZ_ERR_CORRUPT     -5   /* *_host wrappers: some chunk status != 0 */
+
+/* ------------------------------------------------- per-chunk status codes */
+#define MDZ_ST_OK               0
+#define MDZ_ST_OUTPUT_OVERRUN   1   /* decoded data exceeds the output slot */
+#define MDZ_ST_INPUT_OVERRUN    2   /* input ended before the output was complete */
+#define MDZ_ST_BAD_BLOCK_TYPE  -1
+#define MDZ_ST_STORED_LEN      -2
+#define MDZ_ST_BAD_CODE_LENS   -3
+#define MDZ_ST_BAD_REPEAT      -4
+#define MDZ_ST_BAD_LITLEN      -5
+#define MDZ_ST_BAD_DIST        -6
+#define MDZ_ST_BAD_COUNTS      -7
+#define MDZ_ST_NO_EOB          -8
+#define MDZ_ST_BAD_DISTANCE    -9
+#define MDZ_ST_BAD_SYMBOL     -10
+#define MDZ_ST_BAD_HEADER     -11
+
+/* --------------------------------------------------------------- version */
+/* Returns "mdz <version> gfx950". */
+const char* mdz_version(void);
+/* Text of the last HIP error an mdz call on this host thread saw ("" if none). */
+const char* mdz_last_error(void);
+
+/* -------------------------------------------- device plumbing (thin HIP) */
+/* For hosts that do not link HIP themselves (the Python tests and bench). */
+int mdz_device_count(int* count);
+int mdz_dev_malloc(void** ptr, uint64_t bytes);
+int mdz_dev_free(void* ptr);
+int mdz_memcpy_htod(void* dst, const void* src, uint64_t bytes, void* stream);
+int mdz_memcpy_dtoh(void* dst, const void* src, uint64_t bytes, void* stream);
+int mdz_memset_d8(void* dst, int value, uint64_t bytes, void* stream);
+int mdz_stream_create(void** stream);
+int mdz_stream_destroy(void* stream);
+int mdz_stream_sync(void* stream);
+int mdz_device_sync(void);
+int mdz_event_create(void** event);
+int mdz_event_destroy(void* event);
+int mdz_event_record(void* event, void* stream);
+int mdz_event_sync(void* event);
+int mdz_event_elapsed_ms(void* start, void* stop, float* ms);
+void mdz_host_free(void* p);
+
+/* ==================================================== lane-transposed ==== *
+ * RLE v1, RLE v2 and LZSS share one container shape (reference
+ * include/rle_v1/encoder.h:540-588, include/rlev2/encoder_trans.h:291-349,
+ * include/lzss/lzss_gpu2.h:497-509): chunk c's payload is
+ * [blk_off[c], blk_off[c+1]) (128-byte aligned); inside it the 32 lane
+ * streams of col_len[32c+t] bytes are interleaved in 4-byte words, compacted
+ * round-robin over the lanes that still have words. */
+typedef struct {
+    const void*     payload;     /* chunk data (no leading size word) */
+    const uint64_t* blk_off;     /* n_chunks + 1 byte offsets into payload */
+    const uint64_t* col_len;     /* 32 * n_chunks lane-stream byte lengths */
+    const uint8_t*  col_map;     /* LZSS only: 32 * n_chunks sorted->original lane */
+    uint64_t        n_chunks;
+    uint64_t        chunk_bytes; /* uncompressed bytes per chunk */
+    uint32_t        elem_bytes;  /* RLE v1: DATA_TYPE size {1,2,4,8} */
+    uint32_t        lane_line;   /* RLE v1: COMP_COL_LEN bytes per lane line */
+} mdz_lanes_t;
+
+/* Tuning knobs for the lane-transposed decoders; zero fields pick defaults. */
+typedef struct {
+    uint32_t ring_words;         /* LDS staging ring per chunk, 4-byte words (power of 2) */
+    uint32_t reserved[7];
+} mdz_lanes_tune_t;
+
+/* RLE v1 decode -- replaces rle_v1::decompress_gpu<READ_COL_TYPE, DATA_TYPE,
+ * OUT_COL_TYPE, QUEUE_SIZE> (reference include/rle_v1/rlev1.h:99-199) and its
+ * inflate kernel (include/rle_v1/decoder.h:426-470).  Chunk c is written to
+ * dst[c*chunk_bytes, (c+1)*chunk_bytes) clipped to dst_bytes.  status: device
+ * int32[n_chunks] or NULL. */
+int mdz_rlev1_decompress(const mdz_lanes_t* in, void* dst, uint64_t dst_bytes,
+                         int32_t* status, void* stream);
+int mdz_rlev1_decompress_tuned(const mdz_lanes_t* in, void* dst, uint64_t dst_bytes,
+                               int32_t* status, const mdz_lanes_tune_t* tune, void* stream);
+
+/* Host-pointer RLE v1 decode in the reference's signature shape
+ * (include/rle_v1/rlev1.h:99-100): `in` is the reference payload file
+ * [u64 uncompressed_bytes][chunks...]; metadata that the reference reads
+ * from ./input_data/rle_v1_{col_len,blk_offset}.bin is passed explicitly.
+ * *out (uncompressed_bytes) is allocated here; free with mdz_host_free. */
+int mdz_rlev1_decompress_host(const uint8_t* in, uint8_t** out, uint64_t in_n_bytes,
+                              uint64_t* out_n_bytes, int comp_col_len, uint64_t chunk_size,
+                              uint32_t elem_bytes, const uint64_t* col_len,
+                              const uint64_t* blk_off, uint64_t n_chunks);
+
+/* Exactly rle_v1::decompress_gpu's arguments (rlev1.h:99-100), reading the
+ * side files ./input_data/rle_v1_col_len.bin and rle_v1_blk_offset.bin from
+ * the current directory as the reference does (rlev1.h:110-147). */
+int mdz_rle_v1_decompress_gpu(const uint8_t* in, uint8_t** out, uint64_t in_n_bytes,
+                              uint64_t* out_n_bytes, int comp_col_len, uint64_t chunk_size,
+                              uint32_t elem_bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

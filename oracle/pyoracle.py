@@ -1,0 +1,210 @@
+"""ctypes binding of the CPU oracle (libmdz_oracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by ``__graft_entry__.smoke()``
+as the checker, and by bench.py's ``cpu_baseline`` leg.  The product library
+never imports this module.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "libmdz_oracle.so")
+_lib = None
+
+u8p = C.POINTER(C.c_uint8)
+u64p = C.POINTER(C.c_uint64)
+i32p = C.POINTER(C.c_int32)
+
+
+def build(force: bool = False) -> str:
+    """Compile the oracle with its Makefile (gcc); returns the .so path."""
+    if force or not os.path.exists(_LIB_PATH):
+        subprocess.run(["make", "-C", _HERE, "-B" if force else "-s"], check=True,
+                       stdout=subprocess.DEVNULL)
+    return _LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = C.CDLL(_LIB_PATH)
+        L.mdz_rng_seed.argtypes = [C.c_void_p, C.c_uint64]
+        L.mdz_rng_next.restype = C.c_uint64
+        L.mdz_rng_next.argtypes = [C.c_void_p]
+        L.mdz_host_cores.restype = C.c_int
+        L.mdz_free.argtypes = [C.c_void_p]
+        L.mdz_ref_rlev1_lane_encode.restype = C.c_uint64
+        L.mdz_ref_rlev1_lane_encode.argtypes = [C.c_void_p, C.c_uint64, C.c_int, C.c_void_p, C.c_uint64]
+        L.mdz_ref_rlev1_lane_decode.restype = C.c_int
+        L.mdz_ref_rlev1_lane_decode.argtypes = [C.c_void_p, C.c_uint64, C.c_int, C.c_void_p, C.c_uint64]
+        L.mdz_ref_rlev1_encode.restype = C.c_int
+        L.mdz_ref_rlev1_encode.argtypes = [C.c_void_p, C.c_uint64, C.c_uint64, C.c_int, C.c_uint32,
+                                           C.POINTER(u8p), u64p, C.POINTER(u64p), C.POINTER(u64p),
+                                           u64p, C.c_int]
+        L.mdz_ref_rlev1_decode.restype = C.c_int
+        L.mdz_ref_rlev1_decode.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64, C.c_uint64,
+                                           C.c_int, C.c_uint32, C.c_void_p, C.c_uint64, C.c_void_p, C.c_int]
+        L.mdz_ref_brle_chunk_encode.restype = C.c_uint64
+        L.mdz_ref_brle_chunk_encode.argtypes = [C.c_void_p, C.c_uint64, C.c_void_p]
+        L.mdz_ref_brle_chunk_decode.restype = C.c_int
+        L.mdz_ref_brle_chunk_decode.argtypes = [C.c_void_p, C.c_uint64, C.c_void_p, C.c_uint64, u64p]
+        L.mdz_ref_brle_encode.restype = C.c_int
+        L.mdz_ref_brle_encode.argtypes = [C.c_void_p, C.c_uint64, C.c_uint64, C.POINTER(u8p),
+                                          C.POINTER(u64p), u64p, C.c_int]
+        L.mdz_ref_brle_decode.restype = C.c_int
+        L.mdz_ref_brle_decode.argtypes = [C.c_void_p, C.c_void_p, C.c_uint64, C.c_uint64, C.c_void_p,
+                                          C.c_uint64, C.c_void_p, C.c_int]
+        L.mdz_gen_rlev1_u32.argtypes = [C.c_void_p, C.c_uint64, C.c_uint64]
+        L.mdz_gen_brle_u8.argtypes = [C.c_void_p, C.c_uint64, C.c_uint64]
+        _lib = L
+    return _lib
+
+
+def _ptr(a: np.ndarray):
+    return C.c_void_p(a.ctypes.data)
+
+
+def host_cores() -> int:
+    return int(lib().mdz_host_cores())
+
+
+def default_threads() -> int:
+    return max(1, min(host_cores(), 16))
+
+
+# ------------------------------------------------------------------- PRNG --
+def rng_stream(seed: int, n: int) -> np.ndarray:
+    state = (C.c_uint64 * 4)()
+    lib().mdz_rng_seed(state, seed)
+    return np.array([lib().mdz_rng_next(state) for _ in range(n)], dtype=np.uint64)
+
+
+# ----------------------------------------------------------------- RLE v1 --
+_DT = {1: np.uint8, 2: np.uint16, 4: np.uint32, 8: np.uint64}
+
+
+def rlev1_lane_encode(vals: np.ndarray) -> bytes:
+    vals = np.ascontiguousarray(vals)
+    eb = vals.dtype.itemsize
+    cap = len(vals) * 11 + 16
+    out = np.zeros(cap, np.uint8)
+    n = lib().mdz_ref_rlev1_lane_encode(_ptr(vals), len(vals), eb, _ptr(out), cap)
+    return out[:n].tobytes()
+
+
+def rlev1_lane_decode(stream: bytes, n_vals: int, dtype) -> tuple[int, np.ndarray]:
+    src = np.frombuffer(stream, np.uint8).copy() if stream else np.zeros(1, np.uint8)
+    out = np.zeros(n_vals, dtype)
+    st = lib().mdz_ref_rlev1_lane_decode(_ptr(src), len(stream), out.dtype.itemsize, _ptr(out), n_vals)
+    return st, out
+
+
+class RleV1Container:
+    """Lane-transposed RLE v1 container: payload + col_len[32n] + blk_off[n+1]."""
+
+    def __init__(self, payload, col_len, blk_off, n_chunks, chunk, elem_bytes, line, raw_bytes):
+        self.payload, self.col_len, self.blk_off = payload, col_len, blk_off
+        self.n_chunks, self.chunk, self.elem_bytes, self.line = n_chunks, chunk, elem_bytes, line
+        self.raw_bytes = raw_bytes
+
+
+def _take(ptr, n, dtype):
+    if n == 0:
+        return np.zeros(0, dtype)
+    arr = np.ctypeslib.as_array(C.cast(ptr, C.POINTER(np.ctypeslib.as_ctypes_type(dtype))), shape=(n,)).copy()
+    lib().mdz_free(C.cast(ptr, C.c_void_p))
+    return arr
+
+
+def rlev1_encode(data: np.ndarray, chunk: int, line: int, threads: int | None = None) -> RleV1Container:
+    data = np.ascontiguousarray(data)
+    raw = data.view(np.uint8).reshape(-1)
+    pay, col, off = u8p(), u64p(), u64p()
+    pb, nc = C.c_uint64(), C.c_uint64()
+    rc = lib().mdz_ref_rlev1_encode(_ptr(raw), raw.size, chunk, data.dtype.itemsize, line,
+                                    C.byref(pay), C.byref(pb), C.byref(col), C.byref(off),
+                                    C.byref(nc), threads or default_threads())
+    if rc != 0:
+        raise ValueError(f"rlev1 encode rejected parameters (status {rc})")
+    n = nc.value
+    return RleV1Container(_take(pay, pb.value, np.uint8), _take(col, 32 * n, np.uint64),
+                          _take(off, n + 1, np.uint64), n, chunk, data.dtype.itemsize, line, raw.size)
+
+
+def rlev1_decode(c: RleV1Container, out_bytes: int | None = None, threads: int | None = None):
+    out_bytes = c.raw_bytes if out_bytes is None else out_bytes
+    out = np.zeros(max(out_bytes, 1), np.uint8)
+    status = np.zeros(max(c.n_chunks, 1), np.int32)
+    pay = c.payload if c.payload.size else np.zeros(1, np.uint8)
+    rc = lib().mdz_ref_rlev1_decode(_ptr(pay), _ptr(c.blk_off), _ptr(c.col_len), c.n_chunks, c.chunk,
+                                    c.elem_bytes, c.line, _ptr(out), out_bytes, _ptr(status),
+                                    threads or default_threads())
+    return rc, out[:out_bytes], status[:c.n_chunks]
+
+
+# --------------------------------------------------------------- byte RLE --
+def brle_chunk_encode(data: bytes) -> bytes:
+    src = np.frombuffer(data, np.uint8).copy() if data else np.zeros(1, np.uint8)
+    out = np.zeros(len(data) + len(data) // 128 + 2, np.uint8)
+    n = lib().mdz_ref_brle_chunk_encode(_ptr(src), len(data), _ptr(out))
+    return out[:n].tobytes()
+
+
+def brle_chunk_decode(stream: bytes, cap: int) -> tuple[int, bytes]:
+    src = np.frombuffer(stream, np.uint8).copy() if stream else np.zeros(1, np.uint8)
+    out = np.zeros(max(cap, 1), np.uint8)
+    got = C.c_uint64()
+    st = lib().mdz_ref_brle_chunk_decode(_ptr(src), len(stream), _ptr(out), cap, C.byref(got))
+    return st, out[:got.value].tobytes()
+
+
+class BrleContainer:
+    def __init__(self, data, ptr, n_chunks, chunk, raw_bytes):
+        self.data, self.ptr, self.n_chunks, self.chunk, self.raw_bytes = data, ptr, n_chunks, chunk, raw_bytes
+
+    def to_bytes(self) -> bytes:
+        """Reference file layout [u32 n+1][u64 ptr[n+1]][u64 out_bytes][data] (brle.h:278-292)."""
+        return (np.uint32(self.n_chunks + 1).tobytes() + self.ptr.astype('<u8').tobytes()
+                + np.uint64(self.raw_bytes).tobytes() + self.data.tobytes())
+
+
+def brle_encode(data: np.ndarray, chunk: int, threads: int | None = None) -> BrleContainer:
+    raw = np.ascontiguousarray(data).view(np.uint8).reshape(-1)
+    d, p, nc = u8p(), u64p(), C.c_uint64()
+    rc = lib().mdz_ref_brle_encode(_ptr(raw), raw.size, chunk, C.byref(d), C.byref(p), C.byref(nc),
+                                   threads or default_threads())
+    if rc != 0:
+        raise ValueError(f"brle encode rejected parameters (status {rc})")
+    n = nc.value
+    ptr = _take(p, n + 1, np.uint64)
+    return BrleContainer(_take(d, int(ptr[-1]), np.uint8), ptr, n, chunk, raw.size)
+
+
+def brle_decode(c: BrleContainer, out_bytes: int | None = None, threads: int | None = None):
+    out_bytes = c.raw_bytes if out_bytes is None else out_bytes
+    out = np.zeros(max(out_bytes, 1), np.uint8)
+    status = np.zeros(max(c.n_chunks, 1), np.int32)
+    data = c.data if c.data.size else np.zeros(1, np.uint8)
+    rc = lib().mdz_ref_brle_decode(_ptr(data), _ptr(c.ptr), c.n_chunks, c.chunk, _ptr(out), out_bytes,
+                                   _ptr(status), threads or default_threads())
+    return rc, out[:out_bytes], status[:c.n_chunks]
+
+
+# ------------------------------------------------------------- generators --
+def gen_rlev1_u32(n: int, seed: int) -> np.ndarray:
+    out = np.empty(n, np.uint32)
+    lib().mdz_gen_rlev1_u32(_ptr(out), n, seed)
+    return out
+
+
+def gen_brle_u8(n: int, seed: int) -> np.ndarray:
+    out = np.empty(n, np.uint8)
+    lib().mdz_gen_brle_u8(_ptr(out), n, seed)
+    return out
