@@ -49,7 +49,7 @@ class RleV1Workload:
     name = "rlev1_u32_64MiB_plain"
     dtype = "u32"
 
-    def __init__(self, line=32, ring=16384, n_vals=1 << 24, seed=0x4D445A01):
+    def __init__(self, line=128, ring=0, n_vals=1 << 24, seed=0x4D445A01):
         from oracle import pyoracle as O
         from compression_amd.lanes import DeviceLanes
         import compression_amd as mdz
@@ -62,17 +62,21 @@ class RleV1Workload:
         self.dst = mdz.DeviceBuffer(c.raw_bytes)
         self.status = mdz.DeviceBuffer(4 * c.n_chunks)
         self.out_bytes = c.raw_bytes
+        self.max_chunk = int(np.diff(c.blk_off.astype(np.int64)).max())
         n = c.n_chunks
         self.alg_bytes = int(c.blk_off[-1]) + 256 * n + 8 * (n + 1) + c.raw_bytes
         self.config = {"workload": self.name, "codec": "rle_v1", "elem": "u32",
                        "out_bytes": c.raw_bytes, "payload_bytes": int(c.blk_off[-1]),
                        "n_chunks": n, "chunk_bytes": 131072, "comp_col_len": line,
-                       "ring_words": ring, "ratio": round(int(c.blk_off[-1]) / c.raw_bytes, 4)}
+                       "max_chunk_payload": self.max_chunk,
+                       "ratio": round(int(c.blk_off[-1]) / c.raw_bytes, 4)}
+        if ring:
+            self.config["ring_words"] = ring
 
     def step(self, stream):
         from compression_amd import rle_v1
         rle_v1.decompress(self.lanes, self.dst, self.out_bytes, self.status, stream=stream,
-                          ring_words=self.ring)
+                          ring_words=self.ring, max_chunk_bytes=self.max_chunk)
 
     def verify(self):
         got = self.dst.download(self.out_bytes)
@@ -89,7 +93,35 @@ class RleV1Workload:
                           f"oracle C decoder, median of >=3 runs"}
 
 
-WORKLOADS = {"rlev1": RleV1Workload}
+class CopyWorkload:
+    """Calibration: a plain 16-byte-vector device copy of 512 MiB (reads and
+    writes each byte once).  Its FETCH_SIZE/WRITE_SIZE per byte scale the
+    decoders' counters to HBM bytes."""
+    name = "copy_512MiB"
+    dtype = "u8"
+
+    def __init__(self, nbytes=512 << 20, **_):
+        import compression_amd as mdz
+        self.mdz = mdz
+        self.src = mdz.DeviceBuffer(nbytes)
+        self.src.fill(0x5A)
+        self.dst = mdz.DeviceBuffer(nbytes)
+        self.out_bytes = nbytes
+        self.alg_bytes = 2 * nbytes
+        self.config = {"workload": self.name, "bytes": nbytes}
+
+    def step(self, stream):
+        self.mdz._lib.check(self.mdz.lib().mdz_dev_copy(self.dst.ptr, self.src.ptr, self.out_bytes,
+                                                        stream), "mdz_dev_copy")
+
+    def verify(self):
+        return bool((self.dst.download(4096) == 0x5A).all())
+
+    def cpu(self, threads):
+        return None
+
+
+WORKLOADS = {"rlev1": RleV1Workload, "copy": CopyWorkload}
 DEFAULT = "rlev1"
 
 
@@ -99,7 +131,12 @@ def run(args):
 
     if mdz.device_count() < 1:
         raise SystemExit("bench.py needs a GPU")
-    wl = WORKLOADS[args.workload](**({"line": args.line} if args.line else {}))
+    kw = {}
+    if args.line:
+        kw["line"] = args.line
+    if args.ring:
+        kw["ring"] = args.ring
+    wl = WORKLOADS[args.workload](**kw)
     stream = mdz.Stream()
     for _ in range(args.warmup):
         wl.step(stream.ptr)
@@ -128,7 +165,9 @@ def run(args):
         "bit_exact": ok,
     }
     if not args.no_cpu:
-        line["cpu_baseline"] = wl.cpu(threads)
+        cb = wl.cpu(threads)
+        if cb is not None:
+            line["cpu_baseline"] = cb
     print(json.dumps(line), flush=True)
 
 
@@ -139,6 +178,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default=DEFAULT, choices=sorted(WORKLOADS))
     ap.add_argument("--line", type=int, default=0, help="RLE v1 COMP_COL_LEN override")
+    ap.add_argument("--ring", type=int, default=0, help="LDS ring words override")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     args = ap.parse_args()
     if args.gpus != 1:

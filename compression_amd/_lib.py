@@ -37,7 +37,7 @@ class LanesT(C.Structure):
 
 
 class LanesTuneT(C.Structure):
-    _fields_ = [("ring_words", u32), ("reserved", u32 * 7)]
+    _fields_ = [("ring_words", u32), ("max_chunk_bytes", u32), ("reserved", u32 * 6)]
 
 
 # name -> (restype, argtypes); every symbol declared in include/mdz.h
@@ -50,6 +50,7 @@ SIGNATURES = {
     "mdz_memcpy_htod": (C.c_int, [vp, vp, u64, vp]),
     "mdz_memcpy_dtoh": (C.c_int, [vp, vp, u64, vp]),
     "mdz_memset_d8": (C.c_int, [vp, C.c_int, u64, vp]),
+    "mdz_dev_copy": (C.c_int, [vp, vp, u64, vp]),
     "mdz_stream_create": (C.c_int, [C.POINTER(vp)]),
     "mdz_stream_destroy": (C.c_int, [vp]),
     "mdz_stream_sync": (C.c_int, [vp]),

@@ -84,6 +84,9 @@ int mdz_memcpy_dtoh(void* dst, const void* src, uint64_t bytes, void* stream) {
 int mdz_memset_d8(void* dst, int value, uint64_t bytes, void* stream) {
     return herr(hipMemsetAsync(dst, value, bytes, S(stream)));
 }
+int mdz_dev_copy(void* dst, const void* src, uint64_t bytes, void* stream) {
+    return mdz::dev_copy(dst, src, bytes, S(stream));
+}
 int mdz_stream_create(void** stream) {
     hipStream_t s;
     int rc = herr(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));

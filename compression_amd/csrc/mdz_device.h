@@ -58,12 +58,20 @@ struct LaneTable {
     uint32_t total;     // sum of W: words the chunk's lanes occupy
 };
 
-// Builds the table; every lane of the wave calls it (lanes 32..63 mirror
-// 0..31).  `W` is this lane's word count.
+// Orders LDS traffic among the lanes of one wave (no cross-wave effect).
+MDZ_DEV void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
+// Builds the table.  Called by all 64 lanes of ONE wave (lanes 32..63 mirror
+// 0..31); `W` is the calling lane's word count.  Uses only wave-level
+// synchronisation, so other waves of the workgroup need not take part.
 MDZ_DEV void lane_table_build(LaneTable& T, uint32_t W) {
     const uint32_t l = lane_id(), t = l & 31;
     if (l < 32) T.W[t] = W;
-    __syncthreads();
+    wave_sync();
     uint32_t rk = 0;
 #pragma unroll 8
     for (uint32_t u = 0; u < 32; ++u) {
@@ -71,7 +79,7 @@ MDZ_DEV void lane_table_build(LaneTable& T, uint32_t W) {
         rk += (Wu < W) || (Wu == W && u < t);
     }
     if (l < 32) { T.sW[rk] = W; T.rank[t] = rk; }
-    __syncthreads();
+    wave_sync();
     if (l < 32) {
         uint32_t m = 0;
 #pragma unroll 8
@@ -87,7 +95,36 @@ MDZ_DEV void lane_table_build(LaneTable& T, uint32_t W) {
     }
     uint32_t tot = wave_sum_u32(l < 32 ? W : 0);
     if (l == 0) T.total = tot;
-    __syncthreads();
+    wave_sync();
+}
+
+// Position (0-based) of the q-th set bit of m (q < popc(m)).
+MDZ_DEV uint32_t select_bit(uint32_t m, uint32_t q) {
+    uint32_t pos = 0, c;
+    c = __popc(m & 0xffffu); if (q >= c) { q -= c; m >>= 16; pos += 16; }
+    c = __popc(m & 0xffu);   if (q >= c) { q -= c; m >>= 8;  pos += 8; }
+    c = __popc(m & 0xfu);    if (q >= c) { q -= c; m >>= 4;  pos += 4; }
+    c = __popc(m & 0x3u);    if (q >= c) { q -= c; m >>= 2;  pos += 2; }
+    c = m & 1u;              if (q >= c) { pos += 1; }
+    return pos;
+}
+
+// Inverse of the lane-address map: raw word w (< T.total) -> (lane, round).
+MDZ_DEV void lane_of_word(const LaneTable& T, uint32_t w, uint32_t& lane, uint32_t& round) {
+    if (w < T.S[1]) {
+        lane = w & 31; round = w >> 5;          // segment 0: all 32 lanes active
+        return;
+    }
+    uint32_t lo = 0, hi = 31;
+    while (lo < hi) {                            // largest k with S[k] <= w
+        uint32_t mid = (lo + hi + 1) >> 1;
+        if (T.S[mid] <= w) lo = mid; else hi = mid - 1;
+    }
+    const uint32_t k = lo, nk = 32 - k;
+    const uint32_t off = w - T.S[k];
+    const uint32_t q = off / nk;
+    round = (k ? T.sW[k - 1] : 0) + q;
+    lane = select_bit(T.M[k], off - q * nk);
 }
 
 // Per-lane cursor over its own words.
@@ -187,9 +224,14 @@ template <int EB, int SW> struct LineEmitter {
     uint32_t lrem;            // bytes left in the current line
     uint32_t line;
 
-    MDZ_DEV void init(uint8_t* chunk_out, uint64_t lim, uint32_t t, uint32_t line_bytes) {
+    // Starts at byte `lane_off` of lane t's own byte sequence (a multiple of SW).
+    MDZ_DEV void init(uint8_t* chunk_out, uint64_t lim, uint32_t t, uint32_t line_bytes,
+                      uint64_t lane_off) {
         out = chunk_out; limit = lim; line = line_bytes;
-        opos = (uint64_t)t * line_bytes; lrem = line_bytes; na = 0;
+        const uint64_t ln = lane_off / line_bytes, in_line = lane_off - ln * line_bytes;
+        opos = (ln * 32 + t) * line_bytes + in_line;
+        lrem = line_bytes - (uint32_t)in_line;
+        na = 0;
     }
     MDZ_DEV void push(uint64_t v) {
         if constexpr (EB == 4) {

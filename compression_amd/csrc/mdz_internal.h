@@ -17,6 +17,8 @@ int record_hip(hipError_t e, const char* what);
 // kernel is raised once to the largest size requested so far.
 hipError_t set_dynamic_lds(const void* kernel, size_t bytes);
 
+int dev_copy(void* dst, const void* src, uint64_t bytes, hipStream_t s);
+
 int rlev1_decompress(const mdz_lanes_t* in, void* dst, uint64_t dst_bytes, int32_t* status,
                      const mdz_lanes_tune_t* tune, hipStream_t s);
 

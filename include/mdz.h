@@ -61,6 +61,9 @@ int mdz_dev_free(void* ptr);
 int mdz_memcpy_htod(void* dst, const void* src, uint64_t bytes, void* stream);
 int mdz_memcpy_dtoh(void* dst, const void* src, uint64_t bytes, void* stream);
 int mdz_memset_d8(void* dst, int value, uint64_t bytes, void* stream);
+/* Device-to-device copy by a streaming kernel (16-byte vector loads/stores);
+ * also the known-size calibration kernel for HBM traffic counters. */
+int mdz_dev_copy(void* dst, const void* src, uint64_t bytes, void* stream);
 int mdz_stream_create(void** stream);
 int mdz_stream_destroy(void* stream);
 int mdz_stream_sync(void* stream);
@@ -92,8 +95,11 @@ typedef struct {
 
 /* Tuning knobs for the lane-transposed decoders; zero fields pick defaults. */
 typedef struct {
-    uint32_t ring_words;         /* LDS staging ring per chunk, 4-byte words (power of 2) */
-    uint32_t reserved[7];
+    uint32_t ring_words;         /* force the streaming (windowed) path with this LDS ring,
+                                    in 4-byte words (power of 2); 0 = automatic */
+    uint32_t max_chunk_bytes;    /* upper bound on one chunk's payload bytes, if known; it
+                                    sizes the per-chunk LDS area (0 = use the maximum) */
+    uint32_t reserved[6];
 } mdz_lanes_tune_t;
 
 /* RLE v1 decode -- replaces rle_v1::decompress_gpu<READ_COL_TYPE, DATA_TYPE,
