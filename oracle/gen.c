@@ -52,3 +52,48 @@ void mdz_gen_brle_u8(uint8_t* out, uint64_t n, uint64_t seed) {
         for (uint64_t k = 0; k < len && i < n; ++k, ++i) out[i] = (uint8_t)b;
     }
 }
+
+/* RLE v2 int64 column: i.i.d. segments weighted 15/35/35/15.
+ *  SHORT_REPEAT: length U{3..10}, value U[0, 2^32).
+ *  DELTA: length U{10..512}, start U[2^20, 2^40), stride U{-100..100}.
+ *  DIRECT: length U{1..512}, width w in {1,2,4,8,12,16,24,32,64}, values U[0, 2^w).
+ *  PATCHED_BASE: length U{16..128}, base width b U{8..16}, values U[0, 2^b),
+ *    each replaced with p = 0.05 by an outlier of width o U{40..60},
+ *    value U[2^(o-1), 2^o).
+ * Truncated at n. */
+void mdz_gen_rlev2_i64(uint64_t* out, uint64_t n, uint64_t seed) {
+    static const uint32_t widths[9] = {1, 2, 4, 8, 12, 16, 24, 32, 64};
+    mdz_rng r;
+    mdz_rng_seed(&r, seed);
+    uint64_t i = 0;
+    while (i < n) {
+        const uint64_t kind = mdz_rng_below(&r, 100);
+        if (kind < 15) {
+            const uint64_t len = 3 + mdz_rng_below(&r, 8);
+            const uint64_t v = mdz_rng_below(&r, 1ull << 32);
+            for (uint64_t k = 0; k < len && i < n; ++k, ++i) out[i] = v;
+        } else if (kind < 50) {
+            const uint64_t len = 10 + mdz_rng_below(&r, 503);
+            const uint64_t start = (1ull << 20) + mdz_rng_below(&r, (1ull << 40) - (1ull << 20));
+            const int64_t stride = (int64_t)mdz_rng_below(&r, 201) - 100;
+            for (uint64_t k = 0; k < len && i < n; ++k, ++i)
+                out[i] = (uint64_t)((int64_t)start + (int64_t)k * stride);
+        } else if (kind < 85) {
+            const uint64_t len = 1 + mdz_rng_below(&r, 512);
+            const uint32_t w = widths[mdz_rng_below(&r, 9)];
+            for (uint64_t k = 0; k < len && i < n; ++k, ++i)
+                out[i] = w == 64 ? mdz_rng_next(&r) : mdz_rng_below(&r, 1ull << w);
+        } else {
+            const uint64_t len = 16 + mdz_rng_below(&r, 113);
+            const uint32_t b = 8 + (uint32_t)mdz_rng_below(&r, 9);
+            for (uint64_t k = 0; k < len && i < n; ++k, ++i) {
+                if (mdz_rng_below(&r, 100) < 5) {
+                    const uint32_t o = 40 + (uint32_t)mdz_rng_below(&r, 21);
+                    out[i] = (1ull << (o - 1)) + mdz_rng_below(&r, 1ull << (o - 1));
+                } else {
+                    out[i] = mdz_rng_below(&r, 1ull << b);
+                }
+            }
+        }
+    }
+}

@@ -94,6 +94,21 @@ int mdz_ref_rlev1_decode(const uint8_t* payload, const uint64_t* blk_off,
                          int elem_bytes, uint32_t line, uint8_t* out, uint64_t out_bytes,
                          int32_t* status, int threads);
 
+/* ============================================================ RLE v2 ===== *
+ * Lane-transposed ORC RLE v2 (include/rlev2/, theirs side).  Lane t owns
+ * chunk elements k*32*RU + t*RU + j (j < RU, RU = READ_UNIT); values are the
+ * unsigned eb-byte elements widened to 64 bits.  blk_off[n] holds the
+ * uncompressed size, as the reference stores it. */
+uint64_t mdz_ref_rlev2_lane_encode(const uint64_t* vals, uint64_t n, uint8_t* out, uint64_t cap);
+int mdz_ref_rlev2_lane_decode(const uint8_t* in, uint64_t in_len, int elem_bytes, uint64_t* vals,
+                              uint64_t n_vals);
+int mdz_ref_rlev2_encode(const uint8_t* in, uint64_t in_bytes, uint64_t chunk, int elem_bytes,
+                         uint32_t read_unit, uint8_t** payload, uint64_t* payload_bytes,
+                         uint64_t** col_len, uint64_t** blk_off, uint64_t* n_chunks, int threads);
+int mdz_ref_rlev2_decode(const uint8_t* payload, const uint64_t* blk_off, const uint64_t* col_len,
+                         uint64_t n_chunks, uint64_t chunk, int elem_bytes, uint32_t read_unit,
+                         uint8_t* out, uint64_t out_bytes, int32_t* status, int threads);
+
 /* ========================================================== byte RLE ===== *
  * include/brle/brle.h.  Container: [u32 n_ptr=n+1][u64 ptr[n+1]][u64 out_bytes][data]. */
 uint64_t mdz_ref_brle_chunk_encode(const uint8_t* in, uint64_t n, uint8_t* out);
@@ -108,6 +123,9 @@ int mdz_ref_brle_decode(const uint8_t* data, const uint64_t* ptr, uint64_t n_chu
 /* ======================================================== generators ===== */
 /* Config 1: RLE v1 u32 column (runs 70% / literal segments 30% by count). */
 void mdz_gen_rlev1_u32(uint32_t* out, uint64_t n, uint64_t seed);
+/* Config 2(a): int64 column of SHORT_REPEAT / DELTA / DIRECT / PATCHED_BASE
+ * shaped segments weighted 15/35/35/15. */
+void mdz_gen_rlev2_i64(uint64_t* out, uint64_t n, uint64_t seed);
 /* Config 2(b): byte runs, fresh byte != previous, length Geom(1/12). */
 void mdz_gen_brle_u8(uint8_t* out, uint64_t n, uint64_t seed);
 

@@ -61,6 +61,18 @@ def lib():
         L.mdz_ref_brle_decode.restype = C.c_int
         L.mdz_ref_brle_decode.argtypes = [C.c_void_p, C.c_void_p, C.c_uint64, C.c_uint64, C.c_void_p,
                                           C.c_uint64, C.c_void_p, C.c_int]
+        L.mdz_ref_rlev2_lane_encode.restype = C.c_uint64
+        L.mdz_ref_rlev2_lane_encode.argtypes = [C.c_void_p, C.c_uint64, C.c_void_p, C.c_uint64]
+        L.mdz_ref_rlev2_lane_decode.restype = C.c_int
+        L.mdz_ref_rlev2_lane_decode.argtypes = [C.c_void_p, C.c_uint64, C.c_int, C.c_void_p, C.c_uint64]
+        L.mdz_ref_rlev2_encode.restype = C.c_int
+        L.mdz_ref_rlev2_encode.argtypes = [C.c_void_p, C.c_uint64, C.c_uint64, C.c_int, C.c_uint32,
+                                           C.POINTER(u8p), u64p, C.POINTER(u64p), C.POINTER(u64p),
+                                           u64p, C.c_int]
+        L.mdz_ref_rlev2_decode.restype = C.c_int
+        L.mdz_ref_rlev2_decode.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64, C.c_uint64,
+                                           C.c_int, C.c_uint32, C.c_void_p, C.c_uint64, C.c_void_p, C.c_int]
+        L.mdz_gen_rlev2_i64.argtypes = [C.c_void_p, C.c_uint64, C.c_uint64]
         L.mdz_gen_rlev1_u32.argtypes = [C.c_void_p, C.c_uint64, C.c_uint64]
         L.mdz_gen_brle_u8.argtypes = [C.c_void_p, C.c_uint64, C.c_uint64]
         _lib = L
@@ -147,6 +159,65 @@ def rlev1_decode(c: RleV1Container, out_bytes: int | None = None, threads: int |
                                     c.elem_bytes, c.line, _ptr(out), out_bytes, _ptr(status),
                                     threads or default_threads())
     return rc, out[:out_bytes], status[:c.n_chunks]
+
+
+# ----------------------------------------------------------------- RLE v2 --
+def rlev2_lane_encode(vals) -> bytes:
+    v = np.ascontiguousarray(np.asarray(vals, dtype=np.uint64))
+    cap = len(v) * 10 + 64
+    out = np.zeros(cap, np.uint8)
+    n = lib().mdz_ref_rlev2_lane_encode(_ptr(v) if v.size else None, len(v), _ptr(out), cap)
+    return out[:n].tobytes()
+
+
+def rlev2_lane_decode(stream: bytes, n_vals: int, elem_bytes: int = 8) -> tuple[int, np.ndarray]:
+    src = np.frombuffer(stream, np.uint8).copy() if stream else np.zeros(1, np.uint8)
+    out = np.zeros(max(n_vals, 1), np.uint64)
+    st = lib().mdz_ref_rlev2_lane_decode(_ptr(src), len(stream), elem_bytes, _ptr(out), n_vals)
+    return st, out[:n_vals]
+
+
+class RleV2Container:
+    """Lane-transposed RLE v2: payload, col_len[32n], blk_off[n+1] (blk_off[n] = raw size)."""
+
+    def __init__(self, payload, col_len, blk_off, n_chunks, chunk, elem_bytes, read_unit, raw_bytes,
+                 payload_bytes):
+        self.payload, self.col_len, self.blk_off = payload, col_len, blk_off
+        self.n_chunks, self.chunk, self.elem_bytes, self.read_unit = n_chunks, chunk, elem_bytes, read_unit
+        self.raw_bytes, self.payload_bytes = raw_bytes, payload_bytes
+
+
+def rlev2_encode(data: np.ndarray, chunk: int, read_unit: int, threads: int | None = None) -> RleV2Container:
+    data = np.ascontiguousarray(data)
+    raw = data.view(np.uint8).reshape(-1)
+    pay, col, off = u8p(), u64p(), u64p()
+    pb, nc = C.c_uint64(), C.c_uint64()
+    rc = lib().mdz_ref_rlev2_encode(_ptr(raw), raw.size, chunk, data.dtype.itemsize, read_unit,
+                                    C.byref(pay), C.byref(pb), C.byref(col), C.byref(off), C.byref(nc),
+                                    threads or default_threads())
+    if rc != 0:
+        raise ValueError(f"rlev2 encode rejected parameters (status {rc})")
+    n = nc.value
+    return RleV2Container(_take(pay, pb.value, np.uint8), _take(col, 32 * n, np.uint64),
+                          _take(off, n + 1, np.uint64), n, chunk, data.dtype.itemsize, read_unit,
+                          raw.size, pb.value)
+
+
+def rlev2_decode(c: RleV2Container, out_bytes: int | None = None, threads: int | None = None):
+    out_bytes = c.raw_bytes if out_bytes is None else out_bytes
+    out = np.zeros(max(out_bytes, 1), np.uint8)
+    status = np.zeros(max(c.n_chunks, 1), np.int32)
+    pay = c.payload if c.payload.size else np.zeros(1, np.uint8)
+    rc = lib().mdz_ref_rlev2_decode(_ptr(pay), _ptr(c.blk_off), _ptr(c.col_len), c.n_chunks, c.chunk,
+                                    c.elem_bytes, c.read_unit, _ptr(out), out_bytes, _ptr(status),
+                                    threads or default_threads())
+    return rc, out[:out_bytes], status[:c.n_chunks]
+
+
+def gen_rlev2_i64(n: int, seed: int) -> np.ndarray:
+    out = np.empty(n, np.uint64)
+    lib().mdz_gen_rlev2_i64(_ptr(out), n, seed)
+    return out
 
 
 # --------------------------------------------------------------- byte RLE --
