@@ -93,6 +93,96 @@ class RleV1Workload:
                           f"oracle C decoder, median of >=3 runs"}
 
 
+class RleV2Workload:
+    """RLE v2, plain method: 1 GiB int64 column (SHORT_REPEAT / DELTA / DIRECT /
+    PATCHED_BASE segments 15/35/35/15), 128 KiB chunks, READ_UNIT elements per
+    lane line; one workgroup (4 waves) per chunk, each wave decoding whole lane
+    streams cooperatively."""
+    name = "rlev2_i64_1GiB_plain"
+    dtype = "u64"
+
+    def __init__(self, line=512, n_vals=1 << 27, seed=0x4D445A02, **_):
+        from oracle import pyoracle as O
+        from compression_amd.lanes import DeviceLanes
+        import compression_amd as mdz
+        self.O, self.mdz = O, mdz
+        self.ru = line
+        self.data = O.gen_rlev2_i64(n_vals, seed)
+        self.c = c = O.rlev2_encode(self.data, 131072, line)
+        self.lanes = DeviceLanes(c.payload, c.blk_off, c.col_len, c.n_chunks, 131072, 8, line)
+        self.dst = mdz.DeviceBuffer(c.raw_bytes)
+        self.status = mdz.DeviceBuffer(4 * c.n_chunks)
+        self.out_bytes = c.raw_bytes
+        ends = np.append(c.blk_off[1:-1].astype(np.int64), c.payload_bytes)
+        self.max_chunk = int((ends - c.blk_off[:-1].astype(np.int64)).max())
+        n = c.n_chunks
+        self.alg_bytes = c.payload_bytes + 256 * n + 8 * (n + 1) + c.raw_bytes
+        self.config = {"workload": self.name, "codec": "rle_v2", "elem": "u64",
+                       "out_bytes": c.raw_bytes, "payload_bytes": c.payload_bytes,
+                       "n_chunks": n, "chunk_bytes": 131072, "read_unit": line,
+                       "max_chunk_payload": self.max_chunk,
+                       "ratio": round(c.payload_bytes / c.raw_bytes, 4)}
+
+    def step(self, stream):
+        from compression_amd import rlev2
+        rlev2.decompress(self.lanes, self.dst, self.out_bytes, self.status, stream=stream,
+                         max_chunk_bytes=self.max_chunk)
+
+    def verify(self):
+        got = self.dst.download(self.out_bytes)
+        st = self.status.download(dtype=np.int32)
+        return bool((st == 0).all() and np.array_equal(got, self.data.view(np.uint8)))
+
+    def cpu(self, threads):
+        O, c = self.O, self.c
+        t = _cpu_time(lambda: O.rlev2_decode(c, threads=threads))
+        return {"value": round(c.raw_bytes / t / 1e9, 3), "unit": "GB/s", "cores": threads,
+                "kind": "port",
+                "sample": f"whole workload ({c.raw_bytes >> 20} MiB out, {c.n_chunks} chunks), "
+                          f"oracle C decoder, median of >=3 runs"}
+
+
+class BrleWorkload:
+    """Byte RLE, plain method: 256 MiB u8 stream of runs (fresh byte, length
+    Geom(1/12)), 128 KiB chunks; one 256-thread workgroup per chunk."""
+    name = "brle_u8_256MiB_plain"
+    dtype = "u8"
+
+    def __init__(self, n=1 << 28, chunk=131072, seed=0x4D445A03, **_):
+        from oracle import pyoracle as O
+        from compression_amd import brle
+        import compression_amd as mdz
+        self.O, self.mdz = O, mdz
+        self.data = O.gen_brle_u8(n, seed)
+        self.c = c = O.brle_encode(self.data, chunk)
+        self.dev = brle.DeviceBrle(c.data, c.ptr, c.n_chunks, chunk)
+        self.dst = mdz.DeviceBuffer(c.raw_bytes)
+        self.status = mdz.DeviceBuffer(4 * c.n_chunks)
+        self.out_bytes = c.raw_bytes
+        self.alg_bytes = int(c.ptr[-1]) + 8 * (c.n_chunks + 1) + 12 + c.raw_bytes
+        self.config = {"workload": self.name, "codec": "byte_rle", "out_bytes": c.raw_bytes,
+                       "payload_bytes": int(c.ptr[-1]), "n_chunks": c.n_chunks, "chunk_bytes": chunk,
+                       "max_chunk_payload": self.dev.max_chunk,
+                       "ratio": round(int(c.ptr[-1]) / c.raw_bytes, 4)}
+
+    def step(self, stream):
+        from compression_amd import brle
+        brle.decompress(self.dev, self.dst, self.out_bytes, self.status, stream=stream)
+
+    def verify(self):
+        got = self.dst.download(self.out_bytes)
+        st = self.status.download(dtype=np.int32)
+        return bool((st == 0).all() and np.array_equal(got, self.data))
+
+    def cpu(self, threads):
+        O, c = self.O, self.c
+        t = _cpu_time(lambda: O.brle_decode(c, threads=threads))
+        return {"value": round(c.raw_bytes / t / 1e9, 3), "unit": "GB/s", "cores": threads,
+                "kind": "port",
+                "sample": f"whole workload ({c.raw_bytes >> 20} MiB out, {c.n_chunks} chunks), "
+                          f"oracle C decoder, median of >=3 runs"}
+
+
 class CopyWorkload:
     """Calibration: a plain 16-byte-vector device copy of 512 MiB (reads and
     writes each byte once).  Its FETCH_SIZE/WRITE_SIZE per byte scale the
@@ -121,7 +211,8 @@ class CopyWorkload:
         return None
 
 
-WORKLOADS = {"rlev1": RleV1Workload, "copy": CopyWorkload}
+WORKLOADS = {"rlev1": RleV1Workload, "rlev2": RleV2Workload, "brle": BrleWorkload,
+             "copy": CopyWorkload}
 DEFAULT = "rlev1"
 
 

@@ -33,7 +33,8 @@ class MdzError(RuntimeError):
 class LanesT(C.Structure):
     """mirror of mdz_lanes_t (include/mdz.h)"""
     _fields_ = [("payload", vp), ("blk_off", vp), ("col_len", vp), ("col_map", vp),
-                ("n_chunks", u64), ("chunk_bytes", u64), ("elem_bytes", u32), ("lane_line", u32)]
+                ("n_chunks", u64), ("chunk_bytes", u64), ("elem_bytes", u32), ("lane_line", u32),
+                ("payload_bytes", u64)]
 
 
 class LanesTuneT(C.Structure):
@@ -68,6 +69,15 @@ SIGNATURES = {
                                             u64, u32, vp, vp, u64]),
     "mdz_rle_v1_decompress_gpu": (C.c_int, [vp, C.POINTER(vp), u64, C.POINTER(u64), C.c_int,
                                             u64, u32]),
+    "mdz_rlev2_decompress": (C.c_int, [C.POINTER(LanesT), vp, u64, vp, vp]),
+    "mdz_rlev2_decompress_tuned": (C.c_int, [C.POINTER(LanesT), vp, u64, vp,
+                                             C.POINTER(LanesTuneT), vp]),
+    "mdz_rlev2_decompress_host": (C.c_int, [vp, u64, u64, vp, vp, C.POINTER(vp), C.POINTER(u64),
+                                            u64, u32, u32]),
+    "mdz_brle_decompress": (C.c_int, [vp, u64, u64, vp, vp, u64, vp, vp]),
+    "mdz_brle_decompress_tuned": (C.c_int, [vp, u64, u64, vp, vp, u64, vp, u32, vp]),
+    "mdz_brle_decompress_host": (C.c_int, [vp, C.POINTER(vp), u64, C.POINTER(u64), u64]),
+    "mdz_brle_decompress_gpu": (C.c_int, [vp, C.POINTER(vp), u64, C.POINTER(u64)]),
 }
 
 _lib = None

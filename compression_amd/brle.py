@@ -1,0 +1,51 @@
+"""Byte RLE decode -- host-side mirror of the reference's ``brle`` namespace.
+
+``decompress_gpu`` takes the reference container
+[u32 n_ptr][u64 ptr[n_ptr]][u64 out_bytes][data] (include/brle/brle.h:305-342)
+and returns the decoded bytes; ``decompress`` is the device-resident form.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from ._lib import DeviceBuffer, MdzError, check, lib
+from .lanes import as_ptr
+
+
+class DeviceBrle:
+    """ptr[n+1] and the compressed bytes, resident in HBM."""
+
+    def __init__(self, data: np.ndarray, ptr: np.ndarray, n_chunks: int, chunk_bytes: int, stream=None):
+        data = np.ascontiguousarray(data, np.uint8)
+        self.data = DeviceBuffer.from_array(data, pad=16, stream=stream)
+        self.ptr = DeviceBuffer.from_array(np.ascontiguousarray(ptr, np.uint64), stream=stream)
+        self.n_chunks, self.chunk_bytes = int(n_chunks), int(chunk_bytes)
+        p = np.asarray(ptr, np.int64)
+        self.max_chunk = int(np.diff(p).max()) if len(p) > 1 else 0
+        self.data_bytes = data.nbytes
+
+
+def decompress(b: DeviceBrle, dst, dst_bytes: int, status=None, stream=None, tuned: bool = True) -> None:
+    if tuned:
+        rc = lib().mdz_brle_decompress_tuned(b.ptr.ptr, b.n_chunks, b.chunk_bytes, b.data.ptr, as_ptr(dst),
+                                             int(dst_bytes), as_ptr(status), b.max_chunk, as_ptr(stream))
+    else:
+        rc = lib().mdz_brle_decompress(b.ptr.ptr, b.n_chunks, b.chunk_bytes, b.data.ptr, as_ptr(dst),
+                                       int(dst_bytes), as_ptr(status), as_ptr(stream))
+    check(rc, "mdz_brle_decompress")
+
+
+def decompress_gpu(container: bytes, chunk_bytes: int = 256 * 1024) -> bytes:
+    src = np.frombuffer(container, np.uint8)
+    out = C.c_void_p()
+    n = C.c_uint64()
+    rc = lib().mdz_brle_decompress_host(src.ctypes.data, C.byref(out), src.size, C.byref(n), chunk_bytes)
+    if rc not in (0, -5):
+        raise MdzError(rc, "mdz_brle_decompress_host")
+    data = C.string_at(out.value, n.value) if n.value else b""
+    lib().mdz_host_free(out)
+    if rc != 0:
+        raise MdzError(rc, "mdz_brle_decompress_host (corrupt chunk)")
+    return data

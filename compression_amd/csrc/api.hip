@@ -63,6 +63,28 @@ int read_file(const char* path, uint8_t** data, uint64_t* n) {
     return MDZ_OK;
 }
 
+// Synchronises, copies `raw` output bytes and the per-chunk status to the
+// host; *out is malloc'ed.  MDZ_ERR_CORRUPT if any chunk status is nonzero.
+int finish_host(const void* d_out, uint64_t raw, const int32_t* d_st, uint64_t n, uint8_t** out,
+                uint64_t* out_n_bytes) {
+    if (hipDeviceSynchronize() != hipSuccess) return MDZ_ERR_HIP;
+    int32_t* st = (int32_t*)malloc(n * 4 + 4);
+    uint8_t* o = (uint8_t*)malloc(raw ? raw : 1);
+    if (!st || !o) { free(st); free(o); return MDZ_ERR_NOMEM; }
+    if (hipMemcpy(st, d_st, n * 4, hipMemcpyDeviceToHost) != hipSuccess ||
+        (raw && hipMemcpy(o, d_out, raw, hipMemcpyDeviceToHost) != hipSuccess)) {
+        free(st); free(o);
+        return MDZ_ERR_HIP;
+    }
+    int rc = MDZ_OK;
+    for (uint64_t c = 0; c < n; ++c)
+        if (st[c] != MDZ_ST_OK) { rc = MDZ_ERR_CORRUPT; break; }
+    free(st);
+    *out = o;
+    *out_n_bytes = raw;
+    return rc;
+}
+
 }  // namespace
 
 extern "C" {
@@ -142,25 +164,10 @@ int mdz_rlev1_decompress_host(const uint8_t* in, uint8_t** out, uint64_t in_n_by
         hipMemcpy(d_off.p, blk_off, (n_chunks + 1) * 8, hipMemcpyHostToDevice) != hipSuccess)
         return MDZ_ERR_HIP;
     mdz_lanes_t L = {d_pay.p, (const uint64_t*)d_off.p, (const uint64_t*)d_col.p, nullptr,
-                     n_chunks, chunk_size, elem_bytes, (uint32_t)comp_col_len};
+                     n_chunks, chunk_size, elem_bytes, (uint32_t)comp_col_len, pay_bytes};
     rc = mdz::rlev1_decompress(&L, d_out.p, raw, (int32_t*)d_st.p, nullptr, nullptr);
     if (rc) return rc;
-    if (hipDeviceSynchronize() != hipSuccess) return MDZ_ERR_HIP;
-    int32_t* st = (int32_t*)malloc(n_chunks * 4 + 4);
-    uint8_t* o = (uint8_t*)malloc(raw ? raw : 1);
-    if (!st || !o) { free(st); free(o); return MDZ_ERR_NOMEM; }
-    if (hipMemcpy(st, d_st.p, n_chunks * 4, hipMemcpyDeviceToHost) != hipSuccess ||
-        hipMemcpy(o, d_out.p, raw, hipMemcpyDeviceToHost) != hipSuccess) {
-        free(st); free(o);
-        return MDZ_ERR_HIP;
-    }
-    rc = MDZ_OK;
-    for (uint64_t c = 0; c < n_chunks; ++c)
-        if (st[c] != MDZ_ST_OK) { rc = MDZ_ERR_CORRUPT; break; }
-    free(st);
-    *out = o;
-    *out_n_bytes = raw;
-    return rc;
+    return finish_host(d_out.p, raw, (const int32_t*)d_st.p, n_chunks, out, out_n_bytes);
 }
 
 int mdz_rle_v1_decompress_gpu(const uint8_t* in, uint8_t** out, uint64_t in_n_bytes,
@@ -181,6 +188,100 @@ int mdz_rle_v1_decompress_gpu(const uint8_t* in, uint8_t** out, uint64_t in_n_by
     free(col);
     free(off);
     return rc;
+}
+
+// ------------------------------------------------------------------ RLE v2
+int mdz_rlev2_decompress(const mdz_lanes_t* in, void* dst, uint64_t dst_bytes, int32_t* status,
+                         void* stream) {
+    return mdz::rlev2_decompress(in, dst, dst_bytes, status, nullptr, S(stream));
+}
+
+int mdz_rlev2_decompress_tuned(const mdz_lanes_t* in, void* dst, uint64_t dst_bytes,
+                               int32_t* status, const mdz_lanes_tune_t* tune, void* stream) {
+    return mdz::rlev2_decompress(in, dst, dst_bytes, status, tune, S(stream));
+}
+
+int mdz_rlev2_decompress_host(const uint8_t* in, uint64_t in_n_bytes, uint64_t n_chunks,
+                              const uint64_t* blk_off, const uint64_t* col_len, uint8_t** out,
+                              uint64_t* out_n_bytes, uint64_t chunk_size, uint32_t read_unit,
+                              uint32_t elem_bytes) {
+    if (!in || !out || !out_n_bytes || !col_len || !blk_off || !n_chunks) return MDZ_ERR_INVALID;
+    const uint64_t raw = blk_off[n_chunks];
+    if (raw > n_chunks * chunk_size) return MDZ_ERR_INVALID;
+    DevBuf d_pay, d_col, d_off, d_out, d_st;
+    int rc;
+    if ((rc = d_pay.alloc(in_n_bytes + 16)) || (rc = d_col.alloc(32 * n_chunks * 8)) ||
+        (rc = d_off.alloc((n_chunks + 1) * 8)) || (rc = d_out.alloc(raw)) ||
+        (rc = d_st.alloc(n_chunks * 4)))
+        return rc;
+    if (hipMemcpy(d_pay.p, in, in_n_bytes, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(d_col.p, col_len, 32 * n_chunks * 8, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(d_off.p, blk_off, (n_chunks + 1) * 8, hipMemcpyHostToDevice) != hipSuccess)
+        return MDZ_ERR_HIP;
+    mdz_lanes_t L = {d_pay.p, (const uint64_t*)d_off.p, (const uint64_t*)d_col.p, nullptr,
+                     n_chunks, chunk_size, elem_bytes, read_unit, in_n_bytes};
+    rc = mdz::rlev2_decompress(&L, d_out.p, raw, (int32_t*)d_st.p, nullptr, nullptr);
+    if (rc) return rc;
+    return finish_host(d_out.p, raw, (const int32_t*)d_st.p, n_chunks, out, out_n_bytes);
+}
+
+// ---------------------------------------------------------------- byte RLE
+int mdz_brle_decompress(const uint64_t* ptr, uint64_t n_chunks, uint64_t chunk_bytes,
+                        const void* data, void* dst, uint64_t dst_bytes, int32_t* status,
+                        void* stream) {
+    return mdz::brle_decompress(ptr, n_chunks, chunk_bytes, data, dst, dst_bytes, status, 0,
+                                S(stream));
+}
+
+int mdz_brle_decompress_tuned(const uint64_t* ptr, uint64_t n_chunks, uint64_t chunk_bytes,
+                              const void* data, void* dst, uint64_t dst_bytes, int32_t* status,
+                              uint32_t max_chunk_bytes, void* stream) {
+    return mdz::brle_decompress(ptr, n_chunks, chunk_bytes, data, dst, dst_bytes, status,
+                                max_chunk_bytes, S(stream));
+}
+
+int mdz_brle_decompress_host(const uint8_t* in, uint8_t** out, uint64_t in_n_bytes,
+                             uint64_t* out_n_bytes, uint64_t chunk_bytes) {
+    if (!in || !out || !out_n_bytes || in_n_bytes < 4 || chunk_bytes == 0) return MDZ_ERR_INVALID;
+    uint32_t n_ptr;
+    memcpy(&n_ptr, in, 4);
+    if (n_ptr < 2) return MDZ_ERR_INVALID;
+    const uint64_t n = n_ptr - 1, hdr = 4 + 8ull * n_ptr;
+    if (in_n_bytes < hdr + 8) return MDZ_ERR_INVALID;
+    uint64_t* ptr = (uint64_t*)malloc(8ull * n_ptr);
+    if (!ptr) return MDZ_ERR_NOMEM;
+    memcpy(ptr, in + 4, 8ull * n_ptr);
+    uint64_t raw;
+    memcpy(&raw, in + hdr, 8);
+    const uint8_t* data = in + hdr + 8;
+    const uint64_t data_bytes = in_n_bytes - hdr - 8;
+    uint64_t max_chunk = 0;
+    int rc = MDZ_OK;
+    for (uint64_t c = 0; c < n; ++c) {
+        if (ptr[c + 1] < ptr[c] || ptr[c + 1] > data_bytes) { rc = MDZ_ERR_INVALID; break; }
+        if (ptr[c + 1] - ptr[c] > max_chunk) max_chunk = ptr[c + 1] - ptr[c];
+    }
+    if (rc == MDZ_OK && (raw > n * chunk_bytes || max_chunk > 0xffffffffull)) rc = MDZ_ERR_INVALID;
+    DevBuf d_ptr, d_data, d_out, d_st;
+    if (rc == MDZ_OK &&
+        ((rc = d_ptr.alloc(8ull * n_ptr)) || (rc = d_data.alloc(data_bytes + 16)) ||
+         (rc = d_out.alloc(raw)) || (rc = d_st.alloc(4 * n)))) {
+    }
+    if (rc == MDZ_OK &&
+        (hipMemcpy(d_ptr.p, ptr, 8ull * n_ptr, hipMemcpyHostToDevice) != hipSuccess ||
+         hipMemcpy(d_data.p, data, data_bytes, hipMemcpyHostToDevice) != hipSuccess))
+        rc = MDZ_ERR_HIP;
+    free(ptr);
+    if (rc) return rc;
+    rc = mdz::brle_decompress((const uint64_t*)d_ptr.p, n, chunk_bytes, d_data.p, d_out.p, raw,
+                              (int32_t*)d_st.p, (uint32_t)max_chunk, nullptr);
+    if (rc) return rc;
+    return finish_host(d_out.p, raw, (const int32_t*)d_st.p, n, out, out_n_bytes);
+}
+
+int mdz_brle_decompress_gpu(const uint8_t* in, uint8_t** out, uint64_t in_n_bytes,
+                            uint64_t* out_n_bytes) {
+    return mdz_brle_decompress_host(in, out, in_n_bytes, out_n_bytes, 256 * 1024);
 }
 
 }  // extern "C"

@@ -278,3 +278,83 @@ template <int EB, int SW> struct LineEmitter {
 };
 
 }  // namespace mdz
+
+namespace mdz {
+
+// ------------------------------------------------------------------------
+// Whole-chunk staging for the lane-transposed codecs: every thread of the
+// workgroup reads the chunk's interleaved payload with 16-byte loads (each
+// byte once) and scatters each 4-byte word to its lane's contiguous LDS
+// region at d32[lbase[lane]/4 + round].
+MDZ_DEV void stage_chunk_lanes(const LaneTable& T, const uint32_t* lbase, uint32_t* d32,
+                               const u32x4* __restrict__ src, uint32_t tid, uint32_t nthreads) {
+    const uint32_t total = T.total, ngroups = (total + 3) / 4;
+    for (uint32_t g0 = tid; g0 < ngroups; g0 += 4 * nthreads) {
+        u32x4 v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const uint32_t g = g0 + u * nthreads;
+            if (g < ngroups) v[u] = __builtin_nontemporal_load(src + g);
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const uint32_t g = g0 + u * nthreads;
+            if (g >= ngroups) break;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const uint32_t w = 4 * g + i;
+                if (w < total) {
+                    uint32_t lane, round;
+                    lane_of_word(T, w, lane, round);
+                    d32[(lbase[lane] >> 2) + round] = v[u][i];
+                }
+            }
+        }
+    }
+}
+
+// Wave-wide inclusive prefix sums.
+MDZ_DEV uint64_t wave_incl_scan_u64(uint64_t v) {
+    const uint32_t l = lane_id();
+#pragma unroll
+    for (int o = 1; o < kWave; o <<= 1) {
+        const uint32_t ylo = __shfl_up((uint32_t)v, o, kWave);
+        const uint32_t yhi = __shfl_up((uint32_t)(v >> 32), o, kWave);
+        if (l >= (uint32_t)o) v += ((uint64_t)yhi << 32) | ylo;
+    }
+    return v;
+}
+MDZ_DEV uint32_t wave_incl_scan_u32(uint32_t v) {
+    const uint32_t l = lane_id();
+#pragma unroll
+    for (int o = 1; o < kWave; o <<= 1) {
+        const uint32_t y = __shfl_up(v, o, kWave);
+        if (l >= (uint32_t)o) v += y;
+    }
+    return v;
+}
+MDZ_DEV uint64_t shfl_u64(uint64_t v, int src) {
+    const uint32_t lo = __shfl((uint32_t)v, src, kWave), hi = __shfl((uint32_t)(v >> 32), src, kWave);
+    return ((uint64_t)hi << 32) | lo;
+}
+
+// 8 stream bytes (little-endian) at byte p of a 4-byte-aligned LDS region.
+MDZ_DEV uint64_t lds_peek8(const uint8_t* base, uint32_t p) {
+    const uint32_t* w = reinterpret_cast<const uint32_t*>(base) + (p >> 2);
+    const uint32_t w0 = w[0], w1 = w[1], w2 = w[2], k = p & 3;
+    return ((uint64_t)alignbyte(w2, w1, k) << 32) | alignbyte(w1, w0, k);
+}
+
+// MSB-first field of w bits (1..64) starting `bit` bits after byte b0 of a
+// 4-byte-aligned LDS region (reads bytes up to 12 past the field start).
+MDZ_DEV uint64_t lds_bits_be(const uint8_t* base, uint32_t b0, uint64_t bit, uint32_t w) {
+    const uint32_t b = b0 + (uint32_t)(bit >> 3), s = (uint32_t)(bit & 7), k = b & 3;
+    const uint32_t* d = reinterpret_cast<const uint32_t*>(base) + (b >> 2);
+    const uint32_t d0 = d[0], d1 = d[1], d2 = d[2];
+    const uint32_t lo = alignbyte(d1, d0, k), mid = alignbyte(d2, d1, k);
+    const uint64_t be = ((uint64_t)__builtin_bswap32(lo) << 32) | __builtin_bswap32(mid);
+    const uint64_t v = s ? (be << s) | (((d2 >> (8 * k)) & 0xffu) >> (8 - s)) : be;
+    return w >= 64 ? v : v >> (64 - w);
+}
+
+}  // namespace mdz

@@ -22,4 +22,11 @@ int dev_copy(void* dst, const void* src, uint64_t bytes, hipStream_t s);
 int rlev1_decompress(const mdz_lanes_t* in, void* dst, uint64_t dst_bytes, int32_t* status,
                      const mdz_lanes_tune_t* tune, hipStream_t s);
 
+int rlev2_decompress(const mdz_lanes_t* in, void* dst, uint64_t dst_bytes, int32_t* status,
+                     const mdz_lanes_tune_t* tune, hipStream_t s);
+
+int brle_decompress(const uint64_t* ptr, uint64_t n_chunks, uint64_t chunk_bytes, const void* data,
+                    void* dst, uint64_t dst_bytes, int32_t* status, uint32_t max_chunk_bytes,
+                    hipStream_t s);
+
 }  // namespace mdz

@@ -33,7 +33,8 @@ class DeviceLanes:
     def struct(self) -> LanesT:
         return LanesT(self.payload.ptr, self.blk_off.ptr, self.col_len.ptr,
                       self.col_map.ptr if self.col_map is not None else None,
-                      self.n_chunks, self.chunk_bytes, self.elem_bytes, self.lane_line)
+                      self.n_chunks, self.chunk_bytes, self.elem_bytes, self.lane_line,
+                      self.payload_bytes)
 
     def meta_bytes(self) -> int:
         """Metadata bytes a decode reads: col_len, blk_off (+ col_map)."""

@@ -1,0 +1,48 @@
+"""RLE v2 decode -- host-side mirror of the reference's ``rlev2`` namespace.
+
+``decompress_gpu`` keeps the shape of ``rlev2::decompress_gpu``
+(include/rlev2/decoder_trans.h:615-681): payload, n_chunks, blk_off (whose
+last entry is the uncompressed size), col_len -> host bytes.  ``decompress``
+is the device-resident, stream-ordered form.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from ._lib import LanesTuneT, MdzError, check, lib
+from .lanes import DeviceLanes, as_ptr
+
+
+def decompress(lanes: DeviceLanes, dst, dst_bytes: int | None = None, status=None, stream=None,
+               max_chunk_bytes: int = 0) -> None:
+    """Enqueue the decode of every chunk; ``lanes.lane_line`` is READ_UNIT."""
+    s = lanes.struct()
+    n = lanes.out_bytes if dst_bytes is None else int(dst_bytes)
+    if max_chunk_bytes:
+        tune = LanesTuneT(0, max_chunk_bytes)
+        rc = lib().mdz_rlev2_decompress_tuned(C.byref(s), as_ptr(dst), n, as_ptr(status), C.byref(tune),
+                                              as_ptr(stream))
+    else:
+        rc = lib().mdz_rlev2_decompress(C.byref(s), as_ptr(dst), n, as_ptr(status), as_ptr(stream))
+    check(rc, "mdz_rlev2_decompress")
+
+
+def decompress_gpu(payload: bytes, n_chunks: int, blk_off: np.ndarray, col_len: np.ndarray,
+                   chunk_size: int, read_unit: int, elem_bytes: int) -> bytes:
+    src = np.frombuffer(payload, np.uint8)
+    off = np.ascontiguousarray(blk_off, np.uint64)
+    col = np.ascontiguousarray(col_len, np.uint64)
+    out = C.c_void_p()
+    n = C.c_uint64()
+    rc = lib().mdz_rlev2_decompress_host(src.ctypes.data, src.size, n_chunks, off.ctypes.data,
+                                         col.ctypes.data, C.byref(out), C.byref(n), chunk_size,
+                                         read_unit, elem_bytes)
+    if rc not in (0, -5):
+        raise MdzError(rc, "mdz_rlev2_decompress_host")
+    data = C.string_at(out.value, n.value) if n.value else b""
+    lib().mdz_host_free(out)
+    if rc != 0:
+        raise MdzError(rc, "mdz_rlev2_decompress_host (corrupt chunk)")
+    return data

@@ -89,8 +89,12 @@ typedef struct {
     const uint8_t*  col_map;     /* LZSS only: 32 * n_chunks sorted->original lane */
     uint64_t        n_chunks;
     uint64_t        chunk_bytes; /* uncompressed bytes per chunk */
-    uint32_t        elem_bytes;  /* RLE v1: DATA_TYPE size {1,2,4,8} */
-    uint32_t        lane_line;   /* RLE v1: COMP_COL_LEN bytes per lane line */
+    uint32_t        elem_bytes;  /* RLE v1 DATA_TYPE / RLE v2 COMP_TYPE size {1,2,4,8} */
+    uint32_t        lane_line;   /* RLE v1: COMP_COL_LEN bytes per lane line;
+                                    RLE v2: READ_UNIT elements per lane line */
+    uint64_t        payload_bytes; /* payload extent for bounds checks (0 = unchecked);
+                                    RLE v2 needs it for its last chunk, whose
+                                    blk_off[n] holds the uncompressed size */
 } mdz_lanes_t;
 
 /* Tuning knobs for the lane-transposed decoders; zero fields pick defaults. */
@@ -128,6 +132,52 @@ int mdz_rlev1_decompress_host(const uint8_t* in, uint8_t** out, uint64_t in_n_by
 int mdz_rle_v1_decompress_gpu(const uint8_t* in, uint8_t** out, uint64_t in_n_bytes,
                               uint64_t* out_n_bytes, int comp_col_len, uint64_t chunk_size,
                               uint32_t elem_bytes);
+
+/* RLE v2 decode -- replaces rlev2::decompress_gpu<READ_UNIT, queue_size,
+ * COMP_TYPE> (reference include/rlev2/decoder_trans.h:615-681) and its
+ * inflate kernel (include/rlev2/decoder_new.h:618-657).  elem_bytes =
+ * sizeof(COMP_TYPE), lane_line = READ_UNIT; lane t's q-th value is element
+ * (q/RU)*32*RU + t*RU + q%RU of its chunk.  blk_off[n_chunks] is not read
+ * (the reference stores the uncompressed size there); set payload_bytes to
+ * bound the last chunk.  PATCHED_BASE patches use a 64-bit mask (the
+ * reference's 32-bit mask, decoder_new.h:533, drops bits >= 32). */
+int mdz_rlev2_decompress(const mdz_lanes_t* in, void* dst, uint64_t dst_bytes,
+                         int32_t* status, void* stream);
+int mdz_rlev2_decompress_tuned(const mdz_lanes_t* in, void* dst, uint64_t dst_bytes,
+                               int32_t* status, const mdz_lanes_tune_t* tune, void* stream);
+
+/* Host-pointer RLE v2 decode in the reference's signature shape
+ * (decoder_trans.h:615-618): in/in_n_bytes = payload, blk_off[n_chunks+1]
+ * with blk_off[n_chunks] = uncompressed bytes, col_len[32*n_chunks].
+ * *out (blk_off[n_chunks] bytes) is allocated here; free with mdz_host_free. */
+int mdz_rlev2_decompress_host(const uint8_t* in, uint64_t in_n_bytes, uint64_t n_chunks,
+                              const uint64_t* blk_off, const uint64_t* col_len, uint8_t** out,
+                              uint64_t* out_n_bytes, uint64_t chunk_size, uint32_t read_unit,
+                              uint32_t elem_bytes);
+
+/* ========================================================== byte RLE ===== *
+ * Byte RLE decode -- replaces brle::decompress_gpu / kernel_decompress
+ * (reference include/brle/brle.h:26-109,305-342).  Chunk c's compressed bytes
+ * are data[ptr[c], ptr[c+1]) and decode to dst[c*chunk_bytes ...] (at most
+ * chunk_bytes, clipped to dst_bytes).  max_chunk_bytes bounds one chunk's
+ * compressed size (0 = chunk_bytes + chunk_bytes/128 + 1, the format's worst
+ * case); it must fit a workgroup's LDS (about 150 KiB). */
+int mdz_brle_decompress(const uint64_t* ptr, uint64_t n_chunks, uint64_t chunk_bytes,
+                        const void* data, void* dst, uint64_t dst_bytes, int32_t* status,
+                        void* stream);
+int mdz_brle_decompress_tuned(const uint64_t* ptr, uint64_t n_chunks, uint64_t chunk_bytes,
+                              const void* data, void* dst, uint64_t dst_bytes, int32_t* status,
+                              uint32_t max_chunk_bytes, void* stream);
+
+/* brle::decompress_gpu's host form (brle.h:305): `in` is the container
+ * [u32 n_ptr][u64 ptr[n_ptr]][u64 out_bytes][data]; chunk_bytes is the
+ * reference's CHUNK_SIZE (256 KiB there). */
+int mdz_brle_decompress_host(const uint8_t* in, uint8_t** out, uint64_t in_n_bytes,
+                             uint64_t* out_n_bytes, uint64_t chunk_bytes);
+/* Exactly brle::decompress_gpu(in, out, in_n_bytes, out_n_bytes), CHUNK_SIZE
+ * = 256 KiB (brle.h:8). */
+int mdz_brle_decompress_gpu(const uint8_t* in, uint8_t** out, uint64_t in_n_bytes,
+                            uint64_t* out_n_bytes);
 
 #ifdef __cplusplus
 }
