@@ -78,6 +78,10 @@ SIGNATURES = {
     "mdz_brle_decompress_tuned": (C.c_int, [vp, u64, u64, vp, vp, u64, vp, u32, vp]),
     "mdz_brle_decompress_host": (C.c_int, [vp, C.POINTER(vp), u64, C.POINTER(u64), u64]),
     "mdz_brle_decompress_gpu": (C.c_int, [vp, C.POINTER(vp), u64, C.POINTER(u64)]),
+    "mdz_lzss_decompress": (C.c_int, [C.POINTER(LanesT), vp, u64, vp, vp]),
+    "mdz_lzss_decompress_host": (C.c_int, [vp, u64, C.POINTER(vp), C.POINTER(u64), u64,
+                                           C.POINTER(vp), C.POINTER(u64)]),
+    "mdz_meta_split": (C.c_int, [vp, u64, C.POINTER(vp), C.POINTER(u64), C.POINTER(u64)]),
 }
 
 _lib = None

@@ -284,4 +284,60 @@ int mdz_brle_decompress_gpu(const uint8_t* in, uint8_t** out, uint64_t in_n_byte
     return mdz_brle_decompress_host(in, out, in_n_bytes, out_n_bytes, 256 * 1024);
 }
 
+// -------------------------------------------------------------------- LZSS
+int mdz_lzss_decompress(const mdz_lanes_t* in, void* dst, uint64_t dst_bytes, int32_t* status,
+                        void* stream) {
+    return mdz::lzss_decompress(in, dst, dst_bytes, status, S(stream));
+}
+
+int mdz_lzss_decompress_host(const uint8_t* in, uint64_t in_n_bytes,
+                             const uint8_t* const* in_metadata, const uint64_t* in_metadata_lens,
+                             uint64_t in_metadata_num, uint8_t** out, uint64_t* out_n_bytes) {
+    if (in_metadata_num < 4 || !in_metadata || !in_metadata_lens) return -1;
+    if (!in || !out || !out_n_bytes) return MDZ_ERR_INVALID;
+    const uint64_t* lens = in_metadata_lens;
+    if (lens[0] != 8 || lens[1] < 16 || lens[1] % 8) return MDZ_ERR_INVALID;
+    const uint64_t n = lens[1] / 8 - 1;
+    if (lens[2] != 32 * n * 8 || lens[3] != 32 * n) return MDZ_ERR_INVALID;
+    uint64_t chunk;
+    memcpy(&chunk, in_metadata[0], 8);
+    const uint64_t raw = n * chunk;
+    DevBuf d_pay, d_col, d_off, d_map, d_out, d_st;
+    int rc;
+    if ((rc = d_pay.alloc(in_n_bytes + 16)) || (rc = d_col.alloc(lens[2])) ||
+        (rc = d_off.alloc(lens[1])) || (rc = d_map.alloc(lens[3])) || (rc = d_out.alloc(raw)) ||
+        (rc = d_st.alloc(n * 4)))
+        return rc;
+    if (hipMemcpy(d_pay.p, in, in_n_bytes, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(d_off.p, in_metadata[1], lens[1], hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(d_col.p, in_metadata[2], lens[2], hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(d_map.p, in_metadata[3], lens[3], hipMemcpyHostToDevice) != hipSuccess)
+        return MDZ_ERR_HIP;
+    mdz_lanes_t L = {d_pay.p, (const uint64_t*)d_off.p, (const uint64_t*)d_col.p,
+                     (const uint8_t*)d_map.p, n, chunk, 1, 4, in_n_bytes};
+    rc = mdz::lzss_decompress(&L, d_out.p, raw, (int32_t*)d_st.p, nullptr);
+    if (rc) return rc;
+    return finish_host(d_out.p, raw, (const int32_t*)d_st.p, n, out, out_n_bytes);
+}
+
+int mdz_meta_split(const uint8_t* meta, uint64_t meta_bytes, const uint8_t** blobs, uint64_t* lens,
+                   uint64_t* num) {
+    if (!meta || !num || meta_bytes < 8) return MDZ_ERR_INVALID;
+    uint64_t k;
+    memcpy(&k, meta, 8);
+    if (k > (meta_bytes - 8) / 8) return MDZ_ERR_CORRUPT;
+    if (k > *num || (k && (!blobs || !lens))) { *num = k; return MDZ_ERR_INVALID; }
+    uint64_t off = 8 + 8 * k;
+    for (uint64_t i = 0; i < k; ++i) {
+        uint64_t len;
+        memcpy(&len, meta + 8 + 8 * i, 8);
+        if (len > meta_bytes - off) return MDZ_ERR_CORRUPT;
+        blobs[i] = meta + off;
+        lens[i] = len;
+        off += len;
+    }
+    *num = k;
+    return MDZ_OK;
+}
+
 }  // extern "C"

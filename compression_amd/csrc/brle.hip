@@ -219,12 +219,15 @@ __global__ __launch_bounds__(kThreads) void brle_decode_kernel(
             uint32_t il, ol;
             token_at(d, p, il, ol);
             if (p + il > L) { e = MDZ_ST_INPUT_OVERRUN; epos = p; break; }
-            if (sink.pos + ol > cap) { e = MDZ_ST_OUTPUT_OVERRUN; epos = p; break; }
+            // a token that crosses the output end still writes the bytes
+            // that fit (the sink drops the rest), then stops the window
+            const bool over = sink.pos + ol > cap;
             if (cc > 127) {
                 for (uint32_t k = 0; k < ol; ++k) sink.put1(d[p + 1 + k]);
             } else {
                 sink.fill(d[p + 1], ol);
             }
+            if (over) { e = MDZ_ST_OUTPUT_OVERRUN; epos = p; break; }
             p += il;
         }
         sink.finish();

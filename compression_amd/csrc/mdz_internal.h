@@ -29,4 +29,7 @@ int brle_decompress(const uint64_t* ptr, uint64_t n_chunks, uint64_t chunk_bytes
                     void* dst, uint64_t dst_bytes, int32_t* status, uint32_t max_chunk_bytes,
                     hipStream_t s);
 
+int lzss_decompress(const mdz_lanes_t* in, void* dst, uint64_t dst_bytes, int32_t* status,
+                    hipStream_t s);
+
 }  // namespace mdz

@@ -179,6 +179,39 @@ int mdz_brle_decompress_host(const uint8_t* in, uint8_t** out, uint64_t in_n_byt
 int mdz_brle_decompress_gpu(const uint8_t* in, uint8_t** out, uint64_t in_n_bytes,
                             uint64_t* out_n_bytes);
 
+/* ============================================================== LZSS ===== *
+ * LZSS decode -- replaces lzss::kernel_decompress / decompress_func
+ * (reference include/lzss/lzss_gpu2.h:1898-1907,114-325) and the device part
+ * of lzss::decompress_gpu (:2129-2314).  in->col_len holds the lengths in
+ * sorted order and in->col_map[32c+s] the original lane of sorted position s
+ * (lzss_gpu2.h:1063-1085); blk_off[n_chunks] is the payload size.  Lane t's
+ * byte i decodes to dst[c*chunk + 128*(i/4) + 4t + i%4], clipped to
+ * dst_bytes.  chunk_bytes must be a multiple of 128 and at most 128 KiB
+ * (lanes of up to 4 KiB).  Decoding stops when a lane's chunk_bytes/32 bytes
+ * are complete: the reserved flag byte after an 8-token group that ends the
+ * stream is not decoded (the reference decodes it and writes past the lane,
+ * lzss_gpu2.h:200-321).  status[c] (optional): the error of the chunk's
+ * lowest failing lane, BAD_HEADER for bad container metadata. */
+int mdz_lzss_decompress(const mdz_lanes_t* in, void* dst, uint64_t dst_bytes, int32_t* status,
+                        void* stream);
+
+/* lzss::decompress_gpu's signature (lzss_gpu2.h:2129): in = payload,
+ * in_metadata = the four .meta blobs {u64 chunk_size, u64 blk_off[n+1],
+ * u64 col_len[32n], u8 col_map[32n]} with their byte lengths.  Returns -1 when
+ * fewer than 4 blobs are given, as the reference does; *out (n*chunk_size
+ * bytes, allocated here, free with mdz_host_free).  The output stride is the
+ * chunk_size blob (the reference uses its compile-time 128 KiB). */
+int mdz_lzss_decompress_host(const uint8_t* in, uint64_t in_n_bytes,
+                             const uint8_t* const* in_metadata, const uint64_t* in_metadata_lens,
+                             uint64_t in_metadata_num, uint8_t** out, uint64_t* out_n_bytes);
+
+/* Splits a .meta side file image [u64 num][u64 len[num]][blob...] into blob
+ * pointers (into `meta`) and lengths -- the reader in src/lzss/lzss_gpu.cu:
+ * 109-133 (format: include/utils.h:46-118).  *num is in/out: capacity of
+ * blobs/lens in, blob count out.  MDZ_ERR_CORRUPT if the lengths overrun. */
+int mdz_meta_split(const uint8_t* meta, uint64_t meta_bytes, const uint8_t** blobs, uint64_t* lens,
+                   uint64_t* num);
+
 #ifdef __cplusplus
 }
 #endif

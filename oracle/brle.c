@@ -67,7 +67,8 @@ uint64_t mdz_ref_brle_chunk_encode(const uint8_t* in, uint64_t n, uint8_t* out) 
     return pos;
 }
 
-/* brle::decode (brle.h:76-97) with bounds checks. */
+/* brle::decode (brle.h:76-97) with bounds checks.  A token that does not
+ * fit the output writes the bytes that do and reports OUTPUT_OVERRUN. */
 int mdz_ref_brle_chunk_decode(const uint8_t* in, uint64_t in_len, uint8_t* out,
                               uint64_t out_cap, uint64_t* out_len) {
     uint64_t p = 0, o = 0;
@@ -76,13 +77,21 @@ int mdz_ref_brle_chunk_decode(const uint8_t* in, uint64_t in_len, uint8_t* out,
         if (c > 127) {
             uint64_t cnt = (uint64_t)c - 127;
             if (p + cnt > in_len) { *out_len = o; return MDZ_ST_INPUT_OVERRUN; }
-            if (o + cnt > out_cap) { *out_len = o; return MDZ_ST_OUTPUT_OVERRUN; }
+            if (o + cnt > out_cap) {
+                memcpy(out + o, in + p, out_cap - o);
+                *out_len = out_cap;
+                return MDZ_ST_OUTPUT_OVERRUN;
+            }
             memcpy(out + o, in + p, cnt);
             p += cnt; o += cnt;
         } else {
             uint64_t cnt = (uint64_t)c + BRLE_MIN_REPEAT;
             if (p >= in_len) { *out_len = o; return MDZ_ST_INPUT_OVERRUN; }
-            if (o + cnt > out_cap) { *out_len = o; return MDZ_ST_OUTPUT_OVERRUN; }
+            if (o + cnt > out_cap) {
+                memset(out + o, in[p], out_cap - o);
+                *out_len = out_cap;
+                return MDZ_ST_OUTPUT_OVERRUN;
+            }
             memset(out + o, in[p++], cnt);
             o += cnt;
         }

@@ -120,6 +120,24 @@ int mdz_ref_brle_decode(const uint8_t* data, const uint64_t* ptr, uint64_t n_chu
                         uint64_t chunk, uint8_t* out, uint64_t out_bytes,
                         int32_t* status, int threads);
 
+/* ============================================================== LZSS ===== *
+ * Lane-transposed LZSS (include/lzss/lzss_gpu2.h).  Lane t of a chunk is the
+ * bytes chunk[128k + 4t + j]; lanes are sorted by stream length (descending,
+ * ties to the higher lane id) and col_map[32c+s] names the original lane at
+ * sorted position s.  blk_off[n] is the payload size. */
+uint64_t mdz_ref_lzss_lane_encode(const uint8_t* lane, uint64_t n, uint8_t* out, uint64_t cap);
+/* same stream, matches found by the literal find_match scan (lzss_gpu2.h:70-109) */
+uint64_t mdz_ref_lzss_lane_encode_naive(const uint8_t* lane, uint64_t n, uint8_t* out,
+                                        uint64_t cap);
+int mdz_ref_lzss_lane_decode(const uint8_t* in, uint64_t in_len, uint8_t* out, uint64_t out_len);
+int mdz_ref_lzss_encode(const uint8_t* in, uint64_t in_bytes, uint64_t chunk, uint8_t** payload,
+                        uint64_t* payload_bytes, uint64_t** col_len, uint64_t** blk_off,
+                        uint8_t** col_map, uint64_t* n_chunks, int threads);
+int mdz_ref_lzss_decode(const uint8_t* payload, uint64_t payload_bytes, const uint64_t* blk_off,
+                        const uint64_t* col_len, const uint8_t* col_map, uint64_t n_chunks,
+                        uint64_t chunk, uint8_t* out, uint64_t out_bytes, int32_t* status,
+                        int threads);
+
 /* ======================================================== generators ===== */
 /* Config 1: RLE v1 u32 column (runs 70% / literal segments 30% by count). */
 void mdz_gen_rlev1_u32(uint32_t* out, uint64_t n, uint64_t seed);
@@ -128,6 +146,9 @@ void mdz_gen_rlev1_u32(uint32_t* out, uint64_t n, uint64_t seed);
 void mdz_gen_rlev2_i64(uint64_t* out, uint64_t n, uint64_t seed);
 /* Config 2(b): byte runs, fresh byte != previous, length Geom(1/12). */
 void mdz_gen_brle_u8(uint8_t* out, uint64_t n, uint64_t seed);
+/* Config 3: 64-symbol order-2 Markov bytes (Dirichlet(0.3) rows) with
+ * 40% copy events of 4..256 bytes from up to 4096 back. */
+void mdz_gen_lzss_markov(uint8_t* out, uint64_t n, uint64_t seed);
 
 #ifdef __cplusplus
 }

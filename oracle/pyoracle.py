@@ -75,6 +75,20 @@ def lib():
         L.mdz_gen_rlev2_i64.argtypes = [C.c_void_p, C.c_uint64, C.c_uint64]
         L.mdz_gen_rlev1_u32.argtypes = [C.c_void_p, C.c_uint64, C.c_uint64]
         L.mdz_gen_brle_u8.argtypes = [C.c_void_p, C.c_uint64, C.c_uint64]
+        for f in (L.mdz_ref_lzss_lane_encode, L.mdz_ref_lzss_lane_encode_naive):
+            f.restype = C.c_uint64
+            f.argtypes = [C.c_void_p, C.c_uint64, C.c_void_p, C.c_uint64]
+        L.mdz_ref_lzss_lane_decode.restype = C.c_int
+        L.mdz_ref_lzss_lane_decode.argtypes = [C.c_void_p, C.c_uint64, C.c_void_p, C.c_uint64]
+        L.mdz_ref_lzss_encode.restype = C.c_int
+        L.mdz_ref_lzss_encode.argtypes = [C.c_void_p, C.c_uint64, C.c_uint64, C.POINTER(u8p), u64p,
+                                          C.POINTER(u64p), C.POINTER(u64p), C.POINTER(u8p), u64p,
+                                          C.c_int]
+        L.mdz_ref_lzss_decode.restype = C.c_int
+        L.mdz_ref_lzss_decode.argtypes = [C.c_void_p, C.c_uint64, C.c_void_p, C.c_void_p, C.c_void_p,
+                                          C.c_uint64, C.c_uint64, C.c_void_p, C.c_uint64, C.c_void_p,
+                                          C.c_int]
+        L.mdz_gen_lzss_markov.argtypes = [C.c_void_p, C.c_uint64, C.c_uint64]
         _lib = L
     return _lib
 
@@ -266,6 +280,75 @@ def brle_decode(c: BrleContainer, out_bytes: int | None = None, threads: int | N
     rc = lib().mdz_ref_brle_decode(_ptr(data), _ptr(c.ptr), c.n_chunks, c.chunk, _ptr(out), out_bytes,
                                    _ptr(status), threads or default_threads())
     return rc, out[:out_bytes], status[:c.n_chunks]
+
+
+# ------------------------------------------------------------------- LZSS --
+def lzss_lane_encode(lane: bytes, naive: bool = False) -> bytes:
+    a = np.frombuffer(bytes(lane), np.uint8) if len(lane) else np.zeros(1, np.uint8)
+    cap = len(lane) + len(lane) // 8 + 16
+    out = np.zeros(cap, np.uint8)
+    f = lib().mdz_ref_lzss_lane_encode_naive if naive else lib().mdz_ref_lzss_lane_encode
+    n = f(_ptr(a), len(lane), _ptr(out), cap)
+    assert n <= cap
+    return out[:n].tobytes()
+
+
+def lzss_lane_decode(stream: bytes, out_len: int) -> tuple[int, bytes]:
+    a = np.frombuffer(bytes(stream) + b"\0", np.uint8)
+    out = np.zeros(max(out_len, 1), np.uint8)
+    st = lib().mdz_ref_lzss_lane_decode(_ptr(a), len(stream), _ptr(out), out_len)
+    return st, out[:out_len].tobytes()
+
+
+class LzssContainer:
+    """payload + blk_off[n+1] + col_len[32n] (sorted) + col_map[32n]."""
+
+    def __init__(self, payload, col_len, blk_off, col_map, n_chunks, chunk):
+        self.payload, self.col_len, self.blk_off, self.col_map = payload, col_len, blk_off, col_map
+        self.n_chunks, self.chunk = n_chunks, chunk
+        self.raw_bytes = n_chunks * chunk
+
+    def meta_bytes(self) -> bytes:
+        """.meta side file: [u64 4][u64 lens[4]][u64 chunk][blk_off][col_len][col_map]
+        (src/lzss/lzss_gpu.cu:81-103, include/lzss/lzss_gpu2.h:2090-2108)."""
+        blobs = [np.uint64(self.chunk).tobytes(), self.blk_off.astype("<u8").tobytes(),
+                 self.col_len.astype("<u8").tobytes(), self.col_map.astype(np.uint8).tobytes()]
+        return (np.uint64(len(blobs)).tobytes() + np.array([len(b) for b in blobs], "<u8").tobytes()
+                + b"".join(blobs))
+
+
+def lzss_encode(data: np.ndarray, chunk: int, threads: int | None = None) -> LzssContainer:
+    raw = np.ascontiguousarray(data).view(np.uint8).reshape(-1)
+    if raw.size == 0:
+        raw = np.zeros(1, np.uint8)[:0]
+    src = raw if raw.size else np.zeros(1, np.uint8)
+    pay, pb, cl, bo, cm, nc = u8p(), C.c_uint64(), u64p(), u64p(), u8p(), C.c_uint64()
+    rc = lib().mdz_ref_lzss_encode(_ptr(src), raw.size, chunk, C.byref(pay), C.byref(pb), C.byref(cl),
+                                   C.byref(bo), C.byref(cm), C.byref(nc), threads or default_threads())
+    if rc != 0:
+        raise ValueError(f"lzss encode rejected parameters (status {rc})")
+    n = nc.value
+    return LzssContainer(_take(pay, pb.value, np.uint8), _take(cl, 32 * n, np.uint64),
+                         _take(bo, n + 1, np.uint64), _take(cm, 32 * n, np.uint8), n, chunk)
+
+
+def lzss_decode(c: LzssContainer, out_bytes: int | None = None, threads: int | None = None):
+    out_bytes = c.raw_bytes if out_bytes is None else out_bytes
+    out = np.zeros(max(out_bytes, 1), np.uint8)
+    status = np.zeros(max(c.n_chunks, 1), np.int32)
+    pay = c.payload if c.payload.size else np.zeros(1, np.uint8)
+    cl = c.col_len if c.col_len.size else np.zeros(1, np.uint64)
+    cm = c.col_map if c.col_map.size else np.zeros(1, np.uint8)
+    rc = lib().mdz_ref_lzss_decode(_ptr(pay), c.payload.size, _ptr(c.blk_off), _ptr(cl), _ptr(cm),
+                                   c.n_chunks, c.chunk, _ptr(out), out_bytes, _ptr(status),
+                                   threads or default_threads())
+    return rc, out[:out_bytes], status[:c.n_chunks]
+
+
+def gen_lzss_markov(n: int, seed: int) -> np.ndarray:
+    out = np.empty(max(n, 1), np.uint8)
+    lib().mdz_gen_lzss_markov(_ptr(out), n, seed)
+    return out[:n]
 
 
 # ------------------------------------------------------------- generators --
