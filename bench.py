@@ -183,6 +183,49 @@ class BrleWorkload:
                           f"oracle C decoder, median of >=3 runs"}
 
 
+class LzssWorkload:
+    """LZSS, lane-transposed: 1 GiB of 64-symbol order-2 Markov text with 40%
+    copy events, 64 KiB chunks (2 KiB lanes); a workgroup decodes 4 lanes of a
+    chunk, one wave per lane."""
+    name = "lzss_markov_1GiB_64KiB"
+    dtype = "u8"
+
+    def __init__(self, n=1 << 30, chunk=65536, seed=0x4D445A03, **_):
+        from oracle import pyoracle as O
+        from compression_amd.lanes import DeviceLanes
+        import compression_amd as mdz
+        self.O, self.mdz = O, mdz
+        self.data = O.gen_lzss_markov(n, seed)
+        self.c = c = O.lzss_encode(self.data, chunk)
+        self.lanes = DeviceLanes(c.payload, c.blk_off, c.col_len, c.n_chunks, chunk, 1, 4,
+                                 col_map=c.col_map)
+        self.dst = mdz.DeviceBuffer(c.raw_bytes)
+        self.status = mdz.DeviceBuffer(4 * c.n_chunks)
+        self.out_bytes = c.raw_bytes
+        nc = c.n_chunks
+        self.alg_bytes = c.payload.size + 256 * nc + 8 * (nc + 1) + 32 * nc + c.raw_bytes
+        self.config = {"workload": self.name, "codec": "lzss", "out_bytes": c.raw_bytes,
+                       "payload_bytes": int(c.payload.size), "n_chunks": nc, "chunk_bytes": chunk,
+                       "ratio": round(c.payload.size / c.raw_bytes, 4)}
+
+    def step(self, stream):
+        from compression_amd import lzss
+        lzss.decompress(self.lanes, self.dst, self.out_bytes, self.status, stream=stream)
+
+    def verify(self):
+        got = self.dst.download(self.out_bytes)
+        st = self.status.download(dtype=np.int32)
+        return bool((st == 0).all() and np.array_equal(got, self.data))
+
+    def cpu(self, threads):
+        O, c = self.O, self.c
+        t = _cpu_time(lambda: O.lzss_decode(c, threads=threads))
+        return {"value": round(c.raw_bytes / t / 1e9, 3), "unit": "GB/s", "cores": threads,
+                "kind": "port",
+                "sample": f"whole workload ({c.raw_bytes >> 20} MiB out, {c.n_chunks} chunks), "
+                          f"oracle C decoder, median of >=3 runs"}
+
+
 class CopyWorkload:
     """Calibration: a plain 16-byte-vector device copy of 512 MiB (reads and
     writes each byte once).  Its FETCH_SIZE/WRITE_SIZE per byte scale the
@@ -212,7 +255,7 @@ class CopyWorkload:
 
 
 WORKLOADS = {"rlev1": RleV1Workload, "rlev2": RleV2Workload, "brle": BrleWorkload,
-             "copy": CopyWorkload}
+             "lzss": LzssWorkload, "copy": CopyWorkload}
 DEFAULT = "rlev1"
 
 

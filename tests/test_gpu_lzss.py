@@ -88,7 +88,7 @@ def test_overlapping_matches_from_crafted_streams(gpu):
     rc, ref, ref_st = O.lzss_decode(c, threads=1)
     got, st = _run(gpu, c)
     assert rc == 0 and (st == 0).all()
-    assert np.array_equal(ref, assemble([lanes], chunk))
+    assert np.array_equal(ref, np.frombuffer(assemble([lanes], chunk), np.uint8))
     assert np.array_equal(got, ref)
 
 
