@@ -82,6 +82,11 @@ SIGNATURES = {
     "mdz_lzss_decompress_host": (C.c_int, [vp, u64, C.POINTER(vp), C.POINTER(u64), u64,
                                            C.POINTER(vp), C.POINTER(u64)]),
     "mdz_meta_split": (C.c_int, [vp, u64, C.POINTER(vp), C.POINTER(u64), C.POINTER(u64)]),
+    "mdz_inflate_batch": (C.c_int, [vp, vp, C.c_int, C.c_int, vp]),
+    "mdz_inflate_batch_sched": (C.c_int, [vp, vp, C.c_int, C.c_int, vp, u32, vp, vp, vp]),
+    "mdz_inflate_slots": (C.c_int, [C.POINTER(u32)]),
+    "mdz_deflate_decompress_gpu": (C.c_int, [vp, C.POINTER(vp), u64, C.POINTER(u64), vp, u64, vp, u64,
+                                             u64]),
 }
 
 _lib = None

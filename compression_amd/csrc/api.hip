@@ -340,4 +340,80 @@ int mdz_meta_split(const uint8_t* meta, uint64_t meta_bytes, const uint8_t** blo
     return MDZ_OK;
 }
 
+// ----------------------------------------------------------------- Deflate
+int mdz_inflate_batch(const mdz_inflate_in_t* in, mdz_inflate_status_t* out, int count,
+                      int parse_hdr, void* stream) {
+    if (count < 0) return MDZ_ERR_INVALID;
+    return mdz::inflate_batch(in, out, (uint32_t)count, parse_hdr, nullptr, 0, nullptr, nullptr,
+                              S(stream));
+}
+
+int mdz_inflate_batch_sched(const mdz_inflate_in_t* in, mdz_inflate_status_t* out, int count,
+                            int parse_hdr, const uint32_t* order, uint32_t blocks,
+                            uint32_t* cursor, mdz_inflate_cost_t* cost, void* stream) {
+    if (count < 0 || !cursor) return MDZ_ERR_INVALID;
+    return mdz::inflate_batch(in, out, (uint32_t)count, parse_hdr, order, blocks, cursor, cost,
+                              S(stream));
+}
+
+int mdz_inflate_slots(uint32_t* slots) {
+    if (!slots) return MDZ_ERR_INVALID;
+    return mdz::inflate_slots(slots);
+}
+
+int mdz_deflate_decompress_gpu(const uint8_t* in, uint8_t** out, uint64_t in_n_bytes,
+                               uint64_t* out_n_bytes, const uint64_t* col_len_f,
+                               uint64_t col_n_bytes, const uint64_t* blk_offset_f,
+                               uint64_t blk_n_bytes, uint64_t chunk_size) {
+    if (!in || !out || !out_n_bytes || !col_len_f || !blk_offset_f || blk_n_bytes < 24 ||
+        !chunk_size)
+        return MDZ_ERR_INVALID;
+    const uint64_t n = blk_n_bytes / 8 - 2;
+    if (col_n_bytes / 8 < n) return MDZ_ERR_INVALID;
+    const uint64_t data_size = blk_offset_f[0];
+    const uint64_t* off = blk_offset_f + 1;
+    if (data_size > n * chunk_size) return MDZ_ERR_INVALID;
+    for (uint64_t i = 0; i < n; ++i)
+        if (off[i] > in_n_bytes || col_len_f[i] > in_n_bytes - off[i]) return MDZ_ERR_INVALID;
+    DevBuf d_in, d_out, d_desc, d_st;
+    int rc;
+    if ((rc = d_in.alloc(in_n_bytes)) || (rc = d_out.alloc(data_size)) ||
+        (rc = d_desc.alloc(n * sizeof(mdz_inflate_in_t))) ||
+        (rc = d_st.alloc(n * sizeof(mdz_inflate_status_t))))
+        return rc;
+    mdz_inflate_in_t* desc = (mdz_inflate_in_t*)malloc(n * sizeof(mdz_inflate_in_t) + 1);
+    if (!desc) return MDZ_ERR_NOMEM;
+    for (uint64_t i = 0; i < n; ++i) {
+        const uint64_t o = i * chunk_size;
+        desc[i].src = (const uint8_t*)d_in.p + off[i];
+        desc[i].src_bytes = col_len_f[i];
+        desc[i].dst = (uint8_t*)d_out.p + o;
+        desc[i].dst_bytes = o < data_size ? (data_size - o < chunk_size ? data_size - o : chunk_size) : 0;
+    }
+    const bool ok = hipMemcpy(d_in.p, in, in_n_bytes, hipMemcpyHostToDevice) == hipSuccess &&
+                    hipMemcpy(d_desc.p, desc, n * sizeof(mdz_inflate_in_t), hipMemcpyHostToDevice) ==
+                        hipSuccess;
+    free(desc);
+    if (!ok) return MDZ_ERR_HIP;
+    rc = mdz::inflate_batch((const mdz_inflate_in_t*)d_desc.p, (mdz_inflate_status_t*)d_st.p,
+                            (uint32_t)n, 1, nullptr, 0, nullptr, nullptr, nullptr);
+    if (rc) return rc;
+    if (hipDeviceSynchronize() != hipSuccess) return MDZ_ERR_HIP;
+    mdz_inflate_status_t* st = (mdz_inflate_status_t*)malloc(n * sizeof(mdz_inflate_status_t) + 1);
+    uint8_t* o = (uint8_t*)malloc(data_size ? data_size : 1);
+    if (!st || !o) { free(st); free(o); return MDZ_ERR_NOMEM; }
+    if (hipMemcpy(st, d_st.p, n * sizeof(mdz_inflate_status_t), hipMemcpyDeviceToHost) != hipSuccess ||
+        (data_size && hipMemcpy(o, d_out.p, data_size, hipMemcpyDeviceToHost) != hipSuccess)) {
+        free(st); free(o);
+        return MDZ_ERR_HIP;
+    }
+    rc = MDZ_OK;
+    for (uint64_t i = 0; i < n; ++i)
+        if (st[i].status != 0) { rc = MDZ_ERR_CORRUPT; break; }
+    free(st);
+    *out = o;
+    *out_n_bytes = data_size;
+    return rc;
+}
+
 }  // extern "C"

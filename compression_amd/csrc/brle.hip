@@ -10,10 +10,10 @@
 //   2. token discovery in parallel: the bytes are cut into 256 windows;
 //      thread i walks the token chain speculatively from kOverlap bytes
 //      before its window and notes the first token start in its window and
-//      the first one past it.  A wrong starting point falls into step with
-//      the true chain within a few tokens, so adjacent threads agree where
-//      their chains meet; the rare disagreement is repaired in order by one
-//      thread re-walking the affected windows from the true entry;
+//      the first one past it.  A wrong starting point usually falls into
+//      step with the true chain, so adjacent threads agree where their
+//      chains meet; a window that disagrees re-walks from its predecessor's
+//      exit, in parallel rounds until every window agrees;
 //   3. a block-wide prefix sum of the windows' output lengths gives every
 //      thread its output offset;
 //   4. each thread expands its window's tokens through a 16-byte register
@@ -36,8 +36,7 @@ struct BrleTables {
     uint32_t ooff[kThreads];    // output offset of window i
     uint32_t err;               // min over (input position << 8 | status byte)
     uint32_t total;
-    uint32_t bad;               // some window disagreed with its predecessor
-    uint32_t pad;
+    uint32_t any[2];            // repair-round flags
 };
 
 
@@ -159,7 +158,7 @@ __global__ __launch_bounds__(kThreads) void brle_decode_kernel(
             }
         }
     }
-    if (tid == 0) { tb.err = kNone; tb.bad = 0; }
+    if (tid == 0) { tb.err = kNone; tb.any[0] = 0; }
     __syncthreads();
     const uint8_t* d = lds + skew;
 
@@ -174,21 +173,29 @@ __global__ __launch_bounds__(kThreads) void brle_decode_kernel(
         tb.exit_[tid] = ex;
         tb.olen[tid] = olen;
     }
+    // repair rounds: a window whose first token start is not the previous
+    // window's exit re-walks from that exit.  Window 0 starts at the true
+    // entry, so after round r windows 0..r are right; a wrong chain usually
+    // joins the true one inside its window, so few rounds are needed.
+    // The "any window wrong" flag alternates between two words: round r
+    // sets any[r&1] and clears any[(r+1)&1] before the barrier, so each flag
+    // is zero when its round starts and nobody reads it while it is cleared.
     __syncthreads();
-    if (tid > 0 && tb.first[tid] != tb.exit_[tid - 1]) tb.bad = 1;
-    __syncthreads();
-    if (tb.bad && tid == 0) {                 // repair in order from the true entry
-        for (uint32_t i = 1; i < kThreads; ++i) {
-            const uint32_t e = tb.exit_[i - 1];
-            if (tb.first[i] == e) continue;
-            const uint32_t s = min(L, i * W), t = min(L, s + W);
+    for (uint32_t round = 0;; ++round) {
+        const bool wrong = tid > 0 && tb.first[tid] != tb.exit_[tid - 1];
+        const uint32_t e = wrong ? tb.exit_[tid - 1] : 0;
+        if (wrong) tb.any[round & 1] = 1;
+        if (tid == 0) tb.any[(round + 1) & 1] = 0;
+        __syncthreads();
+        if (!tb.any[round & 1]) break;
+        if (wrong) {
             uint32_t first, olen;
-            tb.exit_[i] = walk(d, L, e, s, t, first, olen);
-            tb.first[i] = first;
-            tb.olen[i] = olen;
+            tb.exit_[tid] = walk(d, L, e, ws, we, first, olen);
+            tb.first[tid] = first;
+            tb.olen[tid] = olen;
         }
+        __syncthreads();
     }
-    __syncthreads();
 
     // ---- 3. output offsets (block-wide exclusive scan over 4 waves)
     {

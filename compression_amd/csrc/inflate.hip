@@ -1,0 +1,537 @@
+// Deflate / zlib inflate for gfx950: one wavefront per stream.
+//
+// Replaces the reference's deflate::decompress_gpu inflate kernels (reference
+// include/zlib/zlib.h:832-962; reader, decoder and writer warps of one lane
+// each doing useful work) and cuDF's gpuinflate (include/cudf_deflate.cuh:
+// 1229-1345,1399-1407) behind the same batched descriptor/status interface.
+//
+// Per stream, one 64-lane wave:
+//   * bit reader: a 256-byte input window lives in one VGPR (lane l holds
+//     bytes 4l..4l+3); the 64-bit bit buffer is refilled from it with
+//     readlane, and the next window is prefetched when one is entered;
+//   * Huffman decode: 10-bit (literal/length) and 8-bit (distance) lookup
+//     tables in LDS hold, per bit pattern, the code length, the symbol's
+//     base value and extra-bit count; longer codes take a canonical decode
+//     (count/symbol tables), which also rejects codes an incomplete set does
+//     not define.  Tables are built by the whole wave from the code lengths
+//     (ballot-ranked canonical codes, strided table fill);
+//   * LZ77: symbols are decoded in batches of up to 64 (one record per lane,
+//     at most ~4 KiB of output per batch); the wave then writes the batch's
+//     literals and copies its matches, in order, within a 36.5 KiB LDS ring
+//     that keeps the last 32 KiB of output; byte j of a match reads ring
+//     position start - dist + (j mod dist), so overlapping matches need no
+//     serial step;
+//   * finished output is flushed to HBM in dword stores after each batch.
+// Errors follow zlib's checks and order (invalid block type, stored length,
+// code-set completeness, repeat codes, missing end-of-block, invalid codes,
+// distance too far back) with cuDF's status codes; the zlib header is checked
+// when parse_hdr is set and the Adler-32 trailer is not verified.
+#include "mdz_device.h"
+#include "mdz_internal.h"
+
+namespace mdz {
+namespace {
+
+constexpr uint32_t kRing = 32768 + 4608;    // history + one batch of output
+constexpr uint32_t kBatchOut = 4096;        // a batch closes once this much output is queued
+constexpr uint32_t kLitBits = 10, kDistBits = 8;
+
+// LUT entry: [3:0] code length (0: take the canonical path), [7:4] extra
+// bits, [9:8] kind (0 literal, 1 end of block, 2 length/distance, 3 invalid),
+// [31:16] value (literal byte, base length or base distance).
+enum : uint32_t { kLit = 0, kEob = 1, kRef = 2, kBad = 3 };
+
+__constant__ uint16_t kLenBase[29] = {3, 4, 5, 6, 7, 8, 9, 10, 11, 13, 15, 17, 19, 23, 27, 31,
+                                      35, 43, 51, 59, 67, 83, 99, 115, 131, 163, 195, 227, 258};
+__constant__ uint8_t kLenExtra[29] = {0, 0, 0, 0, 0, 0, 0, 0, 1, 1, 1, 1, 2, 2, 2, 2,
+                                      3, 3, 3, 3, 4, 4, 4, 4, 5, 5, 5, 5, 0};
+__constant__ uint16_t kDistBase[30] = {1, 2, 3, 4, 5, 7, 9, 13, 17, 25, 33, 49, 65, 97, 129, 193,
+                                       257, 385, 513, 769, 1025, 1537, 2049, 3073, 4097, 6145,
+                                       8193, 12289, 16385, 24577};
+__constant__ uint8_t kDistExtra[30] = {0, 0, 0, 0, 1, 1, 2, 2, 3, 3, 4, 4, 5, 5, 6, 6,
+                                       7, 7, 8, 8, 9, 9, 10, 10, 11, 11, 12, 12, 13, 13};
+__constant__ uint8_t kClOrder[19] = {16, 17, 18, 0, 8, 7, 9, 6, 10, 5, 11, 4, 12, 3, 13, 2, 14, 1, 15};
+
+MDZ_DEV uint32_t lit_entry_bits(uint32_t sym) {          // kind / extra / value for a litlen symbol
+    if (sym < 256) return (kLit << 8) | (sym << 16);
+    if (sym == 256) return kEob << 8;
+    if (sym <= 285) return (kRef << 8) | ((uint32_t)kLenExtra[sym - 257] << 4) |
+                           ((uint32_t)kLenBase[sym - 257] << 16);
+    return kBad << 8;
+}
+MDZ_DEV uint32_t dist_entry_bits(uint32_t sym) {
+    if (sym < 30) return (kRef << 8) | ((uint32_t)kDistExtra[sym] << 4) | ((uint32_t)kDistBase[sym] << 16);
+    return kBad << 8;
+}
+
+// One canonical code: counts per length, symbols by (length, value), LUT.
+struct Code {
+    uint16_t cnt[16];
+    uint16_t sym[288];
+};
+
+struct InflateLds {
+    uint32_t lit_lut[1u << kLitBits];
+    uint32_t dist_lut[1u << kDistBits];
+    uint32_t cl_lut[128];                      // code-length code: 7-bit table, complete
+    Code lit, dist, cl;
+    uint8_t lens[320];                         // literal/length then distance code lengths
+    uint8_t ring[kRing];
+};
+
+MDZ_DEV uint32_t bitrev(uint32_t v, uint32_t n) { return __builtin_bitreverse32(v) >> (32 - n); }
+
+// Builds code `c` and its LUT from lens[0..n) (whole wave).  kind: 0 litlen,
+// 1 distance, 2 code lengths.  Returns 0, or -1 for an over-subscribed set or
+// an incomplete one that zlib rejects (incomplete is allowed only for
+// literal/length and distance sets whose longest code has length 1).
+MDZ_DEV int build_code(Code& c, uint32_t* lut, uint32_t lut_bits, const uint8_t* lens, uint32_t n,
+                       uint32_t kind) {
+    const uint32_t l = lane_id();
+    uint32_t cnt = 0;                          // lane k < 16 counts length k
+    for (uint32_t s0 = 0; s0 < n; s0 += 64) {
+        const uint32_t s = s0 + l;
+        const uint32_t len = s < n ? lens[s] : 0;
+#pragma unroll
+        for (uint32_t k = 1; k < 16; ++k) {
+            const uint32_t b = __popcll(__ballot(len == k));
+            if (l == k) cnt += b;
+        }
+    }
+    if (l < 16) c.cnt[l] = (uint16_t)(l == 0 ? 0 : cnt);
+    wave_sync();
+    // completeness (zlib inftrees.c rules), max length, symbol offsets, first codes
+    int left = 1, maxlen = 0;
+    uint32_t offs[16], code = 0, first[16];
+    offs[1] = 0;
+#pragma unroll
+    for (uint32_t k = 1; k < 16; ++k) {
+        const uint32_t ck = c.cnt[k];
+        left = (left << 1) - (int)ck;
+        if (left < 0) return -1;
+        if (ck) maxlen = (int)k;
+        code = (code + (k > 1 ? c.cnt[k - 1] : 0)) << 1;
+        first[k] = code;
+        if (k < 15) offs[k + 1] = offs[k] + ck;
+    }
+    if (left > 0 && (kind == 2 || maxlen != 1) && maxlen != 0) return -1;
+    // every LUT entry starts as "canonical path"; short codes overwrite theirs.
+    // An empty code-length code decodes, as zlib's table does, every pattern
+    // as length 0 with 1 bit.
+    const uint32_t lsize = 1u << lut_bits;
+    const uint32_t init = (kind == 2 && maxlen == 0) ? 1u : 0u;
+    for (uint32_t i = l; i < lsize; i += 64) lut[i] = init;
+    wave_sync();
+    uint32_t run[16];
+#pragma unroll
+    for (uint32_t k = 0; k < 16; ++k) run[k] = 0;
+    for (uint32_t s0 = 0; s0 < n; s0 += 64) {
+        const uint32_t s = s0 + l;
+        const uint32_t len = s < n ? lens[s] : 0;
+        uint32_t rank = 0;
+#pragma unroll
+        for (uint32_t k = 1; k < 16; ++k) {
+            const uint64_t b = __ballot(len == k);
+            if (len == k) rank = run[k] + __popcll(b & ((1ull << l) - 1));
+            run[k] += __popcll(b);
+        }
+        if (len) {
+            uint32_t off = 0, fc = 0;
+#pragma unroll
+            for (uint32_t k = 1; k < 16; ++k)
+                if (k == len) { off = offs[k]; fc = first[k]; }
+            c.sym[off + rank] = (uint16_t)s;
+            if (len <= lut_bits) {
+                const uint32_t e = len | (kind == 0 ? lit_entry_bits(s) : kind == 1 ? dist_entry_bits(s)
+                                                                                    : (s << 16));
+                for (uint32_t j = bitrev(fc + rank, len); j < lsize; j += 1u << len) lut[j] = e;
+            }
+        }
+    }
+    wave_sync();
+    return 0;
+}
+
+struct Reader {
+    const uint8_t* src;
+    uint64_t n;          // stream bytes
+    uint64_t wbyte;      // stream byte offset of the current window's lane-0 dword start (aligned)
+    uint32_t win, nxt;   // current and next 256-byte windows (one dword per lane)
+    uint32_t wpos;       // next dword of `win` to append
+    uint64_t bb;         // bit buffer (LSB first)
+    uint32_t nb;         // valid bits in bb
+    uint64_t cbits;      // bits consumed since the stream start
+
+    // dword k of the aligned window at absolute address a (bytes outside the
+    // stream read as 0, never touched)
+    MDZ_DEV uint32_t load_dword(uint64_t a) const {
+        const uint64_t s = (uint64_t)(uintptr_t)src, e = s + n;
+        if (a >= s && a + 4 <= e) return *reinterpret_cast<const uint32_t*>((uintptr_t)a);
+        uint32_t w = 0;
+        for (uint32_t k = 0; k < 4; ++k)
+            if (a + k >= s && a + k < e) w |= (uint32_t)*reinterpret_cast<const uint8_t*>((uintptr_t)(a + k)) << (8 * k);
+        return w;
+    }
+    MDZ_DEV uint32_t load_window(uint64_t abase) const {
+        return load_dword(abase + 4 * (uint64_t)lane_id());
+    }
+    // positions the reader at stream byte q
+    MDZ_DEV void seek(uint64_t q) {
+        const uint64_t a = ((uint64_t)(uintptr_t)src + q) & ~3ull;
+        win = load_window(a);
+        nxt = load_window(a + 256);
+        wbyte = a;
+        const uint32_t skew = (uint32_t)(((uint64_t)(uintptr_t)src + q) & 3);
+        const uint32_t w0 = __builtin_amdgcn_readlane(win, 0);
+        bb = w0 >> (8 * skew);
+        nb = 32 - 8 * skew;
+        wpos = 1;
+        cbits = 8 * q;
+    }
+    MDZ_DEV void refill() {
+        if (nb > 32) return;
+        const uint32_t w = __builtin_amdgcn_readlane(win, wpos);
+        bb |= (uint64_t)w << nb;
+        nb += 32;
+        if (++wpos == 64) {
+            win = nxt;
+            wbyte += 256;
+            nxt = load_window(wbyte + 256);
+            wpos = 0;
+        }
+    }
+    MDZ_DEV uint32_t peek(uint32_t k) const { return (uint32_t)(bb & ((1ull << k) - 1)); }
+    MDZ_DEV void drop(uint32_t k) { bb >>= k; nb -= k; cbits += k; }
+    MDZ_DEV uint32_t bits(uint32_t k) {             // k <= 32, after refill
+        const uint32_t v = peek(k);
+        drop(k);
+        return v;
+    }
+    MDZ_DEV bool over() const { return cbits > 8 * n; }
+};
+
+// Canonical decode for codes longer than the LUT (or undefined patterns).
+// Returns the symbol or -1; consumes the code's bits.
+MDZ_DEV int slow_decode(const Code& c, Reader& r) {
+    int code = 0, first = 0, index = 0;
+    uint64_t b = r.bb;
+    for (uint32_t len = 1; len < 16; ++len) {
+        code |= (int)(b & 1);
+        b >>= 1;
+        const int count = c.cnt[len];
+        if (code - count < first) {
+            r.drop(len);
+            return c.sym[index + (code - first)];
+        }
+        index += count;
+        first += count;
+        first <<= 1;
+        code <<= 1;
+    }
+    return -1;
+}
+
+MDZ_DEV uint32_t ring_at(uint64_t p) { return (uint32_t)(p % kRing); }
+
+__global__ __launch_bounds__(64) void inflate_kernel(const mdz_inflate_in_t* __restrict__ ins,
+                                                     mdz_inflate_status_t* __restrict__ outs,
+                                                     const uint32_t* __restrict__ order,
+                                                     uint32_t count, int parse_hdr,
+                                                     uint32_t* __restrict__ cursor,
+                                                     mdz_inflate_cost_t* __restrict__ cost) {
+    extern __shared__ __align__(16) uint8_t smem[];
+    InflateLds& S = *reinterpret_cast<InflateLds*>(smem);
+    const uint32_t l = lane_id();
+
+    for (;;) {
+        // next stream: one per block (plain) or from the shared cursor (persistent)
+        uint32_t qi;
+        if (cursor) {
+            uint32_t t = 0;
+            if (l == 0) t = atomicAdd(cursor, 1u);
+            qi = __builtin_amdgcn_readfirstlane(t);
+        } else {
+            qi = blockIdx.x;
+        }
+        if (qi >= count) return;
+        const uint32_t si = order ? order[qi] : qi;
+        const mdz_inflate_in_t in = ins[si];
+        uint8_t* const dst = reinterpret_cast<uint8_t*>(in.dst);
+        const uint64_t cap = in.dst_bytes;
+
+        Reader r;
+        r.src = reinterpret_cast<const uint8_t*>(in.src);
+        r.n = in.src_bytes;
+        int st = MDZ_ST_OK;
+        uint64_t pos = 0, flushed = 0;             // output produced / flushed to dst
+        uint32_t fixed_built = 0;
+        uint32_t n_sym = 0, n_match = 0, n_stored = 0, n_fixed = 0, n_dyn = 0;
+        const uint64_t t0 = clock64();
+
+        r.seek(0);
+        if (parse_hdr) {
+            r.refill();
+            const uint32_t cmf = r.bits(8), flg = r.bits(8);
+            if (r.over()) st = MDZ_ST_INPUT_OVERRUN;
+            else if ((cmf & 15) != 8 || (cmf >> 4) > 7 || (flg & 0x20) || ((cmf << 8) | flg) % 31)
+                st = MDZ_ST_BAD_HEADER;
+        }
+
+        // flushes whole dwords (or bytes for an unaligned dst) of [flushed, upto)
+        auto flush = [&](uint64_t upto, bool all) {
+            uint64_t lim = min(upto, cap);
+            const bool aligned = (((uintptr_t)dst + flushed) & 3) == 0;
+            if (aligned && !all) lim &= ~3ull;
+            if (lim <= flushed) return;
+            if (aligned) {
+                const uint64_t nw = (lim - flushed) / 4;
+                for (uint64_t w = l; w < nw; w += 64) {
+                    const uint64_t p = flushed + 4 * w;
+                    *reinterpret_cast<uint32_t*>(dst + p) = *reinterpret_cast<const uint32_t*>(&S.ring[ring_at(p)]);
+                }
+                for (uint64_t p = flushed + 4 * nw + l; p < lim; p += 64) dst[p] = S.ring[ring_at(p)];
+            } else {
+                for (uint64_t p = flushed + l; p < lim; p += 64) dst[p] = S.ring[ring_at(p)];
+            }
+            flushed = lim;
+        };
+
+        bool final = false;
+        while (st == MDZ_ST_OK && !final) {
+            r.refill();
+            final = r.bits(1);
+            const uint32_t type = r.bits(2);
+            if (r.over()) { st = MDZ_ST_INPUT_OVERRUN; break; }
+            if (type == 3) { st = MDZ_ST_BAD_BLOCK_TYPE; break; }
+            if (type == 0) {                                    // stored
+                ++n_stored;
+                r.drop(r.nb & 7);
+                r.refill();
+                const uint32_t len = r.bits(16), nlen = r.bits(16);
+                if (r.over()) { st = MDZ_ST_INPUT_OVERRUN; break; }
+                if (len != (~nlen & 0xffffu)) { st = MDZ_ST_STORED_LEN; break; }
+                const uint64_t q = r.cbits / 8;
+                uint64_t avail = q <= r.n ? r.n - q : 0, take = min<uint64_t>(len, avail);
+                for (uint64_t done = 0; done < take;) {
+                    const uint64_t k = min<uint64_t>(take - done, kBatchOut);
+                    for (uint64_t j = l; j < k; j += 64) S.ring[ring_at(pos + j)] = r.src[q + done + j];
+                    wave_sync();
+                    const uint64_t room = pos < cap ? cap - pos : 0;
+                    pos += min(k, room);
+                    flush(pos, false);
+                    wave_sync();
+                    done += k;
+                    if (k > room) { st = MDZ_ST_OUTPUT_OVERRUN; break; }
+                }
+                if (st != MDZ_ST_OK) break;
+                if (take < len) { st = MDZ_ST_INPUT_OVERRUN; break; }
+                r.seek(q + len);
+                continue;
+            }
+            if (type == 1) {                                    // fixed tables
+                ++n_fixed;
+                if (!fixed_built) {
+                    for (uint32_t s = l; s < 320; s += 64)
+                        S.lens[s] = s < 144 ? 8 : s < 256 ? 9 : s < 280 ? 7 : s < 288 ? 8 : 5;
+                    wave_sync();
+                    build_code(S.lit, S.lit_lut, kLitBits, S.lens, 288, 0);
+                    build_code(S.dist, S.dist_lut, kDistBits, S.lens + 288, 32, 1);
+                    fixed_built = 1;
+                }
+            } else {                                            // dynamic tables
+                ++n_dyn;
+                fixed_built = 0;
+                r.refill();
+                const uint32_t nlen = r.bits(5) + 257, ndist = r.bits(5) + 1, ncode = r.bits(4) + 4;
+                if (r.over()) { st = MDZ_ST_INPUT_OVERRUN; break; }
+                if (nlen > 286 || ndist > 30) { st = MDZ_ST_BAD_COUNTS; break; }
+                for (uint32_t s = l; s < 19; s += 64) S.lens[s] = 0;
+                wave_sync();
+                for (uint32_t i = 0; i < ncode; ++i) {
+                    r.refill();
+                    const uint32_t v = r.bits(3);
+                    if (l == 0) S.lens[kClOrder[i]] = (uint8_t)v;
+                }
+                wave_sync();
+                if (r.over()) { st = MDZ_ST_INPUT_OVERRUN; break; }
+                if (build_code(S.cl, S.cl_lut, 7, S.lens, 19, 2)) { st = MDZ_ST_BAD_CODE_LENS; break; }
+                // code lengths for nlen + ndist symbols (lens[] reused from 0)
+                uint32_t idx = 0, prev = 0;
+                while (idx < nlen + ndist) {
+                    r.refill();
+                    const uint32_t e = __builtin_amdgcn_readfirstlane(S.cl_lut[r.peek(7)]);
+                    r.drop(e & 15);
+                    const uint32_t sym = e >> 16;
+                    uint32_t rep = 0, val = 0;
+                    if (sym < 16) { val = sym; rep = 1; prev = sym; }
+                    else if (sym == 16) { val = prev; rep = 3 + r.bits(2); }
+                    else if (sym == 17) { rep = 3 + r.bits(3); prev = 0; }
+                    else { rep = 11 + r.bits(7); prev = 0; }
+                    if (r.over()) { st = MDZ_ST_INPUT_OVERRUN; break; }
+                    if (sym == 16 && idx == 0) { st = MDZ_ST_BAD_REPEAT; break; }
+                    if (idx + rep > nlen + ndist) { st = MDZ_ST_BAD_REPEAT; break; }
+                    for (uint32_t j = l; j < rep; j += 64) {
+                        const uint32_t t = idx + j;
+                        S.lens[t < nlen ? t : 288 + (t - nlen)] = (uint8_t)val;
+                    }
+                    idx += rep;
+                }
+                if (st != MDZ_ST_OK) break;
+                for (uint32_t s = nlen + l; s < 288; s += 64) S.lens[s] = 0;
+                for (uint32_t s = 288 + ndist + l; s < 320; s += 64) S.lens[s] = 0;
+                wave_sync();
+                if (S.lens[256] == 0) { st = MDZ_ST_NO_EOB; break; }
+                if (build_code(S.lit, S.lit_lut, kLitBits, S.lens, nlen, 0)) { st = MDZ_ST_BAD_LITLEN; break; }
+                if (build_code(S.dist, S.dist_lut, kDistBits, S.lens + 288, ndist, 1)) {
+                    st = MDZ_ST_BAD_DIST;
+                    break;
+                }
+            }
+
+            // ---- symbols, in batches
+            bool eob = false;
+            while (!eob && st == MDZ_ST_OK) {
+                uint32_t n = 0, bout = 0;
+                uint32_t rec_off = 0, rec_len = 0, rec_val = 0;  // lane k: symbol k of the batch
+                while (n < 64 && bout < kBatchOut) {
+                    r.refill();
+                    uint32_t e = __builtin_amdgcn_readfirstlane(S.lit_lut[r.peek(kLitBits)]);
+                    if ((e & 15) == 0) {
+                        const int sym = slow_decode(S.lit, r);
+                        e = sym < 0 ? (kBad << 8) : lit_entry_bits((uint32_t)sym);
+                    } else {
+                        r.drop(e & 15);
+                    }
+                    const uint32_t kind = (e >> 8) & 3;
+                    if (r.over() || (kind == kBad && r.cbits + 1 > 8 * r.n)) {
+                        st = MDZ_ST_INPUT_OVERRUN;
+                        break;
+                    }
+                    if (kind == kBad) { st = MDZ_ST_BAD_SYMBOL; break; }
+                    if (kind == kEob) { eob = true; break; }
+                    uint32_t len = 1, val = e >> 16;
+                    if (kind == kRef) {
+                        r.refill();
+                        len = val + r.bits((e >> 4) & 15);
+                        uint32_t d = __builtin_amdgcn_readfirstlane(S.dist_lut[r.peek(kDistBits)]);
+                        if ((d & 15) == 0) {
+                            const int sym = slow_decode(S.dist, r);
+                            d = sym < 0 ? (kBad << 8) : dist_entry_bits((uint32_t)sym);
+                        } else {
+                            r.drop(d & 15);
+                        }
+                        if (((d >> 8) & 3) == kBad) {
+                            st = r.over() || r.cbits + 1 > 8 * r.n ? MDZ_ST_INPUT_OVERRUN
+                                                                    : MDZ_ST_BAD_DISTANCE;
+                            break;
+                        }
+                        r.refill();
+                        val = (d >> 16) + r.bits((d >> 4) & 15);
+                        if (r.over()) { st = MDZ_ST_INPUT_OVERRUN; break; }
+                        if (val > pos + bout) { st = MDZ_ST_BAD_DISTANCE; break; }
+                    }
+                    if (l == n) { rec_off = bout; rec_len = len; rec_val = val; }
+                    ++n;
+                    n_match += len > 1;
+                    bout += len;
+                    if (pos + bout > cap) break;       // overrun: nothing after it is decoded
+                }
+                n_sym += n;
+                // ---- emit the batch: literals, then matches in order
+                const uint64_t room = pos < cap ? cap - pos : 0;
+                // matches are at least 3 long, so rec_len == 1 marks a literal
+                if (l < n && rec_len == 1) S.ring[ring_at(pos + rec_off)] = (uint8_t)rec_val;
+                wave_sync();
+                uint64_t mm = __ballot(l < n && rec_len > 1);
+                while (mm) {
+                    const uint32_t m = __builtin_ctzll(mm);
+                    mm &= mm - 1;
+                    const uint32_t off = __builtin_amdgcn_readlane(rec_off, m);
+                    const uint32_t len = __builtin_amdgcn_readlane(rec_len, m);
+                    const uint32_t dist = __builtin_amdgcn_readlane(rec_val, m);
+                    const uint64_t p0 = pos + off, s0 = p0 - dist;
+                    if (dist >= len) {
+                        for (uint32_t j = l; j < len; j += 64) S.ring[ring_at(p0 + j)] = S.ring[ring_at(s0 + j)];
+                    } else {
+                        for (uint32_t j = l; j < len; j += 64) S.ring[ring_at(p0 + j)] = S.ring[ring_at(s0 + j % dist)];
+                    }
+                    wave_sync();
+                }
+                if (bout > room) {
+                    pos += room;
+                    flush(pos, true);
+                    st = MDZ_ST_OUTPUT_OVERRUN;
+                    break;
+                }
+                pos += bout;
+                flush(pos, false);
+                wave_sync();
+            }
+        }
+        flush(pos, true);
+        if (l == 0) {
+            mdz_inflate_status_t o;
+            o.bytes_written = pos;
+            if (cost) {
+                mdz_inflate_cost_t c;
+                c.cycles = clock64() - t0;
+                c.symbols = n_sym;
+                c.matches = n_match;
+                c.stored_blocks = n_stored;
+                c.fixed_blocks = n_fixed;
+                c.dynamic_blocks = n_dyn;
+                c.reserved = 0;
+                cost[si] = c;
+            }
+            o.status = (uint32_t)st;
+            o.reserved = 0;
+            outs[si] = o;
+        }
+        if (!cursor) return;
+        wave_sync();
+    }
+}
+
+__global__ void reset_cursor_kernel(uint32_t* c) { *c = 0; }
+
+}  // namespace
+
+int inflate_slots(uint32_t* slots) {
+    const size_t smem = sizeof(InflateLds);
+    hipError_t e = set_dynamic_lds((const void*)inflate_kernel, smem);
+    if (e != hipSuccess) return record_hip(e, "inflate_kernel LDS");
+    int dev = 0, per_cu = 0, cus = 0;
+    if ((e = hipGetDevice(&dev)) != hipSuccess ||
+        (e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev)) != hipSuccess ||
+        (e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)inflate_kernel, 64,
+                                                          smem)) != hipSuccess)
+        return record_hip(e, "inflate occupancy");
+    *slots = (uint32_t)(per_cu * cus);
+    return MDZ_OK;
+}
+
+int inflate_batch(const mdz_inflate_in_t* in, mdz_inflate_status_t* out, uint32_t count,
+                  int parse_hdr, const uint32_t* order, uint32_t persistent_blocks,
+                  uint32_t* cursor, mdz_inflate_cost_t* cost, hipStream_t s) {
+    if ((!in || !out) && count) return MDZ_ERR_INVALID;
+    if (count == 0) return MDZ_OK;
+    const bool persistent = cursor != nullptr;
+    const size_t smem = sizeof(InflateLds);
+    hipError_t e = set_dynamic_lds((const void*)inflate_kernel, smem);
+    if (e != hipSuccess) return record_hip(e, "inflate_kernel LDS");
+    uint32_t grid = count;
+    if (persistent) {
+        if (!persistent_blocks) {
+            int rc = inflate_slots(&persistent_blocks);
+            if (rc) return rc;
+        }
+        grid = min(persistent_blocks, count);
+        hipLaunchKernelGGL(reset_cursor_kernel, dim3(1), dim3(1), 0, s, cursor);
+        if ((e = hipGetLastError()) != hipSuccess) return record_hip(e, "inflate cursor reset");
+    }
+    hipLaunchKernelGGL(inflate_kernel, dim3(grid), dim3(64), smem, s, in, out, order, count,
+                       parse_hdr, persistent ? cursor : nullptr, cost);
+    return record_hip(hipGetLastError(), "inflate_kernel launch");
+}
+
+}  // namespace mdz

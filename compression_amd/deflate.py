@@ -1,0 +1,104 @@
+"""Deflate / zlib inflate -- host-side mirror of cuDF's ``gpuinflate``
+(include/cudf_deflate.cuh:1399-1407) and the repo's ``deflate::decompress_gpu``
+(include/zlib/zlib.h:966-1082).
+
+``DeviceBatch`` keeps a batch of streams and their output slots resident in
+HBM with the descriptor array the kernels read; ``inflate`` is the plain
+per-stream launch and ``inflate_sched`` the persistent work-queue launch.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from ._lib import DeviceBuffer, MdzError, check, lib
+from .lanes import as_ptr
+
+DESC = np.dtype([("src", "<u8"), ("src_bytes", "<u8"), ("dst", "<u8"), ("dst_bytes", "<u8")])
+STATUS = np.dtype([("bytes_written", "<u8"), ("status", "<i4"), ("reserved", "<u4")])
+COST = np.dtype([("cycles", "<u8"), ("symbols", "<u4"), ("matches", "<u4"), ("stored_blocks", "<u4"),
+                 ("fixed_blocks", "<u4"), ("dynamic_blocks", "<u4"), ("reserved", "<u4")])
+
+
+class DeviceBatch:
+    """Streams src[src_off[i] : src_off[i] + src_len[i]] and output slots of
+    dst_caps[i] bytes, all device-resident."""
+
+    def __init__(self, src: np.ndarray, src_off, src_len, dst_caps, stream=None):
+        src = np.ascontiguousarray(src, np.uint8)
+        self.count = len(src_len)
+        self.src = DeviceBuffer.from_array(src, pad=16, stream=stream)
+        caps = np.ascontiguousarray(dst_caps, np.uint64)
+        self.dst_off = np.zeros(self.count, np.uint64)
+        if self.count:
+            self.dst_off[1:] = np.cumsum((caps + 15) // 16 * 16)[:-1]
+        total = int(self.dst_off[-1] + caps[-1]) if self.count else 0
+        self.dst = DeviceBuffer(max(total, 1))
+        self.caps = caps
+        d = np.zeros(self.count, DESC)
+        d["src"] = self.src.ptr + np.asarray(src_off, np.uint64)
+        d["src_bytes"] = np.asarray(src_len, np.uint64)
+        d["dst"] = self.dst.ptr + self.dst_off
+        d["dst_bytes"] = caps
+        self.desc = DeviceBuffer.from_array(d, stream=stream)
+        self.status = DeviceBuffer(max(self.count, 1) * STATUS.itemsize)
+        self.cursor = DeviceBuffer(4)
+        self.order = None
+        self.cost = None
+
+    def enable_cost_log(self) -> None:
+        self.cost = DeviceBuffer(max(self.count, 1) * COST.itemsize)
+
+    def costs(self) -> np.ndarray:
+        return self.cost.download(self.count * COST.itemsize).view(COST)
+
+    def set_order(self, order) -> None:
+        self.order = DeviceBuffer.from_array(np.ascontiguousarray(order, np.uint32))
+
+    def statuses(self) -> np.ndarray:
+        return self.status.download(self.count * STATUS.itemsize).view(STATUS)
+
+    def output(self, i: int) -> bytes:
+        o = int(self.dst_off[i])
+        return self.dst.download(o + int(self.caps[i]))[o:].tobytes()
+
+    def outputs(self) -> np.ndarray:
+        return self.dst.download()
+
+
+def inflate(b: DeviceBatch, parse_hdr: bool = True, stream=None) -> None:
+    check(lib().mdz_inflate_batch(b.desc.ptr, b.status.ptr, b.count, int(parse_hdr), as_ptr(stream)),
+          "mdz_inflate_batch")
+
+
+def inflate_sched(b: DeviceBatch, parse_hdr: bool = True, blocks: int = 0, stream=None) -> None:
+    check(lib().mdz_inflate_batch_sched(b.desc.ptr, b.status.ptr, b.count, int(parse_hdr),
+                                        b.order.ptr if b.order is not None else None, blocks,
+                                        b.cursor.ptr, b.cost.ptr if b.cost is not None else None,
+                                        as_ptr(stream)), "mdz_inflate_batch_sched")
+
+
+def slots() -> int:
+    v = C.c_uint32()
+    check(lib().mdz_inflate_slots(C.byref(v)), "mdz_inflate_slots")
+    return v.value
+
+
+def decompress_gpu(data: bytes, col_len: np.ndarray, blk_offset: np.ndarray, chunk_size: int) -> bytes:
+    """The repo container: zlib streams, their padded sizes, [size, 0, offsets...]."""
+    src = np.frombuffer(data, np.uint8) if len(data) else np.zeros(1, np.uint8)
+    cl = np.ascontiguousarray(col_len, np.uint64)
+    bo = np.ascontiguousarray(blk_offset, np.uint64)
+    out = C.c_void_p()
+    n = C.c_uint64()
+    rc = lib().mdz_deflate_decompress_gpu(src.ctypes.data, C.byref(out), len(data), C.byref(n),
+                                          cl.ctypes.data, cl.nbytes, bo.ctypes.data, bo.nbytes,
+                                          chunk_size)
+    if rc not in (0, -5):
+        raise MdzError(rc, "mdz_deflate_decompress_gpu")
+    res = C.string_at(out.value, n.value) if n.value else b""
+    lib().mdz_host_free(out)
+    if rc != 0:
+        raise MdzError(rc, "mdz_deflate_decompress_gpu (corrupt stream)")
+    return res

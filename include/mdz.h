@@ -212,6 +212,69 @@ int mdz_lzss_decompress_host(const uint8_t* in, uint64_t in_n_bytes,
 int mdz_meta_split(const uint8_t* meta, uint64_t meta_bytes, const uint8_t** blobs, uint64_t* lens,
                    uint64_t* num);
 
+/* =========================================================== Deflate ===== *
+ * Batched zlib/Deflate inflate -- replaces cudf::io::gpuinflate and its
+ * structs (reference include/cudf_deflate.cuh:236-250,1399-1407) and the
+ * repo's deflate::decompress_gpu kernels (include/zlib/zlib.h:832-962).
+ * Layouts equal gpu_inflate_input_s / gpu_inflate_status_s. */
+typedef struct {
+    const void* src;       /* device pointer to the stream */
+    uint64_t src_bytes;
+    void* dst;             /* device pointer to its output slot */
+    uint64_t dst_bytes;
+} mdz_inflate_in_t;
+
+typedef struct {
+    uint64_t bytes_written;
+    uint32_t status;       /* MDZ_ST_* as uint32 (negative codes wrap) */
+    uint32_t reserved;
+} mdz_inflate_status_t;
+
+/* in/out: device arrays of `count` entries.  parse_hdr != 0: each stream
+ * starts with a 2-byte zlib header (checked; the Adler-32 trailer is not
+ * verified), else raw Deflate.  One stream per workgroup, launched in index
+ * order (the plain per-stream decoder). */
+int mdz_inflate_batch(const mdz_inflate_in_t* in, mdz_inflate_status_t* out, int count,
+                      int parse_hdr, void* stream);
+
+/* Per-stream decode-cost sample the decoder logs for the learned scheduler:
+ * shader-clock cycles from the stream's first to last instruction and what
+ * the stream held. */
+typedef struct {
+    uint64_t cycles;
+    uint32_t symbols;          /* literals + matches */
+    uint32_t matches;
+    uint32_t stored_blocks;
+    uint32_t fixed_blocks;
+    uint32_t dynamic_blocks;
+    uint32_t reserved;
+} mdz_inflate_cost_t;
+
+/* Persistent form: `blocks` workgroups (0 = one per co-resident slot) pull
+ * streams from a device work queue in the order order[0..count) (device u32
+ * array; NULL = index order).  `cursor` is a device u32 the call resets
+ * before the launch; `cost` (device array of count entries, or NULL) receives
+ * each stream's decode-cost sample.  No counterpart in the reference: it is
+ * the batched scheduler's launch. */
+int mdz_inflate_batch_sched(const mdz_inflate_in_t* in, mdz_inflate_status_t* out, int count,
+                            int parse_hdr, const uint32_t* order, uint32_t blocks,
+                            uint32_t* cursor, mdz_inflate_cost_t* cost, void* stream);
+
+/* Workgroups of the persistent inflate kernel that fit on the device at once. */
+int mdz_inflate_slots(uint32_t* slots);
+
+/* deflate::decompress_gpu's host form (include/zlib/zlib.h:966-1082) over the
+ * repo container (orig_compressor.py:44-125): in = concatenated zlib streams,
+ * col_len_f = u64 padded stream sizes [n], blk_offset_f = [u64 data_size]
+ * [u64 0][u64 cumulative padded offsets...] (n+2 entries), chunk_size = output
+ * bytes per stream (the last one is shorter).  *out (data_size bytes) is
+ * allocated here; free with mdz_host_free.  Stored blocks are written (the
+ * reference consumes them without writing, B15). */
+int mdz_deflate_decompress_gpu(const uint8_t* in, uint8_t** out, uint64_t in_n_bytes,
+                               uint64_t* out_n_bytes, const uint64_t* col_len_f,
+                               uint64_t col_n_bytes, const uint64_t* blk_offset_f,
+                               uint64_t blk_n_bytes, uint64_t chunk_size);
+
 #ifdef __cplusplus
 }
 #endif

@@ -380,12 +380,13 @@ static double gamma_small(mdz_rng* r, double a) {
     return g * pow(u, 1.0 / a);
 }
 
-/* Config 3: 64-symbol order-2 Markov text with rows ~ Dirichlet(0.3), mixed
- * with copies: at each step, with p = 0.4 (once there is history) copy
+/* 64-symbol order-2 Markov text with rows ~ Dirichlet(0.3), mixed with
+ * copies: at each step, with probability copy_p (once there is history) copy
  * L ~ U{4..256} bytes from distance D ~ U{1..min(4096, pos)}, byte by byte
  * (overlap allowed); otherwise emit one Markov symbol. */
-void mdz_gen_lzss_markov(uint8_t* out, uint64_t n, uint64_t seed) {
-    enum { SYM = 64, ROWS = SYM * SYM };
+enum { SYM = 64, ROWS = SYM * SYM };
+
+double* mdz_markov_table(uint64_t seed) {
     mdz_rng r;
     mdz_rng_seed(&r, seed);
     double* cdf = (double*)malloc(sizeof(double) * ROWS * SYM);
@@ -396,9 +397,15 @@ void mdz_gen_lzss_markov(uint8_t* out, uint64_t n, uint64_t seed) {
         for (int k = 0; k < SYM; ++k) { acc += c[k] / tot; c[k] = acc; }
         c[SYM - 1] = 2.0;                            /* the last symbol catches rounding */
     }
+    return cdf;
+}
+
+void mdz_markov_gen(const double* cdf, uint8_t* out, uint64_t n, uint64_t seed, double copy_p) {
+    mdz_rng r;
+    mdz_rng_seed(&r, seed);
     uint64_t pos = 0;
     while (pos < n) {
-        if (pos > 0 && mdz_rng_below(&r, 10) < 4) {
+        if (pos > 0 && mdz_rng_unit(&r) < copy_p) {
             uint64_t len = 4 + mdz_rng_below(&r, 253);
             const uint64_t dmax = pos < 4096 ? pos : 4096;
             const uint64_t d = 1 + mdz_rng_below(&r, dmax);
@@ -416,5 +423,12 @@ void mdz_gen_lzss_markov(uint8_t* out, uint64_t n, uint64_t seed) {
         }
         out[pos++] = (uint8_t)lo;
     }
+}
+
+/* Config 3: the Markov text with copy probability 0.4; the table and the
+ * sequence use seeds derived from `seed`. */
+void mdz_gen_lzss_markov(uint8_t* out, uint64_t n, uint64_t seed) {
+    double* cdf = mdz_markov_table(seed);
+    mdz_markov_gen(cdf, out, n, seed ^ 0x9e3779b97f4a7c15ull, 0.4);
     free(cdf);
 }

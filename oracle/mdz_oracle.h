@@ -138,6 +138,27 @@ int mdz_ref_lzss_decode(const uint8_t* payload, uint64_t payload_bytes, const ui
                         uint64_t chunk, uint8_t* out, uint64_t out_bytes, int32_t* status,
                         int threads);
 
+/* =========================================================== Deflate ===== *
+ * zlib is the oracle (RFC 1950/1951).  zlib_hdr: check the 2-byte zlib
+ * header, then inflate raw (the Adler-32 trailer is not verified).  Writes
+ * at most cap bytes; *written = bytes produced.  Status per cuDF's codes:
+ * 1 output full before the end, 2 input ended early, negatives per zlib's
+ * error (invalid block type -1, stored lengths -2, code lengths set -3,
+ * bit length repeat -4, literal/lengths set -5, distances set -6, too many
+ * symbols -7, missing end-of-block -8, invalid distance code / too far back
+ * -9, invalid literal/length code -10, header -11). */
+int mdz_ref_inflate(const uint8_t* src, uint64_t n, uint8_t* dst, uint64_t cap, uint64_t* written,
+                    int zlib_hdr);
+int mdz_ref_inflate_batch(const uint8_t* src, const uint64_t* src_off, const uint64_t* src_len,
+                          uint8_t* dst, const uint64_t* dst_off, const uint64_t* dst_cap,
+                          uint64_t count, int zlib_hdr, uint64_t* written, int32_t* status,
+                          int threads);
+/* Config 4 batch generator (see inflate.c). */
+int mdz_gen_deflate_batch(uint64_t count, uint64_t seed, uint64_t min_size, uint64_t max_size,
+                          uint8_t** src, uint64_t** src_off, uint64_t** src_len, uint8_t** raw,
+                          uint64_t** raw_off, uint64_t** raw_len, uint8_t** mode, int threads);
+const char* mdz_zlib_version(void);
+
 /* ======================================================== generators ===== */
 /* Config 1: RLE v1 u32 column (runs 70% / literal segments 30% by count). */
 void mdz_gen_rlev1_u32(uint32_t* out, uint64_t n, uint64_t seed);
@@ -149,6 +170,10 @@ void mdz_gen_brle_u8(uint8_t* out, uint64_t n, uint64_t seed);
 /* Config 3: 64-symbol order-2 Markov bytes (Dirichlet(0.3) rows) with
  * 40% copy events of 4..256 bytes from up to 4096 back. */
 void mdz_gen_lzss_markov(uint8_t* out, uint64_t n, uint64_t seed);
+/* The Markov source behind config 3 and 4: a transition table (4096 x 64
+ * CDF rows, free with mdz_free) and a sequence with copy probability copy_p. */
+double* mdz_markov_table(uint64_t seed);
+void mdz_markov_gen(const double* cdf, uint8_t* out, uint64_t n, uint64_t seed, double copy_p);
 
 #ifdef __cplusplus
 }

@@ -89,6 +89,18 @@ def lib():
                                           C.c_uint64, C.c_uint64, C.c_void_p, C.c_uint64, C.c_void_p,
                                           C.c_int]
         L.mdz_gen_lzss_markov.argtypes = [C.c_void_p, C.c_uint64, C.c_uint64]
+        L.mdz_ref_inflate.restype = C.c_int
+        L.mdz_ref_inflate.argtypes = [C.c_void_p, C.c_uint64, C.c_void_p, C.c_uint64, u64p, C.c_int]
+        L.mdz_ref_inflate_batch.restype = C.c_int
+        L.mdz_ref_inflate_batch.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                            C.c_void_p, C.c_uint64, C.c_int, C.c_void_p, C.c_void_p,
+                                            C.c_int]
+        L.mdz_gen_deflate_batch.restype = C.c_int
+        L.mdz_gen_deflate_batch.argtypes = [C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint64,
+                                            C.POINTER(u8p), C.POINTER(u64p), C.POINTER(u64p),
+                                            C.POINTER(u8p), C.POINTER(u64p), C.POINTER(u64p),
+                                            C.POINTER(u8p), C.c_int]
+        L.mdz_zlib_version.restype = C.c_char_p
         _lib = L
     return _lib
 
@@ -349,6 +361,64 @@ def gen_lzss_markov(n: int, seed: int) -> np.ndarray:
     out = np.empty(max(n, 1), np.uint8)
     lib().mdz_gen_lzss_markov(_ptr(out), n, seed)
     return out[:n]
+
+
+# ---------------------------------------------------------------- Deflate --
+def zlib_version() -> str:
+    return lib().mdz_zlib_version().decode()
+
+
+def inflate(stream: bytes, cap: int, zlib_hdr: bool = True) -> tuple[int, bytes]:
+    a = np.frombuffer(bytes(stream) + b"\0", np.uint8)
+    out = np.zeros(max(cap, 1), np.uint8)
+    w = C.c_uint64()
+    st = lib().mdz_ref_inflate(_ptr(a), len(stream), _ptr(out), cap, C.byref(w), int(zlib_hdr))
+    return st, out[:w.value].tobytes()
+
+
+class DeflateBatch:
+    """Concatenated zlib streams (16-byte aligned starts) and their raw data."""
+
+    def __init__(self, src, src_off, src_len, raw, raw_off, raw_len, mode):
+        self.src, self.src_off, self.src_len = src, src_off, src_len
+        self.raw, self.raw_off, self.raw_len, self.mode = raw, raw_off, raw_len, mode
+        self.count = len(src_len)
+
+    def stream(self, i) -> bytes:
+        o = int(self.src_off[i])
+        return self.src[o:o + int(self.src_len[i])].tobytes()
+
+    def raw_stream(self, i) -> bytes:
+        o = int(self.raw_off[i])
+        return self.raw[o:o + int(self.raw_len[i])].tobytes()
+
+
+def gen_deflate_batch(count: int, seed: int, min_size: int = 4096, max_size: int = 4 << 20,
+                      threads: int | None = None) -> DeflateBatch:
+    s, so, sl, r, ro, rl, md = u8p(), u64p(), u64p(), u8p(), u64p(), u64p(), u8p()
+    rc = lib().mdz_gen_deflate_batch(count, seed, min_size, max_size, C.byref(s), C.byref(so),
+                                     C.byref(sl), C.byref(r), C.byref(ro), C.byref(rl), C.byref(md),
+                                     threads or default_threads())
+    if rc != 0:
+        raise ValueError("bad deflate batch parameters")
+    so_a = _take(so, count + 1, np.uint64)
+    ro_a = _take(ro, count + 1, np.uint64)
+    return DeflateBatch(_take(s, int(so_a[-1]) + 16, np.uint8), so_a[:-1].copy(), _take(sl, count, np.uint64),
+                        _take(r, int(ro_a[-1]) + 1, np.uint8)[:int(ro_a[-1])], ro_a[:-1].copy(),
+                        _take(rl, count, np.uint64), _take(md, count, np.uint8))
+
+
+def inflate_batch(b: DeflateBatch, caps=None, zlib_hdr: bool = True, threads: int | None = None):
+    caps = b.raw_len.copy() if caps is None else np.ascontiguousarray(caps, np.uint64)
+    dst_off = np.zeros(b.count, np.uint64)
+    dst_off[1:] = np.cumsum(caps)[:-1]
+    out = np.zeros(max(int(caps.sum()), 1), np.uint8)
+    written = np.zeros(b.count, np.uint64)
+    status = np.zeros(b.count, np.int32)
+    rc = lib().mdz_ref_inflate_batch(_ptr(b.src), _ptr(b.src_off), _ptr(b.src_len), _ptr(out),
+                                     _ptr(dst_off), _ptr(caps), b.count, int(zlib_hdr), _ptr(written),
+                                     _ptr(status), threads or default_threads())
+    return rc, out, dst_off, written, status
 
 
 # ------------------------------------------------------------- generators --

@@ -1,0 +1,176 @@
+"""GPU parity: batched zlib inflate through the C ABI vs zlib (bit-exact) and
+zlib's status for corrupt streams."""
+import json
+import os
+import zlib
+
+import numpy as np
+import pytest
+
+from oracle import pyoracle as O
+
+pytestmark = pytest.mark.gpu
+FIX = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "deflate_fixtures.json")))
+
+
+def _batch(streams, caps, src_skew=0):
+    """Packs host streams (optionally at a byte skew) into a device batch."""
+    from compression_amd import deflate
+    offs, blob = [], bytearray()
+    for s in streams:
+        blob += bytes((-len(blob)) % 16 + src_skew)
+        offs.append(len(blob))
+        blob += s
+    return deflate.DeviceBatch(np.frombuffer(bytes(blob) + b"\0", np.uint8), offs,
+                               [len(s) for s in streams], caps)
+
+
+def _run(streams, caps, sched=False, parse_hdr=True, src_skew=0):
+    from compression_amd import deflate
+    b = _batch(streams, caps, src_skew)
+    b.dst.fill(0xEE)
+    if sched:
+        b.set_order(np.arange(b.count, dtype=np.uint32)[::-1].copy())
+        deflate.inflate_sched(b, parse_hdr=parse_hdr)
+    else:
+        deflate.inflate(b, parse_hdr=parse_hdr)
+    st = b.statuses()
+    out = b.outputs()
+    return [out[int(b.dst_off[i]): int(b.dst_off[i]) + int(st["bytes_written"][i])].tobytes()
+            for i in range(b.count)], st
+
+
+def test_fixtures(gpu):
+    vs = FIX["valid"]
+    outs, st = _run([bytes.fromhex(v["stream"]) for v in vs], [len(v["raw"]) // 2 for v in vs])
+    assert (st["status"] == 0).all()
+    for v, o in zip(vs, outs):
+        assert o.hex() == v["raw"], v["name"]
+
+
+def test_fault_statuses(gpu):
+    fs = FIX["faults"]
+    _, st = _run([bytes.fromhex(f["stream"]) for f in fs], [f["cap"] for f in fs])
+    for f, s in zip(fs, st["status"]):
+        assert s == f["status"], f["name"]
+
+
+@pytest.mark.parametrize("sched", [False, True])
+def test_generated_batch(gpu, sched):
+    b = O.gen_deflate_batch(96, 0x4D445A04, 4096, 1 << 20)
+    streams = [b.stream(i) for i in range(b.count)]
+    outs, st = _run(streams, b.raw_len, sched=sched)
+    assert (st["status"] == 0).all()
+    assert (st["bytes_written"] == b.raw_len).all()
+    for i in range(b.count):
+        assert outs[i] == b.raw_stream(i), i
+
+
+def test_edge_streams(gpu):
+    rng = np.random.default_rng(5)
+    datas = [b"", b"x", b"\0" * (1 << 20), rng.integers(0, 256, 300_000).astype(np.uint8).tobytes(),
+             (rng.integers(0, 256, 33_000).astype(np.uint8).tobytes()) * 5,          # dist ~ 33000 > 32768
+             (rng.integers(0, 256, 32_768).astype(np.uint8).tobytes()) * 4,          # dist exactly 32768
+             bytes(range(256)) * 1000]
+    streams, raws = [], []
+    for d in datas:
+        for level in (0, 1, 6, 9):
+            streams.append(zlib.compress(d, level))
+            raws.append(d)
+        c = zlib.compressobj(1, zlib.DEFLATED, 15, 8, zlib.Z_FIXED)
+        streams.append(c.compress(d) + c.flush())
+        raws.append(d)
+    outs, st = _run(streams, [len(r) for r in raws])
+    assert (st["status"] == 0).all()
+    for o, r in zip(outs, raws):
+        assert o == r
+
+
+def test_raw_deflate_unaligned_and_clipped(gpu):
+    rng = np.random.default_rng(8)
+    datas = [rng.integers(0, 9, int(n)).astype(np.uint8).tobytes() for n in rng.integers(1, 70_000, 24)]
+    raw_streams = []
+    for d in datas:
+        c = zlib.compressobj(6, zlib.DEFLATED, -15)
+        raw_streams.append(c.compress(d) + c.flush())
+    for skew in (1, 3):
+        outs, st = _run(raw_streams, [len(d) for d in datas], parse_hdr=False, src_skew=skew)
+        assert (st["status"] == 0).all()
+        assert all(o == d for o, d in zip(outs, datas))
+    caps = [max(0, len(d) - 7) for d in datas]
+    outs, st = _run(raw_streams, caps, parse_hdr=False)
+    for o, d, s, cap in zip(outs, datas, st["status"], caps):
+        assert s == 1 and o == d[:cap]
+
+
+def test_random_corruption_matches_zlib(gpu):
+    rng = np.random.default_rng(13)
+    base = [zlib.compress(rng.integers(0, 20, int(n)).astype(np.uint8).tobytes(), int(lv))
+            for n, lv in zip(rng.integers(100, 20_000, 40), rng.choice([1, 6, 9], 40))]
+    streams, caps = [], []
+    for s in base:
+        for _ in range(5):
+            b = bytearray(s)
+            for _ in range(int(rng.integers(1, 4))):
+                b[int(rng.integers(2, len(b)))] ^= 1 << int(rng.integers(0, 8))
+            streams.append(bytes(b))
+            caps.append(40_000)
+    _, st = _run(streams, caps)
+    for s, c, g in zip(streams, caps, st["status"]):
+        assert g == O.inflate(s, c)[0]
+
+
+def test_full_config4(gpu):
+    b = O.gen_deflate_batch(4096, 0x4D445A04, 4096, 4 << 20, threads=16)
+    from compression_amd import deflate
+    db = deflate.DeviceBatch(b.src, b.src_off, b.src_len, b.raw_len)
+    deflate.inflate(db)
+    st = db.statuses()
+    assert (st["status"] == 0).all() and (st["bytes_written"] == b.raw_len).all()
+    out = db.outputs()
+    for i in range(0, b.count, 97):
+        o = int(db.dst_off[i])
+        assert out[o:o + int(b.raw_len[i])].tobytes() == b.raw_stream(i)
+    order = np.argsort(-b.src_len.astype(np.int64)).astype(np.uint32)
+    db.set_order(order)
+    db.dst.fill(0)
+    deflate.inflate_sched(db)
+    st = db.statuses()
+    assert (st["status"] == 0).all()
+    out2 = db.outputs()
+    assert np.array_equal(out, out2)
+
+
+def test_repo_container_host_wrapper(gpu):
+    from compression_amd import deflate
+    x = O.gen_lzss_markov(5 * 131072 + 777, 21)
+    chunk = 131072
+    comp, sizes = b"", []
+    for i in range(0, x.size, chunk):
+        z = zlib.compress(x[i:i + chunk].tobytes(), 9)
+        z += bytes(-len(z) % 8)
+        comp += z
+        sizes.append(len(z))
+    blk = np.array([x.size, 0] + list(np.cumsum(sizes)), np.uint64)
+    assert deflate.decompress_gpu(comp, np.array(sizes, np.uint64), blk, chunk) == x.tobytes()
+
+
+def test_cost_log(gpu):
+    from compression_amd import deflate
+    b = O.gen_deflate_batch(64, 0x4D445A14, 4096, 1 << 19)
+    db = deflate.DeviceBatch(b.src, b.src_off, b.src_len, b.raw_len)
+    db.enable_cost_log()
+    deflate.inflate_sched(db)
+    st, cost = db.statuses(), db.costs()
+    assert (st["status"] == 0).all()
+    assert (cost["cycles"] > 0).all()
+    for i in range(b.count):
+        m = int(b.mode[i])
+        if m == 0:
+            assert cost["stored_blocks"][i] >= 1 and cost["symbols"][i] == 0
+        else:
+            assert cost["stored_blocks"][i] == 0 and cost["symbols"][i] > 0
+            assert cost["fixed_blocks"][i] + cost["dynamic_blocks"][i] >= 1
+            assert 0 < cost["matches"][i] <= cost["symbols"][i]
+        if m == 1:
+            assert cost["fixed_blocks"][i] >= 1 and cost["dynamic_blocks"][i] == 0
