@@ -231,24 +231,44 @@ class LzssWorkload:
 
 
 class Config2Workload:
-    """RLE v2 + byte RLE: the 1 GiB int64 column and the 256 MiB byte-RLE stream
-    decoded in one step (both codecs enqueued on one stream, 128 KiB chunks)."""
-    name = "rlev2_i64_1GiB+brle_u8_256MiB"
+    """RLE v2 + byte RLE: the 1 GiB int64 column and the 256 MiB byte-RLE
+    stream (128 KiB chunks).  Batched (default): both containers' 10,240
+    chunks decoded by ONE persistent launch that drains a device work queue
+    ordered longest-compressed-chunk first.  Plain: one per-chunk launch per
+    codec."""
     dtype = "u64+u8"
 
-    def __init__(self, line=512, **_):
+    def __init__(self, line=512, batched=True, **_):
+        from compression_amd import batch
         self.parts = [RleV2Workload(line=line), BrleWorkload()]
+        r, b = self.parts
+        self.batched = batched
+        self.name = ("rlev2_i64_1GiB+brle_u8_256MiB_" + ("batched" if batched else "plain"))
         self.out_bytes = sum(p.out_bytes for p in self.parts)
         self.alg_bytes = sum(p.alg_bytes for p in self.parts)
+        self.batch = batch.Batch(r.lanes, r.dst, r.out_bytes, r.status, r.max_chunk,
+                                 b.dev, b.dst, b.out_bytes, b.status)
+        c2, cb = r.c, b.c
+        sizes2 = np.append(c2.blk_off[1:-1].astype(np.int64), c2.payload_bytes) - c2.blk_off[:-1].astype(np.int64)
+        self.batch.set_order(batch.size_order(sizes2, np.diff(cb.ptr.astype(np.int64))))
         self.config = {"workload": self.name, "codecs": ["rle_v2", "byte_rle"],
+                       "launch": "persistent work queue (size order)" if batched else "per-codec per-chunk",
                        "out_bytes": self.out_bytes, "parts": [p.config for p in self.parts]}
 
     def kernels(self):
+        if self.batched:
+            return [("batch_kernel<8> (rle_v2+byte_rle)", self.step, self.alg_bytes)]
         return [(p.name, p.step, p.alg_bytes) for p in self.parts]
 
-    def step(self, stream):
+    def plain_step(self, stream):
         for p in self.parts:
             p.step(stream)
+
+    def step(self, stream):
+        if self.batched:
+            self.batch.submit(stream)
+        else:
+            self.plain_step(stream)
 
     def verify(self):
         return all(p.verify() for p in self.parts)
@@ -290,7 +310,9 @@ class CopyWorkload:
         return None
 
 
-WORKLOADS = {"config2": Config2Workload, "rlev1": RleV1Workload, "rlev2": RleV2Workload,
+WORKLOADS = {"config2": Config2Workload,
+             "config2-plain": lambda **kw: Config2Workload(batched=False, **kw),
+             "rlev1": RleV1Workload, "rlev2": RleV2Workload,
              "brle": BrleWorkload, "lzss": LzssWorkload, "copy": CopyWorkload}
 DEFAULT = "config2"
 
@@ -338,6 +360,15 @@ def run(args):
                            "GBps": round(kbytes / (kms * 1e-3) / 1e9, 1)})
     dom = max(per_kernel, key=lambda k: k["ms"])
     ach = dom["GBps"]
+    plain_ms = None
+    if getattr(wl, "batched", False):               # batched vs plain, same data, same run
+        p0, p1 = mdz.Event(), mdz.Event()
+        p0.record(stream)
+        for _ in range(args.steps):
+            wl.plain_step(stream.ptr)
+        p1.record(stream)
+        p1.sync()
+        plain_ms = p0.elapsed_ms(p1) / args.steps
     threads = max(1, min(O.host_cores(), 16))
     line = {
         "metric": METRIC, "value": round(gbs, 2), "unit": "GB/s", "n_gpus": 1,
@@ -348,6 +379,8 @@ def run(args):
                      "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": TRAFFIC.get(dom["kernel"]),
                      "kernel": dom["kernel"], "alg_bytes_per_launch": dom["alg_bytes"]},
         "kernels": per_kernel,
+        "batched_vs_plain": ({"batched_ms": round(ms, 5), "plain_ms": round(plain_ms, 5),
+                              "speedup": round(plain_ms / ms, 3)} if plain_ms else None),
         "bit_exact": ok,
     }
     if not args.no_cpu:

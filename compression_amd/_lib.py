@@ -37,6 +37,15 @@ class LanesT(C.Structure):
                 ("payload_bytes", u64)]
 
 
+class BatchT(C.Structure):
+    """mdz_batch_t (include/mdz.h)."""
+    _fields_ = [("rlev2", LanesT), ("rlev2_dst", vp), ("rlev2_dst_bytes", u64), ("rlev2_status", vp),
+                ("rlev2_max_chunk", u32), ("brle_max_chunk", u32), ("brle_ptr", vp), ("brle_data", vp),
+                ("brle_n_chunks", u64), ("brle_chunk_bytes", u64), ("brle_dst", vp),
+                ("brle_dst_bytes", u64), ("brle_status", vp), ("items", vp), ("n_items", u32),
+                ("blocks", u32), ("cursor", vp)]
+
+
 class LanesTuneT(C.Structure):
     _fields_ = [("ring_words", u32), ("max_chunk_bytes", u32), ("reserved", u32 * 6)]
 
@@ -85,6 +94,11 @@ SIGNATURES = {
     "mdz_inflate_batch": (C.c_int, [vp, vp, C.c_int, C.c_int, vp]),
     "mdz_inflate_batch_sched": (C.c_int, [vp, vp, C.c_int, C.c_int, vp, u32, vp, vp, vp]),
     "mdz_inflate_slots": (C.c_int, [C.POINTER(u32)]),
+    "mdz_cost_features": (C.c_int, [vp, C.c_int, C.c_int, vp, vp]),
+    "mdz_batch_submit": (C.c_int, [C.POINTER(BatchT), vp]),
+    "mdz_mlp_forward": (C.c_int, [vp, vp, C.c_int, vp, vp]),
+    "mdz_mlp_grad": (C.c_int, [vp, vp, vp, C.c_int, vp, vp, vp]),
+    "mdz_mlp_adam": (C.c_int, [vp, vp, vp, vp, C.c_float, u32, vp]),
     "mdz_deflate_decompress_gpu": (C.c_int, [vp, C.POINTER(vp), u64, C.POINTER(u64), vp, u64, vp, u64,
                                              u64]),
 }

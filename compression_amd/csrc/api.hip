@@ -416,4 +416,29 @@ int mdz_deflate_decompress_gpu(const uint8_t* in, uint8_t** out, uint64_t in_n_b
     return rc;
 }
 
+// ------------------------------------------------------- cost model (MLP)
+int mdz_cost_features(const mdz_inflate_in_t* in, int count, int parse_hdr, float* x,
+                      void* stream) {
+    if (count < 0) return MDZ_ERR_INVALID;
+    return mdz::mlp_features(in, (uint32_t)count, parse_hdr, x, S(stream));
+}
+int mdz_mlp_forward(const float* params, const float* x, int n, float* y, void* stream) {
+    if (n < 0) return MDZ_ERR_INVALID;
+    return mdz::mlp_forward(params, x, (uint32_t)n, y, S(stream));
+}
+int mdz_mlp_grad(const float* params, const float* x, const float* t, int n, float* grad,
+                 float* loss_sum, void* stream) {
+    if (n < 0) return MDZ_ERR_INVALID;
+    return mdz::mlp_grad(params, x, t, (uint32_t)n, grad, loss_sum, S(stream));
+}
+int mdz_mlp_adam(float* params, float* grad, float* m, float* v, float lr, uint32_t step,
+                 void* stream) {
+    return mdz::mlp_adam(params, grad, m, v, lr, step, S(stream));
+}
+
+// ------------------------------------------------ batched persistent decode
+int mdz_batch_submit(const mdz_batch_t* batch, void* stream) {
+    return mdz::batch_submit(batch, S(stream));
+}
+
 }  // extern "C"

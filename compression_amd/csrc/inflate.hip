@@ -255,6 +255,10 @@ __global__ __launch_bounds__(64) void inflate_kernel(const mdz_inflate_in_t* __r
         }
         if (qi >= count) return;
         const uint32_t si = order ? order[qi] : qi;
+        if (si >= count) {                         // a bad order entry names no stream
+            if (!cursor) return;
+            continue;
+        }
         const mdz_inflate_in_t in = ins[si];
         uint8_t* const dst = reinterpret_cast<uint8_t*>(in.dst);
         const uint64_t cap = in.dst_bytes;

@@ -34,6 +34,14 @@ int inflate_batch(const mdz_inflate_in_t* in, mdz_inflate_status_t* out, uint32_
                   uint32_t* cursor, mdz_inflate_cost_t* cost, hipStream_t s);
 int inflate_slots(uint32_t* slots);
 
+int mlp_features(const mdz_inflate_in_t* in, uint32_t n, int parse_hdr, float* x, hipStream_t s);
+int mlp_forward(const float* params, const float* x, uint32_t n, float* y, hipStream_t s);
+int mlp_grad(const float* params, const float* x, const float* t, uint32_t n, float* grad,
+             float* loss_sum, hipStream_t s);
+int mlp_adam(float* params, float* grad, float* m, float* v, float lr, uint32_t step, hipStream_t s);
+
+int batch_submit(const mdz_batch_t* b, hipStream_t s);
+
 int lzss_decompress(const mdz_lanes_t* in, void* dst, uint64_t dst_bytes, int32_t* status,
                     hipStream_t s);
 

@@ -1,0 +1,176 @@
+// Decode-cost model for the batched scheduler: a small MLP (F -> H -> H -> 1,
+// ReLU) with hand-written forward, backward and Adam kernels, fp32.
+//
+// Features come from what a scheduler knows before decoding a Deflate stream
+// (its descriptor and first block header); the target is log2 of the cycles
+// the decoder logged for it (mdz_inflate_cost_t).  The model only orders
+// work: decoded bytes never pass through it.
+//
+// Parameter layout (floats): W1[F][H] b1[H] W2[H][H] b2[H] w3[H] b3.
+// Training is full-batch: forward keeps the activations, backward forms
+// per-sample gradients and reduces them per workgroup in LDS before one
+// atomic add per parameter and block, then Adam updates in place.
+#include "mdz_device.h"
+#include "mdz_internal.h"
+
+namespace mdz {
+namespace {
+
+constexpr int F = MDZ_MLP_FEATURES, H = MDZ_MLP_HIDDEN;
+constexpr int P_W1 = 0, P_B1 = F * H, P_W2 = P_B1 + H, P_B2 = P_W2 + H * H, P_W3 = P_B2 + H,
+              P_B3 = P_W3 + H, P_N = P_B3 + 1;
+static_assert(P_N == MDZ_MLP_PARAMS, "parameter count");
+constexpr int kBlock = 256;
+
+// Features of stream i: log2(1 + src bytes), log2(1 + dst bytes),
+// log2((1 + dst) / (1 + src)), and the first block's type one-hot
+// (stored, fixed, dynamic; all zero if unreadable).
+__global__ void features_kernel(const mdz_inflate_in_t* __restrict__ in, uint32_t n, int parse_hdr,
+                                float* __restrict__ x) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const mdz_inflate_in_t d = in[i];
+    const float ls = log2f(1.0f + (float)d.src_bytes), ld = log2f(1.0f + (float)d.dst_bytes);
+    float* xi = x + (size_t)i * F;
+    xi[0] = ls;
+    xi[1] = ld;
+    xi[2] = ld - ls;
+    const uint64_t at = parse_hdr ? 2 : 0;
+    int type = -1;
+    if (d.src_bytes > at) type = (reinterpret_cast<const uint8_t*>(d.src)[at] >> 1) & 3;
+    xi[3] = type == 0;
+    xi[4] = type == 1;
+    xi[5] = type == 2;
+}
+
+MDZ_DEV void forward_one(const float* __restrict__ p, const float* xi, float* h1, float* h2, float& y) {
+#pragma unroll 4
+    for (int j = 0; j < H; ++j) {
+        float a = p[P_B1 + j];
+#pragma unroll
+        for (int k = 0; k < F; ++k) a += xi[k] * p[P_W1 + k * H + j];
+        h1[j] = a > 0.f ? a : 0.f;
+    }
+    float out = p[P_B3];
+#pragma unroll 4
+    for (int j = 0; j < H; ++j) {
+        float a = p[P_B2 + j];
+        for (int k = 0; k < H; ++k) a += h1[k] * p[P_W2 + k * H + j];
+        h2[j] = a > 0.f ? a : 0.f;
+        out += h2[j] * p[P_W3 + j];
+    }
+    y = out;
+}
+
+__global__ __launch_bounds__(kBlock) void forward_kernel(const float* __restrict__ p,
+                                                         const float* __restrict__ x, uint32_t n,
+                                                         float* __restrict__ y) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float xi[F], h1[H], h2[H], out;
+#pragma unroll
+    for (int k = 0; k < F; ++k) xi[k] = x[(size_t)i * F + k];
+    forward_one(p, xi, h1, h2, out);
+    y[i] = out;
+}
+
+// grad += d/dparams of (1/n) * sum_i (y_i - t_i)^2; loss_sum += sum (y - t)^2.
+__global__ __launch_bounds__(kBlock) void backward_kernel(const float* __restrict__ p,
+                                                          const float* __restrict__ x,
+                                                          const float* __restrict__ t, uint32_t n,
+                                                          float* __restrict__ grad,
+                                                          float* __restrict__ loss_sum) {
+    __shared__ float g[P_N];
+    __shared__ float ls;
+    for (int k = threadIdx.x; k < P_N; k += blockDim.x) g[k] = 0.f;
+    if (threadIdx.x == 0) ls = 0.f;
+    __syncthreads();
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        float xi[F], h1[H], h2[H], y;
+#pragma unroll
+        for (int k = 0; k < F; ++k) xi[k] = x[(size_t)i * F + k];
+        forward_one(p, xi, h1, h2, y);
+        const float r = y - t[i];
+        const float dy = 2.f * r / (float)n;
+        atomicAdd(&ls, r * r);
+        atomicAdd(&g[P_B3], dy);
+        float d2[H];
+        for (int j = 0; j < H; ++j) {
+            atomicAdd(&g[P_W3 + j], dy * h2[j]);
+            d2[j] = h2[j] > 0.f ? dy * p[P_W3 + j] : 0.f;
+            atomicAdd(&g[P_B2 + j], d2[j]);
+        }
+        for (int k = 0; k < H; ++k) {
+            float d1 = 0.f;
+            for (int j = 0; j < H; ++j) {
+                atomicAdd(&g[P_W2 + k * H + j], h1[k] * d2[j]);
+                d1 += p[P_W2 + k * H + j] * d2[j];
+            }
+            d1 = h1[k] > 0.f ? d1 : 0.f;
+            atomicAdd(&g[P_B1 + k], d1);
+#pragma unroll
+            for (int m = 0; m < F; ++m) atomicAdd(&g[P_W1 + m * H + k], xi[m] * d1);
+        }
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < P_N; k += blockDim.x)
+        if (g[k] != 0.f) atomicAdd(&grad[k], g[k]);
+    if (threadIdx.x == 0 && loss_sum) atomicAdd(loss_sum, ls);
+}
+
+__global__ void adam_kernel(float* __restrict__ p, float* __restrict__ grad, float* __restrict__ m,
+                            float* __restrict__ v, float lr, float b1t, float b2t) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= P_N) return;
+    const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+    const float gk = grad[k];
+    const float mk = b1 * m[k] + (1.f - b1) * gk, vk = b2 * v[k] + (1.f - b2) * gk * gk;
+    m[k] = mk;
+    v[k] = vk;
+    p[k] -= lr * (mk / (1.f - b1t)) / (sqrtf(vk / (1.f - b2t)) + eps);
+    grad[k] = 0.f;
+}
+
+__global__ void zero_kernel(float* a, int n) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) a[k] = 0.f;
+}
+
+}  // namespace
+
+int mlp_features(const mdz_inflate_in_t* in, uint32_t n, int parse_hdr, float* x, hipStream_t s) {
+    if (!n) return MDZ_OK;
+    if (!in || !x) return MDZ_ERR_INVALID;
+    hipLaunchKernelGGL(features_kernel, dim3((n + 255) / 256), dim3(256), 0, s, in, n, parse_hdr, x);
+    return record_hip(hipGetLastError(), "mlp features");
+}
+
+int mlp_forward(const float* params, const float* x, uint32_t n, float* y, hipStream_t s) {
+    if (!n) return MDZ_OK;
+    if (!params || !x || !y) return MDZ_ERR_INVALID;
+    hipLaunchKernelGGL(forward_kernel, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, params, x,
+                       n, y);
+    return record_hip(hipGetLastError(), "mlp forward");
+}
+
+int mlp_grad(const float* params, const float* x, const float* t, uint32_t n, float* grad,
+             float* loss_sum, hipStream_t s) {
+    if (!params || !x || !t || !grad) return MDZ_ERR_INVALID;
+    hipLaunchKernelGGL(zero_kernel, dim3((P_N + 255) / 256), dim3(256), 0, s, grad, P_N);
+    if (loss_sum) hipLaunchKernelGGL(zero_kernel, dim3(1), dim3(1), 0, s, loss_sum, 1);
+    if (n)
+        hipLaunchKernelGGL(backward_kernel, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, params,
+                           x, t, n, grad, loss_sum);
+    return record_hip(hipGetLastError(), "mlp backward");
+}
+
+int mlp_adam(float* params, float* grad, float* m, float* v, float lr, uint32_t step, hipStream_t s) {
+    if (!params || !grad || !m || !v || step == 0) return MDZ_ERR_INVALID;
+    const float b1t = powf(0.9f, (float)step), b2t = powf(0.999f, (float)step);
+    hipLaunchKernelGGL(adam_kernel, dim3((P_N + 255) / 256), dim3(256), 0, s, params, grad, m, v, lr,
+                       b1t, b2t);
+    return record_hip(hipGetLastError(), "mlp adam");
+}
+
+}  // namespace mdz

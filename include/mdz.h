@@ -275,6 +275,60 @@ int mdz_deflate_decompress_gpu(const uint8_t* in, uint8_t** out, uint64_t in_n_b
                                uint64_t col_n_bytes, const uint64_t* blk_offset_f,
                                uint64_t blk_n_bytes, uint64_t chunk_size);
 
+/* ================================================ learned cost model ===== *
+ * Scheduler-side MLP (features -> 32 -> 32 -> 1, ReLU, fp32) predicting
+ * log2(decode cycles) of a Deflate stream, trained on the decoder's own cost
+ * log.  It only orders work.  No counterpart in the reference.
+ * Parameters (floats): W1[F][H] b1[H] W2[H][H] b2[H] w3[H] b3. */
+#define MDZ_MLP_FEATURES 6
+#define MDZ_MLP_HIDDEN 32
+#define MDZ_MLP_PARAMS (MDZ_MLP_FEATURES * MDZ_MLP_HIDDEN + MDZ_MLP_HIDDEN + \
+                        MDZ_MLP_HIDDEN * MDZ_MLP_HIDDEN + MDZ_MLP_HIDDEN + MDZ_MLP_HIDDEN + 1)
+
+/* x[count][F] (device): log2(1+src), log2(1+dst), their difference and the
+ * first block type one-hot (stored, fixed, dynamic), read from the stream. */
+int mdz_cost_features(const mdz_inflate_in_t* in, int count, int parse_hdr, float* x,
+                      void* stream);
+/* y[n] = model(x[n][F]) */
+int mdz_mlp_forward(const float* params, const float* x, int n, float* y, void* stream);
+/* grad[P] = d/dparams mean((y - t)^2); *loss_sum (device, optional) = sum (y - t)^2 */
+int mdz_mlp_grad(const float* params, const float* x, const float* t, int n, float* grad,
+                 float* loss_sum, void* stream);
+/* One Adam step (beta 0.9/0.999) with bias correction for step >= 1; clears grad. */
+int mdz_mlp_adam(float* params, float* grad, float* m, float* v, float lr, uint32_t step,
+                 void* stream);
+
+/* ============================================ batched persistent decode ===== *
+ * One launch decodes the chunks of an RLE v2 container and a byte-RLE
+ * container (config 2's pair) through a device work queue: a grid of
+ * `blocks` persistent workgroups (0 = the co-resident maximum) claims items
+ * from `cursor` (device u32, reset by the call).  items (device u32 array,
+ * n_items entries, codec << 31 | chunk, codec 0 = RLE v2, 1 = byte RLE) is
+ * the scheduler's order; NULL = every RLE v2 chunk, then every byte-RLE
+ * chunk.  Results and statuses equal the per-codec entries'.  No counterpart
+ * in the reference (its decoders launch one block per chunk). */
+typedef struct {
+    mdz_lanes_t rlev2;              /* n_chunks = 0: no RLE v2 part */
+    void* rlev2_dst;
+    uint64_t rlev2_dst_bytes;
+    int32_t* rlev2_status;
+    uint32_t rlev2_max_chunk;       /* largest chunk payload (LDS sizing), 0 = LDS maximum */
+    uint32_t brle_max_chunk;        /* 0 = the byte-RLE worst case */
+    const uint64_t* brle_ptr;
+    const void* brle_data;
+    uint64_t brle_n_chunks;         /* 0: no byte-RLE part */
+    uint64_t brle_chunk_bytes;
+    void* brle_dst;
+    uint64_t brle_dst_bytes;
+    int32_t* brle_status;
+    const uint32_t* items;
+    uint32_t n_items;
+    uint32_t blocks;
+    uint32_t* cursor;
+} mdz_batch_t;
+
+int mdz_batch_submit(const mdz_batch_t* batch, void* stream);
+
 #ifdef __cplusplus
 }
 #endif
