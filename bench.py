@@ -54,13 +54,13 @@ class RleV1Workload:
     dtype = "u32"
 
     def __init__(self, line=128, ring=0, n_vals=1 << 24, seed=0x4D445A01):
-        from oracle import pyoracle as O
+        from compression_amd import host as H
         from compression_amd.lanes import DeviceLanes
         import compression_amd as mdz
-        self.O, self.mdz = O, mdz
+        self.mdz = mdz
         self.line, self.ring = line, ring
-        self.data = O.gen_rlev1_u32(n_vals, seed)
-        self.c = O.rlev1_encode(self.data, 131072, line)
+        self.data = H.gen_rlev1_u32(n_vals, seed)
+        self.c = H.rlev1_encode(self.data, 131072, line)
         c = self.c
         self.lanes = DeviceLanes(c.payload, c.blk_off, c.col_len, c.n_chunks, 131072, 4, line)
         self.dst = mdz.DeviceBuffer(c.raw_bytes)
@@ -88,7 +88,8 @@ class RleV1Workload:
         return bool((st == 0).all() and np.array_equal(got, self.data.view(np.uint8)))
 
     def cpu(self, threads):
-        O, c = self.O, self.c
+        from oracle import pyoracle as O   # the CPU baseline leg
+        c = self.c
         out = np.empty(c.raw_bytes, np.uint8)
         t = _cpu_time(lambda: O.rlev1_decode(c, threads=threads))
         return {"value": round(c.raw_bytes / t / 1e9, 3), "unit": "GB/s", "cores": threads,
@@ -106,13 +107,13 @@ class RleV2Workload:
     dtype = "u64"
 
     def __init__(self, line=512, n_vals=1 << 27, seed=0x4D445A02, **_):
-        from oracle import pyoracle as O
+        from compression_amd import host as H
         from compression_amd.lanes import DeviceLanes
         import compression_amd as mdz
-        self.O, self.mdz = O, mdz
+        self.mdz = mdz
         self.ru = line
-        self.data = O.gen_rlev2_i64(n_vals, seed)
-        self.c = c = O.rlev2_encode(self.data, 131072, line)
+        self.data = H.gen_rlev2_i64(n_vals, seed)
+        self.c = c = H.rlev2_encode(self.data, 131072, line)
         self.lanes = DeviceLanes(c.payload, c.blk_off, c.col_len, c.n_chunks, 131072, 8, line)
         self.dst = mdz.DeviceBuffer(c.raw_bytes)
         self.status = mdz.DeviceBuffer(4 * c.n_chunks)
@@ -138,7 +139,8 @@ class RleV2Workload:
         return bool((st == 0).all() and np.array_equal(got, self.data.view(np.uint8)))
 
     def cpu(self, threads):
-        O, c = self.O, self.c
+        from oracle import pyoracle as O   # the CPU baseline leg
+        c = self.c
         t = _cpu_time(lambda: O.rlev2_decode(c, threads=threads))
         return {"value": round(c.raw_bytes / t / 1e9, 3), "unit": "GB/s", "cores": threads,
                 "kind": "port",
@@ -153,12 +155,12 @@ class BrleWorkload:
     dtype = "u8"
 
     def __init__(self, n=1 << 28, chunk=131072, seed=0x4D445A03, **_):
-        from oracle import pyoracle as O
+        from compression_amd import host as H
         from compression_amd import brle
         import compression_amd as mdz
-        self.O, self.mdz = O, mdz
-        self.data = O.gen_brle_u8(n, seed)
-        self.c = c = O.brle_encode(self.data, chunk)
+        self.mdz = mdz
+        self.data = H.gen_brle_u8(n, seed)
+        self.c = c = H.brle_encode(self.data, chunk)
         self.dev = brle.DeviceBrle(c.data, c.ptr, c.n_chunks, chunk)
         self.dst = mdz.DeviceBuffer(c.raw_bytes)
         self.status = mdz.DeviceBuffer(4 * c.n_chunks)
@@ -179,7 +181,8 @@ class BrleWorkload:
         return bool((st == 0).all() and np.array_equal(got, self.data))
 
     def cpu(self, threads):
-        O, c = self.O, self.c
+        from oracle import pyoracle as O   # the CPU baseline leg
+        c = self.c
         t = _cpu_time(lambda: O.brle_decode(c, threads=threads))
         return {"value": round(c.raw_bytes / t / 1e9, 3), "unit": "GB/s", "cores": threads,
                 "kind": "port",
@@ -195,12 +198,12 @@ class LzssWorkload:
     dtype = "u8"
 
     def __init__(self, n=1 << 30, chunk=65536, seed=0x4D445A03, **_):
-        from oracle import pyoracle as O
+        from compression_amd import host as H
         from compression_amd.lanes import DeviceLanes
         import compression_amd as mdz
-        self.O, self.mdz = O, mdz
-        self.data = O.gen_lzss_markov(n, seed)
-        self.c = c = O.lzss_encode(self.data, chunk)
+        self.mdz = mdz
+        self.data = H.gen_lzss_markov(n, seed)
+        self.c = c = H.lzss_encode(self.data, chunk)
         self.lanes = DeviceLanes(c.payload, c.blk_off, c.col_len, c.n_chunks, chunk, 1, 4,
                                  col_map=c.col_map)
         self.dst = mdz.DeviceBuffer(c.raw_bytes)
@@ -222,7 +225,8 @@ class LzssWorkload:
         return bool((st == 0).all() and np.array_equal(got, self.data))
 
     def cpu(self, threads):
-        O, c = self.O, self.c
+        from oracle import pyoracle as O   # the CPU baseline leg
+        c = self.c
         t = _cpu_time(lambda: O.lzss_decode(c, threads=threads))
         return {"value": round(c.raw_bytes / t / 1e9, 3), "unit": "GB/s", "cores": threads,
                 "kind": "port",
@@ -319,7 +323,7 @@ DEFAULT = "config2"
 
 def run(args):
     import compression_amd as mdz
-    from oracle import pyoracle as O
+    from compression_amd import host as H
 
     if mdz.device_count() < 1:
         raise SystemExit("bench.py needs a GPU")
@@ -369,7 +373,7 @@ def run(args):
         p1.record(stream)
         p1.sync()
         plain_ms = p0.elapsed_ms(p1) / args.steps
-    threads = max(1, min(O.host_cores(), 16))
+    threads = max(1, min(H.host_cores(), 16))
     line = {
         "metric": METRIC, "value": round(gbs, 2), "unit": "GB/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 5),

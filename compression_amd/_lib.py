@@ -69,6 +69,7 @@ SIGNATURES = {
     "mdz_event_destroy": (C.c_int, [vp]),
     "mdz_event_record": (C.c_int, [vp, vp]),
     "mdz_event_sync": (C.c_int, [vp]),
+    "mdz_event_query": (C.c_int, [vp]),
     "mdz_event_elapsed_ms": (C.c_int, [vp, vp, C.POINTER(C.c_float)]),
     "mdz_host_free": (None, [vp]),
     "mdz_rlev1_decompress": (C.c_int, [C.POINTER(LanesT), vp, u64, vp, vp]),
@@ -202,6 +203,12 @@ class Event:
 
     def sync(self) -> None:
         check(lib().mdz_event_sync(self.ptr), "mdz_event_sync")
+
+    def query(self) -> bool:
+        rc = lib().mdz_event_query(self.ptr)
+        if rc < 0:
+            check(rc, "mdz_event_query")
+        return rc == 1
 
     def elapsed_ms(self, end: "Event") -> float:
         ms = C.c_float()

@@ -1,8 +1,11 @@
-"""Build recipe for the mdz HIP library (compression_amd/libmdz.so).
+"""Build recipe for the mdz libraries.
 
-Compiles every ``csrc/*.hip`` for gfx950 with hipcc, in parallel, into
-``compression_amd/_build/`` and links the shared library in-tree so it
-travels with the source tree.  Usage: ``python -m compression_amd.build``.
+* ``compression_amd/libmdz.so``: every ``csrc/*.hip`` compiled for gfx950 with
+  hipcc, in parallel, into ``compression_amd/_build/`` and linked in-tree;
+* ``compression_amd/libmdz_host.so``: the host encoders and workload
+  generators (``host/*.c``, gcc, zlib).
+Both are built in-tree so they travel with the source tree.  Usage:
+``python -m compression_amd.build``.
 """
 from __future__ import annotations
 
@@ -17,6 +20,8 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libmdz.so")
+HOST_LIB = os.path.join(HERE, "libmdz_host.so")
+CC = os.environ.get("CC", "gcc")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
@@ -38,7 +43,22 @@ def _compile(src: str) -> str:
     return obj
 
 
+def build_host(verbose: bool = False) -> str:
+    srcs = sorted(glob.glob(os.path.join(HERE, "host", "*.c")))
+    tmp = HOST_LIB + ".tmp"
+    cmd = [CC, "-O2", "-g", "-fPIC", "-std=gnu11", "-Wall", "-Wextra", "-Wno-unused-parameter",
+           "-pthread", f"-I{os.path.join(ROOT, 'include')}", "-shared", "-o", tmp, *srcs, "-lz", "-lm"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"host library build failed:\n{r.stdout}\n{r.stderr}")
+    os.replace(tmp, HOST_LIB)
+    if verbose:
+        print(f"built {HOST_LIB} from {len(srcs)} sources")
+    return HOST_LIB
+
+
 def build(verbose: bool = False) -> str:
+    build_host(verbose)
     os.makedirs(BUILD, exist_ok=True)
     srcs = sorted(glob.glob(os.path.join(CSRC, "*.hip")))
     jobs = min(len(srcs), max(1, min(8, os.cpu_count() or 1)))

@@ -127,6 +127,12 @@ int mdz_event_create(void** event) {
 int mdz_event_destroy(void* event) { return herr(hipEventDestroy((hipEvent_t)event)); }
 int mdz_event_record(void* event, void* stream) { return herr(hipEventRecord((hipEvent_t)event, S(stream))); }
 int mdz_event_sync(void* event) { return herr(hipEventSynchronize((hipEvent_t)event)); }
+int mdz_event_query(void* event) {
+    const hipError_t e = hipEventQuery((hipEvent_t)event);
+    if (e == hipSuccess) return 1;
+    if (e == hipErrorNotReady) { (void)hipGetLastError(); return 0; }
+    return herr(e);
+}
 int mdz_event_elapsed_ms(void* start, void* stop, float* ms) {
     return herr(hipEventElapsedTime(ms, (hipEvent_t)start, (hipEvent_t)stop));
 }

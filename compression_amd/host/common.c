@@ -1,7 +1,7 @@
-/* mdz oracle: PRNG, thread pool, helpers.  TEST INFRASTRUCTURE ONLY (see
- * mdz_oracle.h). */
+/* Host-side utilities: the seeded PRNG of the synthetic workloads
+ * (splitmix64-seeded xoshiro256**) and a small pthread pool. */
 #define _GNU_SOURCE
-#include "mdz_oracle.h"
+#include "mdz_host.h"
 
 #include <pthread.h>
 #include <sched.h>

@@ -1,0 +1,283 @@
+"""Serving loop: mixed-codec decode requests with continuous batching.
+
+Requests (RLE v1, RLE v2, LZSS, Deflate) arrive open-loop.  The loop:
+  * admits waiting requests in arrival order while the HBM budget allows
+    (compressed input + output + metadata of every request on the device);
+  * at each tick folds every admitted, not yet launched request into launches:
+    all pending Deflate chunk streams go into ONE scheduled inflate launch
+    (persistent work queue, longest stream first), lane-transposed requests
+    get one launch each; launches rotate over several HIP streams so new work
+    overlaps work already on the GPU;
+  * records each request's completion with a HIP event; latency is arrival ->
+    output ready on the device; budget and buffers are released then.
+Device buffers come from a caching pool (size-binned free lists), so the loop
+never calls hipFree while kernels run.  Decoding is the library's: the loop
+only decides when and with what work is launched.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import deflate as _deflate
+from ._lib import DeviceBuffer, Event, LanesT, Stream, check, lib
+
+CODECS = ("rlev1", "rlev2", "lzss", "deflate")
+
+
+class Pool:
+    """Caching device allocator: power-of-two bins, reused without hipFree."""
+
+    def __init__(self):
+        self.free: dict[int, list[DeviceBuffer]] = {}
+
+    @staticmethod
+    def _bin(n: int) -> int:
+        return max(256, 1 << (max(int(n), 1) - 1).bit_length())
+
+    def get(self, n: int) -> DeviceBuffer:
+        b = self._bin(n)
+        lst = self.free.get(b)
+        return lst.pop() if lst else DeviceBuffer(b)
+
+    def put(self, buf: DeviceBuffer) -> None:
+        self.free.setdefault(buf.nbytes, []).append(buf)
+
+
+@dataclass
+class Request:
+    """A request's host-side payload: codec, container arrays, expected output."""
+    rid: int
+    codec: str
+    arrival: float                 # seconds from the start of the run
+    raw: np.ndarray                # expected output (bit-exact reference)
+    payload: np.ndarray
+    meta: dict
+    # filled by the loop
+    bufs: list = field(default_factory=list)
+    out: DeviceBuffer | None = None
+    status: DeviceBuffer | None = None
+    done_event: Event | None = None
+    launched: float = -1.0
+    finished: float = -1.0
+
+    @property
+    def out_bytes(self) -> int:
+        return int(self.raw.size)
+
+    def hbm_bytes(self) -> int:
+        return int(self.payload.nbytes + self.raw.size + sum(getattr(v, "nbytes", 0) for v in self.meta.values()))
+
+
+def make_request(rid: int, codec: str, size: int, seed: int, arrival: float = 0.0) -> Request:
+    """Builds a request of `size` decompressed bytes with the codec's generator."""
+    from . import host as O                  # the client side: encoders and generators
+    if codec == "rlev1":
+        n = max(32, size // 4)
+        x = O.gen_rlev1_u32(n, seed)
+        c = O.rlev1_encode(x, 131072, 128, threads=4)
+        return Request(rid, codec, arrival, x.view(np.uint8)[: c.raw_bytes].copy(), c.payload,
+                       {"blk_off": c.blk_off, "col_len": c.col_len, "n": c.n_chunks, "chunk": 131072,
+                        "eb": 4, "line": 128})
+    if codec == "rlev2":
+        n = max(16, size // 8)
+        x = O.gen_rlev2_i64(n, seed)
+        c = O.rlev2_encode(x, 131072, 64, threads=4)
+        return Request(rid, codec, arrival, x.view(np.uint8).copy(), c.payload,
+                       {"blk_off": c.blk_off, "col_len": c.col_len, "n": c.n_chunks, "chunk": 131072,
+                        "eb": 8, "line": 64, "payload_bytes": c.payload_bytes})
+    if codec == "lzss":
+        chunk = 65536
+        n = max(chunk, size // chunk * chunk)
+        x = O.gen_lzss_markov(n, seed)
+        c = O.lzss_encode(x, chunk, threads=4)
+        return Request(rid, codec, arrival, x.copy(), c.payload,
+                       {"blk_off": c.blk_off, "col_len": c.col_len, "col_map": c.col_map,
+                        "n": c.n_chunks, "chunk": chunk})
+    if codec == "deflate":
+        import zlib
+        chunk = 131072
+        x = O.gen_markov(max(size, 1), seed, 0.4)
+        comp, offs, lens = bytearray(), [], []
+        for i in range(0, x.size, chunk):
+            z = zlib.compress(x[i:i + chunk].tobytes(), 6)
+            comp += bytes(-len(comp) % 16)
+            offs.append(len(comp))
+            lens.append(len(z))
+            comp += z
+        return Request(rid, codec, arrival, x, np.frombuffer(bytes(comp) + b"\0", np.uint8),
+                       {"src_off": np.array(offs, np.uint64), "src_len": np.array(lens, np.uint64),
+                        "chunk": chunk})
+    raise ValueError(codec)
+
+
+class Server:
+    def __init__(self, hbm_budget: int = 64 << 30, n_streams: int = 4, verify: bool = False):
+        self.budget = int(hbm_budget)
+        self.in_use = 0
+        self.pool = Pool()
+        self.streams = [Stream() for _ in range(n_streams)]
+        self.next_stream = 0
+        self.verify = verify
+        self.cursor = DeviceBuffer(4)
+
+    # -- buffers ----------------------------------------------------------
+    def _up(self, req: Request, a: np.ndarray, stream) -> DeviceBuffer:
+        a = np.ascontiguousarray(a)
+        b = self.pool.get(a.nbytes + 16)
+        if a.nbytes:
+            check(lib().mdz_memcpy_htod(b.ptr, a.ctypes.data, a.nbytes, stream.ptr), "htod")
+        req.bufs.append(b)
+        return b
+
+    def _stream(self):
+        s = self.streams[self.next_stream]
+        self.next_stream = (self.next_stream + 1) % len(self.streams)
+        return s
+
+    # -- launches ---------------------------------------------------------
+    def _launch_lanes(self, req: Request) -> None:
+        s = self._stream()
+        m = req.meta
+        pay = self._up(req, req.payload, s)
+        off = self._up(req, m["blk_off"], s)
+        col = self._up(req, m["col_len"], s)
+        cmap = self._up(req, m["col_map"], s) if "col_map" in m else None
+        req.out = self.pool.get(req.out_bytes)
+        req.status = self.pool.get(4 * m["n"])
+        L = LanesT(pay.ptr, off.ptr, col.ptr, cmap.ptr if cmap else None, m["n"], m["chunk"],
+                   m.get("eb", 1), m.get("line", 4), m.get("payload_bytes", req.payload.nbytes))
+        fn = {"rlev1": lib().mdz_rlev1_decompress, "rlev2": lib().mdz_rlev2_decompress,
+              "lzss": lib().mdz_lzss_decompress}[req.codec]
+        check(fn(C.byref(L), req.out.ptr, req.out_bytes, req.status.ptr, s.ptr), req.codec)
+        req.done_event = Event()
+        req.done_event.record(s)
+
+    def _launch_deflate(self, reqs: list[Request]) -> None:
+        """All pending Deflate requests' chunk streams in one scheduled launch."""
+        s = self._stream()
+        descs, order_cost = [], []
+        for r in reqs:
+            src = self._up(r, r.payload, s)
+            r.out = self.pool.get(r.out_bytes)
+            m = r.meta
+            for i, (o, n) in enumerate(zip(m["src_off"], m["src_len"])):
+                d0 = i * m["chunk"]
+                descs.append((src.ptr + int(o), int(n), r.out.ptr + d0, min(m["chunk"], r.out_bytes - d0)))
+                order_cost.append(int(n))
+        d = np.array(descs, dtype=_deflate.DESC)
+        count = len(descs)
+        desc = self.pool.get(d.nbytes)
+        check(lib().mdz_memcpy_htod(desc.ptr, d.ctypes.data, d.nbytes, s.ptr), "htod")
+        st = self.pool.get(count * _deflate.STATUS.itemsize)
+        order = np.lexsort((np.arange(count), -np.array(order_cost))).astype(np.uint32)
+        od = self.pool.get(order.nbytes)
+        check(lib().mdz_memcpy_htod(od.ptr, order.ctypes.data, order.nbytes, s.ptr), "htod")
+        cur = self.pool.get(4)
+        check(lib().mdz_inflate_batch_sched(desc.ptr, st.ptr, count, 1, od.ptr, 0, cur.ptr, None, s.ptr),
+              "mdz_inflate_batch_sched")
+        ev = Event()
+        ev.record(s)
+        first = 0
+        for i, r in enumerate(reqs):
+            r.done_event = ev
+            k = len(r.meta["src_off"])
+            r.meta["_batch_status"] = (st, first, k)
+            first += k
+        reqs[-1].bufs.extend([desc, st, od, cur])     # released with the batch's last request
+
+    # -- main loop --------------------------------------------------------
+    def run(self, requests: list[Request], time_limit: float = 120.0) -> dict:
+        pending = sorted(requests, key=lambda r: r.arrival)
+        waiting: list[Request] = []
+        inflight: list[Request] = []
+        done: list[Request] = []
+        self.outputs: dict[int, np.ndarray] = {}
+        self.all_ok = True
+        t0 = time.perf_counter()
+        i = 0
+        while (i < len(pending) or waiting or inflight) and time.perf_counter() - t0 < time_limit:
+            now = time.perf_counter() - t0
+            while i < len(pending) and pending[i].arrival <= now:
+                waiting.append(pending[i])
+                i += 1
+            # admission: arrival order, while the HBM budget allows
+            admitted = []
+            while waiting and self.in_use + waiting[0].hbm_bytes() <= self.budget:
+                r = waiting.pop(0)
+                self.in_use += r.hbm_bytes()
+                admitted.append(r)
+            if not admitted and waiting and not inflight:
+                r = waiting.pop(0)                 # larger than the whole budget: alone
+                self.in_use += r.hbm_bytes()
+                admitted.append(r)
+            # fold: every admitted Deflate request into one launch, the rest one each
+            now = time.perf_counter() - t0
+            for r in admitted:
+                r.launched = now
+                if r.codec != "deflate":
+                    self._launch_lanes(r)
+            defl = [r for r in admitted if r.codec == "deflate"]
+            if defl:
+                self._launch_deflate(defl)
+            inflight.extend(admitted)
+            # completions: verify (optional), then release budget and buffers
+            finished, still = [], []
+            for r in inflight:
+                (finished if r.done_event.query() else still).append(r)
+            inflight = still
+            t_now = time.perf_counter() - t0
+            for r in finished:
+                r.finished = t_now
+                self._complete(r)
+            for r in finished:
+                self._release(r)
+                self.in_use -= r.hbm_bytes()
+            done.extend(finished)
+            if not admitted and not finished:
+                nxt = pending[i].arrival - (time.perf_counter() - t0) if i < len(pending) else 0.0
+                time.sleep(min(max(nxt, 0.0), 0.0002) if inflight or nxt > 0 else 0.0)
+        for s in self.streams:
+            s.sync()
+        lat = np.array([r.finished - r.arrival for r in done]) if done else np.zeros(1)
+        span = (max(r.finished for r in done) - min(r.arrival for r in done)) if done else 0.0
+        return {"completed": len(done), "submitted": len(requests),
+                "bit_exact": self.all_ok if self.verify else None,
+                "p50_ms": float(np.percentile(lat, 50) * 1e3),
+                "p99_ms": float(np.percentile(lat, 99) * 1e3),
+                "GBps": float(sum(r.out_bytes for r in done) / span / 1e9) if span > 0 else 0.0}
+
+    def _complete(self, r: Request) -> None:
+        if not self.verify:
+            return
+        got = r.out.download(r.out_bytes)
+        if r.codec == "deflate":
+            st_buf, first, k = r.meta["_batch_status"]
+            allst = st_buf.download((first + k) * _deflate.STATUS.itemsize).view(_deflate.STATUS)
+            st = allst["status"][first:first + k]
+        else:
+            st = r.status.download(4 * r.meta["n"], np.int32)
+        good = bool((st == 0).all() and np.array_equal(got, r.raw))
+        self.all_ok = self.all_ok and good
+        self.outputs[r.rid] = got
+
+    def _release(self, r: Request) -> None:
+        for b in r.bufs:
+            self.pool.put(b)
+        for b in (r.out, r.status if r.codec != "deflate" else None):
+            if b is not None:
+                self.pool.put(b)
+        r.bufs, r.out, r.status = [], None, None
+
+
+def poisson_arrivals(n: int, rate_per_s: float, seed: int) -> np.ndarray:
+    rng = np.random.default_rng(seed)
+    return np.cumsum(rng.exponential(1.0 / rate_per_s, n))
+
+
+def request_sizes(n: int, seed: int, lo: int = 16 << 10, hi: int = 64 << 20) -> np.ndarray:
+    rng = np.random.default_rng(seed)
+    return np.floor(np.exp(rng.uniform(np.log(lo), np.log(hi), n))).astype(np.int64)

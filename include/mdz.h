@@ -72,6 +72,8 @@ int mdz_event_create(void** event);
 int mdz_event_destroy(void* event);
 int mdz_event_record(void* event, void* stream);
 int mdz_event_sync(void* event);
+/* 1 if the event's work has completed, 0 if not, negative on error. */
+int mdz_event_query(void* event);
 int mdz_event_elapsed_ms(void* start, void* stop, float* ms);
 void mdz_host_free(void* p);
 

@@ -129,86 +129,4 @@ int mdz_ref_inflate_batch(const uint8_t* src, const uint64_t* src_off, const uin
     return 0;
 }
 
-/* --------------------------------------------------------- config 4 ------ */
-
-typedef struct {
-    const double* cdf; uint64_t seed;
-    const uint64_t* raw_off; const uint64_t* raw_len; const double* copy_p; const uint8_t* mode;
-    uint8_t* raw; uint8_t** comp; uint64_t* comp_len;
-} gen_ctx;
-
-static void gen_task(void* v, uint64_t i) {
-    gen_ctx* x = (gen_ctx*)v;
-    uint8_t* r = x->raw + x->raw_off[i];
-    const uint64_t n = x->raw_len[i];
-    mdz_markov_gen(x->cdf, r, n, x->seed + 0x100000001b3ull * (i + 1), x->copy_p[i]);
-    z_stream z;
-    memset(&z, 0, sizeof z);
-    const int level = x->mode[i];
-    deflateInit2(&z, level, Z_DEFLATED, 15, 8, level == 1 ? Z_FIXED : Z_DEFAULT_STRATEGY);
-    const uLong bound = deflateBound(&z, (uLong)n);
-    uint8_t* out = (uint8_t*)malloc(bound + 16);
-    z.next_in = r;
-    z.avail_in = (uInt)n;
-    z.next_out = out;
-    z.avail_out = (uInt)bound;
-    deflate(&z, Z_FINISH);
-    x->comp_len[i] = z.total_out;
-    deflateEnd(&z);
-    x->comp[i] = out;
-}
-
-/* Config 4: `count` zlib streams.  Sizes floor(exp(U[ln min, ln max])) in
- * stream order; copy rate U[0.1, 0.7] per stream; modes by a seeded shuffle:
- * 80% level 6 (dynamic Huffman), 10% level 1 with Z_FIXED (fixed Huffman),
- * 10% level 0 (stored).  Streams are concatenated with each start aligned to
- * 16 bytes.  Outputs are allocated here (free with mdz_free). */
-int mdz_gen_deflate_batch(uint64_t count, uint64_t seed, uint64_t min_size, uint64_t max_size,
-                          uint8_t** src, uint64_t** src_off, uint64_t** src_len, uint8_t** raw,
-                          uint64_t** raw_off, uint64_t** raw_len, uint8_t** mode, int threads) {
-    if (!count || !min_size || max_size < min_size) return MDZ_ST_BAD_HEADER;
-    mdz_rng r;
-    mdz_rng_seed(&r, seed);
-    uint64_t* rl = (uint64_t*)malloc(count * 8);
-    uint64_t* ro = (uint64_t*)malloc((count + 1) * 8);
-    double* cp = (double*)malloc(count * sizeof(double));
-    uint8_t* md = (uint8_t*)malloc(count);
-    const double lo = log((double)min_size), hi = log((double)max_size);
-    ro[0] = 0;
-    for (uint64_t i = 0; i < count; ++i) {
-        uint64_t s = (uint64_t)floor(exp(lo + (hi - lo) * mdz_rng_unit(&r)));
-        if (s < min_size) s = min_size;
-        if (s > max_size) s = max_size;
-        rl[i] = s;
-        ro[i + 1] = ro[i] + s;
-        cp[i] = 0.1 + 0.6 * mdz_rng_unit(&r);
-    }
-    const uint64_t n_fixed = (count + 5) / 10, n_stored = count / 10;
-    for (uint64_t i = 0; i < count; ++i) md[i] = i < n_fixed ? 1 : i < n_fixed + n_stored ? 0 : 6;
-    for (uint64_t i = count - 1; i > 0; --i) {      /* Fisher-Yates */
-        const uint64_t j = mdz_rng_below(&r, i + 1);
-        const uint8_t t = md[i]; md[i] = md[j]; md[j] = t;
-    }
-    double* cdf = mdz_markov_table(seed);
-    uint8_t* rw = (uint8_t*)malloc(ro[count] + 1);
-    uint8_t** comp = (uint8_t**)calloc(count, sizeof(uint8_t*));
-    uint64_t* cl = (uint64_t*)malloc(count * 8);
-    gen_ctx x = {cdf, seed, ro, rl, cp, md, rw, comp, cl};
-    mdz_parallel_for(count, threads, gen_task, &x);
-    uint64_t* so = (uint64_t*)malloc((count + 1) * 8);
-    so[0] = 0;
-    for (uint64_t i = 0; i < count; ++i) so[i + 1] = (so[i] + cl[i] + 15) & ~15ull;
-    uint8_t* s = (uint8_t*)calloc(so[count] + 16, 1);
-    for (uint64_t i = 0; i < count; ++i) {
-        memcpy(s + so[i], comp[i], cl[i]);
-        free(comp[i]);
-    }
-    free(comp);
-    free(cdf);
-    free(cp);
-    *src = s; *src_off = so; *src_len = cl;
-    *raw = rw; *raw_off = ro; *raw_len = rl; *mode = md;
-    return 0;
-}
-
 const char* mdz_zlib_version(void) { return zlibVersion(); }

@@ -1,0 +1,40 @@
+"""GPU: the serving loop (continuous batching over HIP streams, HBM-budget
+admission) returns bit-exact results, and each request's result is the same
+alone as in the mix."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _requests(n, seed, arrivals=None):
+    from compression_amd import serve
+    sizes = serve.request_sizes(n, seed, 16 << 10, 2 << 20)
+    rng = np.random.default_rng(seed)
+    codecs = [serve.CODECS[i] for i in rng.integers(0, 4, n)]
+    arr = arrivals if arrivals is not None else np.zeros(n)
+    return [serve.make_request(i, codecs[i], int(sizes[i]), 1000 + i, float(arr[i])) for i in range(n)]
+
+
+def test_mix_bit_exact_and_same_alone(gpu):
+    from compression_amd import serve
+    reqs = _requests(24, 7, serve.poisson_arrivals(24, 400.0, 8))
+    srv = serve.Server(hbm_budget=256 << 20, n_streams=4, verify=True)
+    res = srv.run(reqs, time_limit=120)
+    assert res["completed"] == 24 and res["bit_exact"]
+    mixed = dict(srv.outputs)
+    for r in reqs:
+        alone = serve.Server(hbm_budget=1 << 30, n_streams=1, verify=True)
+        r2 = serve.make_request(r.rid, r.codec, r.out_bytes, 1000 + r.rid, 0.0)
+        out = alone.run([r2], time_limit=60)
+        assert out["bit_exact"] and np.array_equal(alone.outputs[r.rid], mixed[r.rid])
+
+
+def test_budget_admission_limits_inflight(gpu):
+    from compression_amd import serve
+    reqs = _requests(12, 9)
+    budget = max(r.hbm_bytes() for r in reqs)         # about one request at a time
+    srv = serve.Server(hbm_budget=budget, n_streams=4, verify=True)
+    res = srv.run(reqs, time_limit=120)
+    assert res["completed"] == 12 and res["bit_exact"]
+    assert srv.in_use == 0
