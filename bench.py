@@ -286,6 +286,81 @@ class Config2Workload:
                           "median of >=3 runs each; " + "; ".join(r["sample"] for r in res)}
 
 
+class DeflateWorkload:
+    """Deflate: config 4's uneven batch -- 4096 zlib streams, sizes log-uniform
+    4 KiB..4 MiB (80% level 6, 10% fixed Huffman, 10% stored), one output slot
+    each.  The step is ONE persistent work-queue launch over all streams in
+    longest-predicted-cost-first order, the cost from the learned model
+    (trained here on a separately generated batch the decoder logged).  The
+    plain per-stream launch and the compressed-size order are timed beside
+    it."""
+    dtype = "u8"
+    name = "deflate_4096_streams_uneven"
+
+    def __init__(self, count=4096, **_):
+        from compression_amd import deflate, host as H, sched
+        self.b = b = H.gen_deflate_batch(count, 0x4D445A04, 4096, 4 << 20, threads=16)
+        self.db = deflate.DeviceBatch(b.src, b.src_off, b.src_len, b.raw_len)
+        self.out_bytes = int(b.raw_len.sum())
+        self.alg_bytes = int(b.src_len.sum()) + self.out_bytes + 48 * count
+        # learned order: train on a different batch's decoder-logged costs
+        tb = H.gen_deflate_batch(1024, 0x4D445A44, 4096, 4 << 20, threads=16)
+        tdb = deflate.DeviceBatch(tb.src, tb.src_off, tb.src_len, tb.raw_len)
+        tdb.enable_cost_log()
+        deflate.inflate_sched(tdb)
+        self.model = sched.train_on_log(tdb, tdb.costs()["cycles"], steps=2000)
+        pred = self.model.predict(sched.features(self.db), count)
+        self.order_model = sched.lpt_order(pred)
+        self.order_size = sched.lpt_order(b.src_len.astype(np.float64))
+        self.db.set_order(self.order_model)
+        self.order_db = self.db.order
+        self.db.set_order(self.order_size)
+        self.size_db = self.db.order
+        self.db.order = self.order_db
+        self.config = {"workload": self.name, "codec": "deflate", "streams": count,
+                       "out_bytes": self.out_bytes, "src_bytes": int(b.src_len.sum()),
+                       "modes": {"level6": int((b.mode == 6).sum()), "fixed": int((b.mode == 1).sum()),
+                                 "stored": int((b.mode == 0).sum())},
+                       "launch": "persistent work queue, learned-cost LPT order",
+                       "slots": deflate.slots()}
+
+    def step(self, stream):
+        from compression_amd import deflate
+        deflate.inflate_sched(self.db, stream=stream)
+
+    def plain_step(self, stream):
+        from compression_amd import deflate
+        deflate.inflate(self.db, stream=stream)
+
+    def size_step(self, stream):
+        from compression_amd import deflate
+        self.db.order = self.size_db
+        deflate.inflate_sched(self.db, stream=stream)
+        self.db.order = self.order_db
+
+    batched = True
+
+    def verify(self):
+        st = self.db.statuses()
+        if not ((st["status"] == 0).all() and (st["bytes_written"] == self.b.raw_len).all()):
+            return False
+        out = self.db.outputs()
+        for i in range(0, self.b.count, 41):
+            o = int(self.db.dst_off[i])
+            if out[o:o + int(self.b.raw_len[i])].tobytes() != self.b.raw_stream(i):
+                return False
+        return True
+
+    def cpu(self, threads):
+        from oracle import pyoracle as O   # the CPU baseline leg: zlib inflate
+        b = self.b
+        t = _cpu_time(lambda: O.inflate_batch(b, threads=threads))
+        return {"value": round(self.out_bytes / t / 1e9, 3), "unit": "GB/s", "cores": threads,
+                "kind": "reference",
+                "sample": f"whole batch ({self.out_bytes >> 20} MiB out, {b.count} streams), "
+                          f"zlib {O.zlib_version()} inflate per stream, median of >=3 runs"}
+
+
 class CopyWorkload:
     """Calibration: a plain 16-byte-vector device copy of 512 MiB (reads and
     writes each byte once).  Its FETCH_SIZE/WRITE_SIZE per byte scale the
@@ -317,7 +392,8 @@ class CopyWorkload:
 WORKLOADS = {"config2": Config2Workload,
              "config2-plain": lambda **kw: Config2Workload(batched=False, **kw),
              "rlev1": RleV1Workload, "rlev2": RleV2Workload,
-             "brle": BrleWorkload, "lzss": LzssWorkload, "copy": CopyWorkload}
+             "brle": BrleWorkload, "lzss": LzssWorkload, "deflate": DeflateWorkload,
+             "copy": CopyWorkload}
 DEFAULT = "config2"
 
 
@@ -373,18 +449,29 @@ def run(args):
         p1.record(stream)
         p1.sync()
         plain_ms = p0.elapsed_ms(p1) / args.steps
+    size_ms = None
+    if hasattr(wl, "size_step"):                    # compressed-size order, same launch
+        q0, q1 = mdz.Event(), mdz.Event()
+        q0.record(stream)
+        for _ in range(args.steps):
+            wl.size_step(stream.ptr)
+        q1.record(stream)
+        q1.sync()
+        size_ms = q0.elapsed_ms(q1) / args.steps
     threads = max(1, min(H.host_cores(), 16))
     line = {
         "metric": METRIC, "value": round(gbs, 2), "unit": "GB/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 5),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": wl.dtype,
-        "data": "synthetic (seeded generator, encoded by the C oracle)", "config": wl.config,
+        "data": "synthetic (seeded generators, encoded by the host library)", "config": wl.config,
         "roofline": {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": TRAFFIC.get(dom["kernel"]),
                      "kernel": dom["kernel"], "alg_bytes_per_launch": dom["alg_bytes"]},
         "kernels": per_kernel,
         "batched_vs_plain": ({"batched_ms": round(ms, 5), "plain_ms": round(plain_ms, 5),
-                              "speedup": round(plain_ms / ms, 3)} if plain_ms else None),
+                              "speedup": round(plain_ms / ms, 3),
+                              **({"size_order_ms": round(size_ms, 5)} if size_ms else {})}
+                             if plain_ms else None),
         "bit_exact": ok,
     }
     if not args.no_cpu:
@@ -394,21 +481,73 @@ def run(args):
     print(json.dumps(line), flush=True)
 
 
+def run_serve(args):
+    """Config 5: mixed-codec serving.  A pool of requests (codec uniform over
+    RLE v1 / RLE v2 / LZSS / Deflate, sizes log-uniform 16 KiB..64 MiB) is
+    generated once; saturation is measured with every request arriving at
+    t = 0; then open-loop Poisson arrivals at 10..80% of saturation are
+    replayed with the same requests.  Latency = arrival -> output ready on
+    the device; the 64 GB HBM budget bounds what is admitted."""
+    import compression_amd as mdz
+    from compression_amd import serve
+
+    if mdz.device_count() < 1:
+        raise SystemExit("bench.py needs a GPU")
+    n = args.requests
+    sizes = serve.request_sizes(n, 0x4D445A05)
+    codecs = [serve.CODECS[i] for i in np.random.default_rng(0x4D445A05).integers(0, 4, n)]
+    pool = [serve.make_request(i, codecs[i], int(sizes[i]), 0x4D445A05 + i) for i in range(n)]
+    total = sum(r.out_bytes for r in pool)
+    srv = serve.Server(hbm_budget=64 << 30, n_streams=4, verify=False)
+    srv.run(pool[: min(8, n)], time_limit=60)            # warm-up: allocations, code objects
+    for r in pool:
+        r.arrival = 0.0
+    sat = srv.run(pool, time_limit=120)
+    sat_rps = n / max((total / (sat["GBps"] * 1e9)) if sat["GBps"] else 1.0, 1e-9)
+    sweep = []
+    for frac in (0.1, 0.2, 0.4, 0.6, 0.8):
+        arr = serve.poisson_arrivals(n, frac * sat_rps, 0x4D445A05 + int(frac * 100))
+        for r, a in zip(pool, arr):
+            r.arrival = float(a)
+        res = srv.run(pool, time_limit=120)
+        sweep.append({"load": frac, "p50_ms": round(res["p50_ms"], 3), "p99_ms": round(res["p99_ms"], 3),
+                      "GBps": round(res["GBps"], 2), "completed": res["completed"]})
+    # bit-exactness of the whole pool, once, with verification on
+    chk = serve.Server(hbm_budget=64 << 30, n_streams=4, verify=True)
+    for r in pool:
+        r.arrival = 0.0
+    ok = chk.run(pool, time_limit=180)["bit_exact"]
+    at80 = sweep[-1]
+    print(json.dumps({
+        "metric": METRIC, "value": at80["GBps"], "unit": "GB/s", "n_gpus": 1,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "mixed",
+        "data": "synthetic (seeded generators, host encoders)",
+        "config": {"workload": "serve_mixed_codecs", "requests": n, "out_bytes": total,
+                   "codecs": {c: codecs.count(c) for c in serve.CODECS}, "hbm_budget_GiB": 64,
+                   "streams": 4},
+        "saturation_GBps": round(sat["GBps"], 2), "p50_ms_at_80": at80["p50_ms"],
+        "p99_ms_at_80": at80["p99_ms"], "sweep": sweep, "bit_exact": ok}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--workload", default=DEFAULT, choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default=DEFAULT, choices=sorted(WORKLOADS) + ["serve"])
     ap.add_argument("--line", type=int, default=0, help="RLE v1 COMP_COL_LEN override")
     ap.add_argument("--ring", type=int, default=0, help="LDS ring words override")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--requests", type=int, default=96, help="request pool size for --workload serve")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "mdz decodes on one GPU; chunks shard across GPUs without "
                                    "exchange, but only --gpus 1 is built and measured"}))
         raise SystemExit(2)
-    run(args)
+    if args.workload == "serve":
+        run_serve(args)
+    else:
+        run(args)
 
 
 if __name__ == "__main__":

@@ -68,6 +68,32 @@ class CostModel:
         return y.download(n * 4, np.float32, stream).copy()
 
 
+def init_params(seed: int, target_mean: float = 0.0) -> np.ndarray:
+    """He-initialised weights, small positive hidden biases, output bias at
+    the mean training target."""
+    rng = np.random.default_rng(seed)
+    p = np.zeros(P, np.float32)
+    o = 0
+    p[o:o + F * H] = rng.normal(0, np.sqrt(2.0 / F), F * H); o += F * H
+    p[o:o + H] = 0.1; o += H
+    p[o:o + H * H] = rng.normal(0, np.sqrt(2.0 / H), H * H); o += H * H
+    p[o:o + H] = 0.1; o += H
+    p[o:o + H] = rng.normal(0, np.sqrt(1.0 / H), H); o += H
+    p[o] = target_mean
+    return p
+
+
+def train_on_log(batch, cycles: np.ndarray, steps: int = 2000, seed: int = 7) -> "CostModel":
+    """Fits a model to a decoder-logged batch: features of its streams ->
+    log2(measured cycles)."""
+    x = features(batch)
+    t = np.log2(np.maximum(np.asarray(cycles, np.float64), 1.0)).astype(np.float32)
+    td = DeviceBuffer.from_array(t)
+    m = CostModel(init_params(seed, float(t.mean())))
+    m.train(x, td, batch.count, steps=steps)
+    return m
+
+
 def lpt_order(cost: np.ndarray) -> np.ndarray:
     """Stream indices, largest cost first (ties by index)."""
     return np.lexsort((np.arange(cost.size), -np.asarray(cost, np.float64))).astype(np.uint32)
