@@ -31,7 +31,11 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch of each workload's kernel, from rocprofv3 --pmc
 # FETCH_SIZE and WRITE_SIZE scaled by the copy-kernel calibration; the
 # summaries these come from are under profiles/.
-TRAFFIC: dict = {}
+TRAFFIC: dict = {
+    "batch_kernel<8> (rle_v2+byte_rle)": 2132185088,   # profiles/config2_rlev2_brle_batched.md
+    "rlev2_i64_1GiB_plain": 1416962048,
+    "brle_u8_256MiB_plain": 698067968,
+}
 
 
 def _cpu_time(fn, min_reps=3, max_reps=7, budget_s=20.0):
