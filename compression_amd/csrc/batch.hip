@@ -105,7 +105,7 @@ int batch_submit(const mdz_batch_t* bt, hipStream_t s) {
         if (bt->brle_n_chunks > 0x7fffffffull) return MDZ_ERR_UNSUPPORTED;
         const uint64_t c = bt->brle_chunk_bytes;
         uint64_t need = bt->brle_max_chunk ? bt->brle_max_chunk : c + c / 128 + 1;
-        need = (need + 32 + 255) & ~255ull;
+        need = (need + kBrlePad + 255) & ~255ull;
         const uint64_t lds_max = 160 * 1024 - 16 - sizeof(BrleTables<kBatchThreads>) - 256;
         if (need > lds_max) return MDZ_ERR_UNSUPPORTED;
         b = BrleArgs{(const uint8_t*)bt->brle_data, bt->brle_ptr, bt->brle_n_chunks, c,

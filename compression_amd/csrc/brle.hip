@@ -16,9 +16,12 @@
 //      exit, in parallel rounds until every window agrees;
 //   3. a block-wide prefix sum of the windows' output lengths gives every
 //      thread its output offset;
-//   4. each thread expands its window's tokens through a 16-byte register
-//      window and writes 16-byte aligned stores (bytes at the two ragged
-//      ends of its output range are stored singly).
+//   4. output is cut at 16-byte boundaries of the destination: each thread
+//      writes the aligned blocks from the boundary at or after its window's
+//      output start to the one at or after its end, assembling each block in
+//      registers from token pieces (runs splat, literals read from LDS at
+//      the block offset), so all stores but the chunk's two end blocks are
+//      aligned 16-byte stores.
 #include "brle_dev.h"
 
 namespace mdz {
@@ -40,7 +43,7 @@ int brle_decompress(const uint64_t* ptr, uint64_t n_chunks, uint64_t chunk_bytes
     if (n_chunks > 0x7fffffffull) return MDZ_ERR_UNSUPPORTED;
     // worst case one control byte per 128 literals (brle.h:13)
     uint64_t need = max_chunk_bytes ? max_chunk_bytes : chunk_bytes + chunk_bytes / 128 + 1;
-    need = (need + 32 + 255) & ~255ull;
+    need = (need + kBrlePad + 255) & ~255ull;
     const uint64_t lds_max = 160 * 1024 - sizeof(BrleTables<kBrleThreads>) - 256;
     if (need > lds_max) return MDZ_ERR_UNSUPPORTED;
     const size_t smem = sizeof(BrleTables<kBrleThreads>) + need;

@@ -14,6 +14,24 @@
 
 #define MDZ_DEV __device__ __forceinline__
 
+// Phase timing for tuning builds.  With MDZ_PHASE_PROF defined (never in the
+// library build), thread 0 of each workgroup adds the shader-clock cycles
+// spent between consecutive marks to mdz_phase_cycles[k] (a __device__ array
+// the profiling translation unit defines); otherwise the marks vanish.
+#ifdef MDZ_PHASE_PROF
+extern __device__ unsigned long long mdz_phase_cycles[16];
+#define MDZ_PHASE_START() uint64_t mdz_phase_t_ = clock64()
+#define MDZ_PHASE(k)                                                              \
+    do {                                                                          \
+        const uint64_t mdz_now_ = clock64();                                      \
+        if (threadIdx.x == 0) atomicAdd(&mdz_phase_cycles[k], mdz_now_ - mdz_phase_t_); \
+        mdz_phase_t_ = mdz_now_;                                                  \
+    } while (0)
+#else
+#define MDZ_PHASE_START() ((void)0)
+#define MDZ_PHASE(k) ((void)0)
+#endif
+
 namespace mdz {
 
 constexpr int kWave = 64;

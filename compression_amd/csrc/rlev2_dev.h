@@ -250,6 +250,7 @@ MDZ_DEV void rlev2_chunk(const Rlev2Args& a, uint64_t c, uint32_t* smem) {
     Rlev2Tables& tb = *reinterpret_cast<Rlev2Tables*>(data + data_cap + 16);
 
     const uint32_t tid = threadIdx.x, wave = tid >> 6, l = tid & 63;
+    MDZ_PHASE_START();
     const uint32_t nvals = (uint32_t)(chunk_bytes / EB / 32);
     const uint64_t out_base = c * chunk_bytes;
     const uint64_t limit = dst_bytes > out_base ? min(dst_bytes - out_base, chunk_bytes) : 0;
@@ -283,9 +284,11 @@ MDZ_DEV void rlev2_chunk(const Rlev2Args& a, uint64_t c, uint32_t* smem) {
         if (tid == 0 && status) status[c] = tb.extent_ok ? MDZ_ST_OUTPUT_OVERRUN : MDZ_ST_INPUT_OVERRUN;
         return;
     }
+    MDZ_PHASE(0);
     stage_chunk_lanes(tb.lt, tb.lbase, smem, reinterpret_cast<const u32x4*>(payload + c_off), tid,
                       kThreads);
     __syncthreads();
+    MDZ_PHASE(1);
 
     const uint32_t sh = (ru & (ru - 1)) ? 32u : (uint32_t)__builtin_ctz(ru);
     for (uint32_t t = wave; t < 32; t += kThreads / kWave) {
@@ -293,11 +296,13 @@ MDZ_DEV void rlev2_chunk(const Rlev2Args& a, uint64_t c, uint32_t* smem) {
                                               limit, LanePlace{t, ru, sh});
         if (e != MDZ_ST_OK && l == 0) atomicMin(&tb.err, (t << 8) | ((uint32_t)e & 0xffu));
     }
+    MDZ_PHASE(2);
     __syncthreads();
     if (tid == 0 && status) {
         const uint32_t e = tb.err;
         status[c] = e == 0xffffffffu ? MDZ_ST_OK : (int32_t)(int8_t)(e & 0xff);
     }
+    MDZ_PHASE(3);
 }
 
 }  // namespace

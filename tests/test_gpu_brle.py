@@ -75,3 +75,50 @@ def test_host_wrapper(gpu):
     x = O.gen_brle_u8(700_001, 10)
     c = O.brle_encode(x, 262144, threads=8)
     assert brle.decompress_gpu(c.to_bytes()) == x.tobytes()
+
+
+@pytest.mark.parametrize("chunk,shift", [(4099, 3), (65536, 7), (131072, 1), (1000, 15)])
+def test_unaligned_destination(gpu, chunk, shift):
+    # chunk outputs start off 16-byte boundaries: blocks are cut at the
+    # destination's own alignment, with byte stores only at chunk ends
+    from compression_amd import brle
+    x = O.gen_brle_u8(1_000_003, 11 + shift)
+    c = O.brle_encode(x, chunk, threads=8)
+    b = brle.DeviceBrle(c.data, c.ptr, c.n_chunks, c.chunk)
+    dst = gpu.DeviceBuffer(c.raw_bytes + 64)
+    dst.fill(0xEE)
+    st = gpu.DeviceBuffer(4 * c.n_chunks)
+    brle.decompress(b, dst.ptr + shift, c.raw_bytes, st)
+    got = dst.download(c.raw_bytes + 64)
+    assert (st.download(4 * c.n_chunks, np.int32) == 0).all()
+    assert np.array_equal(got[shift:shift + c.raw_bytes], x)
+    assert (got[:shift] == 0xEE).all() and (got[shift + c.raw_bytes:] == 0xEE).all()
+
+
+def test_truncated_last_token(gpu):
+    # each chunk loses its last 1..3 bytes: the final token overruns the
+    # input; status and every byte before the stop point match the oracle
+    x = O.gen_brle_u8(600_000, 12)
+    c = O.brle_encode(x, 65536, threads=8)
+    keep, ptr, pos = [], [0], 0
+    rng = np.random.default_rng(5)
+    for i in range(c.n_chunks):
+        a, e = int(c.ptr[i]), int(c.ptr[i + 1])
+        e -= int(rng.integers(0, 4))
+        keep.append(c.data[a:e])
+        pos += e - a
+        ptr.append(pos)
+    data = np.concatenate(keep)
+    ptr = np.asarray(ptr, np.uint64)
+    ref = O.BrleContainer(data, ptr, c.n_chunks, c.chunk, c.raw_bytes)
+    rc, ref_out, ref_st = O.brle_decode(ref, threads=4)
+    got, st = _run(gpu, c, data=data, ptr=ptr)
+    assert np.array_equal(st, ref_st) and (st != 0).any()
+    for i in range(c.n_chunks):
+        # the bytes the oracle writes before it stops, and nothing after them
+        a, e = int(ptr[i]), int(ptr[i + 1])
+        _, o1 = O.brle_chunk_decode(data[a:e].tobytes(), c.chunk)
+        lo = i * c.chunk
+        assert got[lo:lo + len(o1)].tobytes() == o1, i
+        hi = min(lo + c.chunk, c.raw_bytes)
+        assert (got[lo + len(o1):hi] == 0xEE).all(), i
