@@ -7,22 +7,28 @@
 // (include/rlev2/decoder_new.h:618-657, decoder_warp :354-611) and host entry
 // rlev2::decompress_gpu (include/rlev2/decoder_trans.h:615-681).
 //
-// One workgroup (4 wave64s) per chunk.  The chunk's interleaved payload is
-// staged into LDS once and de-interleaved into 32 contiguous lane streams.
-// Each wave then decodes whole lane streams cooperatively: the run headers
-// are walked once per wave (uniform control flow), and each run's values are
-// produced 64 at a time, one per SIMD lane:
-//   SHORT_REPEAT  lane i writes the repeated value;
-//   DIRECT        lane i unpacks the i-th MSB-first bit field;
-//   DELTA         fixed stride: base + i*delta; packed deltas: lane i unpacks
-//                 delta i and a wave prefix sum turns deltas into values;
-//   PATCHED_BASE  lane k reads patch entry k, a prefix sum of the gaps places
-//                 it, and every lane ORs in the patches that land on its value
-//                 (64-bit mask: decoder_new.h:533's 32-bit mask drops patch
-//                 bits at positions >= 32).
-// Values are written straight to their final place: lane t's q-th value is
-// element (q / RU) * 32 * RU + t * RU + q % RU of the chunk, so consecutive
-// SIMD lanes store consecutive elements.
+// One workgroup (8 wave64s) per chunk:
+//   1. the chunk's interleaved payload is read with 16-byte loads and each
+//      4-byte word scattered to its lane's contiguous LDS region (a per-
+//      segment destination table and a multiply-high division place it);
+//   2. header walk: 32 threads, one per lane stream, walk the run headers in
+//      parallel, check them as the reference decoder would, and append one
+//      32-byte run descriptor per run to an LDS pool;
+//   3. value pass: every run is cut into 64-value pieces (a packed DELTA run
+//      stays whole: its prefix sum runs over the run) and the 8 waves take
+//      pieces in turn, one value per SIMD lane:
+//        SHORT_REPEAT  lane i writes the repeated value;
+//        DIRECT        lane i unpacks the i-th MSB-first bit field;
+//        DELTA         fixed stride: base + i*delta; packed deltas: lane i
+//                      unpacks delta i and a wave prefix sum makes values;
+//        PATCHED_BASE  lane k reads patch entry k, a prefix sum of the gaps
+//                      places it, and every lane ORs in the patches that land
+//                      on its value (64-bit mask: decoder_new.h:533's 32-bit
+//                      mask drops patch bits at positions >= 32).
+//   Steps 2 and 3 repeat while a lane stream has runs left (the pool holds
+//   kRunPool runs).  Lane t's q-th value is element
+//   (q / RU) * 32 * RU + t * RU + q % RU of the chunk, so consecutive SIMD
+//   lanes store consecutive elements.
 #include "mdz_device.h"
 #include "mdz_internal.h"
 
@@ -96,129 +102,259 @@ MDZ_DEV void put_val(uint8_t* out, uint64_t limit, uint32_t q, const LanePlace& 
     else out[off] = (uint8_t)v;
 }
 
-// Decodes lane stream t (LDS bytes [base, base+end)) into its nvals values.
-// Called by all 64 lanes of one wave; every branch condition is uniform.
-// Status codes follow the oracle's mdz_ref_rlev2_lane_decode.
-template <int EB>
-MDZ_DEV int32_t rlev2_lane_wave(const uint8_t* base, uint32_t end, uint32_t nvals, uint8_t* out,
-                                uint64_t limit, const LanePlace lp) {
-    constexpr uint64_t vmask = EB == 8 ? ~0ull : ((1ull << (8 * EB)) - 1);
-    const uint32_t l = lane_id();
-    uint32_t p = 0, q = 0;
-    int32_t err = MDZ_ST_OK;
-    while (q < nvals) {
-        if (p >= end) return MDZ_ST_INPUT_OVERRUN;
-        const uint64_t x = lds_peek8(base, p);
-        const uint32_t first = (uint32_t)(x & 0xff), kind = first >> 6;
-        if (kind == 0) {                                        // SHORT_REPEAT
-            const uint32_t nb = ((first >> 3) & 7) + 1, cnt = (first & 7) + 3;
-            if (p + 1 + nb > end) return MDZ_ST_INPUT_OVERRUN;
-            if (cnt > nvals - q) return MDZ_ST_OUTPUT_OVERRUN;
-            const uint64_t v = lds_bits_be(base, p + 1, 0, 8 * nb) & vmask;
-            if (l < cnt) put_val<EB>(out, limit, q + l, lp, v);
-            p += 1 + nb;
-            q += cnt;
-            continue;
+constexpr uint32_t kRunPool = 768;              // run descriptors per epoch
+
+// Bit width of a 5-bit width code (the reference's FixedBitSizes decode):
+// 1..24, then 26, 28, 30, 32, then 40, 48, 56, 64.
+MDZ_DEV uint32_t decode_width(uint32_t c) {
+    return c < 24 ? c + 1 : c < 28 ? 26 + 2 * (c - 24) : 40 + 8 * (c - 28);
+}
+
+// Byte length (1..10) of the unsigned LEB128 at byte p, or 0 with *err as
+// uvarint() reports it.
+MDZ_DEV uint32_t uvarint_len(const uint8_t* base, uint32_t p, uint32_t end, int32_t& err) {
+    const uint64_t lo = lds_peek8(base, p), hi = lds_peek8(base, p + 8);
+    const uint64_t tl = ~lo & 0x8080808080808080ull, th = ~hi & 0x8080ull;
+    const uint32_t len = tl ? (__builtin_ctzll(tl) >> 3) + 1 : th ? 9 + (__builtin_ctzll(th) >> 3) : 0;
+    const uint32_t avail = end > p ? end - p : 0;
+    if (len == 0 || len > avail) {
+        err = (len == 0 && avail >= 10) ? MDZ_ST_BAD_SYMBOL : MDZ_ST_INPUT_OVERRUN;
+        return 0;
+    }
+    return len;
+}
+
+// One run, as the header walk leaves it for the value pass.
+struct Run {
+    uint32_t p;      // lane byte offset: SHORT_REPEAT value bytes, DIRECT values,
+                     // DELTA base varint, PATCHED_BASE base bytes
+    uint32_t q;      // index of the run's first value in its lane
+    uint32_t meta;   // kind | code << 2 | w << 7 | (len - 1) << 14 | lane << 23
+    uint32_t aux;    // SHORT_REPEAT: nb; DELTA: n1 | n2 << 4;
+                     // PATCHED_BASE: bw | pw << 4 | pgw << 11 | pll << 15 | cfb << 20
+};
+
+// Walks one run header at byte *p of a lane stream of `end` bytes whose next
+// value is q: checks it as the oracle's mdz_ref_rlev2_lane_decode does (in
+// the same order), fills r and advances *p.  Patch positions of a
+// PATCHED_BASE run are checked later, by patch_list_check().
+MDZ_DEV int32_t walk_run(const uint8_t* base, uint32_t end, uint32_t& p, uint32_t q, uint32_t nvals,
+                         uint32_t lane, Run& r) {
+    if (p >= end) return MDZ_ST_INPUT_OVERRUN;
+    const uint64_t x = lds_peek8(base, p);
+    const uint32_t first = (uint32_t)(x & 0xff), kind = first >> 6;
+    r.q = q;
+    if (kind == 0) {                                            // SHORT_REPEAT
+        const uint32_t nb = ((first >> 3) & 7) + 1, cnt = (first & 7) + 3;
+        if (p + 1 + nb > end) return MDZ_ST_INPUT_OVERRUN;
+        if (cnt > nvals - q) return MDZ_ST_OUTPUT_OVERRUN;
+        r.p = p + 1;
+        r.meta = (cnt - 1) << 14 | lane << 23;
+        r.aux = nb;
+        p += 1 + nb;
+        return MDZ_ST_OK;
+    }
+    if (p + 1 >= end) return MDZ_ST_INPUT_OVERRUN;
+    const uint32_t code = (first >> 1) & 0x1f, w = decode_width(code);
+    const uint32_t len = ((((first & 1) << 8)) | (uint32_t)((x >> 8) & 0xff)) + 1;
+    r.meta = kind | code << 2 | w << 7 | (len - 1) << 14 | lane << 23;
+    if (len > nvals - q) {
+        // DIRECT, DELTA and PATCHED_BASE all check the value count first,
+        // except that PATCHED_BASE checks its 4-byte header before it
+        if (kind != 2 || p + 3 < end) return MDZ_ST_OUTPUT_OVERRUN;
+    }
+    if (kind == 1) {                                            // DIRECT
+        const uint32_t bytes = (len * w + 7) / 8;
+        if (p + 2 + bytes > end) return MDZ_ST_INPUT_OVERRUN;
+        r.p = p + 2;
+        r.aux = 0;
+        p += 2 + bytes;
+        return MDZ_ST_OK;
+    }
+    if (kind == 3) {                                            // DELTA
+        if (len < 2) return MDZ_ST_BAD_SYMBOL;
+        int32_t err = MDZ_ST_OK;
+        const uint32_t n1 = uvarint_len(base, p + 2, end, err);
+        if (!n1) return err;
+        const uint32_t n2 = uvarint_len(base, p + 2 + n1, end, err);
+        if (!n2) return err;
+        const uint32_t pd = p + 2 + n1 + n2;
+        r.p = p + 2;
+        r.aux = n1 | n2 << 4;
+        if (code == 0) {
+            p = pd;
+        } else {
+            const uint32_t bytes = ((len - 2) * w + 7) / 8;
+            if (pd + bytes > end) return MDZ_ST_INPUT_OVERRUN;
+            p = pd + bytes;
         }
-        if (p + 1 >= end) return MDZ_ST_INPUT_OVERRUN;
-        const uint32_t code = (first >> 1) & 0x1f, w = kDecW[code];
-        const uint32_t len = ((((first & 1) << 8)) | (uint32_t)((x >> 8) & 0xff)) + 1;
-        if (kind == 1) {                                        // DIRECT
-            if (len > nvals - q) return MDZ_ST_OUTPUT_OVERRUN;
-            const uint32_t bytes = (len * w + 7) / 8;
-            if (p + 2 + bytes > end) return MDZ_ST_INPUT_OVERRUN;
-            for (uint32_t i0 = 0; i0 < len; i0 += kWave) {
-                const uint32_t i = i0 + l;
-                if (i < len)
-                    put_val<EB>(out, limit, q + i, lp,
-                                lds_bits_be(base, p + 2, (uint64_t)i * w, w) & vmask);
-            }
-            p += 2 + bytes;
-            q += len;
-        } else if (kind == 3) {                                 // DELTA
-            if (len > nvals - q) return MDZ_ST_OUTPUT_OVERRUN;
-            if (len < 2) return MDZ_ST_BAD_SYMBOL;
-            uint64_t bv, zd;
-            const uint32_t n1 = uvarint(base, p + 2, end, bv, err);
-            if (!n1) return err;
-            const uint32_t n2 = uvarint(base, p + 2 + n1, end, zd, err);
-            if (!n2) return err;
-            const uint32_t pd = p + 2 + n1 + n2;
-            const uint64_t dt = ((zd >> 1) ^ (0 - (zd & 1))) & vmask;
-            const uint64_t v0 = bv & vmask, v1 = (v0 + dt) & vmask;
-            if (code == 0) {                                    // fixed stride
-                for (uint32_t i0 = 0; i0 < len; i0 += kWave) {
-                    const uint32_t i = i0 + l;
-                    if (i < len) put_val<EB>(out, limit, q + i, lp, (v0 + (uint64_t)i * dt) & vmask);
-                }
-                p = pd;
-            } else {
-                const uint32_t nd = len - 2, bytes = (nd * w + 7) / 8;
-                if (pd + bytes > end) return MDZ_ST_INPUT_OVERRUN;
-                if (l < 2) put_val<EB>(out, limit, q + l, lp, l ? v1 : v0);
-                // multiplier = (DATA_TYPE)delta > 0 ? 1 : -1 on the unsigned type
-                const bool neg = dt == 0;
-                uint64_t carry = v1;
-                for (uint32_t k0 = 0; k0 < nd; k0 += kWave) {
-                    const uint32_t k = k0 + l;
-                    uint64_t d = k < nd ? lds_bits_be(base, pd, (uint64_t)k * w, w) : 0;
-                    if (neg) d = 0 - d;
-                    const uint64_t incl = wave_incl_scan_u64(d);
-                    if (k < nd) put_val<EB>(out, limit, q + 2 + k, lp, (carry + incl) & vmask);
-                    carry += shfl_u64(incl, kWave - 1);
-                }
-                p = pd + bytes;
-            }
-            q += len;
-        } else {                                                // PATCHED_BASE
-            if (p + 3 >= end) return MDZ_ST_INPUT_OVERRUN;
-            const uint32_t third = (uint32_t)((x >> 16) & 0xff), fourth = (uint32_t)((x >> 24) & 0xff);
-            const uint32_t bw = ((third >> 5) & 7) + 1, pw = kDecW[third & 0x1f];
-            const uint32_t pgw = ((fourth >> 5) & 7) + 1, pll = fourth & 0x1f;
-            if (len > nvals - q) return MDZ_ST_OUTPUT_OVERRUN;
-            if (p + 4 + bw > end) return MDZ_ST_INPUT_OVERRUN;
-            const uint64_t bv = lds_bits_be(base, p + 4, 0, 8 * bw);
-            const uint32_t pb = p + 4 + bw, vbytes = (len * w + 7) / 8;
-            if (pb + vbytes > end) return MDZ_ST_INPUT_OVERRUN;
-            const uint32_t cfb = closest_bits(pw + pgw), pp = pb + vbytes;
-            // patch entry k in lane k; the first entry (in order) that cannot
-            // be read or lands past the run decides the status
-            const uint64_t pmask = pw >= 64 ? ~0ull : ((1ull << pw) - 1);
-            const bool readable = (uint64_t)pp * 8 + (uint64_t)(l + 1) * cfb <= (uint64_t)end * 8;
-            const uint64_t e = (l < pll && readable) ? lds_bits_be(base, pp, (uint64_t)l * cfb, cfb) : 0;
-            const uint32_t gap = (l < pll && pw < 64) ? (uint32_t)(e >> pw) : 0;
-            const uint32_t pos = wave_incl_scan_u32(gap);
-            const uint64_t bad = __ballot(l < pll && (!readable || pos >= len));
-            if (bad) {
-                const uint32_t k = (uint32_t)__builtin_ctzll(bad);
-                return __shfl((int)readable, (int)k, kWave) ? MDZ_ST_BAD_SYMBOL : MDZ_ST_INPUT_OVERRUN;
-            }
-            const uint64_t pv = e & pmask;
-            for (uint32_t i0 = 0; i0 < len; i0 += kWave) {
-                const uint32_t i = i0 + l;
-                uint64_t v = i < len ? lds_bits_be(base, pb, (uint64_t)i * w, w) : 0;
-                for (uint32_t k = 0; k < pll; ++k) {
-                    const uint32_t pk = __shfl(pos, (int)k, kWave);
-                    const uint64_t vk = shfl_u64(pv, (int)k);
-                    if (pk == i) v |= w >= 64 ? 0 : vk << w;
-                }
-                if (i < len) put_val<EB>(out, limit, q + i, lp, (v + bv) & vmask);
-            }
-            p = pp + (pll * cfb + 7) / 8;
-            q += len;
-        }
+        return MDZ_ST_OK;
+    }
+    // PATCHED_BASE
+    if (p + 3 >= end) return MDZ_ST_INPUT_OVERRUN;
+    const uint32_t third = (uint32_t)((x >> 16) & 0xff), fourth = (uint32_t)((x >> 24) & 0xff);
+    const uint32_t bw = ((third >> 5) & 7) + 1, pw = decode_width(third & 0x1f);
+    const uint32_t pgw = ((fourth >> 5) & 7) + 1, pll = fourth & 0x1f;
+    if (p + 4 + bw > end) return MDZ_ST_INPUT_OVERRUN;
+    const uint32_t pb = p + 4 + bw, vbytes = (len * w + 7) / 8;
+    if (pb + vbytes > end) return MDZ_ST_INPUT_OVERRUN;
+    const uint32_t cfb = closest_bits(pw + pgw);
+    r.p = p + 4;
+    r.aux = bw | pw << 4 | pgw << 11 | pll << 15 | cfb << 20;
+    p = pb + vbytes + (pll * cfb + 7) / 8;
+    return MDZ_ST_OK;
+}
+
+// Header chain step of the walk: the byte length and value count of the run
+// at byte p, from the 24 bytes at p alone (one LDS read).  Exact for every
+// run that passes walk_run(); for a malformed run it only has to stop:
+// *stop is set when the chain cannot go on past this run.
+MDZ_DEV void chain_step(const uint8_t* base, uint32_t p, uint32_t end, uint32_t& size, uint32_t& cnt,
+                        bool& stop) {
+    const uint32_t* w32 = reinterpret_cast<const uint32_t*>(base) + (p >> 2);
+    const uint32_t k = p & 3;
+    const uint32_t d0 = w32[0], d1 = w32[1], d2 = w32[2], d3 = w32[3], d4 = w32[4], d5 = w32[5],
+                   d6 = w32[6];
+    const uint32_t u0 = alignbyte(d1, d0, k), u1 = alignbyte(d2, d1, k), u2 = alignbyte(d3, d2, k),
+                   u3 = alignbyte(d4, d3, k), u4 = alignbyte(d5, d4, k), u5 = alignbyte(d6, d5, k);
+    const uint32_t b0 = u0 & 0xff, b1 = (u0 >> 8) & 0xff, b2 = (u0 >> 16) & 0xff, b3 = u0 >> 24;
+    const uint32_t kind = b0 >> 6;
+    const uint32_t code = (b0 >> 1) & 31, w = decode_width(code);
+    const uint32_t len = (((b0 & 1) << 8) | b1) + 1;
+    const uint32_t vbytes = (len * w + 7) / 8;
+    uint32_t sz, n = len;
+    bool bad = false;
+    if (kind == 0) {                                         // SHORT_REPEAT
+        sz = 2 + ((b0 >> 3) & 7);
+        n = (b0 & 7) + 3;
+    } else if (kind == 1) {                                  // DIRECT
+        sz = 2 + vbytes;
+    } else if (kind == 2) {                                  // PATCHED_BASE
+        const uint32_t bw = ((b2 >> 5) & 7) + 1, pw = decode_width(b2 & 31);
+        const uint32_t pgw = ((b3 >> 5) & 7) + 1, pll = b3 & 31;
+        sz = 4 + bw + vbytes + (pll * closest_bits(pw + pgw) + 7) / 8;
+    } else {                                                 // DELTA: two LEB128 varints
+        // terminator bytes (bit 7 clear) of bytes 0..23, one bit each
+        auto term4 = [](uint32_t u) { return (((~u >> 7) & 0x01010101u) * 0x01020408u) >> 24; };
+        const uint32_t tm = term4(u0) | term4(u1) << 4 | term4(u2) << 8 | term4(u3) << 12 |
+                            term4(u4) << 16 | term4(u5) << 20;
+        const uint32_t t1 = (tm >> 2) & 0x3ff;
+        const uint32_t n1 = t1 ? (uint32_t)__builtin_ctz(t1) + 1 : 0;
+        const uint32_t t2 = (tm >> (2 + n1)) & 0x3ff;
+        const uint32_t n2 = t2 ? (uint32_t)__builtin_ctz(t2) + 1 : 0;
+        bad = !n1 || !n2 || len < 2;
+        sz = 2 + n1 + n2 + (code ? ((len - 2) * w + 7) / 8 : 0);
+    }
+    size = sz;
+    cnt = n;
+    stop = bad || p >= end || p + sz > end;
+}
+
+// Patch-list check of a PATCHED_BASE run (one thread): the first entry
+// that cannot be read, or that lands past the run, decides the status
+// (readability only falls as the entry index grows).  Returns MDZ_ST_OK
+// when every entry is good.
+MDZ_DEV int32_t patch_list_check(const uint8_t* base, uint32_t end, const Run& r) {
+    const uint32_t w = (r.meta >> 7) & 127, len = ((r.meta >> 14) & 511) + 1;
+    const uint32_t bw = r.aux & 15, pw = (r.aux >> 4) & 127, pll = (r.aux >> 15) & 31,
+                   cfb = (r.aux >> 20) & 127;
+    const uint32_t pp = r.p + bw + (len * w + 7) / 8;
+    uint32_t pos = 0;
+    for (uint32_t k = 0; k < pll; ++k) {
+        if ((uint64_t)pp * 8 + (uint64_t)(k + 1) * cfb > (uint64_t)end * 8) return MDZ_ST_INPUT_OVERRUN;
+        const uint64_t e = lds_bits_be(base, pp, (uint64_t)k * cfb, cfb);
+        pos += pw < 64 ? (uint32_t)(e >> pw) : 0u;
+        if (pos >= len) return MDZ_ST_BAD_SYMBOL;
     }
     return MDZ_ST_OK;
 }
 
+// Decodes run r (all 64 lanes of one wave; r is wave-uniform).
+template <int EB>
+MDZ_DEV void run_values(const Run& rr, const uint8_t* data, const uint32_t* lbase, uint8_t* out,
+                        uint64_t limit, uint32_t ru, uint32_t sh) {
+    constexpr uint64_t vmask = EB == 8 ? ~0ull : ((1ull << (8 * EB)) - 1);
+    const uint32_t l = lane_id();
+    const uint32_t meta = __builtin_amdgcn_readfirstlane(rr.meta);
+    const uint32_t aux = __builtin_amdgcn_readfirstlane(rr.aux);
+    const uint32_t p = __builtin_amdgcn_readfirstlane(rr.p), q = __builtin_amdgcn_readfirstlane(rr.q);
+    const uint32_t kind = meta & 3, code = (meta >> 2) & 31, w = (meta >> 7) & 127;
+    const uint32_t len = ((meta >> 14) & 511) + 1, t = meta >> 23;
+    const uint8_t* const base = data + lbase[t];
+    const LanePlace lp{t, ru, sh};
+    if (kind == 0) {                                            // SHORT_REPEAT
+        const uint64_t v = lds_bits_be(base, p, 0, 8 * aux) & vmask;
+        if (l < len) put_val<EB>(out, limit, q + l, lp, v);
+    } else if (kind == 1) {                                     // DIRECT
+        for (uint32_t i0 = 0; i0 < len; i0 += kWave) {
+            const uint32_t i = i0 + l;
+            if (i < len) put_val<EB>(out, limit, q + i, lp, lds_bits_be(base, p, (uint64_t)i * w, w) & vmask);
+        }
+    } else if (kind == 3) {                                     // DELTA
+        const uint32_t n1 = aux & 15, n2 = aux >> 4;
+        const uint64_t a = gather7(lds_peek8(base, p)), zd = gather7(lds_peek8(base, p + n1));
+        const uint64_t hi1 = n1 > 8 ? (lds_peek8(base, p + 8) & 0x7f) | ((lds_peek8(base, p + 8) >> 1) & (0x7full << 7)) : 0;
+        const uint64_t hi2 = n2 > 8 ? (lds_peek8(base, p + n1 + 8) & 0x7f) | ((lds_peek8(base, p + n1 + 8) >> 1) & (0x7full << 7)) : 0;
+        const uint64_t bv = (n1 < 8 ? a & ((1ull << (7 * n1)) - 1) : a) |
+                            (n1 > 8 ? (hi1 & ((1ull << (7 * (n1 - 8))) - 1)) << 56 : 0);
+        const uint64_t zz = (n2 < 8 ? zd & ((1ull << (7 * n2)) - 1) : zd) |
+                            (n2 > 8 ? (hi2 & ((1ull << (7 * (n2 - 8))) - 1)) << 56 : 0);
+        const uint64_t v0 = bv & vmask, dt = ((zz >> 1) ^ (0 - (zz & 1))) & vmask;
+        if (code == 0) {
+            for (uint32_t i0 = 0; i0 < len; i0 += kWave) {
+                const uint32_t i = i0 + l;
+                if (i < len) put_val<EB>(out, limit, q + i, lp, (v0 + (uint64_t)i * dt) & vmask);
+            }
+        } else {
+            const uint32_t nd = len - 2, pd = p + n1 + n2;
+            const uint64_t v1 = (v0 + dt) & vmask;
+            if (l < 2) put_val<EB>(out, limit, q + l, lp, l ? v1 : v0);
+            // multiplier = (DATA_TYPE)delta > 0 ? 1 : -1 on the unsigned type
+            const bool neg = dt == 0;
+            uint64_t carry = v1;
+            for (uint32_t k0 = 0; k0 < nd; k0 += kWave) {
+                const uint32_t k = k0 + l;
+                uint64_t d = k < nd ? lds_bits_be(base, pd, (uint64_t)k * w, w) : 0;
+                if (neg) d = 0 - d;
+                const uint64_t incl = wave_incl_scan_u64(d);
+                if (k < nd) put_val<EB>(out, limit, q + 2 + k, lp, (carry + incl) & vmask);
+                carry += shfl_u64(incl, kWave - 1);
+            }
+        }
+    } else {                                                    // PATCHED_BASE
+        const uint32_t bw = aux & 15, pw = (aux >> 4) & 127, pll = (aux >> 15) & 31,
+                       cfb = (aux >> 20) & 127;
+        const uint64_t bv = lds_bits_be(base, p, 0, 8 * bw);
+        const uint32_t pb = p + bw, pp = pb + (len * w + 7) / 8;
+        const uint64_t e = l < pll ? lds_bits_be(base, pp, (uint64_t)l * cfb, cfb) : 0;
+        const uint32_t gap = (l < pll && pw < 64) ? (uint32_t)(e >> pw) : 0;
+        const uint32_t pos = wave_incl_scan_u32(gap);
+        const uint64_t pv = pw >= 64 ? e : e & ((1ull << pw) - 1);
+        for (uint32_t i0 = 0; i0 < len; i0 += kWave) {
+            const uint32_t i = i0 + l;
+            uint64_t v = i < len ? lds_bits_be(base, pb, (uint64_t)i * w, w) : 0;
+            for (uint32_t k = 0; k < pll; ++k) {
+                const uint32_t pk = __shfl(pos, (int)k, kWave);
+                const uint64_t vk = shfl_u64(pv, (int)k);
+                if (pk == i) v |= w >= 64 ? 0 : vk << w;
+            }
+            if (i < len) put_val<EB>(out, limit, q + i, lp, (v + bv) & vmask);
+        }
+    }
+}
+
 struct Rlev2Tables {
-    uint32_t lbase[32];
-    uint32_t cl[32];
-    uint32_t err;        // min over lane << 8 | status byte
+    uint32_t lbase[32];      // LDS byte offset of each lane stream (16-byte aligned)
+    uint32_t cl[32];         // lane stream bytes
+    uint32_t W[32];          // lane stream 4-byte words
+    uint32_t lerr[32];       // first failing run of each lane: q << 8 | status byte
     uint32_t fits;
     uint32_t extent_ok;
-    uint32_t pad;
-    LaneTable lt;
+    uint32_t nruns;          // runs in the pool this epoch
+    uint32_t more;           // a lane stream has runs left after this epoch
+    uint32_t maxw;           // rounds of the interleave (longest lane's words)
+    uint32_t pad[3];
+    Run runs[kRunPool];
 };
 
 // Launch arguments of one RLE v2 container decode.
@@ -234,10 +370,42 @@ struct Rlev2Args {
     int32_t* status;
 };
 
+// Scatters the chunk's interleaved words into the lane regions.  Round r of
+// the interleave holds one word of every lane longer than r words, in lane
+// order, and starts at raw word R(r) = sum over lanes s of min(W_s, r).
+// Each half-wave takes one round at a time: thread t of the half computes
+// R(r) (a 32-lane sum), its rank among the active lanes (a ballot), loads
+// its word with a coalesced 4-byte load and stores it at word r of its
+// lane's region.
+MDZ_DEV void stage_rounds(const Rlev2Tables& tb, uint32_t* d32, const uint32_t* __restrict__ src,
+                          uint32_t tid, uint32_t nthreads) {
+    const uint32_t t = tid & 31, h = (tid >> 5) & 1, halves = nthreads / 32;
+    const uint32_t Wt = tb.W[t], lb = tb.lbase[t] >> 2, maxw = tb.maxw;
+    const uint32_t below = (1u << t) - 1u;
+    for (uint32_t r0 = tid >> 5; r0 < maxw; r0 += 4 * halves) {
+        uint32_t v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const uint32_t r = r0 + u * halves;
+            uint32_t R = min(Wt, r);
+#pragma unroll
+            for (int o = 1; o < 32; o <<= 1) R += __shfl_xor(R, o, kWave);
+            const uint32_t act = (uint32_t)(__ballot(Wt > r) >> (32 * h));
+            v[u] = Wt > r ? __builtin_nontemporal_load(src + R + __popc(act & below)) : 0u;
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const uint32_t r = r0 + u * halves;
+            if (Wt > r) d32[lb + r] = v[u];
+        }
+    }
+}
+
 // Decodes chunk c with the whole workgroup (kThreads threads); smem holds
 // data_cap + 16 bytes of staged payload followed by the tables.
 template <int EB>
 MDZ_DEV void rlev2_chunk(const Rlev2Args& a, uint64_t c, uint32_t* smem) {
+    constexpr uint64_t vmask = EB == 8 ? ~0ull : ((1ull << (8 * EB)) - 1);
     const uint8_t* __restrict__ payload = a.payload;
     const uint64_t* __restrict__ blk_off = a.blk_off;
     const uint64_t* __restrict__ col_len = a.col_len;
@@ -261,20 +429,25 @@ MDZ_DEV void rlev2_chunk(const Rlev2Args& a, uint64_t c, uint32_t* smem) {
     if (wave == 0) {
         const uint32_t t = l & 31;
         const uint32_t cl = (uint32_t)col_len[c * 32 + t];
-        lane_table_build(tb.lt, (cl + 3) / 4);
-        const uint32_t r16 = (cl + 15) & ~15u;
-        uint32_t incl = l < 32 ? r16 : 0;
+        const uint32_t W = (cl + 3) / 4, r16 = (cl + 15) & ~15u;
+        uint32_t incl = r16, tot = W, mx = W;
 #pragma unroll
         for (int o = 1; o < 32; o <<= 1) {
-            uint32_t y = __shfl_up(incl, o, kWave);
-            if ((l & 31) >= (uint32_t)o) incl += y;
+            const uint32_t y = __shfl_up(incl, o, kWave);
+            if (t >= (uint32_t)o) incl += y;
+            tot += __shfl_xor(tot, o, kWave);
+            mx = max(mx, (uint32_t)__shfl_xor(mx, o, kWave));
         }
-        if (l < 32) { tb.lbase[t] = incl - r16; tb.cl[t] = cl; }
+        if (l < 32) {
+            tb.lbase[t] = incl - r16;
+            tb.cl[t] = cl;
+            tb.W[t] = W;
+            tb.lerr[t] = 0xffffffffu;
+        }
         if (l == 31) tb.fits = incl <= data_cap;
         if (l == 0) {
-            tb.err = 0xffffffffu;
-            tb.extent_ok = !(c_off & 15) && c_end >= c_off &&
-                           (uint64_t)tb.lt.total * 4 <= c_end - c_off;
+            tb.maxw = mx;
+            tb.extent_ok = !(c_off & 15) && c_end >= c_off && (uint64_t)tot * 4 <= c_end - c_off;
         }
     }
     __syncthreads();
@@ -285,24 +458,88 @@ MDZ_DEV void rlev2_chunk(const Rlev2Args& a, uint64_t c, uint32_t* smem) {
         return;
     }
     MDZ_PHASE(0);
-    stage_chunk_lanes(tb.lt, tb.lbase, smem, reinterpret_cast<const u32x4*>(payload + c_off), tid,
-                      kThreads);
+    stage_rounds(tb, smem, reinterpret_cast<const uint32_t*>(payload + c_off), tid, kThreads);
     __syncthreads();
     MDZ_PHASE(1);
 
     const uint32_t sh = (ru & (ru - 1)) ? 32u : (uint32_t)__builtin_ctz(ru);
-    for (uint32_t t = wave; t < 32; t += kThreads / kWave) {
-        const int32_t e = rlev2_lane_wave<EB>(data + tb.lbase[t], tb.cl[t], nvals, dst + out_base,
-                                              limit, LanePlace{t, ru, sh});
-        if (e != MDZ_ST_OK && l == 0) atomicMin(&tb.err, (t << 8) | ((uint32_t)e & 0xffu));
+    uint8_t* const out = dst + out_base;
+    // header-chain state of lane stream tid (threads 0..31 of wave 0)
+    uint32_t wp = 0, wq = 0;
+    bool done = tid >= 32;
+    for (;;) {
+        if (wave == 0) {
+            // the walk is the chunk's critical path: raise this wave's issue
+            // priority over the workgroup's value waves while it runs
+            __builtin_amdgcn_s_setprio(2);
+            const uint8_t* const base = data + tb.lbase[l & 31];
+            const uint32_t end = tb.cl[l & 31];
+            uint32_t n = 0;
+            for (;;) {
+                const bool has = !done && wq < nvals;
+                uint32_t size = 0, cnt = 0;
+                bool stop = true;
+                if (has) chain_step(base, wp, end, size, cnt, stop);
+                const uint64_t want = __ballot(has);
+                if (!want) break;
+                const uint32_t slot = n + __builtin_amdgcn_mbcnt_hi((uint32_t)(want >> 32),
+                                                                    __builtin_amdgcn_mbcnt_lo((uint32_t)want, 0u));
+                if (has && slot < kRunPool) {
+                    // every run the chain reaches is recorded, a malformed one
+                    // too: the check pass finds its status
+                    *reinterpret_cast<uint4*>(&tb.runs[slot]) = make_uint4(wp, wq, tid << 23, 0u);
+                    wp += size;
+                    wq += cnt;
+                    done = stop;
+                }
+                n += (uint32_t)__popcll(want);
+                if (n >= kRunPool) { n = kRunPool; break; }
+            }
+            __builtin_amdgcn_s_setprio(0);
+            done = done || wq >= nvals;
+            const uint64_t left = __ballot(!done);
+            if (l == 0) {
+                tb.nruns = n;
+                tb.more = left != 0;
+            }
+        }
+        __syncthreads();
+        MDZ_PHASE(2);
+        const uint32_t nruns = tb.nruns, more = tb.more;
+        // check pass, one run per thread: the oracle's checks in its order
+        // (patch lists included) fill in the descriptor; a lane's first
+        // failing run (lowest q) ends it
+        for (uint32_t i = tid; i < nruns; i += kThreads) {
+            Run& rr = tb.runs[i];
+            const uint32_t t = rr.meta >> 23, q = rr.q;
+            const uint8_t* const base = data + tb.lbase[t];
+            const uint32_t end = tb.cl[t];
+            uint32_t pp = rr.p;
+            Run r;
+            int32_t e = walk_run(base, end, pp, q, nvals, t, r);
+            if (e == MDZ_ST_OK && (r.meta & 3) == 2) e = patch_list_check(base, end, r);
+            if (e != MDZ_ST_OK) atomicMin(&tb.lerr[t], q << 8 | ((uint32_t)e & 0xffu));
+            else rr = r;
+        }
+        __syncthreads();
+        for (uint32_t r = wave; r < nruns; r += kThreads / kWave) {
+            const Run& rr = tb.runs[r];
+            const uint32_t t = rr.meta >> 23;
+            if (rr.q >= (tb.lerr[t] >> 8)) continue;        // at or after its lane's first error
+            run_values<EB>(rr, data, tb.lbase, out, limit, ru, sh);
+        }
+        __syncthreads();
+        MDZ_PHASE(3);
+        if (!more) break;
     }
-    MDZ_PHASE(2);
-    __syncthreads();
     if (tid == 0 && status) {
-        const uint32_t e = tb.err;
-        status[c] = e == 0xffffffffu ? MDZ_ST_OK : (int32_t)(int8_t)(e & 0xff);
+        int32_t st = MDZ_ST_OK;
+        for (uint32_t t = 0; t < 32; ++t) {
+            const uint32_t e = tb.lerr[t];
+            if (e != 0xffffffffu) { st = (int32_t)(int8_t)(e & 0xff); break; }
+        }
+        status[c] = st;
     }
-    MDZ_PHASE(3);
 }
 
 }  // namespace

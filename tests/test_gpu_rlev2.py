@@ -87,3 +87,32 @@ def test_host_wrapper(gpu):
     c = O.rlev2_encode(x, 65536, 128, threads=8)
     out = rlev2.decompress_gpu(c.payload.tobytes(), c.n_chunks, c.blk_off, c.col_len, 65536, 128, 8)
     assert out == x.tobytes()
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_random_corruption_matches_oracle(gpu, seed):
+    # one random byte overwritten in most chunks (header or data) and a few
+    # lane lengths changed: every chunk's status matches the oracle, and
+    # chunks the oracle decodes cleanly are bit-exact
+    x = O.gen_rlev2_i64(1 << 18, 40 + seed)
+    c = O.rlev2_encode(x, 32768, 64, threads=8)
+    rng = np.random.default_rng(seed)
+    pay = c.payload.copy()
+    col = c.col_len.copy()
+    for ch in range(c.n_chunks):
+        lo = int(c.blk_off[ch])
+        hi = int(c.blk_off[ch + 1]) if ch + 1 < c.n_chunks else c.payload_bytes
+        if rng.random() < 0.8 and hi > lo:
+            pay[lo + int(rng.integers(0, hi - lo))] = int(rng.integers(0, 256))
+        if rng.random() < 0.2:
+            t = ch * 32 + int(rng.integers(0, 32))
+            col[t] = max(0, int(col[t]) - int(rng.integers(1, 40)))
+    got, st = _run(gpu, c, payload=pay, col_len=col)
+    ref = O.RleV2Container(pay, col, c.blk_off, c.n_chunks, c.chunk, c.elem_bytes, c.read_unit,
+                           c.raw_bytes, c.payload_bytes)
+    _, ref_out, ref_st = O.rlev2_decode(ref, threads=4)
+    assert np.array_equal(st, ref_st)
+    assert (st != 0).any()
+    for ch in np.flatnonzero(ref_st == 0):
+        lo, hi = ch * c.chunk, min((ch + 1) * c.chunk, c.raw_bytes)
+        assert np.array_equal(got[lo:hi], ref_out[lo:hi]), ch
