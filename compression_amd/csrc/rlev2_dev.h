@@ -29,6 +29,8 @@
 //   kRunPool runs).  Lane t's q-th value is element
 //   (q / RU) * 32 * RU + t * RU + q % RU of the chunk, so consecutive SIMD
 //   lanes store consecutive elements.
+#include <type_traits>
+
 #include "mdz_device.h"
 #include "mdz_internal.h"
 
@@ -103,11 +105,20 @@ MDZ_DEV void put_val(uint8_t* out, uint64_t limit, uint32_t q, const LanePlace& 
 }
 
 constexpr uint32_t kRunPool = 768;              // run descriptors per epoch
+constexpr uint32_t kSegBlocks = 160 * 1024 / 128;  // 32-word blocks of the largest LDS payload
 
 // Bit width of a 5-bit width code (the reference's FixedBitSizes decode):
 // 1..24, then 26, 28, 30, 32, then 40, 48, 56, 64.
 MDZ_DEV uint32_t decode_width(uint32_t c) {
-    return c < 24 ? c + 1 : c < 28 ? 26 + 2 * (c - 24) : 40 + 8 * (c - 28);
+    // masks instead of compares: c >= 24 iff c + 8 >= 32, c >= 28 iff c + 4 >= 32
+    const uint32_t ge24 = 0u - ((c + 8) >> 5), ge28 = 0u - ((c + 4) >> 5);
+    return c + 1 + ((c - 23) & ge24) + ((6 * c - 162) & ge28);
+}
+
+// closest_bits() as mask arithmetic (the walk keeps it free of branches).
+MDZ_DEV uint32_t closest_bits_sel(uint32_t b) {
+    const uint32_t gt24 = 0u - (uint32_t)(b > 24), gt32 = 0u - (uint32_t)(b > 32);
+    return max(b, 1u) + ((b & 1) & gt24 & ~gt32) + (((8 - (b & 7)) & 7) & gt32);
 }
 
 // Byte length (1..10) of the unsigned LEB128 at byte p, or 0 with *err as
@@ -232,17 +243,21 @@ MDZ_DEV void chain_step(const uint8_t* base, uint32_t p, uint32_t end, uint32_t&
     } else if (kind == 2) {                                  // PATCHED_BASE
         const uint32_t bw = ((b2 >> 5) & 7) + 1, pw = decode_width(b2 & 31);
         const uint32_t pgw = ((b3 >> 5) & 7) + 1, pll = b3 & 31;
-        sz = 4 + bw + vbytes + (pll * closest_bits(pw + pgw) + 7) / 8;
+        sz = 4 + bw + vbytes + (pll * closest_bits_sel(pw + pgw) + 7) / 8;
     } else {                                                 // DELTA: two LEB128 varints
         // terminator bytes (bit 7 clear) of bytes 0..23, one bit each
-        auto term4 = [](uint32_t u) { return (((~u >> 7) & 0x01010101u) * 0x01020408u) >> 24; };
+        // (bits 0, 8, 16, 24 of t gathered to bits 0..3)
+        auto term4 = [](uint32_t u) {
+            const uint32_t t = (~u >> 7) & 0x01010101u;
+            return (t | (t >> 7) | (t >> 14) | (t >> 21)) & 0xfu;
+        };
         const uint32_t tm = term4(u0) | term4(u1) << 4 | term4(u2) << 8 | term4(u3) << 12 |
                             term4(u4) << 16 | term4(u5) << 20;
         const uint32_t t1 = (tm >> 2) & 0x3ff;
-        const uint32_t n1 = t1 ? (uint32_t)__builtin_ctz(t1) + 1 : 0;
+        const uint32_t n1 = (uint32_t)__builtin_ctz(t1 | 0x400) + 1;      // 11 when absent
         const uint32_t t2 = (tm >> (2 + n1)) & 0x3ff;
-        const uint32_t n2 = t2 ? (uint32_t)__builtin_ctz(t2) + 1 : 0;
-        bad = !n1 || !n2 || len < 2;
+        const uint32_t n2 = (uint32_t)__builtin_ctz(t2 | 0x400) + 1;
+        bad = t1 == 0 || t2 == 0 || len < 2;
         sz = 2 + n1 + n2 + (code ? ((len - 2) * w + 7) / 8 : 0);
     }
     size = sz;
@@ -269,76 +284,130 @@ MDZ_DEV int32_t patch_list_check(const uint8_t* base, uint32_t end, const Run& r
     return MDZ_ST_OK;
 }
 
-// Decodes run r (all 64 lanes of one wave; r is wave-uniform).
+// Wave-uniform LDS reads: every lane reads the same address and the result
+// goes to scalar registers, so the arithmetic on it is scalar.
+MDZ_DEV uint32_t lds_u32_uniform(const uint8_t* data, uint32_t word) {
+    return __builtin_amdgcn_readfirstlane(reinterpret_cast<const uint32_t*>(data)[word]);
+}
+// 8 little-endian bytes at byte p (uniform p).
+MDZ_DEV uint64_t lds_le64_uniform(const uint8_t* data, uint32_t p) {
+    const uint32_t w = p >> 2, k = (p & 3) * 8;
+    const uint64_t lo = lds_u32_uniform(data, w) | (uint64_t)lds_u32_uniform(data, w + 1) << 32;
+    const uint64_t hi = lds_u32_uniform(data, w + 2);
+    return k ? (lo >> k) | (hi << (64 - k)) : lo;
+}
+// Big-endian value of the nb (1..8) bytes at byte p (uniform p).
+MDZ_DEV uint64_t lds_be_uniform(const uint8_t* data, uint32_t p, uint32_t nb) {
+    return __builtin_bswap64(lds_le64_uniform(data, p)) >> (64 - 8 * nb);
+}
+// Unsigned LEB128 of n (1..10) bytes at byte p (uniform p).
+MDZ_DEV uint64_t uvarint_uniform(const uint8_t* data, uint32_t p, uint32_t n) {
+    uint64_t v = gather7(lds_le64_uniform(data, p));
+    if (n < 8) {
+        v &= (1ull << (7 * n)) - 1;
+    } else if (n > 8) {
+        const uint64_t hi = lds_le64_uniform(data, p + 8);
+        const uint64_t h = ((hi & 0x7f) | ((hi >> 1) & (0x7full << 7))) & ((1ull << (7 * (n - 8))) - 1);
+        v |= h << 56;
+    }
+    return v;
+}
+
+// MSB-first field of w bits (1..64) starting `bit` bits after byte b0 of the
+// 4-byte-aligned LDS array (32-bit bit offset; reads up to 12 bytes past the
+// field start).
+MDZ_DEV uint64_t lds_bits_be32(const uint8_t* base, uint32_t b0, uint32_t bit, uint32_t w) {
+    const uint32_t b = b0 + (bit >> 3), s = bit & 7, k = b & 3;
+    const uint32_t* d = reinterpret_cast<const uint32_t*>(base) + (b >> 2);
+    const uint32_t d0 = d[0], d1 = d[1], d2 = d[2];
+    const uint32_t lo = alignbyte(d1, d0, k), mid = alignbyte(d2, d1, k);
+    const uint64_t be = ((uint64_t)__builtin_bswap32(lo) << 32) | __builtin_bswap32(mid);
+    const uint64_t v = s ? (be << s) | (((d2 >> (8 * k)) & 0xffu) >> (8 - s)) : be;
+    return w >= 64 ? v : v >> (64 - w);
+}
+
+// Decodes run r (all 64 lanes of one wave; r is wave-uniform, r.p an
+// absolute LDS byte offset).  A run whose values lie in one READ_UNIT line
+// and inside the output is stored through a per-run pointer; otherwise each
+// value is placed and clipped on its own.
 template <int EB>
-MDZ_DEV void run_values(const Run& rr, const uint8_t* data, const uint32_t* lbase, uint8_t* out,
-                        uint64_t limit, uint32_t ru, uint32_t sh) {
+MDZ_DEV void run_values(const Run& rr, const uint8_t* data, uint8_t* out, uint64_t limit, uint32_t ru,
+                        uint32_t sh) {
     constexpr uint64_t vmask = EB == 8 ? ~0ull : ((1ull << (8 * EB)) - 1);
+    typedef typename std::conditional<EB == 8, uint64_t,
+            typename std::conditional<EB == 4, uint32_t,
+            typename std::conditional<EB == 2, uint16_t, uint8_t>::type>::type>::type elem_t;
     const uint32_t l = lane_id();
     const uint32_t meta = __builtin_amdgcn_readfirstlane(rr.meta);
     const uint32_t aux = __builtin_amdgcn_readfirstlane(rr.aux);
     const uint32_t p = __builtin_amdgcn_readfirstlane(rr.p), q = __builtin_amdgcn_readfirstlane(rr.q);
     const uint32_t kind = meta & 3, code = (meta >> 2) & 31, w = (meta >> 7) & 127;
     const uint32_t len = ((meta >> 14) & 511) + 1, t = meta >> 23;
-    const uint8_t* const base = data + lbase[t];
     const LanePlace lp{t, ru, sh};
+    elem_t* o = nullptr;
+    if (sh < 32) {
+        const uint32_t in_line = q & (ru - 1);
+        const uint64_t e0 = ((uint64_t)((q >> sh) * 32 + t) << sh) | in_line;
+        if (in_line + len <= ru && (e0 + len) * EB <= limit) o = reinterpret_cast<elem_t*>(out) + e0;
+    }
+    auto put = [&](uint32_t i, uint64_t v) {
+        if (o) o[i] = (elem_t)v;
+        else put_val<EB>(out, limit, q + i, lp, v);
+    };
     if (kind == 0) {                                            // SHORT_REPEAT
-        const uint64_t v = lds_bits_be(base, p, 0, 8 * aux) & vmask;
-        if (l < len) put_val<EB>(out, limit, q + l, lp, v);
+        const uint64_t v = lds_be_uniform(data, p, aux) & vmask;
+        if (l < len) put(l, v);
     } else if (kind == 1) {                                     // DIRECT
         for (uint32_t i0 = 0; i0 < len; i0 += kWave) {
             const uint32_t i = i0 + l;
-            if (i < len) put_val<EB>(out, limit, q + i, lp, lds_bits_be(base, p, (uint64_t)i * w, w) & vmask);
+            if (i < len) put(i, lds_bits_be32(data, p, i * w, w) & vmask);
         }
     } else if (kind == 3) {                                     // DELTA
         const uint32_t n1 = aux & 15, n2 = aux >> 4;
-        const uint64_t a = gather7(lds_peek8(base, p)), zd = gather7(lds_peek8(base, p + n1));
-        const uint64_t hi1 = n1 > 8 ? (lds_peek8(base, p + 8) & 0x7f) | ((lds_peek8(base, p + 8) >> 1) & (0x7full << 7)) : 0;
-        const uint64_t hi2 = n2 > 8 ? (lds_peek8(base, p + n1 + 8) & 0x7f) | ((lds_peek8(base, p + n1 + 8) >> 1) & (0x7full << 7)) : 0;
-        const uint64_t bv = (n1 < 8 ? a & ((1ull << (7 * n1)) - 1) : a) |
-                            (n1 > 8 ? (hi1 & ((1ull << (7 * (n1 - 8))) - 1)) << 56 : 0);
-        const uint64_t zz = (n2 < 8 ? zd & ((1ull << (7 * n2)) - 1) : zd) |
-                            (n2 > 8 ? (hi2 & ((1ull << (7 * (n2 - 8))) - 1)) << 56 : 0);
-        const uint64_t v0 = bv & vmask, dt = ((zz >> 1) ^ (0 - (zz & 1))) & vmask;
+        const uint64_t v0 = uvarint_uniform(data, p, n1) & vmask;
+        const uint64_t zz = uvarint_uniform(data, p + n1, n2);
+        const uint64_t dt = ((zz >> 1) ^ (0 - (zz & 1))) & vmask;
         if (code == 0) {
             for (uint32_t i0 = 0; i0 < len; i0 += kWave) {
                 const uint32_t i = i0 + l;
-                if (i < len) put_val<EB>(out, limit, q + i, lp, (v0 + (uint64_t)i * dt) & vmask);
+                if (i < len) put(i, (v0 + (uint64_t)i * dt) & vmask);
             }
         } else {
             const uint32_t nd = len - 2, pd = p + n1 + n2;
             const uint64_t v1 = (v0 + dt) & vmask;
-            if (l < 2) put_val<EB>(out, limit, q + l, lp, l ? v1 : v0);
+            if (l < 2) put(l, l ? v1 : v0);
             // multiplier = (DATA_TYPE)delta > 0 ? 1 : -1 on the unsigned type
             const bool neg = dt == 0;
             uint64_t carry = v1;
             for (uint32_t k0 = 0; k0 < nd; k0 += kWave) {
                 const uint32_t k = k0 + l;
-                uint64_t d = k < nd ? lds_bits_be(base, pd, (uint64_t)k * w, w) : 0;
+                uint64_t d = k < nd ? lds_bits_be32(data, pd, k * w, w) : 0;
                 if (neg) d = 0 - d;
                 const uint64_t incl = wave_incl_scan_u64(d);
-                if (k < nd) put_val<EB>(out, limit, q + 2 + k, lp, (carry + incl) & vmask);
+                if (k < nd) put(2 + k, (carry + incl) & vmask);
                 carry += shfl_u64(incl, kWave - 1);
             }
         }
     } else {                                                    // PATCHED_BASE
         const uint32_t bw = aux & 15, pw = (aux >> 4) & 127, pll = (aux >> 15) & 31,
                        cfb = (aux >> 20) & 127;
-        const uint64_t bv = lds_bits_be(base, p, 0, 8 * bw);
+        const uint64_t bv = lds_be_uniform(data, p, bw);
         const uint32_t pb = p + bw, pp = pb + (len * w + 7) / 8;
-        const uint64_t e = l < pll ? lds_bits_be(base, pp, (uint64_t)l * cfb, cfb) : 0;
+        const uint64_t e = l < pll ? lds_bits_be32(data, pp, l * cfb, cfb) : 0;
         const uint32_t gap = (l < pll && pw < 64) ? (uint32_t)(e >> pw) : 0;
         const uint32_t pos = wave_incl_scan_u32(gap);
         const uint64_t pv = pw >= 64 ? e : e & ((1ull << pw) - 1);
         for (uint32_t i0 = 0; i0 < len; i0 += kWave) {
             const uint32_t i = i0 + l;
-            uint64_t v = i < len ? lds_bits_be(base, pb, (uint64_t)i * w, w) : 0;
+            uint64_t v = i < len ? lds_bits_be32(data, pb, i * w, w) : 0;
             for (uint32_t k = 0; k < pll; ++k) {
-                const uint32_t pk = __shfl(pos, (int)k, kWave);
-                const uint64_t vk = shfl_u64(pv, (int)k);
+                const uint32_t pk = __builtin_amdgcn_readlane(pos, k);
+                const uint32_t vlo = (uint32_t)__builtin_amdgcn_readlane((uint32_t)pv, k);
+                const uint32_t vhi = (uint32_t)__builtin_amdgcn_readlane((uint32_t)(pv >> 32), k);
+                const uint64_t vk = (uint64_t)vhi << 32 | vlo;
                 if (pk == i) v |= w >= 64 ? 0 : vk << w;
             }
-            if (i < len) put_val<EB>(out, limit, q + i, lp, (v + bv) & vmask);
+            if (i < len) put(i, (v + bv) & vmask);
         }
     }
 }
@@ -346,16 +415,30 @@ MDZ_DEV void run_values(const Run& rr, const uint8_t* data, const uint32_t* lbas
 struct Rlev2Tables {
     uint32_t lbase[32];      // LDS byte offset of each lane stream (16-byte aligned)
     uint32_t cl[32];         // lane stream bytes
-    uint32_t W[32];          // lane stream 4-byte words
     uint32_t lerr[32];       // first failing run of each lane: q << 8 | status byte
+    // The interleave in segments: segment k holds rounds sW[k-1] .. sW[k]-1,
+    // in which the 32 - k lanes of mask M[k] are active; it starts at raw
+    // word S[k].
+    uint32_t sW[32];         // lane word counts, ascending
+    uint32_t S[32];
+    uint32_t M[32];
     uint32_t fits;
     uint32_t extent_ok;
     uint32_t nruns;          // runs in the pool this epoch
     uint32_t more;           // a lane stream has runs left after this epoch
-    uint32_t maxw;           // rounds of the interleave (longest lane's words)
+    uint32_t total;          // words of all lanes
     uint32_t pad[3];
-    Run runs[kRunPool];
+    union {
+        Run runs[kRunPool];
+        struct {                 // staging only
+            uint32_t dest[32][32];   // segment k, j-th active lane -> its LDS word base + the
+                                     // segment's first round
+            uint8_t segof[kSegBlocks];  // segment of the first raw word of each 32-word block
+        } st;
+    };
 };
+static_assert(sizeof(Run) * kRunPool >= sizeof(uint32_t) * 32 * 32 + kSegBlocks,
+              "staging tables fit the run pool");
 
 // Launch arguments of one RLE v2 container decode.
 struct Rlev2Args {
@@ -370,33 +453,61 @@ struct Rlev2Args {
     int32_t* status;
 };
 
-// Scatters the chunk's interleaved words into the lane regions.  Round r of
-// the interleave holds one word of every lane longer than r words, in lane
-// order, and starts at raw word R(r) = sum over lanes s of min(W_s, r).
-// Each half-wave takes one round at a time: thread t of the half computes
-// R(r) (a 32-lane sum), its rank among the active lanes (a ballot), loads
-// its word with a coalesced 4-byte load and stores it at word r of its
-// lane's region.
-MDZ_DEV void stage_rounds(const Rlev2Tables& tb, uint32_t* d32, const uint32_t* __restrict__ src,
-                          uint32_t tid, uint32_t nthreads) {
-    const uint32_t t = tid & 31, h = (tid >> 5) & 1, halves = nthreads / 32;
-    const uint32_t Wt = tb.W[t], lb = tb.lbase[t] >> 2, maxw = tb.maxw;
-    const uint32_t below = (1u << t) - 1u;
-    for (uint32_t r0 = tid >> 5; r0 < maxw; r0 += 4 * halves) {
-        uint32_t v[4];
+// Segment of raw word w: the number of segment starts S[1..31] at or below
+// w (the starts ascend, so this is the last segment starting at or below w).
+MDZ_DEV uint32_t segment_of(const uint32_t* S, uint32_t w) {
+    uint32_t k = 0;
+#pragma unroll
+    for (int j = 1; j < 32; ++j) k += w >= S[j];
+    return k;
+}
+
+// Scatters the chunk's interleaved words into the lane regions with 16-byte
+// loads: raw word w of segment k is the j-th active lane's word of round
+// sW[k-1] + (w - S[k]) / (32 - k), j = (w - S[k]) mod (32 - k), stored at
+// dest[k][j] + (w - S[k]) / (32 - k).  A group's segment comes from the
+// table of 32-word blocks; the quotient is found once per group and then
+// stepped word by word.
+MDZ_DEV void stage_lanes(const Rlev2Tables& tb, uint32_t* d32, const u32x4* __restrict__ src,
+                         uint32_t tid, uint32_t nthreads) {
+    const uint32_t total = tb.total, ngroups = (total + 3) / 4;
+    auto next_start = [&](uint32_t k) { return k < 31 ? tb.S[k + 1] : total; };
+    for (uint32_t g0 = tid; g0 < ngroups; g0 += 4 * nthreads) {
+        u32x4 v[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
-            const uint32_t r = r0 + u * halves;
-            uint32_t R = min(Wt, r);
-#pragma unroll
-            for (int o = 1; o < 32; o <<= 1) R += __shfl_xor(R, o, kWave);
-            const uint32_t act = (uint32_t)(__ballot(Wt > r) >> (32 * h));
-            v[u] = Wt > r ? __builtin_nontemporal_load(src + R + __popc(act & below)) : 0u;
+            const uint32_t g = g0 + u * nthreads;
+            if (g < ngroups) v[u] = __builtin_nontemporal_load(src + g);
         }
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
-            const uint32_t r = r0 + u * halves;
-            if (Wt > r) d32[lb + r] = v[u];
+            const uint32_t g = g0 + u * nthreads;
+            if (g >= ngroups) break;
+            const uint32_t w0 = 4 * g;
+            uint32_t k = tb.st.segof[g >> 3];
+            uint32_t sn = next_start(k);
+            while (w0 >= sn) sn = next_start(++k);
+            uint32_t nk = 32 - k, off = w0 - tb.S[k];
+            uint32_t r = (uint32_t)((float)off * __builtin_amdgcn_rcpf((float)nk));
+            if (r * nk > off) --r;
+            else if ((r + 1) * nk <= off) ++r;
+            uint32_t j = off - r * nk;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const uint32_t w = w0 + e;
+                if (w >= total) break;
+                if (w >= sn) {                       // the group crosses a segment start
+                    while (w >= sn) sn = next_start(++k);
+                    nk = 32 - k;
+                    off = w - tb.S[k];
+                    r = (uint32_t)((float)off * __builtin_amdgcn_rcpf((float)nk));
+                    if (r * nk > off) --r;
+                    else if ((r + 1) * nk <= off) ++r;
+                    j = off - r * nk;
+                }
+                d32[tb.st.dest[k][j] + r] = v[u][e];
+                if (++j == nk) { j = 0; ++r; }
+            }
         }
     }
 }
@@ -438,17 +549,39 @@ MDZ_DEV void rlev2_chunk(const Rlev2Args& a, uint64_t c, uint32_t* smem) {
             tot += __shfl_xor(tot, o, kWave);
             mx = max(mx, (uint32_t)__shfl_xor(mx, o, kWave));
         }
+        // sort the lane lengths: rank = lanes shorter, ties to the lower lane
+        uint32_t rk = 0;
+#pragma unroll
+        for (int u = 0; u < 32; ++u) {
+            const uint32_t wu = __builtin_amdgcn_readlane(W, u);
+            rk += (wu < W) || (wu == W && (uint32_t)u < t);
+        }
         if (l < 32) {
             tb.lbase[t] = incl - r16;
             tb.cl[t] = cl;
-            tb.W[t] = W;
             tb.lerr[t] = 0xffffffffu;
+            tb.sW[rk] = W;
         }
+        // segment k's active lanes: rank >= k
+        uint32_t mk = 0;
+#pragma unroll
+        for (int k = 0; k < 32; ++k) {
+            const uint32_t m = (uint32_t)__ballot(l < 32 && rk >= (uint32_t)k);
+            mk = t == (uint32_t)k ? m : mk;
+        }
+        wave_sync();
+        // S[k] = raw words before round sW[k-1] = sum over lanes of min(W, sW[k-1])
+        const uint32_t prev = t ? tb.sW[t - 1] : 0;
+        uint32_t sk = 0;
+#pragma unroll
+        for (int u = 0; u < 32; ++u) sk += min((uint32_t)__builtin_amdgcn_readlane(W, u), prev);
+        if (l < 32) { tb.S[t] = sk; tb.M[t] = mk; }
         if (l == 31) tb.fits = incl <= data_cap;
         if (l == 0) {
-            tb.maxw = mx;
+            tb.total = tot;
             tb.extent_ok = !(c_off & 15) && c_end >= c_off && (uint64_t)tot * 4 <= c_end - c_off;
         }
+        (void)mx;
     }
     __syncthreads();
     if (!tb.extent_ok || !tb.fits) {
@@ -457,8 +590,21 @@ MDZ_DEV void rlev2_chunk(const Rlev2Args& a, uint64_t c, uint32_t* smem) {
         if (tid == 0 && status) status[c] = tb.extent_ok ? MDZ_ST_OUTPUT_OVERRUN : MDZ_ST_INPUT_OVERRUN;
         return;
     }
+    for (uint32_t x = tid; x < 32 * 32; x += kThreads) {
+        const uint32_t k = x >> 5, jj = x & 31;
+        if (jj < 32 - k)
+            tb.st.dest[k][jj] = (tb.lbase[select_bit(tb.M[k], jj)] >> 2) + (k ? tb.sW[k - 1] : 0);
+    }
+    {
+        uint32_t S[32];
+#pragma unroll
+        for (int j = 0; j < 32; ++j) S[j] = __builtin_amdgcn_readfirstlane(tb.S[j]);
+        for (uint32_t b = tid; b < (tb.total + 31) / 32; b += kThreads)
+            tb.st.segof[b] = (uint8_t)segment_of(S, 32 * b);
+    }
+    __syncthreads();
     MDZ_PHASE(0);
-    stage_rounds(tb, smem, reinterpret_cast<const uint32_t*>(payload + c_off), tid, kThreads);
+    stage_lanes(tb, smem, reinterpret_cast<const u32x4*>(payload + c_off), tid, kThreads);
     __syncthreads();
     MDZ_PHASE(1);
 
@@ -518,15 +664,19 @@ MDZ_DEV void rlev2_chunk(const Rlev2Args& a, uint64_t c, uint32_t* smem) {
             Run r;
             int32_t e = walk_run(base, end, pp, q, nvals, t, r);
             if (e == MDZ_ST_OK && (r.meta & 3) == 2) e = patch_list_check(base, end, r);
-            if (e != MDZ_ST_OK) atomicMin(&tb.lerr[t], q << 8 | ((uint32_t)e & 0xffu));
-            else rr = r;
+            if (e != MDZ_ST_OK) {
+                atomicMin(&tb.lerr[t], q << 8 | ((uint32_t)e & 0xffu));
+            } else {
+                r.p += tb.lbase[t];                      // absolute LDS offset for the value pass
+                rr = r;
+            }
         }
         __syncthreads();
         for (uint32_t r = wave; r < nruns; r += kThreads / kWave) {
             const Run& rr = tb.runs[r];
             const uint32_t t = rr.meta >> 23;
             if (rr.q >= (tb.lerr[t] >> 8)) continue;        // at or after its lane's first error
-            run_values<EB>(rr, data, tb.lbase, out, limit, ru, sh);
+            run_values<EB>(rr, data, out, limit, ru, sh);
         }
         __syncthreads();
         MDZ_PHASE(3);
