@@ -10,11 +10,13 @@
 //     bytes 4l..4l+3); the 64-bit bit buffer is refilled from it with
 //     readlane, and the next window is prefetched when one is entered;
 //   * Huffman decode: 10-bit (literal/length) and 8-bit (distance) lookup
-//     tables in LDS hold, per bit pattern, the code length, the symbol's
-//     base value and extra-bit count; longer codes take a canonical decode
+//     tables hold, per bit pattern, the code length, the symbol's base value
+//     and extra-bit count; longer codes take a canonical decode
 //     (count/symbol tables), which also rejects codes an incomplete set does
-//     not define.  Tables are built by the whole wave from the code lengths
-//     (ballot-ranked canonical codes, strided table fill);
+//     not define.  Tables are built in LDS by the whole wave from the code
+//     lengths (ballot-ranked canonical codes, strided table fill), then
+//     copied to 20 registers per lane so that each lookup is an indexed
+//     register move and a readlane;
 //   * LZ77: symbols are decoded in batches of up to 64 (one record per lane,
 //     at most ~4 KiB of output per batch); the wave then writes the batch's
 //     literals and copies its matches, in order, within a 36.5 KiB LDS ring
@@ -233,6 +235,14 @@ MDZ_DEV int slow_decode(const Code& c, Reader& r) {
 
 MDZ_DEV uint32_t ring_at(uint64_t p) { return (uint32_t)(p % kRing); }
 
+// Entry idx (wave-uniform) of a lookup table held in registers, entry i in
+// lane i % 64 of v[i / 64].
+template <uint32_t N>
+MDZ_DEV uint32_t lut_pick(const uint32_t (&v)[N], uint32_t idx) {
+    const uint32_t e = __builtin_amdgcn_readfirstlane(idx >> 6);
+    return (uint32_t)__builtin_amdgcn_readlane(v[e], idx & 63);
+}
+
 __global__ __launch_bounds__(64) void inflate_kernel(const mdz_inflate_in_t* __restrict__ ins,
                                                      mdz_inflate_status_t* __restrict__ outs,
                                                      const uint32_t* __restrict__ order,
@@ -287,15 +297,22 @@ __global__ __launch_bounds__(64) void inflate_kernel(const mdz_inflate_in_t* __r
             const bool aligned = (((uintptr_t)dst + flushed) & 3) == 0;
             if (aligned && !all) lim &= ~3ull;
             if (lim <= flushed) return;
+            // at most one batch (< kRing bytes) is pending: ring indices are
+            // the flush start's plus an offset, wrapped once
+            const uint32_t rf = ring_at(flushed);
+            auto at = [&](uint64_t p) {
+                const uint32_t x = rf + (uint32_t)(p - flushed);
+                return x >= kRing ? x - kRing : x;
+            };
             if (aligned) {
                 const uint64_t nw = (lim - flushed) / 4;
                 for (uint64_t w = l; w < nw; w += 64) {
                     const uint64_t p = flushed + 4 * w;
-                    *reinterpret_cast<uint32_t*>(dst + p) = *reinterpret_cast<const uint32_t*>(&S.ring[ring_at(p)]);
+                    *reinterpret_cast<uint32_t*>(dst + p) = *reinterpret_cast<const uint32_t*>(&S.ring[at(p)]);
                 }
-                for (uint64_t p = flushed + 4 * nw + l; p < lim; p += 64) dst[p] = S.ring[ring_at(p)];
+                for (uint64_t p = flushed + 4 * nw + l; p < lim; p += 64) dst[p] = S.ring[at(p)];
             } else {
-                for (uint64_t p = flushed + l; p < lim; p += 64) dst[p] = S.ring[ring_at(p)];
+                for (uint64_t p = flushed + l; p < lim; p += 64) dst[p] = S.ring[at(p)];
             }
             flushed = lim;
         };
@@ -392,14 +409,22 @@ __global__ __launch_bounds__(64) void inflate_kernel(const mdz_inflate_in_t* __r
                 }
             }
 
-            // ---- symbols, in batches
+            // ---- symbols, in batches.  The two lookup tables are copied into
+            // registers (entry i in lane i % 64 of register i / 64): a lookup
+            // with the wave-uniform bit pattern is then an indexed register
+            // move and a readlane, with no LDS round trip on the decode chain.
+            uint32_t litv[(1u << kLitBits) / 64], distv[(1u << kDistBits) / 64];
+#pragma unroll
+            for (uint32_t e = 0; e < (1u << kLitBits) / 64; ++e) litv[e] = S.lit_lut[e * 64 + l];
+#pragma unroll
+            for (uint32_t e = 0; e < (1u << kDistBits) / 64; ++e) distv[e] = S.dist_lut[e * 64 + l];
             bool eob = false;
             while (!eob && st == MDZ_ST_OK) {
                 uint32_t n = 0, bout = 0;
                 uint32_t rec_off = 0, rec_len = 0, rec_val = 0;  // lane k: symbol k of the batch
                 while (n < 64 && bout < kBatchOut) {
                     r.refill();
-                    uint32_t e = __builtin_amdgcn_readfirstlane(S.lit_lut[r.peek(kLitBits)]);
+                    uint32_t e = lut_pick(litv, r.peek(kLitBits));
                     if ((e & 15) == 0) {
                         const int sym = slow_decode(S.lit, r);
                         e = sym < 0 ? (kBad << 8) : lit_entry_bits((uint32_t)sym);
@@ -417,7 +442,7 @@ __global__ __launch_bounds__(64) void inflate_kernel(const mdz_inflate_in_t* __r
                     if (kind == kRef) {
                         r.refill();
                         len = val + r.bits((e >> 4) & 15);
-                        uint32_t d = __builtin_amdgcn_readfirstlane(S.dist_lut[r.peek(kDistBits)]);
+                        uint32_t d = lut_pick(distv, r.peek(kDistBits));
                         if ((d & 15) == 0) {
                             const int sym = slow_decode(S.dist, r);
                             d = sym < 0 ? (kBad << 8) : dist_entry_bits((uint32_t)sym);
@@ -441,11 +466,17 @@ __global__ __launch_bounds__(64) void inflate_kernel(const mdz_inflate_in_t* __r
                     if (pos + bout > cap) break;       // overrun: nothing after it is decoded
                 }
                 n_sym += n;
-                // ---- emit the batch: literals, then matches in order
+                // ---- emit the batch: literals, then matches in order.  The
+                // workgroup is this one wave and its LDS operations execute in
+                // order, so a match reads what earlier literals and matches of
+                // the batch stored without a wait in between; ring positions
+                // are kept below 2 * kRing and wrapped with one compare.
                 const uint64_t room = pos < cap ? cap - pos : 0;
+                const uint32_t rb = ring_at(pos);          // ring index of the batch start
+                auto wrap = [](uint32_t x) { return x >= kRing ? x - kRing : x; };
                 // matches are at least 3 long, so rec_len == 1 marks a literal
-                if (l < n && rec_len == 1) S.ring[ring_at(pos + rec_off)] = (uint8_t)rec_val;
-                wave_sync();
+                if (l < n && rec_len == 1) S.ring[wrap(rb + rec_off)] = (uint8_t)rec_val;
+                __builtin_amdgcn_wave_barrier();
                 uint64_t mm = __ballot(l < n && rec_len > 1);
                 while (mm) {
                     const uint32_t m = __builtin_ctzll(mm);
@@ -453,13 +484,23 @@ __global__ __launch_bounds__(64) void inflate_kernel(const mdz_inflate_in_t* __r
                     const uint32_t off = __builtin_amdgcn_readlane(rec_off, m);
                     const uint32_t len = __builtin_amdgcn_readlane(rec_len, m);
                     const uint32_t dist = __builtin_amdgcn_readlane(rec_val, m);
-                    const uint64_t p0 = pos + off, s0 = p0 - dist;
+                    const uint32_t d0 = wrap(rb + off);                  // destination start
+                    const uint32_t s0 = wrap(d0 + kRing - dist);         // source start (dist <= 32768)
                     if (dist >= len) {
-                        for (uint32_t j = l; j < len; j += 64) S.ring[ring_at(p0 + j)] = S.ring[ring_at(s0 + j)];
+                        for (uint32_t j = l; j < len; j += 64) S.ring[wrap(d0 + j)] = S.ring[wrap(s0 + j)];
                     } else {
-                        for (uint32_t j = l; j < len; j += 64) S.ring[ring_at(p0 + j)] = S.ring[ring_at(s0 + j % dist)];
+                        // overlapping copy: byte j repeats source byte j mod dist;
+                        // j < 258, so a float reciprocal and one correction step
+                        // give the exact quotient
+                        const float rd = __builtin_amdgcn_rcpf((float)dist);
+                        for (uint32_t j = l; j < len; j += 64) {
+                            uint32_t qj = (uint32_t)((float)j * rd);
+                            if (qj * dist > j) --qj;
+                            else if ((qj + 1) * dist <= j) ++qj;
+                            S.ring[wrap(d0 + j)] = S.ring[wrap(s0 + (j - qj * dist))];
+                        }
                     }
-                    wave_sync();
+                    __builtin_amdgcn_wave_barrier();
                 }
                 if (bout > room) {
                     pos += room;

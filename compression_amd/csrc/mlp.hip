@@ -7,9 +7,10 @@
 // work: decoded bytes never pass through it.
 //
 // Parameter layout (floats): W1[F][H] b1[H] W2[H][H] b2[H] w3[H] b3.
-// Training is full-batch: forward keeps the activations, backward forms
-// per-sample gradients and reduces them per workgroup in LDS before one
-// atomic add per parameter and block, then Adam updates in place.
+// Training is full-batch: backward stages each sample's activations and
+// deltas in LDS, sums every parameter's outer-product terms over the
+// workgroup's samples (one thread per parameter) and adds them with one
+// atomic per parameter and block; Adam then updates in place.
 #include "mdz_device.h"
 #include "mdz_internal.h"
 
@@ -75,48 +76,76 @@ __global__ __launch_bounds__(kBlock) void forward_kernel(const float* __restrict
 }
 
 // grad += d/dparams of (1/n) * sum_i (y_i - t_i)^2; loss_sum += sum (y - t)^2.
-__global__ __launch_bounds__(kBlock) void backward_kernel(const float* __restrict__ p,
-                                                          const float* __restrict__ x,
-                                                          const float* __restrict__ t, uint32_t n,
-                                                          float* __restrict__ grad,
-                                                          float* __restrict__ loss_sum) {
-    __shared__ float g[P_N];
+// Phase 1: one thread per sample runs the forward pass and the backward
+// deltas and leaves x, h1, h2, d1 (dL/d pre-activation 1), d2 and dy in LDS.
+// Phase 2: each thread owns parameters and sums their outer-product terms
+// over the block's samples (W1 = x^T d1, W2 = h1^T d2, w3 = h2^T dy, biases
+// the column sums), then adds them to grad with one atomic per parameter.
+constexpr int kBwd = 96;                        // samples (and threads) per block (52 KB LDS)
+__global__ __launch_bounds__(kBwd) void backward_kernel(const float* __restrict__ p,
+                                                        const float* __restrict__ x,
+                                                        const float* __restrict__ t, uint32_t n,
+                                                        float* __restrict__ grad,
+                                                        float* __restrict__ loss_sum) {
+    __shared__ float xs[kBwd][F], h1s[kBwd][H], h2s[kBwd][H], d1s[kBwd][H], d2s[kBwd][H], dys[kBwd];
     __shared__ float ls;
-    for (int k = threadIdx.x; k < P_N; k += blockDim.x) g[k] = 0.f;
-    if (threadIdx.x == 0) ls = 0.f;
+    const uint32_t tid = threadIdx.x, i = blockIdx.x * kBwd + tid;
+    if (tid == 0) ls = 0.f;
     __syncthreads();
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) {
-        float xi[F], h1[H], h2[H], y;
+    {
+        float xi[F], h1[H], h2[H], y = 0.f, dy = 0.f;
 #pragma unroll
-        for (int k = 0; k < F; ++k) xi[k] = x[(size_t)i * F + k];
+        for (int k = 0; k < F; ++k) xi[k] = i < n ? x[(size_t)i * F + k] : 0.f;
         forward_one(p, xi, h1, h2, y);
-        const float r = y - t[i];
-        const float dy = 2.f * r / (float)n;
-        atomicAdd(&ls, r * r);
-        atomicAdd(&g[P_B3], dy);
-        float d2[H];
-        for (int j = 0; j < H; ++j) {
-            atomicAdd(&g[P_W3 + j], dy * h2[j]);
-            d2[j] = h2[j] > 0.f ? dy * p[P_W3 + j] : 0.f;
-            atomicAdd(&g[P_B2 + j], d2[j]);
+        if (i < n) {
+            const float r = y - t[i];
+            dy = 2.f * r / (float)n;
+            atomicAdd(&ls, r * r);
         }
+        float d2[H];
+#pragma unroll
+        for (int j = 0; j < H; ++j) {
+            d2[j] = h2[j] > 0.f ? dy * p[P_W3 + j] : 0.f;
+            d2s[tid][j] = d2[j];
+            h2s[tid][j] = i < n ? h2[j] : 0.f;
+        }
+#pragma unroll
         for (int k = 0; k < H; ++k) {
             float d1 = 0.f;
-            for (int j = 0; j < H; ++j) {
-                atomicAdd(&g[P_W2 + k * H + j], h1[k] * d2[j]);
-                d1 += p[P_W2 + k * H + j] * d2[j];
-            }
-            d1 = h1[k] > 0.f ? d1 : 0.f;
-            atomicAdd(&g[P_B1 + k], d1);
 #pragma unroll
-            for (int m = 0; m < F; ++m) atomicAdd(&g[P_W1 + m * H + k], xi[m] * d1);
+            for (int j = 0; j < H; ++j) d1 += p[P_W2 + k * H + j] * d2[j];
+            d1s[tid][k] = h1[k] > 0.f ? d1 : 0.f;
+            h1s[tid][k] = h1[k];
         }
+#pragma unroll
+        for (int k = 0; k < F; ++k) xs[tid][k] = xi[k];
+        dys[tid] = dy;
     }
     __syncthreads();
-    for (int k = threadIdx.x; k < P_N; k += blockDim.x)
-        if (g[k] != 0.f) atomicAdd(&grad[k], g[k]);
-    if (threadIdx.x == 0 && loss_sum) atomicAdd(loss_sum, ls);
+    const uint32_t ns = min((uint32_t)kBwd, n - blockIdx.x * kBwd);
+    for (int k = tid; k < P_N; k += kBwd) {
+        float g = 0.f;
+        if (k < P_B1) {                                   // W1[m][j]
+            const int m = k / H, j = k % H;
+            for (uint32_t s = 0; s < ns; ++s) g += xs[s][m] * d1s[s][j];
+        } else if (k < P_W2) {                            // b1[j]
+            const int j = k - P_B1;
+            for (uint32_t s = 0; s < ns; ++s) g += d1s[s][j];
+        } else if (k < P_B2) {                            // W2[a][b]
+            const int a = (k - P_W2) / H, b = (k - P_W2) % H;
+            for (uint32_t s = 0; s < ns; ++s) g += h1s[s][a] * d2s[s][b];
+        } else if (k < P_W3) {                            // b2[b]
+            const int b = k - P_B2;
+            for (uint32_t s = 0; s < ns; ++s) g += d2s[s][b];
+        } else if (k < P_B3) {                            // w3[j]
+            const int j = k - P_W3;
+            for (uint32_t s = 0; s < ns; ++s) g += dys[s] * h2s[s][j];
+        } else {                                          // b3
+            for (uint32_t s = 0; s < ns; ++s) g += dys[s];
+        }
+        if (g != 0.f) atomicAdd(&grad[k], g);
+    }
+    if (tid == 0 && loss_sum) atomicAdd(loss_sum, ls);
 }
 
 __global__ void adam_kernel(float* __restrict__ p, float* __restrict__ grad, float* __restrict__ m,
@@ -160,7 +189,7 @@ int mlp_grad(const float* params, const float* x, const float* t, uint32_t n, fl
     hipLaunchKernelGGL(zero_kernel, dim3((P_N + 255) / 256), dim3(256), 0, s, grad, P_N);
     if (loss_sum) hipLaunchKernelGGL(zero_kernel, dim3(1), dim3(1), 0, s, loss_sum, 1);
     if (n)
-        hipLaunchKernelGGL(backward_kernel, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, params,
+        hipLaunchKernelGGL(backward_kernel, dim3((n + kBwd - 1) / kBwd), dim3(kBwd), 0, s, params,
                            x, t, n, grad, loss_sum);
     return record_hip(hipGetLastError(), "mlp backward");
 }
