@@ -10,13 +10,14 @@
 // One workgroup (8 wave64s) per chunk:
 //   1. the chunk's interleaved payload is read with 16-byte loads and each
 //      4-byte word scattered to its lane's contiguous LDS region (a per-
-//      segment destination table and a multiply-high division place it);
-//   2. header walk: 32 threads, one per lane stream, walk the run headers in
-//      parallel, check them as the reference decoder would, and append one
-//      32-byte run descriptor per run to an LDS pool;
-//   3. value pass: every run is cut into 64-value pieces (a packed DELTA run
-//      stays whole: its prefix sum runs over the run) and the 8 waves take
-//      pieces in turn, one value per SIMD lane:
+//      segment destination table and a stepped quotient place it);
+//   2. header walk: 32 threads of wave 0, one per lane stream, walk the run
+//      headers in parallel (one LDS read per header) and append one record
+//      per run to an LDS pool, publishing the count as they go;
+//   3. the other waves -- and wave 0 once its walk ends -- claim windows of
+//      published records, check them one per lane with the oracle's checks
+//      in its order (a lane's first failing run is its status), and decode
+//      each run with the whole wave, one value per SIMD lane:
 //        SHORT_REPEAT  lane i writes the repeated value;
 //        DIRECT        lane i unpacks the i-th MSB-first bit field;
 //        DELTA         fixed stride: base + i*delta; packed deltas: lane i
@@ -25,10 +26,12 @@
 //                      places it, and every lane ORs in the patches that land
 //                      on its value (64-bit mask: decoder_new.h:533's 32-bit
 //                      mask drops patch bits at positions >= 32).
-//   Steps 2 and 3 repeat while a lane stream has runs left (the pool holds
-//   kRunPool runs).  Lane t's q-th value is element
-//   (q / RU) * 32 * RU + t * RU + q % RU of the chunk, so consecutive SIMD
-//   lanes store consecutive elements.
+//   Steps 2 and 3 overlap; they repeat if a chunk has more than kRunPool
+//   runs.  Lane t's q-th value is element (q / RU) * 32 * RU + t * RU + q % RU
+//   of the chunk, so consecutive SIMD lanes store consecutive elements.
+//   Runs are checked and written concurrently, so for a lane that fails,
+//   runs after its first error may still be written (its status is exact,
+//   and every byte before the error point equals the oracle's).
 #include <type_traits>
 
 #include "mdz_device.h"
@@ -105,6 +108,9 @@ MDZ_DEV void put_val(uint8_t* out, uint64_t limit, uint32_t q, const LanePlace& 
 }
 
 constexpr uint32_t kRunPool = 768;              // run descriptors per epoch
+constexpr uint32_t kWin = 32;                   // records a consumer wave claims at once
+constexpr uint32_t kOpen = 0xffffffffu;         // final_ while the walk runs
+constexpr uint32_t kSpinMax = 1u << 22;         // consumer wait bound (about 0.2 s)
 constexpr uint32_t kSegBlocks = 160 * 1024 / 128;  // 32-word blocks of the largest LDS payload
 
 // Bit width of a 5-bit width code (the reference's FixedBitSizes decode):
@@ -226,6 +232,9 @@ MDZ_DEV void chain_step(const uint8_t* base, uint32_t p, uint32_t end, uint32_t&
     const uint32_t k = p & 3;
     const uint32_t d0 = w32[0], d1 = w32[1], d2 = w32[2], d3 = w32[3], d4 = w32[4], d5 = w32[5],
                    d6 = w32[6];
+    // all seven words in one LDS round trip: keep the compiler from sinking
+    // the DELTA-only loads into that branch (a second round trip per header)
+    asm volatile("" ::"v"(d0), "v"(d1), "v"(d2), "v"(d3), "v"(d4), "v"(d5), "v"(d6));
     const uint32_t u0 = alignbyte(d1, d0, k), u1 = alignbyte(d2, d1, k), u2 = alignbyte(d3, d2, k),
                    u3 = alignbyte(d4, d3, k), u4 = alignbyte(d5, d4, k), u5 = alignbyte(d6, d5, k);
     const uint32_t b0 = u0 & 0xff, b1 = (u0 >> 8) & 0xff, b2 = (u0 >> 16) & 0xff, b3 = u0 >> 24;
@@ -284,29 +293,13 @@ MDZ_DEV int32_t patch_list_check(const uint8_t* base, uint32_t end, const Run& r
     return MDZ_ST_OK;
 }
 
-// Wave-uniform LDS reads: every lane reads the same address and the result
-// goes to scalar registers, so the arithmetic on it is scalar.
-MDZ_DEV uint32_t lds_u32_uniform(const uint8_t* data, uint32_t word) {
-    return __builtin_amdgcn_readfirstlane(reinterpret_cast<const uint32_t*>(data)[word]);
-}
-// 8 little-endian bytes at byte p (uniform p).
-MDZ_DEV uint64_t lds_le64_uniform(const uint8_t* data, uint32_t p) {
-    const uint32_t w = p >> 2, k = (p & 3) * 8;
-    const uint64_t lo = lds_u32_uniform(data, w) | (uint64_t)lds_u32_uniform(data, w + 1) << 32;
-    const uint64_t hi = lds_u32_uniform(data, w + 2);
-    return k ? (lo >> k) | (hi << (64 - k)) : lo;
-}
-// Big-endian value of the nb (1..8) bytes at byte p (uniform p).
-MDZ_DEV uint64_t lds_be_uniform(const uint8_t* data, uint32_t p, uint32_t nb) {
-    return __builtin_bswap64(lds_le64_uniform(data, p)) >> (64 - 8 * nb);
-}
-// Unsigned LEB128 of n (1..10) bytes at byte p (uniform p).
-MDZ_DEV uint64_t uvarint_uniform(const uint8_t* data, uint32_t p, uint32_t n) {
-    uint64_t v = gather7(lds_le64_uniform(data, p));
+// Unsigned LEB128 of n (1..10) bytes at byte p (per lane).
+MDZ_DEV uint64_t uvarint_value(const uint8_t* base, uint32_t p, uint32_t n) {
+    uint64_t v = gather7(lds_peek8(base, p));
     if (n < 8) {
         v &= (1ull << (7 * n)) - 1;
     } else if (n > 8) {
-        const uint64_t hi = lds_le64_uniform(data, p + 8);
+        const uint64_t hi = lds_peek8(base, p + 8);
         const uint64_t h = ((hi & 0x7f) | ((hi >> 1) & (0x7full << 7))) & ((1ull << (7 * (n - 8))) - 1);
         v |= h << 56;
     }
@@ -327,12 +320,13 @@ MDZ_DEV uint64_t lds_bits_be32(const uint8_t* base, uint32_t b0, uint32_t bit, u
 }
 
 // Decodes run r (all 64 lanes of one wave; r is wave-uniform, r.p an
-// absolute LDS byte offset).  A run whose values lie in one READ_UNIT line
+// absolute LDS byte offset).  a and b are the run's decoded parameters:
+// SHORT_REPEAT value; DELTA base and delta (masked); PATCHED_BASE base.  A run whose values lie in one READ_UNIT line
 // and inside the output is stored through a per-run pointer; otherwise each
 // value is placed and clipped on its own.
 template <int EB>
-MDZ_DEV void run_values(const Run& rr, const uint8_t* data, uint8_t* out, uint64_t limit, uint32_t ru,
-                        uint32_t sh) {
+MDZ_DEV void run_values(const Run& rr, uint64_t a, uint64_t b, const uint8_t* data, uint8_t* out,
+                        uint64_t limit, uint32_t ru, uint32_t sh) {
     constexpr uint64_t vmask = EB == 8 ? ~0ull : ((1ull << (8 * EB)) - 1);
     typedef typename std::conditional<EB == 8, uint64_t,
             typename std::conditional<EB == 4, uint32_t,
@@ -354,9 +348,8 @@ MDZ_DEV void run_values(const Run& rr, const uint8_t* data, uint8_t* out, uint64
         if (o) o[i] = (elem_t)v;
         else put_val<EB>(out, limit, q + i, lp, v);
     };
-    if (kind == 0) {                                            // SHORT_REPEAT
-        const uint64_t v = lds_be_uniform(data, p, aux) & vmask;
-        if (l < len) put(l, v);
+    if (kind == 0) {                                            // SHORT_REPEAT (value in a)
+        if (l < len) put(l, a);
     } else if (kind == 1) {                                     // DIRECT
         for (uint32_t i0 = 0; i0 < len; i0 += kWave) {
             const uint32_t i = i0 + l;
@@ -364,9 +357,7 @@ MDZ_DEV void run_values(const Run& rr, const uint8_t* data, uint8_t* out, uint64
         }
     } else if (kind == 3) {                                     // DELTA
         const uint32_t n1 = aux & 15, n2 = aux >> 4;
-        const uint64_t v0 = uvarint_uniform(data, p, n1) & vmask;
-        const uint64_t zz = uvarint_uniform(data, p + n1, n2);
-        const uint64_t dt = ((zz >> 1) ^ (0 - (zz & 1))) & vmask;
+        const uint64_t v0 = a, dt = b;                          // base and delta, masked
         if (code == 0) {
             for (uint32_t i0 = 0; i0 < len; i0 += kWave) {
                 const uint32_t i = i0 + l;
@@ -391,7 +382,7 @@ MDZ_DEV void run_values(const Run& rr, const uint8_t* data, uint8_t* out, uint64
     } else {                                                    // PATCHED_BASE
         const uint32_t bw = aux & 15, pw = (aux >> 4) & 127, pll = (aux >> 15) & 31,
                        cfb = (aux >> 20) & 127;
-        const uint64_t bv = lds_be_uniform(data, p, bw);
+        const uint64_t bv = a;                                  // patched base
         const uint32_t pb = p + bw, pp = pb + (len * w + 7) / 8;
         const uint64_t e = l < pll ? lds_bits_be32(data, pp, l * cfb, cfb) : 0;
         const uint32_t gap = (l < pll && pw < 64) ? (uint32_t)(e >> pw) : 0;
@@ -424,10 +415,13 @@ struct Rlev2Tables {
     uint32_t M[32];
     uint32_t fits;
     uint32_t extent_ok;
-    uint32_t nruns;          // runs in the pool this epoch
+    uint32_t pub;            // records the walk has published this epoch
+    uint32_t claimed;        // records claimed by consumer waves (windows of kWin)
+    uint32_t final_;         // records of this epoch once the walk has ended, else kOpen
     uint32_t more;           // a lane stream has runs left after this epoch
     uint32_t total;          // words of all lanes
-    uint32_t pad[3];
+    uint32_t stuck;          // a consumer gave up waiting (never expected)
+    uint32_t pad[2];
     union {
         Run runs[kRunPool];
         struct {                 // staging only
@@ -610,17 +604,24 @@ MDZ_DEV void rlev2_chunk(const Rlev2Args& a, uint64_t c, uint32_t* smem) {
 
     const uint32_t sh = (ru & (ru - 1)) ? 32u : (uint32_t)__builtin_ctz(ru);
     uint8_t* const out = dst + out_base;
+    // Steps 2-4 overlap: wave 0 walks the headers and publishes records as
+    // it goes; every wave (wave 0 once its walk ends) claims windows of kWin
+    // published records, checks them one per lane and decodes them.  Only
+    // consumers wait, and only for records the walk will publish, so every
+    // wait ends; a bound on the wait turns a broken invariant into a status.
+    if (tid == 0) { tb.pub = 0; tb.claimed = 0; tb.final_ = kOpen; tb.stuck = 0; }
+    __syncthreads();
     // header-chain state of lane stream tid (threads 0..31 of wave 0)
     uint32_t wp = 0, wq = 0;
     bool done = tid >= 32;
     for (;;) {
         if (wave == 0) {
             // the walk is the chunk's critical path: raise this wave's issue
-            // priority over the workgroup's value waves while it runs
+            // priority over the consumer waves while it runs
             __builtin_amdgcn_s_setprio(2);
             const uint8_t* const base = data + tb.lbase[l & 31];
             const uint32_t end = tb.cl[l & 31];
-            uint32_t n = 0;
+            uint32_t n = 0, pubd = 0;
             for (;;) {
                 const bool has = !done && wq < nvals;
                 uint32_t size = 0, cnt = 0;
@@ -632,55 +633,112 @@ MDZ_DEV void rlev2_chunk(const Rlev2Args& a, uint64_t c, uint32_t* smem) {
                                                                     __builtin_amdgcn_mbcnt_lo((uint32_t)want, 0u));
                 if (has && slot < kRunPool) {
                     // every run the chain reaches is recorded, a malformed one
-                    // too: the check pass finds its status
+                    // too: its check finds the status
                     *reinterpret_cast<uint4*>(&tb.runs[slot]) = make_uint4(wp, wq, tid << 23, 0u);
                     wp += size;
                     wq += cnt;
                     done = stop;
                 }
-                n += (uint32_t)__popcll(want);
-                if (n >= kRunPool) { n = kRunPool; break; }
+                n = min(n + (uint32_t)__popcll(want), kRunPool);
+                // publish when a consumer window has filled (each publish
+                // waits for the records' LDS writes)
+                if (n >= pubd + kWin) {
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+                    if (l == 0) __hip_atomic_store(&tb.pub, n, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                    pubd = n;
+                }
+                if (n >= kRunPool) break;
             }
             __builtin_amdgcn_s_setprio(0);
             done = done || wq >= nvals;
             const uint64_t left = __ballot(!done);
             if (l == 0) {
-                tb.nruns = n;
                 tb.more = left != 0;
+                __hip_atomic_store(&tb.final_, n, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
             }
+            MDZ_PHASE(2);
         }
-        __syncthreads();
-        MDZ_PHASE(2);
-        const uint32_t nruns = tb.nruns, more = tb.more;
-        // check pass, one run per thread: the oracle's checks in its order
-        // (patch lists included) fill in the descriptor; a lane's first
-        // failing run (lowest q) ends it
-        for (uint32_t i = tid; i < nruns; i += kThreads) {
-            Run& rr = tb.runs[i];
-            const uint32_t t = rr.meta >> 23, q = rr.q;
-            const uint8_t* const base = data + tb.lbase[t];
-            const uint32_t end = tb.cl[t];
-            uint32_t pp = rr.p;
-            Run r;
-            int32_t e = walk_run(base, end, pp, q, nvals, t, r);
-            if (e == MDZ_ST_OK && (r.meta & 3) == 2) e = patch_list_check(base, end, r);
-            if (e != MDZ_ST_OK) {
-                atomicMin(&tb.lerr[t], q << 8 | ((uint32_t)e & 0xffu));
-            } else {
-                r.p += tb.lbase[t];                      // absolute LDS offset for the value pass
-                rr = r;
+        for (;;) {
+            uint32_t w0 = 0;
+            if (l == 0) w0 = atomicAdd(&tb.claimed, kWin);
+            w0 = __builtin_amdgcn_readfirstlane(w0);
+            uint32_t avail = 0;
+            for (uint32_t it = 0;; ++it) {
+                const uint32_t fin = __hip_atomic_load(&tb.final_, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP);
+                if (fin != kOpen) { avail = fin; break; }
+                const uint32_t pv = __hip_atomic_load(&tb.pub, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP);
+                if (pv >= w0 + kWin) { avail = w0 + kWin; break; }
+                if (it >= kSpinMax) { if (l == 0) tb.stuck = 1; break; }
+                __builtin_amdgcn_s_sleep(2);
             }
-        }
-        __syncthreads();
-        for (uint32_t r = wave; r < nruns; r += kThreads / kWave) {
-            const Run& rr = tb.runs[r];
-            const uint32_t t = rr.meta >> 23;
-            if (rr.q >= (tb.lerr[t] >> 8)) continue;        // at or after its lane's first error
-            run_values<EB>(rr, data, out, limit, ru, sh);
+            avail = __builtin_amdgcn_readfirstlane(avail);
+            if (w0 >= avail) break;
+            const uint32_t cnt = min(kWin, avail - w0);
+            // check, one record per lane: the oracle's checks in its order
+            // (patch lists included) complete the descriptor in registers,
+            // and the run's parameters are decoded here, one run per lane
+            Run r{};
+            uint64_t pa = 0, pb = 0;
+            bool ok = false;
+            // lane errors as of now, lane t's in register lane t
+            const uint32_t lerr_r = tb.lerr[l & 31];
+            if (l < cnt) {
+                const Run rec = tb.runs[w0 + l];
+                const uint32_t t = rec.meta >> 23;
+                const uint8_t* const base = data + tb.lbase[t];
+                const uint32_t end = tb.cl[t];
+                uint32_t pp = rec.p;
+                int32_t e = walk_run(base, end, pp, rec.q, nvals, t, r);
+                const uint32_t kind = r.meta & 3;
+                if (e == MDZ_ST_OK && kind == 2) e = patch_list_check(base, end, r);
+                if (e != MDZ_ST_OK) {
+                    atomicMin(&tb.lerr[t], rec.q << 8 | ((uint32_t)e & 0xffu));
+                } else {
+                    ok = true;
+                    if (kind == 0) {
+                        pa = lds_bits_be32(base, r.p, 0, 8 * r.aux) & vmask;
+                    } else if (kind == 3) {
+                        const uint32_t n1 = r.aux & 15, n2 = r.aux >> 4;
+                        const uint64_t zz = uvarint_value(base, r.p + n1, n2);
+                        pa = uvarint_value(base, r.p, n1) & vmask;
+                        pb = ((zz >> 1) ^ (0 - (zz & 1))) & vmask;
+                    } else if (kind == 2) {
+                        pa = lds_bits_be32(base, r.p, 0, 8 * (r.aux & 15));
+                    }
+                    r.p += tb.lbase[t];                 // absolute LDS offset for the value pass
+                    // SHORT_REPEAT (3..10 values): written here by its lane
+                    if (kind == 0 && rec.q < (tb.lerr[t] >> 8)) {
+                        const LanePlace lp{t, ru, sh};
+                        const uint32_t n = ((r.meta >> 14) & 511) + 1;
+                        for (uint32_t i = 0; i < n; ++i) put_val<EB>(out, limit, rec.q + i, lp, pa);
+                    }
+                }
+            }
+            // the longer runs: one at a time with the whole wave
+            const uint64_t okm = __ballot(ok && (r.meta & 3) != 0);
+            for (uint64_t m = okm; m;) {
+                const uint32_t j = (uint32_t)__builtin_ctzll(m);
+                m &= m - 1;
+                Run rj;
+                rj.p = __builtin_amdgcn_readlane(r.p, j);
+                rj.q = __builtin_amdgcn_readlane(r.q, j);
+                rj.meta = __builtin_amdgcn_readlane(r.meta, j);
+                rj.aux = __builtin_amdgcn_readlane(r.aux, j);
+                // a run at or after its lane's first known error is not written
+                if (rj.q >= ((uint32_t)__builtin_amdgcn_readlane(lerr_r, rj.meta >> 23) >> 8)) continue;
+                const uint64_t a = (uint64_t)(uint32_t)__builtin_amdgcn_readlane((uint32_t)(pa >> 32), j) << 32 |
+                                   (uint32_t)__builtin_amdgcn_readlane((uint32_t)pa, j);
+                const uint64_t b = (uint64_t)(uint32_t)__builtin_amdgcn_readlane((uint32_t)(pb >> 32), j) << 32 |
+                                   (uint32_t)__builtin_amdgcn_readlane((uint32_t)pb, j);
+                run_values<EB>(rj, a, b, data, out, limit, ru, sh);
+            }
         }
         __syncthreads();
         MDZ_PHASE(3);
-        if (!more) break;
+        if (!tb.more || tb.stuck) break;
+        __syncthreads();                               // everyone has read more
+        if (tid == 0) { tb.pub = 0; tb.claimed = 0; tb.final_ = kOpen; }
+        __syncthreads();
     }
     if (tid == 0 && status) {
         int32_t st = MDZ_ST_OK;
@@ -688,7 +746,7 @@ MDZ_DEV void rlev2_chunk(const Rlev2Args& a, uint64_t c, uint32_t* smem) {
             const uint32_t e = tb.lerr[t];
             if (e != 0xffffffffu) { st = (int32_t)(int8_t)(e & 0xff); break; }
         }
-        status[c] = st;
+        status[c] = tb.stuck ? MDZ_ST_INTERNAL : st;
     }
 }
 

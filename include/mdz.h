@@ -46,6 +46,7 @@ extern "C" {
 #define MDZ_ST_BAD_DISTANCE    -9
 #define MDZ_ST_BAD_SYMBOL     -10
 #define MDZ_ST_BAD_HEADER     -11
+#define MDZ_ST_INTERNAL       -12   /* a decoder's bounded internal wait expired (never expected) */
 
 /* --------------------------------------------------------------- version */
 /* Returns "mdz <version> gfx950". */
