@@ -256,9 +256,7 @@ class Config2Workload:
         self.alg_bytes = sum(p.alg_bytes for p in self.parts)
         self.batch = batch.Batch(r.lanes, r.dst, r.out_bytes, r.status, r.max_chunk,
                                  b.dev, b.dst, b.out_bytes, b.status)
-        c2, cb = r.c, b.c
-        sizes2 = np.append(c2.blk_off[1:-1].astype(np.int64), c2.payload_bytes) - c2.blk_off[:-1].astype(np.int64)
-        self.batch.set_order(batch.size_order(sizes2, np.diff(cb.ptr.astype(np.int64))))
+        self.batch.set_order(self.batch.size_order())
         self.config = {"workload": self.name, "codecs": ["rle_v2", "byte_rle"],
                        "launch": "persistent work queue (size order)" if batched else "per-codec per-chunk",
                        "out_bytes": self.out_bytes, "parts": [p.config for p in self.parts]}

@@ -40,14 +40,16 @@ class LanesT(C.Structure):
 class BatchT(C.Structure):
     """mdz_batch_t (include/mdz.h)."""
     _fields_ = [("rlev2", LanesT), ("rlev2_dst", vp), ("rlev2_dst_bytes", u64), ("rlev2_status", vp),
-                ("rlev2_max_chunk", u32), ("brle_max_chunk", u32), ("brle_ptr", vp), ("brle_data", vp),
+                ("rlev2_max_chunk", u32), ("brle_max_chunk", u32), ("rlev2_max_part", u32),
+                ("reserved0", u32), ("brle_ptr", vp), ("brle_data", vp),
                 ("brle_n_chunks", u64), ("brle_chunk_bytes", u64), ("brle_dst", vp),
                 ("brle_dst_bytes", u64), ("brle_status", vp), ("items", vp), ("n_items", u32),
                 ("blocks", u32), ("cursor", vp)]
 
 
 class LanesTuneT(C.Structure):
-    _fields_ = [("ring_words", u32), ("max_chunk_bytes", u32), ("reserved", u32 * 6)]
+    _fields_ = [("ring_words", u32), ("max_chunk_bytes", u32), ("max_part_bytes", u32),
+                ("reserved", u32 * 5)]
 
 
 # name -> (restype, argtypes); every symbol declared in include/mdz.h

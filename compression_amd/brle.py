@@ -24,6 +24,7 @@ class DeviceBrle:
         self.n_chunks, self.chunk_bytes = int(n_chunks), int(chunk_bytes)
         p = np.asarray(ptr, np.int64)
         self.max_chunk = int(np.diff(p).max()) if len(p) > 1 else 0
+        self.chunk_sizes = np.diff(p) if len(p) > 1 else np.zeros(0, np.int64)
         self.data_bytes = data.nbytes
 
 

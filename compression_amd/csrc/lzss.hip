@@ -39,6 +39,7 @@
 // fill; a finishing pass turns the key back into the status code.
 #include "mdz_device.h"
 #include "mdz_internal.h"
+#include "status_dev.h"
 
 #include <climits>
 
@@ -66,20 +67,6 @@ struct LzssHdr {
     uint32_t pad[2];
 };
 static_assert(sizeof(LzssHdr) % 16 == 0, "LDS carve alignment");
-
-__global__ void status_fill_kernel(int32_t* st, uint64_t n) {
-    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
-         i += (uint64_t)gridDim.x * blockDim.x)
-        st[i] = INT_MAX;
-}
-
-__global__ void status_finish_kernel(int32_t* st, uint64_t n) {
-    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
-         i += (uint64_t)gridDim.x * blockDim.x) {
-        const int32_t v = st[i];
-        st[i] = v == INT_MAX ? MDZ_ST_OK : (int32_t)(int8_t)(v & 0xff);
-    }
-}
 
 MDZ_DEV void report(int32_t* status, uint64_t c, uint32_t lane, int code) {
     if (status) atomicMin(&status[c], (int32_t)((lane << 8) | ((uint32_t)code & 0xffu)));

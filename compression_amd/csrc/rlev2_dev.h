@@ -107,7 +107,6 @@ MDZ_DEV void put_val(uint8_t* out, uint64_t limit, uint32_t q, const LanePlace& 
     else out[off] = (uint8_t)v;
 }
 
-constexpr uint32_t kRunPool = 768;              // run descriptors per epoch
 constexpr uint32_t kWin = 32;                   // records a consumer wave claims at once
 constexpr uint32_t kOpen = 0xffffffffu;         // final_ while the walk runs
 constexpr uint32_t kSpinMax = 1u << 22;         // consumer wait bound (about 0.2 s)
@@ -254,19 +253,38 @@ MDZ_DEV void chain_step(const uint8_t* base, uint32_t p, uint32_t end, uint32_t&
         const uint32_t pgw = ((b3 >> 5) & 7) + 1, pll = b3 & 31;
         sz = 4 + bw + vbytes + (pll * closest_bits_sel(pw + pgw) + 7) / 8;
     } else {                                                 // DELTA: two LEB128 varints
-        // terminator bytes (bit 7 clear) of bytes 0..23, one bit each
-        // (bits 0, 8, 16, 24 of t gathered to bits 0..3)
-        auto term4 = [](uint32_t u) {
-            const uint32_t t = (~u >> 7) & 0x01010101u;
-            return (t | (t >> 7) | (t >> 14) | (t >> 21)) & 0xfu;
-        };
-        const uint32_t tm = term4(u0) | term4(u1) << 4 | term4(u2) << 8 | term4(u3) << 12 |
-                            term4(u4) << 16 | term4(u5) << 20;
-        const uint32_t t1 = (tm >> 2) & 0x3ff;
-        const uint32_t n1 = (uint32_t)__builtin_ctz(t1 | 0x400) + 1;      // 11 when absent
-        const uint32_t t2 = (tm >> (2 + n1)) & 0x3ff;
-        const uint32_t n2 = (uint32_t)__builtin_ctz(t2 | 0x400) + 1;
-        bad = t1 == 0 || t2 == 0 || len < 2;
+        // bytes 2..9 and 10..17 as 64-bit words: the first varint's end is
+        // the first clear bit 7 among them, the second's the first after it
+        const uint64_t x0 = (uint64_t)alignbyte(u2, u1, 2) << 32 | alignbyte(u1, u0, 2);
+        const uint64_t x1 = (uint64_t)alignbyte(u4, u3, 2) << 32 | alignbyte(u3, u2, 2);
+        const uint32_t x2 = alignbyte(u5, u4, 2);
+        const uint64_t t1 = ~x0 & 0x8080808080808080ull;
+        uint32_t n1;
+        if (t1) {
+            n1 = ((uint32_t)__builtin_ctzll(t1) >> 3) + 1;
+        } else {
+            const uint32_t tt = ~(uint32_t)x1 & 0x8080u;
+            n1 = tt ? 9 + ((uint32_t)__builtin_ctz(tt) >> 3) : 0;
+        }
+        // the 8 bytes after the first varint (bytes 2 + n1 ..)
+        const uint32_t sh1 = 8 * (n1 & 7);
+        const uint64_t lo = n1 < 8 ? x0 : x1, hi = n1 < 8 ? x1 : (uint64_t)x2;
+        const uint64_t y = sh1 ? (lo >> sh1) | (hi << (64 - sh1)) : lo;
+        const uint64_t t2 = ~y & 0x8080808080808080ull;
+        uint32_t n2;
+        if (t2) {
+            n2 = ((uint32_t)__builtin_ctzll(t2) >> 3) + 1;
+        } else {                                             // a second varint of 9 or 10 bytes
+            auto term4 = [](uint32_t u) {
+                const uint32_t t = (~u >> 7) & 0x01010101u;
+                return (t | (t >> 7) | (t >> 14) | (t >> 21)) & 0xfu;
+            };
+            const uint32_t tm = term4(u0) | term4(u1) << 4 | term4(u2) << 8 | term4(u3) << 12 |
+                                term4(u4) << 16 | term4(u5) << 20;
+            const uint32_t tt = (tm >> (2 + n1)) & 0x3ff;
+            n2 = tt ? (uint32_t)__builtin_ctz(tt) + 1 : 0;
+        }
+        bad = n1 == 0 || n2 == 0 || len < 2;
         sz = 2 + n1 + n2 + (code ? ((len - 2) * w + 7) / 8 : 0);
     }
     size = sz;
@@ -403,8 +421,15 @@ MDZ_DEV void run_values(const Run& rr, uint64_t a, uint64_t b, const uint8_t* da
     }
 }
 
+constexpr uint32_t kNoLane = 0xffffffffu;      // lbase / dest of a lane outside the part
+
+// Run pool per workgroup: a whole chunk (32 lanes) or one part of NL lanes.
+template <uint32_t NL> struct PartPool { static constexpr uint32_t value = NL == 32 ? 768 : 384; };
+
+template <uint32_t POOL>
 struct Rlev2Tables {
-    uint32_t lbase[32];      // LDS byte offset of each lane stream (16-byte aligned)
+    uint32_t lbase[32];      // LDS byte offset of each lane stream of the part (16-byte
+                             // aligned), kNoLane for lanes of other parts
     uint32_t cl[32];         // lane stream bytes
     uint32_t lerr[32];       // first failing run of each lane: q << 8 | status byte
     // The interleave in segments: segment k holds rounds sW[k-1] .. sW[k]-1,
@@ -423,15 +448,15 @@ struct Rlev2Tables {
     uint32_t stuck;          // a consumer gave up waiting (never expected)
     uint32_t pad[2];
     union {
-        Run runs[kRunPool];
+        Run runs[POOL];
         struct {                 // staging only
             uint32_t dest[32][32];   // segment k, j-th active lane -> its LDS word base + the
-                                     // segment's first round
+                                     // segment's first round (kNoLane: another part's lane)
             uint8_t segof[kSegBlocks];  // segment of the first raw word of each 32-word block
         } st;
     };
 };
-static_assert(sizeof(Run) * kRunPool >= sizeof(uint32_t) * 32 * 32 + kSegBlocks,
+static_assert(sizeof(Run) * PartPool<8>::value >= sizeof(uint32_t) * 32 * 32 + kSegBlocks,
               "staging tables fit the run pool");
 
 // Launch arguments of one RLE v2 container decode.
@@ -456,13 +481,14 @@ MDZ_DEV uint32_t segment_of(const uint32_t* S, uint32_t w) {
     return k;
 }
 
-// Scatters the chunk's interleaved words into the lane regions with 16-byte
-// loads: raw word w of segment k is the j-th active lane's word of round
+// Scatters the chunk's interleaved words into the lane regions of the part
+// with 16-byte loads: raw word w of segment k is the j-th active lane's word of round
 // sW[k-1] + (w - S[k]) / (32 - k), j = (w - S[k]) mod (32 - k), stored at
 // dest[k][j] + (w - S[k]) / (32 - k).  A group's segment comes from the
 // table of 32-word blocks; the quotient is found once per group and then
 // stepped word by word.
-MDZ_DEV void stage_lanes(const Rlev2Tables& tb, uint32_t* d32, const u32x4* __restrict__ src,
+template <class Tables>
+MDZ_DEV void stage_lanes(const Tables& tb, uint32_t* d32, const u32x4* __restrict__ src,
                          uint32_t tid, uint32_t nthreads) {
     const uint32_t total = tb.total, ngroups = (total + 3) / 4;
     auto next_start = [&](uint32_t k) { return k < 31 ? tb.S[k + 1] : total; };
@@ -499,17 +525,76 @@ MDZ_DEV void stage_lanes(const Rlev2Tables& tb, uint32_t* d32, const u32x4* __re
                     else if ((r + 1) * nk <= off) ++r;
                     j = off - r * nk;
                 }
-                d32[tb.st.dest[k][j] + r] = v[u][e];
+                const uint32_t dw = tb.st.dest[k][j];
+                if (dw != kNoLane) d32[dw + r] = v[u][e];     // words of other parts are dropped
                 if (++j == nk) { j = 0; ++r; }
             }
         }
     }
 }
 
-// Decodes chunk c with the whole workgroup (kThreads threads); smem holds
-// data_cap + 16 bytes of staged payload followed by the tables.
-template <int EB>
-MDZ_DEV void rlev2_chunk(const Rlev2Args& a, uint64_t c, uint32_t* smem) {
+// Stages only the part's lanes (NL < 32).  Thread (j, x) takes a contiguous
+// block of rounds of lane t0 + j.  Round r lies in segment
+// k = #{lanes with W <= r} (sW ascending), whose active lanes are M[k], and
+// the word's raw index is base_k + r * (32 - k) with
+// base_k = S[k] - sW[k-1] * (32 - k) + popc(M[k] below the lane): affine in r
+// inside a segment, so the tables are read only where a segment starts.  A
+// thread issues up to 16 loads before their LDS stores.
+template <uint32_t NL, uint32_t NT, class Tables>
+MDZ_DEV void stage_part(const Tables& tb, uint32_t* d32, const uint32_t* __restrict__ src, uint32_t t0,
+                        uint32_t tid) {
+    constexpr uint32_t XS = NT / NL;              // threads per lane
+    const uint32_t j = tid % NL, x = tid / NL, t = t0 + j;
+    const uint32_t cl = tb.cl[t], Wt = (cl + 3) / 4, lb = tb.lbase[t] >> 2;
+    const uint32_t below = (1u << t) - 1u;
+    const uint32_t rb = (Wt + XS - 1) / XS;
+    uint32_t r = x * rb;
+    const uint32_t rend = min(Wt, r + rb);
+    if (r >= rend) return;
+    // the segment of round r: count of sorted lengths <= r (scalar copies)
+    uint32_t k = 0;
+#pragma unroll
+    for (int i = 0; i < 32; ++i) k += __builtin_amdgcn_readfirstlane(tb.sW[i]) <= r;
+    uint32_t nbnd, base, stride;
+    auto enter = [&]() {                          // k < 32 while lane t is active
+        nbnd = k < 31 ? tb.sW[k] : 0xffffffffu;
+        stride = 32 - k;
+        base = tb.S[k] - (k ? tb.sW[k - 1] : 0) * stride + __popc(tb.M[k] & below);
+    };
+    enter();
+    while (r < rend) {
+        uint32_t v[16], at[16];
+        uint32_t nb = 0;
+#pragma unroll
+        for (uint32_t u = 0; u < 16; ++u) {
+            if (r < rend) {
+                if (r >= nbnd) {                  // next segment (a shorter lane ended)
+                    while (k < 31 && tb.sW[k] <= r) ++k;
+                    enter();
+                }
+                v[u] = __builtin_nontemporal_load(src + base + r * stride);
+                at[u] = lb + r;
+                ++nb;
+                ++r;
+            }
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < 16; ++u)
+            if (u < nb) d32[at[u]] = v[u];
+    }
+}
+
+// Decodes lanes [part*NL, part*NL + NL) of chunk c with one workgroup of NT
+// threads; smem holds data_cap + 16 bytes of staged payload (the part's
+// lanes) followed by the tables.  With NL = 32 the workgroup decodes the
+// whole chunk and writes its status; with NL < 32 each part adds its first
+// failing lane to status[c] as an atomic minimum of (lane << 8 | code), which
+// status_finish_kernel turns into the code (status_fill_kernel clears it).
+template <int EB, uint32_t NL, uint32_t NT>
+MDZ_DEV void rlev2_part(const Rlev2Args& a, uint64_t c, uint32_t part, uint32_t* smem) {
+    static_assert(NL >= 1 && NL <= 32 && 32 % NL == 0 && NT % kWave == 0 && NT >= 128, "part shape");
+    constexpr uint32_t kRunPool = PartPool<NL>::value;
+    typedef Rlev2Tables<kRunPool> Tables;
     constexpr uint64_t vmask = EB == 8 ? ~0ull : ((1ull << (8 * EB)) - 1);
     const uint8_t* __restrict__ payload = a.payload;
     const uint64_t* __restrict__ blk_off = a.blk_off;
@@ -520,7 +605,8 @@ MDZ_DEV void rlev2_chunk(const Rlev2Args& a, uint64_t c, uint32_t* smem) {
     const uint64_t dst_bytes = a.dst_bytes;
     int32_t* __restrict__ status = a.status;
     uint8_t* const data = reinterpret_cast<uint8_t*>(smem);
-    Rlev2Tables& tb = *reinterpret_cast<Rlev2Tables*>(data + data_cap + 16);
+    Tables& tb = *reinterpret_cast<Tables*>(data + data_cap + 16);
+    const uint32_t t0 = part * NL;                 // first lane of the part
 
     const uint32_t tid = threadIdx.x, wave = tid >> 6, l = tid & 63;
     MDZ_PHASE_START();
@@ -534,7 +620,8 @@ MDZ_DEV void rlev2_chunk(const Rlev2Args& a, uint64_t c, uint32_t* smem) {
     if (wave == 0) {
         const uint32_t t = l & 31;
         const uint32_t cl = (uint32_t)col_len[c * 32 + t];
-        const uint32_t W = (cl + 3) / 4, r16 = (cl + 15) & ~15u;
+        const bool own = t - t0 < NL;
+        const uint32_t W = (cl + 3) / 4, r16 = own ? (cl + 15) & ~15u : 0u;
         uint32_t incl = r16, tot = W, mx = W;
 #pragma unroll
         for (int o = 1; o < 32; o <<= 1) {
@@ -551,7 +638,7 @@ MDZ_DEV void rlev2_chunk(const Rlev2Args& a, uint64_t c, uint32_t* smem) {
             rk += (wu < W) || (wu == W && (uint32_t)u < t);
         }
         if (l < 32) {
-            tb.lbase[t] = incl - r16;
+            tb.lbase[t] = own ? incl - r16 : kNoLane;
             tb.cl[t] = cl;
             tb.lerr[t] = 0xffffffffu;
             tb.sW[rk] = W;
@@ -581,24 +668,35 @@ MDZ_DEV void rlev2_chunk(const Rlev2Args& a, uint64_t c, uint32_t* smem) {
     if (!tb.extent_ok || !tb.fits) {
         // lanes longer than the chunk's payload, or a chunk larger than the
         // LDS budget (the host sizes the budget from the largest chunk)
-        if (tid == 0 && status) status[c] = tb.extent_ok ? MDZ_ST_OUTPUT_OVERRUN : MDZ_ST_INPUT_OVERRUN;
+        if (tid == 0 && status) {
+            const int32_t e = tb.extent_ok ? MDZ_ST_OUTPUT_OVERRUN : MDZ_ST_INPUT_OVERRUN;
+            if constexpr (NL == 32) status[c] = e;
+            else atomicMin(&status[c], (int32_t)(t0 << 8 | ((uint32_t)e & 0xff)));
+        }
         return;
     }
-    for (uint32_t x = tid; x < 32 * 32; x += kThreads) {
-        const uint32_t k = x >> 5, jj = x & 31;
-        if (jj < 32 - k)
-            tb.st.dest[k][jj] = (tb.lbase[select_bit(tb.M[k], jj)] >> 2) + (k ? tb.sW[k - 1] : 0);
-    }
-    {
-        uint32_t S[32];
+    if constexpr (NL == 32) {
+        for (uint32_t x = tid; x < 32 * 32; x += NT) {
+            const uint32_t k = x >> 5, jj = x & 31;
+            if (jj < 32 - k) {
+                const uint32_t lb = tb.lbase[select_bit(tb.M[k], jj)];
+                tb.st.dest[k][jj] = lb == kNoLane ? kNoLane : (lb >> 2) + (k ? tb.sW[k - 1] : 0);
+            }
+        }
+        {
+            uint32_t S[32];
 #pragma unroll
-        for (int j = 0; j < 32; ++j) S[j] = __builtin_amdgcn_readfirstlane(tb.S[j]);
-        for (uint32_t b = tid; b < (tb.total + 31) / 32; b += kThreads)
-            tb.st.segof[b] = (uint8_t)segment_of(S, 32 * b);
+            for (int j = 0; j < 32; ++j) S[j] = __builtin_amdgcn_readfirstlane(tb.S[j]);
+            for (uint32_t b = tid; b < (tb.total + 31) / 32; b += NT)
+                tb.st.segof[b] = (uint8_t)segment_of(S, 32 * b);
+        }
+        __syncthreads();
+        MDZ_PHASE(0);
+        stage_lanes(tb, smem, reinterpret_cast<const u32x4*>(payload + c_off), tid, NT);
+    } else {
+        MDZ_PHASE(0);
+        stage_part<NL, NT>(tb, smem, reinterpret_cast<const uint32_t*>(payload + c_off), t0, tid);
     }
-    __syncthreads();
-    MDZ_PHASE(0);
-    stage_lanes(tb, smem, reinterpret_cast<const u32x4*>(payload + c_off), tid, kThreads);
     __syncthreads();
     MDZ_PHASE(1);
 
@@ -611,16 +709,17 @@ MDZ_DEV void rlev2_chunk(const Rlev2Args& a, uint64_t c, uint32_t* smem) {
     // wait ends; a bound on the wait turns a broken invariant into a status.
     if (tid == 0) { tb.pub = 0; tb.claimed = 0; tb.final_ = kOpen; tb.stuck = 0; }
     __syncthreads();
-    // header-chain state of lane stream tid (threads 0..31 of wave 0)
+    // header-chain state of lane stream t0 + tid (threads 0..NL-1 of wave 0)
     uint32_t wp = 0, wq = 0;
-    bool done = tid >= 32;
+    bool done = tid >= NL;
     for (;;) {
         if (wave == 0) {
             // the walk is the chunk's critical path: raise this wave's issue
             // priority over the consumer waves while it runs
             __builtin_amdgcn_s_setprio(2);
-            const uint8_t* const base = data + tb.lbase[l & 31];
-            const uint32_t end = tb.cl[l & 31];
+            const uint32_t wl = t0 + l % NL;           // lanes past NL idle on a valid lane
+            const uint8_t* const base = data + tb.lbase[wl];
+            const uint32_t end = tb.cl[wl];
             uint32_t n = 0, pubd = 0;
             for (;;) {
                 const bool has = !done && wq < nvals;
@@ -634,7 +733,7 @@ MDZ_DEV void rlev2_chunk(const Rlev2Args& a, uint64_t c, uint32_t* smem) {
                 if (has && slot < kRunPool) {
                     // every run the chain reaches is recorded, a malformed one
                     // too: its check finds the status
-                    *reinterpret_cast<uint4*>(&tb.runs[slot]) = make_uint4(wp, wq, tid << 23, 0u);
+                    *reinterpret_cast<uint4*>(&tb.runs[slot]) = make_uint4(wp, wq, wl << 23, 0u);
                     wp += size;
                     wq += cnt;
                     done = stop;
@@ -741,12 +840,17 @@ MDZ_DEV void rlev2_chunk(const Rlev2Args& a, uint64_t c, uint32_t* smem) {
         __syncthreads();
     }
     if (tid == 0 && status) {
-        int32_t st = MDZ_ST_OK;
-        for (uint32_t t = 0; t < 32; ++t) {
+        uint32_t fl = 32, code = 0;                   // the part's first failing lane
+        for (uint32_t t = t0; t < t0 + NL; ++t) {
             const uint32_t e = tb.lerr[t];
-            if (e != 0xffffffffu) { st = (int32_t)(int8_t)(e & 0xff); break; }
+            if (e != 0xffffffffu) { fl = t; code = e & 0xff; break; }
         }
-        status[c] = tb.stuck ? MDZ_ST_INTERNAL : st;
+        if (tb.stuck) { fl = t0; code = (uint32_t)MDZ_ST_INTERNAL & 0xff; }
+        if constexpr (NL == 32) {
+            status[c] = fl < 32 ? (int32_t)(int8_t)code : MDZ_ST_OK;
+        } else {
+            if (fl < 32) atomicMin(&status[c], (int32_t)(fl << 8 | code));
+        }
     }
 }
 

@@ -29,6 +29,15 @@ class DeviceLanes:
         self.n_chunks, self.chunk_bytes = int(n_chunks), int(chunk_bytes)
         self.elem_bytes, self.lane_line = int(elem_bytes), int(lane_line)
         self.payload_bytes = payload.nbytes
+        # bytes of each chunk's lane quarters (lanes 8q..8q+7) once staged,
+        # every lane rounded up to 16: sizes the per-quarter LDS area
+        cl = np.asarray(col_len, np.int64)[:32 * self.n_chunks]
+        if cl.size == 32 * self.n_chunks and self.n_chunks:
+            self.part_bytes = ((cl + 15) & ~15).reshape(self.n_chunks, 4, 8).sum(axis=2)
+            self.max_part = int(self.part_bytes.max())
+        else:
+            self.part_bytes = np.zeros((self.n_chunks, 4), np.int64)
+            self.max_part = 0
 
     def struct(self) -> LanesT:
         return LanesT(self.payload.ptr, self.blk_off.ptr, self.col_len.ptr,

@@ -106,7 +106,11 @@ typedef struct {
                                     in 4-byte words (power of 2); 0 = automatic */
     uint32_t max_chunk_bytes;    /* upper bound on one chunk's payload bytes, if known; it
                                     sizes the per-chunk LDS area (0 = use the maximum) */
-    uint32_t reserved[6];
+    uint32_t max_part_bytes;     /* RLE v2: upper bound, over chunks and their 4 lane quarters
+                                    (lanes 8q..8q+7), of sum(round_up(col_len, 16)); when set,
+                                    each quarter is decoded by its own workgroup (4 per chunk,
+                                    smaller LDS, more chunks in flight).  0 = whole chunks */
+    uint32_t reserved[5];
 } mdz_lanes_tune_t;
 
 /* RLE v1 decode -- replaces rle_v1::decompress_gpu<READ_COL_TYPE, DATA_TYPE,
@@ -317,6 +321,9 @@ typedef struct {
     int32_t* rlev2_status;
     uint32_t rlev2_max_chunk;       /* largest chunk payload (LDS sizing), 0 = LDS maximum */
     uint32_t brle_max_chunk;        /* 0 = the byte-RLE worst case */
+    uint32_t rlev2_max_part;        /* as mdz_lanes_tune_t.max_part_bytes: nonzero decodes RLE v2
+                                       in lane quarters (256-thread workgroups throughout) */
+    uint32_t reserved0;
     const uint64_t* brle_ptr;
     const void* brle_data;
     uint64_t brle_n_chunks;         /* 0: no byte-RLE part */

@@ -20,7 +20,7 @@ def _parts(gpu, n_vals, n_bytes, ru=64, chunk=131072):
     return x, c2, lanes, y, cb, db
 
 
-def _batch(gpu, lanes, c2, db, cb):
+def _batch(gpu, lanes, c2, db, cb, quarters=False):
     from compression_amd import batch
     d1, s1 = gpu.DeviceBuffer(c2.raw_bytes), gpu.DeviceBuffer(4 * c2.n_chunks)
     d2, s2 = gpu.DeviceBuffer(cb.raw_bytes), gpu.DeviceBuffer(4 * cb.n_chunks)
@@ -28,21 +28,20 @@ def _batch(gpu, lanes, c2, db, cb):
     d2.fill(0xCC)
     s1.fill(0x7F)
     s2.fill(0x7F)
-    b = batch.Batch(lanes, d1, c2.raw_bytes, s1, 0, db, d2, cb.raw_bytes, s2)
+    b = batch.Batch(lanes, d1, c2.raw_bytes, s1, 0, db, d2, cb.raw_bytes, s2, quarters=quarters)
     return b, (d1, s1, d2, s2)
 
 
+@pytest.mark.parametrize("quarters", [True, False])
 @pytest.mark.parametrize("order", ["index", "size", "random"])
-def test_batch_matches_data(gpu, order):
+def test_batch_matches_data(gpu, order, quarters):
     from compression_amd import batch
     x, c2, lanes, y, cb, db = _parts(gpu, 1 << 21, 1 << 23)
-    b, (d1, s1, d2, s2) = _batch(gpu, lanes, c2, db, cb)
+    b, (d1, s1, d2, s2) = _batch(gpu, lanes, c2, db, cb, quarters)
     if order == "size":
-        b.set_order(batch.size_order(np.diff(np.append(c2.blk_off[:-1], c2.payload_bytes).astype(np.int64)),
-                                     np.diff(cb.ptr.astype(np.int64))))
+        b.set_order(b.size_order())
     elif order == "random":
-        items = np.concatenate([batch.item(0, np.arange(c2.n_chunks)), batch.item(1, np.arange(cb.n_chunks))])
-        b.set_order(np.random.default_rng(1).permutation(items))
+        b.set_order(np.random.default_rng(1).permutation(b.all_items()))
     b.submit()
     assert (s1.download(dtype=np.int32) == 0).all() and (s2.download(dtype=np.int32) == 0).all()
     assert np.array_equal(d1.download(c2.raw_bytes), x.view(np.uint8))

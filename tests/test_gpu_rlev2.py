@@ -7,7 +7,7 @@ from oracle import pyoracle as O
 pytestmark = pytest.mark.gpu
 
 
-def _run(mdz, c, dst_bytes=None, payload=None, col_len=None, max_chunk=0):
+def _run(mdz, c, dst_bytes=None, payload=None, col_len=None, max_chunk=0, quarters=False):
     from compression_amd import rlev2
     from compression_amd.lanes import DeviceLanes
     pay = c.payload if payload is None else payload
@@ -17,17 +17,18 @@ def _run(mdz, c, dst_bytes=None, payload=None, col_len=None, max_chunk=0):
     dst = mdz.DeviceBuffer(max(n, 1))
     dst.fill(0xCD)
     st = mdz.DeviceBuffer(4 * c.n_chunks)
-    rlev2.decompress(lanes, dst, n, st, max_chunk_bytes=max_chunk)
+    rlev2.decompress(lanes, dst, n, st, max_chunk_bytes=max_chunk, quarters=quarters)
     return dst.download(n), st.download(4 * c.n_chunks, np.int32)
 
 
+@pytest.mark.parametrize("quarters", [True, False])
 @pytest.mark.parametrize("dtype,ru", [(np.uint64, 1), (np.uint64, 8), (np.uint64, 64),
                                       (np.uint64, 512), (np.uint32, 64), (np.uint16, 32),
                                       (np.uint8, 16)])
-def test_config2_shapes(gpu, dtype, ru):
+def test_config2_shapes(gpu, dtype, ru, quarters):
     x = O.gen_rlev2_i64(1 << 18, 0x4D445A02).astype(dtype)
     c = O.rlev2_encode(x, 131072, ru, threads=8)
-    got, st = _run(gpu, c)
+    got, st = _run(gpu, c, quarters=quarters)
     assert (st == 0).all()
     assert np.array_equal(got, x.view(np.uint8))
 
@@ -89,8 +90,9 @@ def test_host_wrapper(gpu):
     assert out == x.tobytes()
 
 
+@pytest.mark.parametrize("quarters", [True, False])
 @pytest.mark.parametrize("seed", [1, 2, 3])
-def test_random_corruption_matches_oracle(gpu, seed):
+def test_random_corruption_matches_oracle(gpu, seed, quarters):
     # one random byte overwritten in most chunks (header or data) and a few
     # lane lengths changed: every chunk's status matches the oracle, and
     # chunks the oracle decodes cleanly are bit-exact
@@ -107,7 +109,7 @@ def test_random_corruption_matches_oracle(gpu, seed):
         if rng.random() < 0.2:
             t = ch * 32 + int(rng.integers(0, 32))
             col[t] = max(0, int(col[t]) - int(rng.integers(1, 40)))
-    got, st = _run(gpu, c, payload=pay, col_len=col)
+    got, st = _run(gpu, c, payload=pay, col_len=col, quarters=quarters)
     ref = O.RleV2Container(pay, col, c.blk_off, c.n_chunks, c.chunk, c.elem_bytes, c.read_unit,
                            c.raw_bytes, c.payload_bytes)
     _, ref_out, ref_st = O.rlev2_decode(ref, threads=4)
