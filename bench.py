@@ -32,8 +32,8 @@ HBM_PEAK_GBS = 8000.0
 # FETCH_SIZE and WRITE_SIZE scaled by the copy-kernel calibration; the
 # summaries these come from are under profiles/.
 TRAFFIC: dict = {
-    "batch_kernel<8> (rle_v2+byte_rle)": 1780582931,   # profiles/config2_rlev2_brle_batched.md
-    "rlev2_i64_1GiB_plain": 1418166298,
+    "batch_kernel<8> (rle_v2+byte_rle)": 1792273658,   # profiles/config2_rlev2_brle_batched.md
+    "rlev2_i64_1GiB_plain": 1430079078,
     "brle_u8_256MiB_plain": 547122419,
 }
 
