@@ -5,18 +5,36 @@
 // (decoder.h:296-411): int8 h < 0 -> -h LEB128 literals; h >= 0 -> int8 delta,
 // LEB128 base, h+3 values base + i*delta (64-bit arithmetic, truncated to T).
 //
-// One workgroup (4 wave64s) per chunk:
-//   1. stage: all 256 threads read the chunk's interleaved payload from HBM
-//      once with 16-byte loads and scatter each 4-byte word to its lane's
-//      contiguous region in LDS (the inverse of the compacted round-robin
-//      interleave, via the lane-address table);
-//   2. scan: one thread per lane stream walks its tokens -- literal groups
-//      are skipped by counting LEB128 terminators 16 bytes at a time -- and
-//      records the decoder state at every K-th value (8 segments per lane);
-//   3. decode: 256 threads, one per (lane, segment), each decode exactly K
-//      values from their checkpoint and write their lines with 16-byte stores.
-// A chunk whose payload does not fit the LDS budget falls back to a
-// single-wave decoder that streams the chunk through an LDS window.
+// Pseudo-varints.  Cut a lane stream after every LEB128 terminator (a byte
+// with bit 7 clear); call the pieces pseudo-varints.  Every token starts at
+// a cut: a literal header has bit 7 set, so it joins the first literal's
+// pseudo-varint and a group of n literals spans exactly n of them; a run
+// header (bit 7 clear) is one by itself, and the delta byte is one too (bit 7
+// clear) or joins the base value's (bit 7 set), so a run spans 3 or 2.
+//
+// One workgroup (8 wave64s) per chunk:
+//   1. setup and staging: the chunk's interleaved words are scattered into
+//      per-lane LDS regions (the same code as RLE v2);
+//   2. cut table: 16 threads per lane count the lane's terminators, then list
+//      where each pseudo-varint starts: cut[t][0] = 0, cut[t][i+1] = one past
+//      the i-th terminator (16-bit entries);
+//   3. walk: 32 threads of wave 0, one per lane, follow the token chain in
+//      pseudo-varint steps (one table read and one data read per token),
+//      check each header as the oracle does, and record for every group of
+//      G values the token covering its first value;
+//   4. decode: every thread builds P-byte pieces of the chunk's output in
+//      output order (coalesced stores).  From its group's token it steps to
+//      the token covering the piece; a literal value is decoded straight from
+//      its pseudo-varint (start and length from the table), a run value is
+//      base + i*delta.
+// A lane's status is its first error in stream order.  Header errors (walk)
+// and value errors (decode) carry the key (value index, step) and are
+// combined with an atomic minimum; the chunk reports its lowest failing lane,
+// as the oracle does.  Values after a lane's first error are unspecified.
+// A chunk whose lanes do not fit the LDS budget, with a lane stream over
+// 65535 bytes, or whose cut table does not fit, is decoded by a single-wave
+// decoder that streams the payload through an LDS window.
+#include "lanes_dev.h"
 #include "mdz_device.h"
 #include "mdz_internal.h"
 
@@ -24,9 +42,10 @@ namespace mdz {
 namespace {
 
 enum : uint32_t { kHdr = 0, kDelta = 1, kBase = 2, kRun = 3, kLit = 4, kDone = 5 };
-constexpr uint32_t kSegs = 16;           // segments per lane stream
-constexpr uint32_t kThreads = 32 * kSegs;
-constexpr uint32_t kBad = 0xffffffffu;
+constexpr uint32_t kNT = 512;            // threads per workgroup
+constexpr uint32_t kGroups = 64;         // value groups (walk checkpoints) per lane
+constexpr uint32_t kCutThreads = kNT / 32;   // threads per lane building the cut table
+constexpr uint32_t kNoErr = 0xffffffffu;
 
 template <int EB> struct VarintSpec;
 template <> struct VarintSpec<1> { static constexpr uint32_t kMax = 2; };
@@ -74,130 +93,6 @@ MDZ_DEV uint32_t varint_decode(uint64_t lo, uint64_t hi, uint32_t avail, uint64_
             v |= h << 56;
         }
     }
-    val = v;
-    return len;
-}
-
-// 8 stream bytes starting at byte p of a 16-byte-aligned LDS lane region.
-MDZ_DEV uint64_t peek8(const uint8_t* base, uint32_t p) {
-    const uint32_t* w = reinterpret_cast<const uint32_t*>(base) + (p >> 2);
-    const uint32_t w0 = w[0], w1 = w[1], w2 = w[2], k = p & 3;
-    return ((uint64_t)alignbyte(w2, w1, k) << 32) | alignbyte(w1, w0, k);
-}
-
-template <int EB>
-MDZ_DEV uint32_t read_varint(const uint8_t* base, uint32_t p, uint32_t end, uint64_t& val,
-                             int32_t& err) {
-    const uint64_t lo = peek8(base, p);
-    uint64_t hi = 0;
-    if constexpr (EB == 8) {
-        if ((~lo & 0x8080808080808080ull) == 0) hi = peek8(base, p + 8);
-    }
-    return varint_decode<EB>(lo, hi, end > p ? end - p : 0, val, err);
-}
-
-// ---------------------------------------------------------------- tables --
-struct SplitTables {
-    uint32_t lbase[32];      // lane region byte offsets in the data area (16-aligned)
-    uint32_t cl[32];
-    uint32_t err;            // min over (lane*kSegs+segment) << 8 | status byte
-    uint32_t fits;           // lane regions fit the LDS data area
-    uint32_t extent_ok;      // lane lengths lie within the chunk's payload extent
-    uint32_t nblk;           // 16-byte blocks in the data area in use
-    union {
-        LaneTable lt;                        // stage phase
-        struct {                             // scan -> decode hand-off
-            uint32_t p[kSegs - 1][32];
-            uint32_t m[kSegs - 1][32];      // mode | rem << 8 | delta << 24
-            uint32_t vlo[kSegs - 1][32];
-            uint32_t vhi[kSegs - 1][32];
-        } ck;
-    };
-};
-
-MDZ_DEV uint32_t pack_err(uint32_t t, uint32_t j, int32_t e) {
-    return ((t * kSegs + j) << 8) | ((uint32_t)e & 0xffu);
-}
-
-// LEB128 terminators (bytes with bit 7 clear) in a 16-byte block, as four
-// dword masks, ignoring bytes before `from`.
-MDZ_DEV void block_terms(u32x4 x, uint32_t from, uint32_t m[4]) {
-    m[0] = ~x.x & 0x80808080u; m[1] = ~x.y & 0x80808080u;
-    m[2] = ~x.z & 0x80808080u; m[3] = ~x.w & 0x80808080u;
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        const int lo = (int)from - 4 * i;
-        if (lo >= 4) m[i] = 0;
-        else if (lo > 0) m[i] &= 0xffffffffu << (8 * lo);
-    }
-}
-
-// Byte offset (within the block) just past the k-th set terminator (k >= 1).
-MDZ_DEV uint32_t block_select(const uint32_t m[4], uint32_t k) {
-    uint32_t di = 0, mm = m[0], c = __popc(m[0]);
-    if (k > c) { k -= c; di = 1; mm = m[1]; c = __popc(m[1]);
-        if (k > c) { k -= c; di = 2; mm = m[2]; c = __popc(m[2]);
-            if (k > c) { k -= c; di = 3; mm = m[3]; } } }
-    return di * 4 + (select_bit(mm, k - 1) >> 3) + 1;
-}
-
-// Position just past the k-th LEB128 terminator at or after q in a lane
-// stream (`base`, `end` bytes).  `bcnt` holds the terminator count of every
-// 16-byte block of the LDS data area; the lane's blocks start at `blk0`.
-// Whole blocks are skipped 16 counts per LDS read.  kBad if the stream ends
-// first.
-MDZ_DEV uint32_t skip_varints(const uint8_t* base, const uint8_t* bcnt, uint32_t blk0, uint32_t q,
-                              uint32_t k, uint32_t end) {
-    if (k == 0) return q;
-    if (q >= end) return kBad;
-    const u32x4* blk = reinterpret_cast<const u32x4*>(base);
-    const uint32_t last = (end - 1) >> 4;
-    uint32_t b = q >> 4;
-    uint32_t m[4];
-    block_terms(blk[b], q & 15, m);
-    const uint32_t c = __popc(m[0]) + __popc(m[1]) + __popc(m[2]) + __popc(m[3]);
-    if (c < k) {
-        k -= c;
-        uint32_t gb = blk0 + b + 1;                 // global block index
-        const uint32_t glast = blk0 + last;
-        for (;;) {                                  // 16 block counts per LDS read
-            if (gb > glast) return kBad;
-            const uint32_t a = gb & ~15u;
-            const u32x4 cn = *reinterpret_cast<const u32x4*>(bcnt + a);
-            const uint32_t w[4] = {cn.x, cn.y, cn.z, cn.w};
-            bool found = false;
-            for (uint32_t i = gb - a; i < 16; ++i) {
-                if (a + i > glast) return kBad;
-                const uint32_t ci = (w[i >> 2] >> (8 * (i & 3))) & 0xffu;
-                if (ci >= k) { gb = a + i; found = true; break; }
-                k -= ci;
-            }
-            if (found) break;
-            gb = a + 16;
-        }
-        b = gb - blk0;
-        block_terms(blk[b], 0, m);
-    }
-    const uint32_t pos = b * 16 + block_select(m, k);
-    return pos <= end ? pos : kBad;
-}
-
-// 32-bit LEB128 for T of up to 4 bytes: bytes 0..3 in lo, byte 4 in b4.
-// Same results and status codes as varint_decode.
-template <int EB>
-MDZ_DEV uint32_t varint32(uint32_t lo, uint32_t b4, uint32_t avail, uint32_t& val, int32_t& err) {
-    constexpr uint32_t kMax = VarintSpec<EB>::kMax;
-    constexpr uint32_t tmask = kMax >= 4 ? 0x80808080u : kMax == 3 ? 0x808080u : 0x8080u;
-    const uint32_t t = ~lo & tmask;
-    uint32_t len = t ? (__builtin_ctz(t) >> 3) + 1 : (kMax >= 5 && !(b4 & 0x80) ? 5 : 0);
-    if (len == 0 || len > avail) {
-        err = (len == 0 && avail >= kMax) ? MDZ_ST_BAD_SYMBOL : MDZ_ST_INPUT_OVERRUN;
-        return 0;
-    }
-    uint32_t v = (lo & 0x7fu) | ((lo >> 1) & 0x3f80u) | ((lo >> 2) & 0x1fc000u) |
-                 ((lo >> 3) & 0xfe00000u);
-    if (len < 4) v = __builtin_amdgcn_ubfe(v, 0, 7 * len);
-    else if (len == 5) v |= b4 << 28;
     val = v;
     return len;
 }
@@ -294,312 +189,388 @@ MDZ_DEV int32_t rlev1_windowed(const u32x4* __restrict__ src, LaneTable& T, uint
     return __shfl(err, (int)__builtin_ctzll(bad), kWave);
 }
 
-// ------------------------------------------------------------ the kernel --
-// Scan: records the decoder state at values K, 2K, ... of lane t's stream.
-template <int EB>
-MDZ_DEV void scan_lane(SplitTables& tb, const uint8_t* base, const uint8_t* bcnt, uint32_t blk0,
-                       uint32_t end, uint32_t t, uint32_t nvals, uint32_t K) {
-    uint32_t p = 0, v = 0, jn = 1, target = K;
-    int32_t e = MDZ_ST_OK;
-    auto record = [&](uint32_t pos, uint32_t mode, uint32_t rem, uint64_t rv, int32_t d) {
-        tb.ck.p[jn - 1][t] = pos;
-        tb.ck.m[jn - 1][t] = mode | (rem << 8) | (((uint32_t)d & 0xffu) << 24);
-        tb.ck.vlo[jn - 1][t] = (uint32_t)rv;
-        tb.ck.vhi[jn - 1][t] = (uint32_t)(rv >> 32);
-        ++jn;
-        target += K;
+
+// ---------------------------------------------------------------- tables --
+struct V1Tables {
+    uint32_t lbase[32];      // lane region byte offsets in the data area (16-aligned)
+    uint32_t cl[32];         // lane stream bytes
+    uint32_t lerr[32];       // first error of each lane: (4 * value + step) << 8 | status byte
+    uint32_t sW[32];         // interleave segments (lane_setup)
+    uint32_t S[32];
+    uint32_t M[32];
+    uint32_t nterm[32];      // terminators of each lane stream
+    uint32_t cut0[32];       // first cut-table entry of each lane
+    uint32_t vstop[32];      // values the walk covered (later values are not decoded)
+    uint32_t total;          // words of all lanes
+    uint32_t fits;           // lane regions fit the data budget
+    uint32_t extent_ok;      // lanes lie inside the chunk's payload extent
+    uint32_t used;           // bytes of the lane regions
+    uint32_t fast;           // the chunk takes the table path
+    uint32_t pad[3];
+    union {
+        struct {                                 // staging
+            uint32_t dest[32][32];
+            uint8_t segof[kSegBlocks];
+        } st;
+        uint32_t ck[32][kGroups];                // walk -> decode: token k | v << 16
+        LaneTable lt;                            // windowed decoder
     };
-    while (jn < kSegs) {
-        if (v == target) { record(p, kHdr, 0, 0, 0); continue; }
-        if (p >= end) break;
-        const uint64_t x = peek8(base, p);
-        const uint32_t h = (uint32_t)(x & 0xff);
-        if (h & 0x80) {
-            const uint32_t n = 256 - h;
-            if (n > nvals - v) break;
-            uint32_t q = p + 1, done = 0;
-            while (jn < kSegs && target < v + n) {
-                const uint32_t k = target - v - done;
-                q = skip_varints(base, bcnt, blk0, q, k, end);
-                if (q == kBad) break;
-                done += k;
-                record(q, kLit, n - done, 0, 0);
-            }
-            if (q == kBad) break;
-            q = skip_varints(base, bcnt, blk0, q, n - done, end);
-            if (q == kBad) break;
-            p = q;
-            v += n;
-        } else {
-            const uint32_t cnt = h + 3;
-            if (cnt > nvals - v || p + 1 >= end) break;
-            const int32_t d = (int32_t)(int8_t)(x >> 8);
-            uint64_t bv;
-            const uint32_t len = read_varint<EB>(base, p + 2, end, bv, e);
-            if (len == 0) break;
-            const uint32_t pe = p + 2 + len;
-            while (jn < kSegs && target < v + cnt) {
-                const uint32_t i = target - v;
-                record(pe, kRun, cnt - i, bv + (uint64_t)(int64_t)d * i, d);
-            }
-            p = pe;
-            v += cnt;
-        }
-    }
-    while (jn < kSegs) record(0, kDone, 0, 0, 0);    // past a damaged token
+};
+static_assert(sizeof(V1Tables) % 16 == 0, "data area alignment");
+
+MDZ_DEV uint32_t err_key(uint32_t v, uint32_t step, int32_t code) {
+    return ((4 * v + step) << 8) | ((uint32_t)code & 0xffu);
 }
 
-// Decode values [v0, v0+K) of one lane for T of up to 4 bytes: 32-bit
-// arithmetic, the next token's header folded into the step of its first value.
-template <int EB, int SW>
-MDZ_DEV int32_t decode_seg32(const uint8_t* base, uint32_t end, uint32_t p, uint32_t mode,
-                             uint32_t rem, uint32_t rv, uint32_t d, uint32_t v0, uint32_t K,
-                             uint32_t nvals, LineEmitter<EB, SW>& em) {
-    for (uint32_t i = 0; i < K; ++i) {
-        if (rem == 0) {
-            if (p >= end) return MDZ_ST_INPUT_OVERRUN;
-            const uint64_t x = peek8(base, p);
-            const uint32_t lo = (uint32_t)x, h = lo & 0xffu;
-            rem = (h & 0x80) ? 256 - h : h + 3;
-            if (rem > nvals - v0 - i) return MDZ_ST_OUTPUT_OVERRUN;
-            if (h & 0x80) {
-                mode = kLit;
-                p += 1;
-            } else {
-                if (p + 1 >= end) return MDZ_ST_INPUT_OVERRUN;
-                d = (uint32_t)(int32_t)(int8_t)(lo >> 8);
-                int32_t e = MDZ_ST_OK;
-                const uint32_t hi = (uint32_t)(x >> 32);
-                const uint32_t len = varint32<EB>(alignbyte(hi, lo, 2), (hi >> 16) & 0xffu,
-                                                  end - p - 2, rv, e);
-                if (len == 0) return e;
-                p += 2 + len;
-                mode = kRun;
-            }
+// Two bytes at byte p of a 4-byte-aligned LDS region (byte p in bits 0..7).
+MDZ_DEV uint32_t lds_u16_at(const uint8_t* base, uint32_t p) {
+    const uint32_t* w = reinterpret_cast<const uint32_t*>(base) + (p >> 2);
+    return alignbyte(w[1], w[0], p & 3) & 0xffffu;
+}
+
+// Value of the LEB128 of `len` bytes (1 <= len <= its maximum) at byte a of
+// a 4-byte-aligned LDS region.  Bits past 64 are dropped, as the oracle does.
+template <int EB>
+MDZ_DEV uint64_t leb_at(const uint8_t* base, uint32_t a, uint32_t len) {
+    const uint32_t* w = reinterpret_cast<const uint32_t*>(base) + (a >> 2);
+    const uint32_t k = a & 3;
+    if constexpr (EB <= 4) {
+        const uint32_t w0 = w[0], w1 = w[1];
+        const uint32_t lo = alignbyte(w1, w0, k);
+        uint32_t v = (lo & 0x7fu) | ((lo >> 1) & 0x3f80u) | ((lo >> 2) & 0x1fc000u) |
+                     ((lo >> 3) & 0xfe00000u);
+        if (len < 4) v &= (1u << (7 * len)) - 1u;
+        else if (len > 4) v |= ((w1 >> (8 * k)) & 0x7fu) << 28;
+        return v;
+    } else {
+        const uint32_t w0 = w[0], w1 = w[1], w2 = w[2], w3 = w[3];
+        const uint64_t lo = ((uint64_t)alignbyte(w2, w1, k) << 32) | alignbyte(w1, w0, k);
+        uint64_t v = gather7x8(lo);
+        if (len < 8) {
+            v &= (1ull << (7 * len)) - 1;
+        } else if (len > 8) {
+            const uint32_t hi = alignbyte(w3, w2, k);
+            uint64_t h = (hi & 0x7fu) | ((hi >> 1) & (0x7fu << 7));
+            h &= (1ull << (7 * (len - 8))) - 1;
+            v |= h << 56;
         }
-        uint32_t val;
-        if (mode == kLit) {
-            const uint64_t x = peek8(base, p);
-            int32_t e = MDZ_ST_OK;
-            const uint32_t len = varint32<EB>((uint32_t)x, (uint32_t)(x >> 32) & 0xffu,
-                                              end - p, val, e);
-            if (len == 0) return e;
-            p += len;
-        } else {
-            val = rv;
-            rv += d;
-        }
-        em.push(val);
-        --rem;
+        return v;
     }
+}
+
+// Pseudo-varint j of a lane read as a LEB128 value, skipping its first byte
+// when `skip` (a literal or delta byte joined to it).  j < nterm: complete,
+// its length is cut[j+1] - start; j == nterm: the lane's unterminated tail.
+// Returns the status the oracle's LEB128 reader gives there.
+template <int EB>
+MDZ_DEV int32_t pseudo_value(const uint8_t* base, const uint16_t* cut, uint32_t j, uint32_t nterm,
+                             uint32_t cl, uint32_t skip, uint64_t& val) {
+    constexpr uint32_t kMax = VarintSpec<EB>::kMax;
+    const uint32_t a = cut[j] + skip;
+    if (j >= nterm) {
+        val = 0;
+        return cl - a >= kMax ? MDZ_ST_BAD_SYMBOL : MDZ_ST_INPUT_OVERRUN;
+    }
+    const uint32_t len = cut[j + 1] - a;
+    if (len > kMax) { val = 0; return MDZ_ST_BAD_SYMBOL; }
+    val = leb_at<EB>(base, a, len);
     return MDZ_ST_OK;
 }
 
-// Generic decode (any T, 64-bit arithmetic).
-template <int EB, int SW>
-MDZ_DEV int32_t decode_seg64(const uint8_t* base, uint32_t end, uint32_t p, uint32_t mode,
-                             uint32_t rem, uint64_t rv, int64_t d, uint32_t v0, uint32_t K,
-                             uint32_t nvals, LineEmitter<EB, SW>& em) {
+// One token of a lane: pseudo-varint k starts it; its first two bytes are
+// h (header) and, for a run, the delta byte.
+struct Token {
+    uint32_t k, v;           // first pseudo-varint, first value
+    uint32_t hd;             // header | delta << 8
+    MDZ_DEV bool lit() const { return hd & 0x80u; }
+    MDZ_DEV uint32_t count() const { return lit() ? 256u - (hd & 0xffu) : (hd & 0xffu) + 3u; }
+    MDZ_DEV uint32_t next_k() const { return k + (lit() ? count() : (hd & 0x8000u) ? 2u : 3u); }
+};
+
+// Writes one output piece of P bytes (words w, byte i of the piece in
+// w[i / 4] bits 8 * (i % 4)) at chunk offset o, clipped to `limit`.
+template <int P>
+MDZ_DEV void store_piece(uint8_t* out, uint32_t o, uint64_t limit, const uint32_t (&w)[4]) {
+    uint8_t* p = out + o;
+    if (o + P <= limit) {
+        if constexpr (P == 16) *reinterpret_cast<uint4*>(p) = make_uint4(w[0], w[1], w[2], w[3]);
+        else if constexpr (P == 8) *reinterpret_cast<uint2*>(p) = make_uint2(w[0], w[1]);
+        else if constexpr (P == 4) *reinterpret_cast<uint32_t*>(p) = w[0];
+        else if constexpr (P == 2) *reinterpret_cast<uint16_t*>(p) = (uint16_t)w[0];
+        else *p = (uint8_t)w[0];
+    } else {
+        for (uint32_t i = 0; i < (uint32_t)P && o + i < limit; ++i)
+            p[i] = (uint8_t)(w[i >> 2] >> (8 * (i & 3)));
+    }
+}
+
+// Decodes the piece of lane t holding values [v0, v0 + P / EB).  The walk
+// covered values below vs; `ck` is the token covering value (v0 >> gsh) << gsh.
+// Value errors go to lerr[t] as an atomic minimum.
+template <int EB, int P>
+MDZ_DEV void decode_piece(const uint8_t* base, const uint16_t* cut, uint32_t nterm, uint32_t cl,
+                          uint32_t ck, uint32_t v0, uint32_t vs, uint32_t* lerr,
+                          uint32_t (&w)[4]) {
+    constexpr int NV = P / EB;
+    constexpr uint64_t vmask = EB == 8 ? ~0ull : ((1ull << (8 * EB)) - 1);
+    Token tk;
+    tk.k = ck & 0xffffu;
+    tk.v = ck >> 16;
+    tk.hd = lds_u16_at(base, cut[tk.k]);
+    while (tk.v + tk.count() <= v0) {             // tokens between the checkpoint and v0
+        const uint32_t n = tk.count();
+        tk.k = tk.next_k();
+        tk.v += n;
+        tk.hd = lds_u16_at(base, cut[tk.k]);
+    }
+    uint64_t rbase = 0;
+    bool have_base = false;
     int32_t e = MDZ_ST_OK;
-    for (uint32_t i = 0; i < K;) {
-        if (mode == kHdr) {
-            if (p >= end) return MDZ_ST_INPUT_OVERRUN;
-            const uint64_t x = peek8(base, p);
-            const uint32_t h = (uint32_t)(x & 0xff);
-            if (h & 0x80) {
-                rem = 256 - h;
-                if (rem > nvals - v0 - i) return MDZ_ST_OUTPUT_OVERRUN;
-                mode = kLit;
-                p += 1;
-            } else {
-                rem = h + 3;
-                if (rem > nvals - v0 - i) return MDZ_ST_OUTPUT_OVERRUN;
-                if (p + 1 >= end) return MDZ_ST_INPUT_OVERRUN;
-                d = (int64_t)(int8_t)(x >> 8);
-                const uint32_t len = read_varint<EB>(base, p + 2, end, rv, e);
-                if (len == 0) return e;
-                p += 2 + len;
-                mode = kRun;
-            }
-            continue;
+    uint32_t ekey = kNoErr;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) w[i] = 0;
+#pragma unroll
+    for (int i = 0; i < NV; ++i) {
+        const uint32_t vi = v0 + i;
+        if (vi >= vs) break;
+        if (vi >= tk.v + tk.count()) {            // the next token starts inside the piece
+            const uint32_t n = tk.count();
+            tk.k = tk.next_k();
+            tk.v += n;
+            tk.hd = lds_u16_at(base, cut[tk.k]);
+            have_base = false;
         }
         uint64_t val;
-        if (mode == kRun) {
-            val = rv;
-            rv += (uint64_t)d;
+        if (tk.lit()) {
+            const uint32_t j = tk.k + (vi - tk.v);
+            if (j > nterm) break;                 // past the lane's unterminated tail (an
+                                                  // earlier value reports the error)
+            e = pseudo_value<EB>(base, cut, j, nterm, cl, j == tk.k, val);
+            if (e != MDZ_ST_OK && ekey == kNoErr) ekey = err_key(vi, 2, e);
         } else {
-            const uint32_t len = read_varint<EB>(base, p, end, val, e);
-            if (len == 0) return e;
-            p += len;
+            if (!have_base) {
+                const bool joined = tk.hd & 0x8000u;    // delta byte joined to the base's
+                e = pseudo_value<EB>(base, cut, tk.k + (joined ? 1 : 2), nterm, cl, joined, rbase);
+                if (e != MDZ_ST_OK && ekey == kNoErr) ekey = err_key(tk.v, 2, e);
+                rbase &= vmask;
+                have_base = true;
+            }
+            const int64_t d = (int64_t)(int8_t)(tk.hd >> 8);
+            val = rbase + (uint64_t)(vi - tk.v) * (uint64_t)d;
         }
-        em.push(val);
-        ++i;
-        if (--rem == 0) mode = kHdr;
+        val &= vmask;
+        if constexpr (EB == 8) {
+            w[2 * i] = (uint32_t)val;
+            w[2 * i + 1] = (uint32_t)(val >> 32);
+        } else if constexpr (EB == 4) {
+            w[i] = (uint32_t)val;
+        } else {
+            w[(i * EB) >> 2] |= (uint32_t)val << (8 * ((i * EB) & 3));
+        }
     }
-    return MDZ_ST_OK;
+    if (ekey != kNoErr) atomicMin(lerr, ekey);
 }
 
-template <int EB, int SW>
-__global__ __launch_bounds__(kThreads) void rlev1_decode_kernel(
+// ------------------------------------------------------------ the kernel --
+// `lds_bytes` is the dynamic LDS size; `data_cap` bounds the lane regions
+// for the table path; `ring_words` (a power of two) sizes the window of the
+// streaming decoder.
+template <int EB, int P>
+__global__ __launch_bounds__(kNT) void rlev1_decode_kernel(
     const uint8_t* __restrict__ payload, const uint64_t* __restrict__ blk_off,
     const uint64_t* __restrict__ col_len, uint64_t n_chunks, uint64_t chunk_bytes,
     uint32_t line, uint8_t* __restrict__ dst, uint64_t dst_bytes, int32_t* __restrict__ status,
-    uint32_t data_cap) {
+    uint32_t data_cap, uint32_t ring_words, uint32_t lds_bytes) {
     extern __shared__ __align__(16) uint32_t smem[];
-    uint8_t* const data = reinterpret_cast<uint8_t*>(smem);
-    uint8_t* const bcnt = data + data_cap + 16;                 // data_cap / 16 counts
-    SplitTables& tb = *reinterpret_cast<SplitTables*>(bcnt + data_cap / 16);
+    V1Tables& tb = *reinterpret_cast<V1Tables*>(smem);
+    uint8_t* const data = reinterpret_cast<uint8_t*>(smem) + sizeof(V1Tables);
 
     const uint64_t c = blockIdx.x;
     const uint32_t tid = threadIdx.x, wave = tid >> 6, l = tid & 63;
+    MDZ_PHASE_START();
     const uint32_t nvals = (uint32_t)(chunk_bytes / 32 / EB);
-    const uint32_t K = nvals / kSegs;
     const uint64_t out_base = c * chunk_bytes;
     const uint64_t limit = dst_bytes > out_base ? min(dst_bytes - out_base, chunk_bytes) : 0;
-    const uint64_t c_off = blk_off[c], c_len = blk_off[c + 1] - c_off;
+    const uint64_t c_off = blk_off[c], c_end = blk_off[c + 1];
     const u32x4* src = reinterpret_cast<const u32x4*>(payload + c_off);
 
-    // ---- lane table and lane regions (wave 0)
+    // ---- 1. lane geometry (wave 0)
     if (wave == 0) {
-        const uint32_t t = l & 31;
-        const uint32_t cl = (uint32_t)col_len[c * 32 + t];
-        lane_table_build(tb.lt, (cl + 3) / 4);
-        const uint32_t r16 = (cl + 15) & ~15u;
-        uint32_t incl = l < 32 ? r16 : 0;
-#pragma unroll
-        for (int o = 1; o < 32; o <<= 1) {
-            uint32_t y = __shfl_up(incl, o, kWave);
-            if ((l & 31) >= (uint32_t)o) incl += y;
-        }
-        if (l < 32) { tb.lbase[t] = incl - r16; tb.cl[t] = cl; }
-        if (l == 31) { tb.fits = incl <= data_cap; tb.nblk = incl / 16; }
+        const uint32_t used = lane_setup<32>(tb, col_len, c, 0, data_cap, c_off, c_end);
+        const uint32_t cl = (uint32_t)col_len[c * 32 + (l & 31)];
+        const bool big = __ballot(cl > 0xffffu) != 0 || nvals > 0xffffu;
         if (l == 0) {
-            tb.err = 0xffffffffu;
-            tb.extent_ok = (uint64_t)tb.lt.total * 4 <= c_len && !(c_off & 15);
+            tb.used = used;
+            tb.fast = !big;
         }
     }
     __syncthreads();
-
     if (!tb.extent_ok) {
         if (tid == 0 && status) status[c] = MDZ_ST_INPUT_OVERRUN;
         return;
     }
-    if (!tb.fits) {
+    const uint32_t used = tb.used;
+    uint16_t* const cut = reinterpret_cast<uint16_t*>(data + used + 16);
+    const uint32_t cut_cap = (lds_bytes - (uint32_t)sizeof(V1Tables) - used - 16) / 2;
+    bool fast = tb.fits && tb.fast;
+    if (fast) {
+        stage_tables(tb, tid, kNT);
+        __syncthreads();
+        MDZ_PHASE(0);
+        stage_lanes(tb, reinterpret_cast<uint32_t*>(data), src, tid, kNT);
+        __syncthreads();
+        MDZ_PHASE(1);
+
+        // ---- 2. cut table: thread j of lane t takes a contiguous share of its blocks
+        const uint32_t t = tid / kCutThreads, j = tid % kCutThreads;
+        const uint32_t cl = tb.cl[t], nb = (cl + 15) / 16;
+        const uint32_t b0 = j * nb / kCutThreads, b1 = (j + 1) * nb / kCutThreads;
+        const u32x4* blk = reinterpret_cast<const u32x4*>(data + tb.lbase[t]);
+        auto term_mask = [&](uint32_t b, uint32_t i, uint32_t x) {
+            uint32_t m = ~x & 0x80808080u;
+            const int rest = (int)cl - (int)(16 * b + 4 * i);   // stream bytes in this word
+            if (rest < 4) m = rest <= 0 ? 0u : m & ((1u << (8 * rest)) - 1u);
+            return m;
+        };
+        uint32_t n = 0;
+        for (uint32_t b = b0; b < b1; ++b) {
+            const u32x4 x = blk[b];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) n += __popc(term_mask(b, i, x[i]));
+        }
+        uint32_t incl = n;
+#pragma unroll
+        for (int o = 1; o < (int)kCutThreads; o <<= 1) {
+            const uint32_t y = __shfl_up(incl, o, kCutThreads);
+            if (j >= (uint32_t)o) incl += y;
+        }
+        if (j == kCutThreads - 1) tb.nterm[t] = incl;
+        __syncthreads();
         if (wave == 0) {
-            uint32_t ring_words = 256;
-            while (ring_words * 2 <= data_cap / 4) ring_words *= 2;
-            int32_t st = rlev1_windowed<EB, SW>(src, tb.lt, tb.cl[l & 31], smem, ring_words,
-                                                dst + out_base, limit, nvals, line);
+            const uint32_t nt = l < 32 ? tb.nterm[l] + 1 : 0;
+            const uint32_t ex = wave_incl_scan_u32(nt) - nt;
+            if (l < 32) tb.cut0[l] = ex;
+            if (l == 31) tb.fast = ex + nt <= cut_cap;
+        }
+        __syncthreads();
+        fast = tb.fast;
+        if (fast) {
+            uint16_t* ct = cut + tb.cut0[t];
+            if (j == 0) ct[0] = 0;
+            uint32_t idx = incl - n + 1;
+            for (uint32_t b = b0; b < b1; ++b) {
+                const u32x4 x = blk[b];
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    uint32_t m = term_mask(b, i, x[i]);
+                    while (m) {
+                        const uint32_t bit = __builtin_ctz(m);
+                        m &= m - 1;
+                        ct[idx++] = (uint16_t)(16 * b + 4 * i + (bit >> 3) + 1);
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        MDZ_PHASE(2);
+    }
+    if (!fast) {
+        // ---- streaming decoder (wave 0)
+        if (wave == 0) {
+            const uint32_t cl = tb.cl[l & 31];
+            lane_table_build(tb.lt, (cl + 3) / 4);
+            int32_t st = rlev1_windowed<EB, P>(src, tb.lt, cl, reinterpret_cast<uint32_t*>(data),
+                                              ring_words, dst + out_base, limit, nvals, line);
             if (l == 0 && status) status[c] = st;
         }
         return;
     }
 
-    // ---- 1. stage + de-interleave into per-lane contiguous LDS regions
-    {
-        const uint32_t total = tb.lt.total, ngroups = (total + 3) / 4;
-        uint32_t* d32 = smem;
-        for (uint32_t g0 = tid; g0 < ngroups; g0 += 4 * kThreads) {
-            u32x4 v[4];
-#pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                const uint32_t g = g0 + u * kThreads;
-                if (g < ngroups) v[u] = __builtin_nontemporal_load(src + g);
-            }
-#pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                const uint32_t g = g0 + u * kThreads;
-                if (g >= ngroups) break;
-#pragma unroll
-                for (int i = 0; i < 4; ++i) {
-                    const uint32_t w = 4 * g + i;
-                    if (w < total) {
-                        uint32_t lane, round;
-                        lane_of_word(tb.lt, w, lane, round);
-                        d32[(tb.lbase[lane] >> 2) + round] = v[u][i];
-                    }
-                }
-            }
-        }
-    }
-    __syncthreads();
-    // ---- per-16-byte-block LEB128 terminator counts (bytes past a lane's
-    //      end count too; the scan bounds every position by the lane length)
-    for (uint32_t b = tid; b < tb.nblk; b += kThreads) {
-        const u32x4 x = reinterpret_cast<const u32x4*>(data)[b];
-        bcnt[b] = (uint8_t)(__popc(~x.x & 0x80808080u) + __popc(~x.y & 0x80808080u) +
-                            __popc(~x.z & 0x80808080u) + __popc(~x.w & 0x80808080u));
-    }
-    __syncthreads();
-
-    // ---- 2. scan: one thread per lane stream, 4 per wave
-    if (l < 32 / (kThreads / 64)) {
-        const uint32_t t = l * (kThreads / 64) + wave;
-        scan_lane<EB>(tb, data + tb.lbase[t], bcnt, tb.lbase[t] / 16, tb.cl[t], t, nvals, K);
-    }
-    __syncthreads();
-
-    // ---- 3. decode: thread (lane t, segment j) writes values [jK, (j+1)K)
-    {
-        const uint32_t t = tid & 31, j = tid >> 5;
+    // ---- 3. walk: one thread per lane stream
+    uint32_t gsh = 0;                            // groups of G = 2^gsh values, at most kGroups
+    while (((nvals + (1u << gsh) - 1) >> gsh) > kGroups) ++gsh;
+    if (wave == 0 && l < 32) {
+        const uint32_t t = l;
         const uint8_t* base = data + tb.lbase[t];
-        const uint32_t end = tb.cl[t];
-        uint32_t p = 0, mode = kHdr, rem = 0, vlo = 0, vhi = 0, dd = 0;
-        if (j) {
-            p = tb.ck.p[j - 1][t];
-            const uint32_t m = tb.ck.m[j - 1][t];
-            mode = m & 0xff;
-            rem = (m >> 8) & 0xffff;
-            dd = (uint32_t)(int32_t)(int8_t)(m >> 24);
-            vlo = tb.ck.vlo[j - 1][t];
-            vhi = tb.ck.vhi[j - 1][t];
+        const uint16_t* ct = cut + tb.cut0[t];
+        const uint32_t nterm = tb.nterm[t], cl = tb.cl[t], G = 1u << gsh;
+        uint32_t k = 0, v = 0, err = kNoErr;
+        while (v < nvals && k <= nterm) {
+            const uint32_t s = ct[k];
+            if (s >= cl) { err = err_key(v, 0, MDZ_ST_INPUT_OVERRUN); break; }
+            Token tk;
+            tk.k = k;
+            tk.v = v;
+            tk.hd = lds_u16_at(base, s);
+            const uint32_t cnt = tk.count();
+            if (cnt > nvals - v) { err = err_key(v, 0, MDZ_ST_OUTPUT_OVERRUN); break; }
+            if (!tk.lit() && s + 1 >= cl) { err = err_key(v, 1, MDZ_ST_INPUT_OVERRUN); break; }
+            for (uint32_t g = (v + G - 1) >> gsh; (g << gsh) < v + cnt; ++g) tb.ck[t][g] = k | v << 16;
+            k = tk.next_k();
+            v += cnt;
         }
-        int32_t e = MDZ_ST_OK;
-        if (mode != kDone) {
-            const uint32_t v0 = j * K;
-            LineEmitter<EB, SW> em;
-            em.init(dst + out_base, limit, t, line, (uint64_t)v0 * EB);
-            if constexpr (EB <= 4) {
-                e = decode_seg32<EB, SW>(base, end, p, mode, rem, vlo, dd, v0, K, nvals, em);
-            } else {
-                e = decode_seg64<EB, SW>(base, end, p, mode, rem,
-                                         (uint64_t)vlo | ((uint64_t)vhi << 32),
-                                         (int64_t)(int32_t)dd, v0, K, nvals, em);
-            }
-        }
-        if (e != MDZ_ST_OK) atomicMin(&tb.err, pack_err(t, j, e));
+        tb.vstop[t] = v;
+        tb.lerr[t] = err;
     }
     __syncthreads();
+    MDZ_PHASE(3);
+
+    // ---- 4. decode: P-byte pieces in output order
+    {
+        const float inv_line = __builtin_amdgcn_rcpf((float)line);
+        const uint32_t npieces = (uint32_t)(chunk_bytes / P);
+        uint8_t* const out = dst + out_base;
+        for (uint32_t q = tid; q < npieces; q += kNT) {
+            const uint32_t o = q * P;      // pieces past `limit` are decoded for the status only
+            uint32_t R = (uint32_t)((float)o * inv_line);     // line of the piece
+            if (R * line > o) --R;
+            else if ((R + 1) * line <= o) ++R;
+            const uint32_t t = R & 31, v0 = ((R >> 5) * line + (o - R * line)) / EB;
+            const uint32_t vs = tb.vstop[t];
+            if (v0 >= vs) continue;
+            uint32_t w[4];
+            decode_piece<EB, P>(data + tb.lbase[t], cut + tb.cut0[t], tb.nterm[t], tb.cl[t],
+                                tb.ck[t][v0 >> gsh], v0, vs, &tb.lerr[t], w);
+            store_piece<P>(out, o, limit, w);
+        }
+    }
+    __syncthreads();
+    MDZ_PHASE(4);
     if (tid == 0 && status) {
-        const uint32_t e = tb.err;
-        status[c] = e == 0xffffffffu ? MDZ_ST_OK : (int32_t)(int8_t)(e & 0xff);
+        int32_t st = MDZ_ST_OK;
+        for (int t = 0; t < 32; ++t) {
+            const uint32_t e = tb.lerr[t];
+            if (e != kNoErr) { st = (int32_t)(int8_t)(e & 0xffu); break; }
+        }
+        status[c] = st;
     }
 }
 
-template <int EB, int SW>
+template <int EB, int P>
 hipError_t launch(const mdz_lanes_t& in, void* dst, uint64_t dst_bytes, int32_t* status,
-                  uint32_t data_cap, hipStream_t s) {
-    const size_t smem = (size_t)data_cap + 16 + data_cap / 16 + sizeof(SplitTables);
-    auto k = rlev1_decode_kernel<EB, SW>;
-    hipError_t e = set_dynamic_lds((const void*)k, smem);
+                  uint32_t lds, uint32_t data_cap, uint32_t ring_words, hipStream_t s) {
+    auto k = rlev1_decode_kernel<EB, P>;
+    hipError_t e = set_dynamic_lds((const void*)k, lds);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k, dim3((uint32_t)in.n_chunks), dim3(kThreads), smem, s,
+    hipLaunchKernelGGL(k, dim3((uint32_t)in.n_chunks), dim3(kNT), lds, s,
                        (const uint8_t*)in.payload, in.blk_off, in.col_len, in.n_chunks,
-                       in.chunk_bytes, in.lane_line, (uint8_t*)dst, dst_bytes, status, data_cap);
+                       in.chunk_bytes, in.lane_line, (uint8_t*)dst, dst_bytes, status, data_cap,
+                       ring_words, lds);
     return hipGetLastError();
 }
 
 template <int EB>
 hipError_t launch_eb(const mdz_lanes_t& in, void* dst, uint64_t dst_bytes, int32_t* status,
-                     uint32_t data_cap, uint32_t sw, hipStream_t s) {
-    if (sw == 16) return launch<EB, 16>(in, dst, dst_bytes, status, data_cap, s);
-    if (sw == 8) return launch<EB, 8>(in, dst, dst_bytes, status, data_cap, s);
+                     uint32_t lds, uint32_t data_cap, uint32_t ring_words, uint32_t p,
+                     hipStream_t s) {
+    if (p == 16) return launch<EB, 16>(in, dst, dst_bytes, status, lds, data_cap, ring_words, s);
+    if (p == 8) return launch<EB, 8>(in, dst, dst_bytes, status, lds, data_cap, ring_words, s);
     if constexpr (EB <= 4)
-        if (sw == 4) return launch<EB, 4>(in, dst, dst_bytes, status, data_cap, s);
+        if (p == 4) return launch<EB, 4>(in, dst, dst_bytes, status, lds, data_cap, ring_words, s);
     if constexpr (EB <= 2)
-        if (sw == 2) return launch<EB, 2>(in, dst, dst_bytes, status, data_cap, s);
+        if (p == 2) return launch<EB, 2>(in, dst, dst_bytes, status, lds, data_cap, ring_words, s);
     if constexpr (EB == 1)
-        if (sw == 1) return launch<EB, 1>(in, dst, dst_bytes, status, data_cap, s);
+        if (p == 1) return launch<EB, 1>(in, dst, dst_bytes, status, lds, data_cap, ring_words, s);
     return hipErrorInvalidValue;
 }
 
@@ -611,38 +582,48 @@ int rlev1_decompress(const mdz_lanes_t* in, void* dst, uint64_t dst_bytes, int32
         return MDZ_ERR_INVALID;
     const uint32_t eb = in->elem_bytes, line = in->lane_line;
     const uint64_t chunk = in->chunk_bytes;
-    // each lane stream holds a multiple of 8 values (8 decode segments)
     if (!(eb == 1 || eb == 2 || eb == 4 || eb == 8) || line == 0 || line % eb || chunk % 32 ||
-        (chunk / 32) % line || (chunk / 32 / eb) % kSegs || chunk / 32 / eb < kSegs ||
-        chunk / 32 / eb > 0xffffffu)
+        (chunk / 32) % line || chunk / 32 / eb == 0 || chunk / 32 / eb > 0xffffffu)
         return MDZ_ERR_INVALID;
     if (in->n_chunks == 0) return MDZ_OK;
     if (in->n_chunks > 0x7fffffffull) return MDZ_ERR_UNSUPPORTED;
     if (((uintptr_t)in->payload & 15) != 0) return MDZ_ERR_INVALID;
-    // LDS data area: the caller's bound on one chunk's payload (+ 12 bytes of
-    // 16-byte alignment per lane region), else as much as a workgroup gets.
-    const uint32_t lds_max = ((160 * 1024 - 16 - (uint32_t)sizeof(SplitTables) - 512) * 16 / 17);
-    uint32_t cap = lds_max & ~255u;
+    // LDS: the tables, the lane regions and the cut table (at most one entry
+    // per payload byte, plus one per lane).  With a bound on one chunk's
+    // payload the allocation is sized for it, else the whole budget is taken.
+    constexpr uint32_t kLdsMax = 160 * 1024;
+    const uint32_t fixed = (uint32_t)sizeof(V1Tables) + 16;
+    uint32_t lds = kLdsMax;
     if (tune && tune->max_chunk_bytes) {
-        uint64_t want = ((uint64_t)tune->max_chunk_bytes + 32 * 12 + 255) & ~255ull;
-        if (want < cap) cap = (uint32_t)want;
+        const uint64_t regions = ((uint64_t)tune->max_chunk_bytes + 32 * 12 + 15) & ~15ull;
+        const uint64_t want = fixed + regions + 2 * ((uint64_t)tune->max_chunk_bytes + 32) + 16;
+        if (want < lds) lds = (uint32_t)((want + 255) & ~255ull);
     }
-    if (tune && tune->ring_words) {          // forces the windowed path for testing
+    uint32_t ring_words = 0;
+    if (tune && tune->ring_words) {          // a small window, for testing the streaming path
         if (tune->ring_words < 256 || (tune->ring_words & (tune->ring_words - 1)) ||
-            tune->ring_words * 4ull > lds_max)
+            tune->ring_words * 4ull > kLdsMax - fixed)
             return MDZ_ERR_INVALID;
-        cap = tune->ring_words * 4;
+        ring_words = tune->ring_words;
+        lds = max(lds, fixed + ring_words * 4);
     }
-    // widest store that divides the line, the segment and the output alignment
-    const uint64_t seg_bytes = chunk / 32 / kSegs;
-    uint32_t sw = 16;
-    while (sw > eb && ((line % sw) || ((uintptr_t)dst % sw) || (seg_bytes % sw))) sw >>= 1;
+    lds = max(lds, fixed + 4096);
+    uint32_t data_cap = lds - fixed - 64;
+    if (ring_words) {
+        data_cap = min(data_cap, ring_words * 4);
+    } else {                                 // the largest power of two in the data area
+        ring_words = 256;
+        while (ring_words * 8ull <= lds - fixed) ring_words *= 2;
+    }
+    // widest piece that divides the line and the output alignment
+    uint32_t p = 16;
+    while (p > eb && ((line % p) || ((uintptr_t)dst % p))) p >>= 1;
     hipError_t e;
     switch (eb) {
-    case 1: e = launch_eb<1>(*in, dst, dst_bytes, status, cap, sw, s); break;
-    case 2: e = launch_eb<2>(*in, dst, dst_bytes, status, cap, sw, s); break;
-    case 4: e = launch_eb<4>(*in, dst, dst_bytes, status, cap, sw, s); break;
-    default: e = launch_eb<8>(*in, dst, dst_bytes, status, cap, sw, s); break;
+    case 1: e = launch_eb<1>(*in, dst, dst_bytes, status, lds, data_cap, ring_words, p, s); break;
+    case 2: e = launch_eb<2>(*in, dst, dst_bytes, status, lds, data_cap, ring_words, p, s); break;
+    case 4: e = launch_eb<4>(*in, dst, dst_bytes, status, lds, data_cap, ring_words, p, s); break;
+    default: e = launch_eb<8>(*in, dst, dst_bytes, status, lds, data_cap, ring_words, p, s); break;
     }
     return record_hip(e, "rlev1_decode_kernel launch");
 }

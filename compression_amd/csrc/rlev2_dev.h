@@ -34,6 +34,7 @@
 //   and every byte before the error point equals the oracle's).
 #include <type_traits>
 
+#include "lanes_dev.h"
 #include "mdz_device.h"
 #include "mdz_internal.h"
 
@@ -110,7 +111,6 @@ MDZ_DEV void put_val(uint8_t* out, uint64_t limit, uint32_t q, const LanePlace& 
 constexpr uint32_t kWin = 32;                   // records a consumer wave claims at once
 constexpr uint32_t kOpen = 0xffffffffu;         // final_ while the walk runs
 constexpr uint32_t kSpinMax = 1u << 22;         // consumer wait bound (about 0.2 s)
-constexpr uint32_t kSegBlocks = 160 * 1024 / 128;  // 32-word blocks of the largest LDS payload
 
 // Bit width of a 5-bit width code (the reference's FixedBitSizes decode):
 // 1..24, then 26, 28, 30, 32, then 40, 48, 56, 64.
@@ -421,7 +421,6 @@ MDZ_DEV void run_values(const Run& rr, uint64_t a, uint64_t b, const uint8_t* da
     }
 }
 
-constexpr uint32_t kNoLane = 0xffffffffu;      // lbase / dest of a lane outside the part
 
 // Run pool per workgroup: a whole chunk (32 lanes) or one part of NL lanes.
 template <uint32_t NL> struct PartPool { static constexpr uint32_t value = NL == 32 ? 768 : 384; };
@@ -472,118 +471,6 @@ struct Rlev2Args {
     int32_t* status;
 };
 
-// Segment of raw word w: the number of segment starts S[1..31] at or below
-// w (the starts ascend, so this is the last segment starting at or below w).
-MDZ_DEV uint32_t segment_of(const uint32_t* S, uint32_t w) {
-    uint32_t k = 0;
-#pragma unroll
-    for (int j = 1; j < 32; ++j) k += w >= S[j];
-    return k;
-}
-
-// Scatters the chunk's interleaved words into the lane regions of the part
-// with 16-byte loads: raw word w of segment k is the j-th active lane's word of round
-// sW[k-1] + (w - S[k]) / (32 - k), j = (w - S[k]) mod (32 - k), stored at
-// dest[k][j] + (w - S[k]) / (32 - k).  A group's segment comes from the
-// table of 32-word blocks; the quotient is found once per group and then
-// stepped word by word.
-template <class Tables>
-MDZ_DEV void stage_lanes(const Tables& tb, uint32_t* d32, const u32x4* __restrict__ src,
-                         uint32_t tid, uint32_t nthreads) {
-    const uint32_t total = tb.total, ngroups = (total + 3) / 4;
-    auto next_start = [&](uint32_t k) { return k < 31 ? tb.S[k + 1] : total; };
-    for (uint32_t g0 = tid; g0 < ngroups; g0 += 4 * nthreads) {
-        u32x4 v[4];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            const uint32_t g = g0 + u * nthreads;
-            if (g < ngroups) v[u] = __builtin_nontemporal_load(src + g);
-        }
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            const uint32_t g = g0 + u * nthreads;
-            if (g >= ngroups) break;
-            const uint32_t w0 = 4 * g;
-            uint32_t k = tb.st.segof[g >> 3];
-            uint32_t sn = next_start(k);
-            while (w0 >= sn) sn = next_start(++k);
-            uint32_t nk = 32 - k, off = w0 - tb.S[k];
-            uint32_t r = (uint32_t)((float)off * __builtin_amdgcn_rcpf((float)nk));
-            if (r * nk > off) --r;
-            else if ((r + 1) * nk <= off) ++r;
-            uint32_t j = off - r * nk;
-#pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                const uint32_t w = w0 + e;
-                if (w >= total) break;
-                if (w >= sn) {                       // the group crosses a segment start
-                    while (w >= sn) sn = next_start(++k);
-                    nk = 32 - k;
-                    off = w - tb.S[k];
-                    r = (uint32_t)((float)off * __builtin_amdgcn_rcpf((float)nk));
-                    if (r * nk > off) --r;
-                    else if ((r + 1) * nk <= off) ++r;
-                    j = off - r * nk;
-                }
-                const uint32_t dw = tb.st.dest[k][j];
-                if (dw != kNoLane) d32[dw + r] = v[u][e];     // words of other parts are dropped
-                if (++j == nk) { j = 0; ++r; }
-            }
-        }
-    }
-}
-
-// Stages only the part's lanes (NL < 32).  Thread (j, x) takes a contiguous
-// block of rounds of lane t0 + j.  Round r lies in segment
-// k = #{lanes with W <= r} (sW ascending), whose active lanes are M[k], and
-// the word's raw index is base_k + r * (32 - k) with
-// base_k = S[k] - sW[k-1] * (32 - k) + popc(M[k] below the lane): affine in r
-// inside a segment, so the tables are read only where a segment starts.  A
-// thread issues up to 16 loads before their LDS stores.
-template <uint32_t NL, uint32_t NT, class Tables>
-MDZ_DEV void stage_part(const Tables& tb, uint32_t* d32, const uint32_t* __restrict__ src, uint32_t t0,
-                        uint32_t tid) {
-    constexpr uint32_t XS = NT / NL;              // threads per lane
-    const uint32_t j = tid % NL, x = tid / NL, t = t0 + j;
-    const uint32_t cl = tb.cl[t], Wt = (cl + 3) / 4, lb = tb.lbase[t] >> 2;
-    const uint32_t below = (1u << t) - 1u;
-    const uint32_t rb = (Wt + XS - 1) / XS;
-    uint32_t r = x * rb;
-    const uint32_t rend = min(Wt, r + rb);
-    if (r >= rend) return;
-    // the segment of round r: count of sorted lengths <= r (scalar copies)
-    uint32_t k = 0;
-#pragma unroll
-    for (int i = 0; i < 32; ++i) k += __builtin_amdgcn_readfirstlane(tb.sW[i]) <= r;
-    uint32_t nbnd, base, stride;
-    auto enter = [&]() {                          // k < 32 while lane t is active
-        nbnd = k < 31 ? tb.sW[k] : 0xffffffffu;
-        stride = 32 - k;
-        base = tb.S[k] - (k ? tb.sW[k - 1] : 0) * stride + __popc(tb.M[k] & below);
-    };
-    enter();
-    while (r < rend) {
-        uint32_t v[16], at[16];
-        uint32_t nb = 0;
-#pragma unroll
-        for (uint32_t u = 0; u < 16; ++u) {
-            if (r < rend) {
-                if (r >= nbnd) {                  // next segment (a shorter lane ended)
-                    while (k < 31 && tb.sW[k] <= r) ++k;
-                    enter();
-                }
-                v[u] = __builtin_nontemporal_load(src + base + r * stride);
-                at[u] = lb + r;
-                ++nb;
-                ++r;
-            }
-        }
-#pragma unroll
-        for (uint32_t u = 0; u < 16; ++u)
-            if (u < nb) d32[at[u]] = v[u];
-    }
-}
-
 // Decodes lanes [part*NL, part*NL + NL) of chunk c with one workgroup of NT
 // threads; smem holds data_cap + 16 bytes of staged payload (the part's
 // lanes) followed by the tables.  With NL = 32 the workgroup decodes the
@@ -617,53 +504,7 @@ MDZ_DEV void rlev2_part(const Rlev2Args& a, uint64_t c, uint32_t part, uint32_t*
     // chunk c ends at blk_off[c+1], except the last, whose end is the payload's
     const uint64_t c_end = c + 1 < n_chunks ? blk_off[c + 1] : (payload_bytes ? payload_bytes : ~0ull);
 
-    if (wave == 0) {
-        const uint32_t t = l & 31;
-        const uint32_t cl = (uint32_t)col_len[c * 32 + t];
-        const bool own = t - t0 < NL;
-        const uint32_t W = (cl + 3) / 4, r16 = own ? (cl + 15) & ~15u : 0u;
-        uint32_t incl = r16, tot = W, mx = W;
-#pragma unroll
-        for (int o = 1; o < 32; o <<= 1) {
-            const uint32_t y = __shfl_up(incl, o, kWave);
-            if (t >= (uint32_t)o) incl += y;
-            tot += __shfl_xor(tot, o, kWave);
-            mx = max(mx, (uint32_t)__shfl_xor(mx, o, kWave));
-        }
-        // sort the lane lengths: rank = lanes shorter, ties to the lower lane
-        uint32_t rk = 0;
-#pragma unroll
-        for (int u = 0; u < 32; ++u) {
-            const uint32_t wu = __builtin_amdgcn_readlane(W, u);
-            rk += (wu < W) || (wu == W && (uint32_t)u < t);
-        }
-        if (l < 32) {
-            tb.lbase[t] = own ? incl - r16 : kNoLane;
-            tb.cl[t] = cl;
-            tb.lerr[t] = 0xffffffffu;
-            tb.sW[rk] = W;
-        }
-        // segment k's active lanes: rank >= k
-        uint32_t mk = 0;
-#pragma unroll
-        for (int k = 0; k < 32; ++k) {
-            const uint32_t m = (uint32_t)__ballot(l < 32 && rk >= (uint32_t)k);
-            mk = t == (uint32_t)k ? m : mk;
-        }
-        wave_sync();
-        // S[k] = raw words before round sW[k-1] = sum over lanes of min(W, sW[k-1])
-        const uint32_t prev = t ? tb.sW[t - 1] : 0;
-        uint32_t sk = 0;
-#pragma unroll
-        for (int u = 0; u < 32; ++u) sk += min((uint32_t)__builtin_amdgcn_readlane(W, u), prev);
-        if (l < 32) { tb.S[t] = sk; tb.M[t] = mk; }
-        if (l == 31) tb.fits = incl <= data_cap;
-        if (l == 0) {
-            tb.total = tot;
-            tb.extent_ok = !(c_off & 15) && c_end >= c_off && (uint64_t)tot * 4 <= c_end - c_off;
-        }
-        (void)mx;
-    }
+    if (wave == 0) lane_setup<NL>(tb, col_len, c, t0, data_cap, c_off, c_end);
     __syncthreads();
     if (!tb.extent_ok || !tb.fits) {
         // lanes longer than the chunk's payload, or a chunk larger than the
@@ -676,20 +517,7 @@ MDZ_DEV void rlev2_part(const Rlev2Args& a, uint64_t c, uint32_t part, uint32_t*
         return;
     }
     if constexpr (NL == 32) {
-        for (uint32_t x = tid; x < 32 * 32; x += NT) {
-            const uint32_t k = x >> 5, jj = x & 31;
-            if (jj < 32 - k) {
-                const uint32_t lb = tb.lbase[select_bit(tb.M[k], jj)];
-                tb.st.dest[k][jj] = lb == kNoLane ? kNoLane : (lb >> 2) + (k ? tb.sW[k - 1] : 0);
-            }
-        }
-        {
-            uint32_t S[32];
-#pragma unroll
-            for (int j = 0; j < 32; ++j) S[j] = __builtin_amdgcn_readfirstlane(tb.S[j]);
-            for (uint32_t b = tid; b < (tb.total + 31) / 32; b += NT)
-                tb.st.segof[b] = (uint8_t)segment_of(S, 32 * b);
-        }
+        stage_tables(tb, tid, NT);
         __syncthreads();
         MDZ_PHASE(0);
         stage_lanes(tb, smem, reinterpret_cast<const u32x4*>(payload + c_off), tid, NT);
