@@ -49,7 +49,7 @@ class BatchT(C.Structure):
 
 class LanesTuneT(C.Structure):
     _fields_ = [("ring_words", u32), ("max_chunk_bytes", u32), ("max_part_bytes", u32),
-                ("reserved", u32 * 5)]
+                ("max_lane_tokens", u32), ("reserved", u32 * 4)]
 
 
 # name -> (restype, argtypes); every symbol declared in include/mdz.h

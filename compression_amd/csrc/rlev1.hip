@@ -20,13 +20,18 @@
 //      the i-th terminator (16-bit entries);
 //   3. walk: 32 threads of wave 0, one per lane, follow the token chain in
 //      pseudo-varint steps (one table read and one data read per token),
-//      check each header as the oracle does, and record for every group of
-//      G values the token covering its first value;
-//   4. decode: every thread builds P-byte pieces of the chunk's output in
-//      output order (coalesced stores).  From its group's token it steps to
-//      the token covering the piece; a literal value is decoded straight from
-//      its pseudo-varint (start and length from the table), a run value is
-//      base + i*delta.
+//      check each header as the oracle does, and list every token (first
+//      pseudo-varint, first value, header) in LDS after the cut table;
+//   4. one thread per listed token decodes its run base and marks the token
+//      covering the first value of every unit of 32 values;
+//   5. one thread per unit loads its tokens at once and decodes all 32
+//      values without branches: a literal straight from its pseudo-varint
+//      (start and length from the cut table), a run value as base + i*delta;
+//      every load of a unit is independent of the others, so they overlap.
+//      The unit's bytes are stored as P-byte pieces.
+// If a lane has more tokens than its list holds, the chunk is walked again
+// with a checkpoint per value group and decoded in P-byte pieces in output
+// order, each stepping from its group's checkpoint to its token.
 // A lane's status is its first error in stream order.  Header errors (walk)
 // and value errors (decode) carry the key (value index, step) and are
 // combined with an atomic minimum; the chunk reports its lowest failing lane,
@@ -34,6 +39,8 @@
 // A chunk whose lanes do not fit the LDS budget, with a lane stream over
 // 65535 bytes, or whose cut table does not fit, is decoded by a single-wave
 // decoder that streams the payload through an LDS window.
+#include <type_traits>
+
 #include "lanes_dev.h"
 #include "mdz_device.h"
 #include "mdz_internal.h"
@@ -46,6 +53,9 @@ constexpr uint32_t kNT = 512;            // threads per workgroup
 constexpr uint32_t kGroups = 64;         // value groups (walk checkpoints) per lane
 constexpr uint32_t kCutThreads = kNT / 32;   // threads per lane building the cut table
 constexpr uint32_t kNoErr = 0xffffffffu;
+constexpr uint32_t kUnit = 32;           // values per decode unit (token path)
+constexpr uint32_t kUnitMax = 128;       // units per lane on the token path
+constexpr uint32_t kMT = 4;              // tokens a unit loads at once
 
 template <int EB> struct VarintSpec;
 template <> struct VarintSpec<1> { static constexpr uint32_t kMax = 2; };
@@ -201,18 +211,22 @@ struct V1Tables {
     uint32_t nterm[32];      // terminators of each lane stream
     uint32_t cut0[32];       // first cut-table entry of each lane
     uint32_t vstop[32];      // values the walk covered (later values are not decoded)
+    uint32_t ntok[32];       // tokens listed per lane (token path)
     uint32_t total;          // words of all lanes
     uint32_t fits;           // lane regions fit the data budget
     uint32_t extent_ok;      // lanes lie inside the chunk's payload extent
     uint32_t used;           // bytes of the lane regions
     uint32_t fast;           // the chunk takes the table path
-    uint32_t pad[3];
+    uint32_t tok_off;        // byte offset of the token lists in LDS
+    uint32_t cap;            // tokens each lane's list holds
+    uint32_t overflow;       // a lane has more tokens than its list holds
     union {
         struct {                                 // staging
             uint32_t dest[32][32];
             uint8_t segof[kSegBlocks];
         } st;
-        uint32_t ck[32][kGroups];                // walk -> decode: token k | v << 16
+        uint32_t ck[32][kGroups];                // checkpoint path: token k | v << 16
+        uint16_t ui[32][kUnitMax];               // token path: token covering value 32u
         LaneTable lt;                            // windowed decoder
     };
 };
@@ -299,8 +313,9 @@ MDZ_DEV void store_piece(uint8_t* out, uint32_t o, uint64_t limit, const uint32_
         else if constexpr (P == 2) *reinterpret_cast<uint16_t*>(p) = (uint16_t)w[0];
         else *p = (uint8_t)w[0];
     } else {
-        for (uint32_t i = 0; i < (uint32_t)P && o + i < limit; ++i)
-            p[i] = (uint8_t)(w[i >> 2] >> (8 * (i & 3)));
+#pragma unroll
+        for (uint32_t i = 0; i < (uint32_t)P; ++i)      // unrolled: w stays in registers
+            if (o + i < limit) p[i] = (uint8_t)(w[i >> 2] >> (8 * (i & 3)));
     }
 }
 
@@ -371,16 +386,183 @@ MDZ_DEV void decode_piece(const uint8_t* base, const uint16_t* cut, uint32_t nte
     if (ekey != kNoErr) atomicMin(lerr, ekey);
 }
 
+// Run base per listed token: the value type, 64-bit for 8-byte elements.
+template <int EB> using BaseT = typename std::conditional<EB == 8, uint64_t, uint32_t>::type;
+
+// Status of literal pseudo-varint j read from byte a0 with `len` bytes (the
+// oracle's LEB128 reader): j == nterm is the unterminated tail of the lane.
+template <int EB>
+MDZ_DEV int32_t lit_status(uint32_t j, uint32_t nterm, uint32_t cl, uint32_t a0, uint32_t len) {
+    constexpr uint32_t kMax = VarintSpec<EB>::kMax;
+    return j >= nterm ? (cl - a0 >= kMax ? MDZ_ST_BAD_SYMBOL : MDZ_ST_INPUT_OVERRUN)
+                      : (len > kMax ? MDZ_ST_BAD_SYMBOL : MDZ_ST_OK);
+}
+
+// Places value i of a piece into its words (w starts zeroed).
+template <int EB>
+MDZ_DEV void put_value(uint32_t (&w)[4], uint32_t i, BaseT<EB> x) {
+    if constexpr (EB == 8) {
+        w[2 * i] = (uint32_t)x;
+        w[2 * i + 1] = (uint32_t)(x >> 32);
+    } else if constexpr (EB == 4) {
+        w[i] = (uint32_t)x;
+    } else {
+        w[(i * EB) >> 2] |= (uint32_t)x << (8 * ((i * EB) & 3));
+    }
+}
+
+// One listed token reduced to what a value needs: its first value tv, a
+// literal flag, and A, D: a literal value i is pseudo-varint i - A, a run
+// value is A + i * D.  Passed by value so the per-value selection stays in
+// registers.
+template <int EB> struct TokVal {
+    BaseT<EB> A, D;
+    uint32_t tv;
+    uint32_t lit;
+};
+
+template <int EB>
+MDZ_DEV TokVal<EB> tok_val(const uint32_t* kvs, const uint16_t* hds, const BaseT<EB>* bss,
+                           uint32_t nt, uint32_t idx, uint32_t vs) {
+    typedef BaseT<EB> VT;
+    const uint32_t ic = min(idx, nt - 1);
+    const uint32_t kv = kvs[ic], h = hds[ic];
+    TokVal<EB> r;
+    r.tv = idx < nt ? kv >> 16 : vs;
+    r.lit = h & 0x80u;
+    const VT d = (VT)(int64_t)(int8_t)(h >> 8);
+    r.A = r.lit ? (VT)(r.tv - (kv & 0xffffu)) : (VT)(bss[ic] - (VT)r.tv * d);
+    r.D = d;
+    return r;
+}
+
+template <int EB>
+MDZ_DEV TokVal<EB> pick(bool c, TokVal<EB> x, TokVal<EB> y) {
+    TokVal<EB> r;
+    r.A = c ? x.A : y.A;
+    r.D = c ? x.D : y.D;
+    r.tv = c ? x.tv : y.tv;
+    r.lit = c ? x.lit : y.lit;
+    return r;
+}
+
+// Value vi of a lane from its token m (selected by the caller): a literal
+// read from its pseudo-varint, or a run value.  `ok` is false for a literal
+// past the lane's unterminated tail (an earlier value reports that error); a
+// literal's LEB128 status is returned in `le`.
+template <int EB>
+MDZ_DEV BaseT<EB> unit_value(TokVal<EB> m, uint32_t vi, const uint8_t* base, const uint16_t* cut,
+                             uint32_t nterm, uint32_t cl, bool& ok, int32_t& le) {
+    typedef BaseT<EB> VT;
+    le = MDZ_ST_OK;
+    ok = true;
+    if (!m.lit) return m.A + (VT)vi * m.D;
+    // pseudo-varint j; reads clamped to the lane's table
+    const uint32_t j = vi - (uint32_t)m.A;
+    const uint32_t a0 = cut[min(j, nterm)] + (m.tv == vi);
+    const uint32_t len = cut[min(j + 1, nterm)] - a0;
+    ok = j <= nterm;
+    le = lit_status<EB>(j, nterm, cl, a0, len);
+    return (VT)leb_at<EB>(base, a0, len);
+}
+
+// Decodes values [V, V + n) of lane t (n <= kUnit, V a multiple of kUnit)
+// from its token list and stores them as P-byte pieces, one piece at a time.
+// i0 = the token covering V; tokens are taken kMT = 4 at a time (T0..T3,
+// and e = the first value after them).  A piece inside the current batch
+// decodes its values unrolled (their loads overlap); a piece reaching past
+// it (rare) goes value by value, loading batches as needed.
+template <int EB, int P>
+MDZ_DEV void decode_unit(const uint8_t* base, const uint16_t* cut, const uint32_t* kvs,
+                         const uint16_t* hds, const BaseT<EB>* bss, uint32_t nt, uint32_t i0,
+                         uint32_t nterm, uint32_t cl, uint32_t V, uint32_t n, uint32_t vs,
+                         uint32_t* lerr, uint8_t* out, uint64_t limit, uint32_t t,
+                         uint32_t line) {
+    static_assert(kMT == 4, "four tokens per batch");
+    typedef BaseT<EB> VT;
+    constexpr uint32_t NV = P / EB;                 // values per piece
+    constexpr VT vmask = EB == 8 ? ~(VT)0 : (VT)((1ull << (8 * EB)) - 1);
+    const uint32_t vend = min(V + n, vs);
+    uint32_t ekey = kNoErr, ib = i0;
+    TokVal<EB> T0, T1, T2, T3;
+    uint32_t e;
+    auto load = [&]() {
+        T0 = tok_val<EB>(kvs, hds, bss, nt, ib, vs);
+        T1 = tok_val<EB>(kvs, hds, bss, nt, ib + 1, vs);
+        T2 = tok_val<EB>(kvs, hds, bss, nt, ib + 2, vs);
+        T3 = tok_val<EB>(kvs, hds, bss, nt, ib + 3, vs);
+        e = ib + 4 < nt ? kvs[ib + 4] >> 16 : vs;
+    };
+    auto value = [&](uint32_t vi, bool& ok, int32_t& le) {
+        const TokVal<EB> m = pick<EB>(vi >= T3.tv, T3, pick<EB>(vi >= T2.tv, T2,
+                                      pick<EB>(vi >= T1.tv, T1, T0)));
+        return unit_value<EB>(m, vi, base, cut, nterm, cl, ok, le) & vmask;
+    };
+    load();
+    const uint32_t lb = V * EB;
+    uint32_t r = lb / line, off = lb - r * line;
+    for (uint32_t q = 0; q < n / NV; ++q) {
+        const uint32_t v0 = V + q * NV;
+        uint32_t w[4] = {0, 0, 0, 0};
+        if (min(v0 + NV, vend) <= e) {
+#pragma unroll
+            for (uint32_t i = 0; i < NV; ++i) {
+                const uint32_t vi = v0 + i;
+                bool ok;
+                int32_t le;
+                const VT val = value(vi, ok, le);
+                const bool act = vi < vend;
+                if (act && le != MDZ_ST_OK) ekey = min(ekey, err_key(vi, 2, le));
+                put_value<EB>(w, i, act && ok ? val : (VT)0);
+            }
+        } else {
+            // value by value; the piece's bytes shift in from the top of w
+            for (uint32_t i = 0; i < NV; ++i) {
+                const uint32_t vi = v0 + i;
+                while (vi >= e && vi < vend) {
+                    ib += kMT;
+                    load();
+                }
+                bool ok;
+                int32_t le;
+                const VT val = value(vi, ok, le);
+                const bool act = vi < vend;
+                if (act && le != MDZ_ST_OK) ekey = min(ekey, err_key(vi, 2, le));
+                const VT x = act && ok ? val : (VT)0;
+                if constexpr (EB == 8) {
+                    w[0] = w[2]; w[1] = w[3];
+                    w[2] = (uint32_t)x; w[3] = (uint32_t)(x >> 32);
+                } else if constexpr (EB == 4) {
+                    w[0] = w[1]; w[1] = w[2]; w[2] = w[3]; w[3] = (uint32_t)x;
+                } else {
+                    w[0] = alignbyte(w[1], w[0], EB); w[1] = alignbyte(w[2], w[1], EB);
+                    w[2] = alignbyte(w[3], w[2], EB);
+                    w[3] = (w[3] >> (8 * EB)) | ((uint32_t)x << (32 - 8 * EB));
+                }
+            }
+            // the piece now sits in the top P bytes of w: move it to the bottom
+            if constexpr (P == 8) { w[0] = w[2]; w[1] = w[3]; }
+            else if constexpr (P == 4) { w[0] = w[3]; }
+            else if constexpr (P == 2) { w[0] = w[3] >> 16; }
+            else if constexpr (P == 1) { w[0] = w[3] >> 24; }
+        }
+        store_piece<P>(out, (r * 32 + t) * line + off, limit, w);
+        off += P;
+        if (off == line) { off = 0; ++r; }
+    }
+    if (ekey != kNoErr) atomicMin(lerr, ekey);
+}
+
 // ------------------------------------------------------------ the kernel --
 // `lds_bytes` is the dynamic LDS size; `data_cap` bounds the lane regions
 // for the table path; `ring_words` (a power of two) sizes the window of the
-// streaming decoder.
+// streaming decoder; `tok_limit` bounds the per-lane token lists.
 template <int EB, int P>
 __global__ __launch_bounds__(kNT) void rlev1_decode_kernel(
     const uint8_t* __restrict__ payload, const uint64_t* __restrict__ blk_off,
     const uint64_t* __restrict__ col_len, uint64_t n_chunks, uint64_t chunk_bytes,
     uint32_t line, uint8_t* __restrict__ dst, uint64_t dst_bytes, int32_t* __restrict__ status,
-    uint32_t data_cap, uint32_t ring_words, uint32_t lds_bytes) {
+    uint32_t data_cap, uint32_t ring_words, uint32_t lds_bytes, uint32_t tok_limit) {
     extern __shared__ __align__(16) uint32_t smem[];
     V1Tables& tb = *reinterpret_cast<V1Tables*>(smem);
     uint8_t* const data = reinterpret_cast<uint8_t*>(smem) + sizeof(V1Tables);
@@ -450,7 +632,15 @@ __global__ __launch_bounds__(kNT) void rlev1_decode_kernel(
             const uint32_t nt = l < 32 ? tb.nterm[l] + 1 : 0;
             const uint32_t ex = wave_incl_scan_u32(nt) - nt;
             if (l < 32) tb.cut0[l] = ex;
-            if (l == 31) tb.fast = ex + nt <= cut_cap;
+            if (l == 31) {
+                tb.fast = ex + nt <= cut_cap;
+                // token lists after the cut table: as many tokens per lane as fit
+                const uint32_t off = (uint32_t)sizeof(V1Tables) + used + 16 + ((2 * (ex + nt) + 15) & ~15u);
+                constexpr uint32_t rec = sizeof(BaseT<EB>) + 4 + 2;
+                tb.tok_off = off;
+                tb.cap = min(off < lds_bytes ? (lds_bytes - off) / (32 * rec) : 0u, tok_limit);
+                tb.overflow = 0;
+            }
         }
         __syncthreads();
         fast = tb.fast;
@@ -486,56 +676,132 @@ __global__ __launch_bounds__(kNT) void rlev1_decode_kernel(
         return;
     }
 
-    // ---- 3. walk: one thread per lane stream
-    uint32_t gsh = 0;                            // groups of G = 2^gsh values, at most kGroups
-    while (((nvals + (1u << gsh) - 1) >> gsh) > kGroups) ++gsh;
-    if (wave == 0 && l < 32) {
-        const uint32_t t = l;
-        const uint8_t* base = data + tb.lbase[t];
-        const uint16_t* ct = cut + tb.cut0[t];
-        const uint32_t nterm = tb.nterm[t], cl = tb.cl[t], G = 1u << gsh;
-        uint32_t k = 0, v = 0, err = kNoErr;
-        while (v < nvals && k <= nterm) {
-            const uint32_t s = ct[k];
-            if (s >= cl) { err = err_key(v, 0, MDZ_ST_INPUT_OVERRUN); break; }
-            Token tk;
-            tk.k = k;
-            tk.v = v;
-            tk.hd = lds_u16_at(base, s);
-            const uint32_t cnt = tk.count();
-            if (cnt > nvals - v) { err = err_key(v, 0, MDZ_ST_OUTPUT_OVERRUN); break; }
-            if (!tk.lit() && s + 1 >= cl) { err = err_key(v, 1, MDZ_ST_INPUT_OVERRUN); break; }
-            for (uint32_t g = (v + G - 1) >> gsh; (g << gsh) < v + cnt; ++g) tb.ck[t][g] = k | v << 16;
-            k = tk.next_k();
-            v += cnt;
+    // ---- 3. walk: one thread per lane stream.  The token path lists every
+    //      token (tkv = first pseudo-varint | first value << 16, thd = header |
+    //      delta << 8, tbs = run base, filled in step 4) after the cut table,
+    //      as many per lane as fit; if a lane has more, the chunk is walked
+    //      again with a checkpoint per value group (checkpoint path).
+    const uint32_t upl = (nvals + kUnit - 1) / kUnit;      // decode units per lane
+    const uint32_t cap = tb.cap;
+    BaseT<EB>* const tbs = reinterpret_cast<BaseT<EB>*>(reinterpret_cast<uint8_t*>(smem) + tb.tok_off);
+    uint32_t* const tkv = reinterpret_cast<uint32_t*>(tbs + 32 * cap);
+    uint16_t* const thd = reinterpret_cast<uint16_t*>(tkv + 32 * cap);
+    bool tokens = upl <= kUnitMax && cap >= kMT;
+    if (tokens) {
+        if (wave == 0 && l < 32) {
+            const uint32_t t = l;
+            const uint8_t* base = data + tb.lbase[t];
+            const uint16_t* ct = cut + tb.cut0[t];
+            uint32_t* kvl = tkv + t * cap;
+            uint16_t* hdl = thd + t * cap;
+            const uint32_t nterm = tb.nterm[t], cl = tb.cl[t];
+            uint32_t k = 0, v = 0, i = 0, err = kNoErr;
+            while (v < nvals && k <= nterm) {
+                const uint32_t s = ct[k];
+                if (s >= cl) { err = err_key(v, 0, MDZ_ST_INPUT_OVERRUN); break; }
+                Token tk;
+                tk.k = k;
+                tk.v = v;
+                tk.hd = lds_u16_at(base, s);
+                const uint32_t cnt = tk.count();
+                if (cnt > nvals - v) { err = err_key(v, 0, MDZ_ST_OUTPUT_OVERRUN); break; }
+                if (!tk.lit() && s + 1 >= cl) { err = err_key(v, 1, MDZ_ST_INPUT_OVERRUN); break; }
+                if (i == cap) { tb.overflow = 1; break; }
+                kvl[i] = k | v << 16;
+                hdl[i] = (uint16_t)tk.hd;
+                ++i;
+                k = tk.next_k();
+                v += cnt;
+            }
+            tb.ntok[t] = i;
+            tb.vstop[t] = v;
+            tb.lerr[t] = err;
         }
-        tb.vstop[t] = v;
-        tb.lerr[t] = err;
+        __syncthreads();
+        tokens = !tb.overflow;
     }
-    __syncthreads();
     MDZ_PHASE(3);
+    uint8_t* const out = dst + out_base;
+    if (tokens) {
+        // ---- 4. one thread per listed token: run base (and its status), and the
+        //      token covering the first value of each unit
+        for (uint32_t x = tid; x < 32 * cap; x += kNT) {
+            const uint32_t t = x & 31, i = x >> 5, nt = tb.ntok[t];
+            if (i >= nt) continue;
+            const uint32_t kv = tkv[t * cap + i], k = kv & 0xffffu, v = kv >> 16;
+            const uint32_t vn = i + 1 < nt ? tkv[t * cap + i + 1] >> 16 : tb.vstop[t];
+            const uint32_t hd = thd[t * cap + i];
+            if (!(hd & 0x80u)) {
+                const uint32_t joined = (hd >> 15) & 1u;     // delta byte joined to the base's
+                uint64_t b;
+                const int32_t e = pseudo_value<EB>(data + tb.lbase[t], cut + tb.cut0[t],
+                                                   k + 2 - joined, tb.nterm[t], tb.cl[t], joined, b);
+                if (e != MDZ_ST_OK) atomicMin(&tb.lerr[t], err_key(v, 2, e));
+                tbs[t * cap + i] = (BaseT<EB>)b;
+            }
+            for (uint32_t g = (v + kUnit - 1) / kUnit; g * kUnit < vn; ++g) tb.ui[t][g] = (uint16_t)i;
+        }
+        __syncthreads();
+        MDZ_PHASE(4);
+        // ---- 5. one thread per unit of kUnit values of a lane
+        for (uint32_t u = tid; u < 32 * upl; u += kNT) {
+            const uint32_t t = u & 31, V = (u >> 5) * kUnit, vs = tb.vstop[t];
+            if (V >= vs) continue;
+            decode_unit<EB, P>(data + tb.lbase[t], cut + tb.cut0[t], tkv + t * cap, thd + t * cap,
+                               tbs + t * cap, tb.ntok[t], tb.ui[t][u >> 5], tb.nterm[t],
+                               tb.cl[t], V, min(kUnit, nvals - V), vs, &tb.lerr[t], out, limit, t,
+                               line);
+        }
+    } else {
+        uint32_t gsh = 0;                            // groups of G = 2^gsh values, at most kGroups
+        while (((nvals + (1u << gsh) - 1) >> gsh) > kGroups) ++gsh;
+        if (wave == 0 && l < 32) {
+            const uint32_t t = l;
+            const uint8_t* base = data + tb.lbase[t];
+            const uint16_t* ct = cut + tb.cut0[t];
+            const uint32_t nterm = tb.nterm[t], cl = tb.cl[t], G = 1u << gsh;
+            uint32_t k = 0, v = 0, err = kNoErr;
+            while (v < nvals && k <= nterm) {
+                const uint32_t s = ct[k];
+                if (s >= cl) { err = err_key(v, 0, MDZ_ST_INPUT_OVERRUN); break; }
+                Token tk;
+                tk.k = k;
+                tk.v = v;
+                tk.hd = lds_u16_at(base, s);
+                const uint32_t cnt = tk.count();
+                if (cnt > nvals - v) { err = err_key(v, 0, MDZ_ST_OUTPUT_OVERRUN); break; }
+                if (!tk.lit() && s + 1 >= cl) { err = err_key(v, 1, MDZ_ST_INPUT_OVERRUN); break; }
+                for (uint32_t g = (v + G - 1) >> gsh; (g << gsh) < v + cnt; ++g) tb.ck[t][g] = k | v << 16;
+                k = tk.next_k();
+                v += cnt;
+            }
+            tb.vstop[t] = v;
+            tb.lerr[t] = err;
+        }
+        __syncthreads();
+        MDZ_PHASE(4);
 
-    // ---- 4. decode: P-byte pieces in output order
-    {
-        const float inv_line = __builtin_amdgcn_rcpf((float)line);
-        const uint32_t npieces = (uint32_t)(chunk_bytes / P);
-        uint8_t* const out = dst + out_base;
-        for (uint32_t q = tid; q < npieces; q += kNT) {
-            const uint32_t o = q * P;      // pieces past `limit` are decoded for the status only
-            uint32_t R = (uint32_t)((float)o * inv_line);     // line of the piece
-            if (R * line > o) --R;
-            else if ((R + 1) * line <= o) ++R;
-            const uint32_t t = R & 31, v0 = ((R >> 5) * line + (o - R * line)) / EB;
-            const uint32_t vs = tb.vstop[t];
-            if (v0 >= vs) continue;
-            uint32_t w[4];
-            decode_piece<EB, P>(data + tb.lbase[t], cut + tb.cut0[t], tb.nterm[t], tb.cl[t],
-                                tb.ck[t][v0 >> gsh], v0, vs, &tb.lerr[t], w);
-            store_piece<P>(out, o, limit, w);
+        // P-byte pieces in output order, each from its group's checkpoint
+        {
+            const float inv_line = __builtin_amdgcn_rcpf((float)line);
+            const uint32_t npieces = (uint32_t)(chunk_bytes / P);
+            for (uint32_t q = tid; q < npieces; q += kNT) {
+                const uint32_t o = q * P;      // pieces past `limit` are decoded for the status only
+                uint32_t R = (uint32_t)((float)o * inv_line);     // line of the piece
+                if (R * line > o) --R;
+                else if ((R + 1) * line <= o) ++R;
+                const uint32_t t = R & 31, v0 = ((R >> 5) * line + (o - R * line)) / EB;
+                const uint32_t vs = tb.vstop[t];
+                if (v0 >= vs) continue;
+                uint32_t w[4];
+                decode_piece<EB, P>(data + tb.lbase[t], cut + tb.cut0[t], tb.nterm[t], tb.cl[t],
+                                    tb.ck[t][v0 >> gsh], v0, vs, &tb.lerr[t], w);
+                store_piece<P>(out, o, limit, w);
+            }
         }
     }
     __syncthreads();
-    MDZ_PHASE(4);
+    MDZ_PHASE(5);
     if (tid == 0 && status) {
         int32_t st = MDZ_ST_OK;
         for (int t = 0; t < 32; ++t) {
@@ -548,29 +814,32 @@ __global__ __launch_bounds__(kNT) void rlev1_decode_kernel(
 
 template <int EB, int P>
 hipError_t launch(const mdz_lanes_t& in, void* dst, uint64_t dst_bytes, int32_t* status,
-                  uint32_t lds, uint32_t data_cap, uint32_t ring_words, hipStream_t s) {
+                  uint32_t lds, uint32_t data_cap, uint32_t ring_words, uint32_t tok_limit,
+                  hipStream_t s) {
     auto k = rlev1_decode_kernel<EB, P>;
     hipError_t e = set_dynamic_lds((const void*)k, lds);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k, dim3((uint32_t)in.n_chunks), dim3(kNT), lds, s,
                        (const uint8_t*)in.payload, in.blk_off, in.col_len, in.n_chunks,
                        in.chunk_bytes, in.lane_line, (uint8_t*)dst, dst_bytes, status, data_cap,
-                       ring_words, lds);
+                       ring_words, lds, tok_limit);
     return hipGetLastError();
 }
 
 template <int EB>
 hipError_t launch_eb(const mdz_lanes_t& in, void* dst, uint64_t dst_bytes, int32_t* status,
-                     uint32_t lds, uint32_t data_cap, uint32_t ring_words, uint32_t p,
-                     hipStream_t s) {
-    if (p == 16) return launch<EB, 16>(in, dst, dst_bytes, status, lds, data_cap, ring_words, s);
-    if (p == 8) return launch<EB, 8>(in, dst, dst_bytes, status, lds, data_cap, ring_words, s);
+                     uint32_t lds, uint32_t data_cap, uint32_t ring_words, uint32_t tok_limit,
+                     uint32_t p, hipStream_t s) {
+#define MDZ_V1_LAUNCH(P) launch<EB, P>(in, dst, dst_bytes, status, lds, data_cap, ring_words, tok_limit, s)
+    if (p == 16) return MDZ_V1_LAUNCH(16);
+    if (p == 8) return MDZ_V1_LAUNCH(8);
     if constexpr (EB <= 4)
-        if (p == 4) return launch<EB, 4>(in, dst, dst_bytes, status, lds, data_cap, ring_words, s);
+        if (p == 4) return MDZ_V1_LAUNCH(4);
     if constexpr (EB <= 2)
-        if (p == 2) return launch<EB, 2>(in, dst, dst_bytes, status, lds, data_cap, ring_words, s);
+        if (p == 2) return MDZ_V1_LAUNCH(2);
     if constexpr (EB == 1)
-        if (p == 1) return launch<EB, 1>(in, dst, dst_bytes, status, lds, data_cap, ring_words, s);
+        if (p == 1) return MDZ_V1_LAUNCH(1);
+#undef MDZ_V1_LAUNCH
     return hipErrorInvalidValue;
 }
 
@@ -618,12 +887,13 @@ int rlev1_decompress(const mdz_lanes_t* in, void* dst, uint64_t dst_bytes, int32
     // widest piece that divides the line and the output alignment
     uint32_t p = 16;
     while (p > eb && ((line % p) || ((uintptr_t)dst % p))) p >>= 1;
+    const uint32_t tok_limit = tune && tune->max_lane_tokens ? tune->max_lane_tokens : 0xffffffffu;
     hipError_t e;
     switch (eb) {
-    case 1: e = launch_eb<1>(*in, dst, dst_bytes, status, lds, data_cap, ring_words, p, s); break;
-    case 2: e = launch_eb<2>(*in, dst, dst_bytes, status, lds, data_cap, ring_words, p, s); break;
-    case 4: e = launch_eb<4>(*in, dst, dst_bytes, status, lds, data_cap, ring_words, p, s); break;
-    default: e = launch_eb<8>(*in, dst, dst_bytes, status, lds, data_cap, ring_words, p, s); break;
+    case 1: e = launch_eb<1>(*in, dst, dst_bytes, status, lds, data_cap, ring_words, tok_limit, p, s); break;
+    case 2: e = launch_eb<2>(*in, dst, dst_bytes, status, lds, data_cap, ring_words, tok_limit, p, s); break;
+    case 4: e = launch_eb<4>(*in, dst, dst_bytes, status, lds, data_cap, ring_words, tok_limit, p, s); break;
+    default: e = launch_eb<8>(*in, dst, dst_bytes, status, lds, data_cap, ring_words, tok_limit, p, s); break;
     }
     return record_hip(e, "rlev1_decode_kernel launch");
 }

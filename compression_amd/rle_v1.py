@@ -16,15 +16,17 @@ from .lanes import DeviceLanes, as_ptr
 
 
 def decompress(lanes: DeviceLanes, dst, dst_bytes: int | None = None, status=None, stream=None,
-               ring_words: int = 0, max_chunk_bytes: int = 0) -> None:
+               ring_words: int = 0, max_chunk_bytes: int = 0, max_lane_tokens: int = 0) -> None:
     """Enqueue the decode of every chunk of ``lanes`` into device buffer ``dst``.
 
     ``max_chunk_bytes`` (largest chunk payload, if the caller knows it) sizes
-    the per-chunk LDS area; ``ring_words`` forces the streaming path."""
+    the per-chunk LDS area; ``ring_words`` forces the streaming path and
+    ``max_lane_tokens`` bounds the per-lane token lists (a chunk with a longer
+    lane takes the checkpoint decode)."""
     s = lanes.struct()
     n = lanes.out_bytes if dst_bytes is None else int(dst_bytes)
-    if ring_words or max_chunk_bytes:
-        tune = LanesTuneT(ring_words, max_chunk_bytes)
+    if ring_words or max_chunk_bytes or max_lane_tokens:
+        tune = LanesTuneT(ring_words, max_chunk_bytes, 0, max_lane_tokens)
         rc = lib().mdz_rlev1_decompress_tuned(C.byref(s), as_ptr(dst), n, as_ptr(status), C.byref(tune),
                                               as_ptr(stream))
     else:

@@ -110,7 +110,10 @@ typedef struct {
                                     (lanes 8q..8q+7), of sum(round_up(col_len, 16)); when set,
                                     each quarter is decoded by its own workgroup (4 per chunk,
                                     smaller LDS, more chunks in flight).  0 = whole chunks */
-    uint32_t reserved[5];
+    uint32_t max_lane_tokens;    /* RLE v1: at most this many tokens listed per lane stream
+                                    (0 = as many as fit in LDS); a chunk with a longer lane
+                                    takes the checkpoint decode.  For testing that path */
+    uint32_t reserved[4];
 } mdz_lanes_tune_t;
 
 /* RLE v1 decode -- replaces rle_v1::decompress_gpu<READ_COL_TYPE, DATA_TYPE,

@@ -9,7 +9,8 @@ from oracle import pyoracle as O
 pytestmark = pytest.mark.gpu
 
 
-def _roundtrip(mdz, data: np.ndarray, chunk: int, line: int, ring: int = 0, dst_bytes=None):
+def _roundtrip(mdz, data: np.ndarray, chunk: int, line: int, ring: int = 0, dst_bytes=None,
+               max_lane_tokens: int = 0):
     from compression_amd import rle_v1
     from compression_amd.lanes import DeviceLanes
     c = O.rlev1_encode(data, chunk, line, threads=8)
@@ -18,7 +19,7 @@ def _roundtrip(mdz, data: np.ndarray, chunk: int, line: int, ring: int = 0, dst_
     dst = mdz.DeviceBuffer(max(n, 1))
     dst.fill(0xAB)
     st = mdz.DeviceBuffer(4 * c.n_chunks)
-    rle_v1.decompress(lanes, dst, n, st, ring_words=ring)
+    rle_v1.decompress(lanes, dst, n, st, ring_words=ring, max_lane_tokens=max_lane_tokens)
     got = dst.download(n)
     status = st.download(4 * c.n_chunks, np.int32)
     rc, ref, ref_st = O.rlev1_decode(c, out_bytes=n, threads=8)
@@ -77,7 +78,18 @@ def test_config1_full_size(gpu):
     assert np.array_equal(got, x.view(np.uint8))
 
 
-def test_corrupt_streams_report_status(gpu):
+@pytest.mark.parametrize("dtype", [np.uint8, np.uint16, np.uint32, np.uint64])
+@pytest.mark.parametrize("max_tokens", [1, 6])
+def test_checkpoint_path(gpu, dtype, max_tokens):
+    # token lists too short for the lanes: the chunk is walked again with a
+    # checkpoint per value group (1: below the batch size, so never listed)
+    x = O.gen_rlev1_u32(300_000, 13).astype(dtype)
+    c, got, st, ref, _ = _roundtrip(gpu, x, 32768, 32, max_lane_tokens=max_tokens)
+    assert (st == 0).all() and np.array_equal(got, x.view(np.uint8))
+
+
+@pytest.mark.parametrize("max_tokens", [0, 6])
+def test_corrupt_streams_report_status(gpu, max_tokens):
     from compression_amd import rle_v1
     from compression_amd.lanes import DeviceLanes
     x = O.gen_rlev1_u32(1 << 16, 3)
@@ -90,7 +102,7 @@ def test_corrupt_streams_report_status(gpu):
     lanes = DeviceLanes(pay, c.blk_off, col, c.n_chunks, 32768, 4, 32)
     dst = gpu.DeviceBuffer(c.raw_bytes)
     st = gpu.DeviceBuffer(4 * c.n_chunks)
-    rle_v1.decompress(lanes, dst, c.raw_bytes, st)
+    rle_v1.decompress(lanes, dst, c.raw_bytes, st, max_lane_tokens=max_tokens)
     s = st.download(4 * c.n_chunks, np.int32)
     assert s[0] != 0 and s[1] != 0 and (s[2:] == 0).all()
     ref = O.RleV1Container(payload=pay, col_len=col, blk_off=c.blk_off, n_chunks=c.n_chunks,
@@ -98,6 +110,43 @@ def test_corrupt_streams_report_status(gpu):
     _, _, ref_st = O.rlev1_decode(ref, threads=4)
     assert np.array_equal(s, ref_st)          # same status codes as the oracle
     assert np.array_equal(dst.download()[2 * 32768:], x.view(np.uint8)[2 * 32768:])
+
+
+@pytest.mark.parametrize("dtype", [np.uint8, np.uint32, np.uint64])
+@pytest.mark.parametrize("max_tokens", [0, 6])
+def test_random_corruption_matches_oracle(gpu, dtype, max_tokens):
+    # byte damage in most chunks: every chunk's status equals the oracle's,
+    # and undamaged chunks decode bit-exactly
+    from compression_amd import rle_v1
+    from compression_amd.lanes import DeviceLanes
+    x = O.gen_rlev1_u32(1 << 18, 17).astype(dtype)
+    chunk = 16384
+    c = O.rlev1_encode(x, chunk, 32, threads=4)
+    pay = c.payload.copy()
+    rng = np.random.default_rng(23)
+    damaged = set()
+    for ch in range(c.n_chunks):
+        if ch % 4 == 0:
+            continue
+        lo, hi = int(c.blk_off[ch]), int(c.blk_off[ch + 1])
+        for _ in range(1 + ch % 3):
+            pos = int(rng.integers(lo, hi))
+            pay[pos] = (int(pay[pos]) ^ int(rng.integers(1, 256))) & 0xFF
+        damaged.add(ch)
+    lanes = DeviceLanes(pay, c.blk_off, c.col_len, c.n_chunks, chunk, c.elem_bytes, 32)
+    dst = gpu.DeviceBuffer(c.raw_bytes)
+    st = gpu.DeviceBuffer(4 * c.n_chunks)
+    rle_v1.decompress(lanes, dst, c.raw_bytes, st, max_lane_tokens=max_tokens)
+    s = st.download(4 * c.n_chunks, np.int32)
+    ref = O.RleV1Container(payload=pay, col_len=c.col_len, blk_off=c.blk_off, n_chunks=c.n_chunks,
+                           chunk=chunk, elem_bytes=c.elem_bytes, line=32, raw_bytes=c.raw_bytes)
+    _, ref_out, ref_st = O.rlev1_decode(ref, threads=4)
+    assert np.array_equal(s, ref_st)
+    assert (ref_st[sorted(damaged)] != 0).sum() > len(damaged) // 2      # most damage is seen
+    got = dst.download()
+    for ch in range(c.n_chunks):
+        if ref_st[ch] == 0:
+            assert np.array_equal(got[ch * chunk:(ch + 1) * chunk], ref_out[ch * chunk:(ch + 1) * chunk]), ch
 
 
 def test_host_wrappers(gpu, tmp_path):
