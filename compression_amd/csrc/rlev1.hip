@@ -12,10 +12,11 @@
 // header (bit 7 clear) is one by itself, and the delta byte is one too (bit 7
 // clear) or joins the base value's (bit 7 set), so a run spans 3 or 2.
 //
-// One workgroup (8 wave64s) per chunk:
+// One workgroup (16 wave64s) per chunk; a chunk fills most of a CU's LDS, so
+// the workgroup is as wide as the hardware allows:
 //   1. setup and staging: the chunk's interleaved words are scattered into
 //      per-lane LDS regions (the same code as RLE v2);
-//   2. cut table: 16 threads per lane count the lane's terminators, then list
+//   2. cut table: 32 threads per lane count the lane's terminators, then list
 //      where each pseudo-varint starts: cut[t][0] = 0, cut[t][i+1] = one past
 //      the i-th terminator (16-bit entries);
 //   3. walk: 32 threads of wave 0, one per lane, follow the token chain in
@@ -49,7 +50,7 @@ namespace mdz {
 namespace {
 
 enum : uint32_t { kHdr = 0, kDelta = 1, kBase = 2, kRun = 3, kLit = 4, kDone = 5 };
-constexpr uint32_t kNT = 512;            // threads per workgroup
+constexpr uint32_t kNT = 1024;           // threads per workgroup
 constexpr uint32_t kGroups = 64;         // value groups (walk checkpoints) per lane
 constexpr uint32_t kCutThreads = kNT / 32;   // threads per lane building the cut table
 constexpr uint32_t kNoErr = 0xffffffffu;
