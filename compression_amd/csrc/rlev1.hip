@@ -468,7 +468,7 @@ MDZ_DEV BaseT<EB> unit_value(TokVal<EB> m, uint32_t vi, const uint8_t* base, con
 }
 
 // Decodes values [V, V + n) of lane t (n <= kUnit, V a multiple of kUnit)
-// from its token list and stores them as P-byte pieces, one piece at a time.
+// from its token list and stores them as P-byte pieces.
 // i0 = the token covering V; tokens are taken kMT = 4 at a time (T0..T3,
 // and e = the first value after them).  A piece inside the current batch
 // decodes its values unrolled (their loads overlap); a piece reaching past
@@ -500,56 +500,72 @@ MDZ_DEV void decode_unit(const uint8_t* base, const uint16_t* cut, const uint32_
         return unit_value<EB>(m, vi, base, cut, nterm, cl, ok, le) & vmask;
     };
     load();
+    // Groups of up to 8 pieces (one 128-byte line of 16-byte pieces) are kept
+    // in registers and stored together, so each line is written within a few
+    // cycles (lines written piece by piece over a whole unit are evicted from
+    // L2 half-written).
+    constexpr uint32_t NP = kUnit * EB / P;         // pieces per unit
+    constexpr uint32_t NG = NP < 8 ? NP : 8;        // pieces per group
     const uint32_t lb = V * EB;
     uint32_t r = lb / line, off = lb - r * line;
-    for (uint32_t q = 0; q < n / NV; ++q) {
-        const uint32_t v0 = V + q * NV;
-        uint32_t w[4] = {0, 0, 0, 0};
-        if (min(v0 + NV, vend) <= e) {
+    for (uint32_t g0 = 0; g0 < NP && g0 * NV < n; g0 += NG) {
+        uint32_t ub[NG][4];
 #pragma unroll
-            for (uint32_t i = 0; i < NV; ++i) {
-                const uint32_t vi = v0 + i;
-                bool ok;
-                int32_t le;
-                const VT val = value(vi, ok, le);
-                const bool act = vi < vend;
-                if (act && le != MDZ_ST_OK) ekey = min(ekey, err_key(vi, 2, le));
-                put_value<EB>(w, i, act && ok ? val : (VT)0);
-            }
-        } else {
-            // value by value; the piece's bytes shift in from the top of w
-            for (uint32_t i = 0; i < NV; ++i) {
-                const uint32_t vi = v0 + i;
-                while (vi >= e && vi < vend) {
-                    ib += kMT;
-                    load();
+        for (uint32_t gq = 0; gq < NG; ++gq) {
+            const uint32_t q = g0 + gq, v0 = V + q * NV;
+            uint32_t (&w)[4] = ub[gq];
+            w[0] = w[1] = w[2] = w[3] = 0;
+            if (q * NV >= n) continue;
+            if (min(v0 + NV, vend) <= e) {
+#pragma unroll
+                for (uint32_t i = 0; i < NV; ++i) {
+                    const uint32_t vi = v0 + i;
+                    bool ok;
+                    int32_t le;
+                    const VT val = value(vi, ok, le);
+                    const bool act = vi < vend;
+                    if (act && le != MDZ_ST_OK) ekey = min(ekey, err_key(vi, 2, le));
+                    put_value<EB>(w, i, act && ok ? val : (VT)0);
                 }
-                bool ok;
-                int32_t le;
-                const VT val = value(vi, ok, le);
-                const bool act = vi < vend;
-                if (act && le != MDZ_ST_OK) ekey = min(ekey, err_key(vi, 2, le));
-                const VT x = act && ok ? val : (VT)0;
-                if constexpr (EB == 8) {
-                    w[0] = w[2]; w[1] = w[3];
-                    w[2] = (uint32_t)x; w[3] = (uint32_t)(x >> 32);
-                } else if constexpr (EB == 4) {
-                    w[0] = w[1]; w[1] = w[2]; w[2] = w[3]; w[3] = (uint32_t)x;
-                } else {
-                    w[0] = alignbyte(w[1], w[0], EB); w[1] = alignbyte(w[2], w[1], EB);
-                    w[2] = alignbyte(w[3], w[2], EB);
-                    w[3] = (w[3] >> (8 * EB)) | ((uint32_t)x << (32 - 8 * EB));
+            } else {
+                // value by value; the piece's bytes shift in from the top of w
+                for (uint32_t i = 0; i < NV; ++i) {
+                    const uint32_t vi = v0 + i;
+                    while (vi >= e && vi < vend) {
+                        ib += kMT;
+                        load();
+                    }
+                    bool ok;
+                    int32_t le;
+                    const VT val = value(vi, ok, le);
+                    const bool act = vi < vend;
+                    if (act && le != MDZ_ST_OK) ekey = min(ekey, err_key(vi, 2, le));
+                    const VT x = act && ok ? val : (VT)0;
+                    if constexpr (EB == 8) {
+                        w[0] = w[2]; w[1] = w[3];
+                        w[2] = (uint32_t)x; w[3] = (uint32_t)(x >> 32);
+                    } else if constexpr (EB == 4) {
+                        w[0] = w[1]; w[1] = w[2]; w[2] = w[3]; w[3] = (uint32_t)x;
+                    } else {
+                        w[0] = alignbyte(w[1], w[0], EB); w[1] = alignbyte(w[2], w[1], EB);
+                        w[2] = alignbyte(w[3], w[2], EB);
+                        w[3] = (w[3] >> (8 * EB)) | ((uint32_t)x << (32 - 8 * EB));
+                    }
                 }
+                // the piece now sits in the top P bytes of w: move it to the bottom
+                if constexpr (P == 8) { w[0] = w[2]; w[1] = w[3]; }
+                else if constexpr (P == 4) { w[0] = w[3]; }
+                else if constexpr (P == 2) { w[0] = w[3] >> 16; }
+                else if constexpr (P == 1) { w[0] = w[3] >> 24; }
             }
-            // the piece now sits in the top P bytes of w: move it to the bottom
-            if constexpr (P == 8) { w[0] = w[2]; w[1] = w[3]; }
-            else if constexpr (P == 4) { w[0] = w[3]; }
-            else if constexpr (P == 2) { w[0] = w[3] >> 16; }
-            else if constexpr (P == 1) { w[0] = w[3] >> 24; }
         }
-        store_piece<P>(out, (r * 32 + t) * line + off, limit, w);
-        off += P;
-        if (off == line) { off = 0; ++r; }
+#pragma unroll
+        for (uint32_t gq = 0; gq < NG; ++gq) {
+            if ((g0 + gq) * NV >= n) break;
+            store_piece<P>(out, (r * 32 + t) * line + off, limit, ub[gq]);
+            off += P;
+            if (off == line) { off = 0; ++r; }
+        }
     }
     if (ekey != kNoErr) atomicMin(lerr, ekey);
 }
