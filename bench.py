@@ -58,7 +58,7 @@ class RleV1Workload:
     name = "rlev1_u32_64MiB_plain"
     dtype = "u32"
 
-    def __init__(self, line=128, ring=0, n_vals=1 << 24, seed=0x4D445A01):
+    def __init__(self, line=128, ring=0, n_vals=1 << 24, seed=0x4D445A01, **_):
         from compression_amd import host as H
         from compression_amd.lanes import DeviceLanes
         import compression_amd as mdz
@@ -247,20 +247,47 @@ class Config2Workload:
     codec."""
     dtype = "u64+u8"
 
-    def __init__(self, line=512, batched=True, **_):
-        from compression_amd import batch
+    def __init__(self, line=512, batched=True, tuned=True, graphed=True, **_):
+        import compression_amd as mdz
+        from compression_amd import batch, graph, tune
         self.parts = [RleV2Workload(line=line), BrleWorkload()]
         r, b = self.parts
         self.batched = batched
         self.name = ("rlev2_i64_1GiB+brle_u8_256MiB_" + ("batched" if batched else "plain"))
         self.out_bytes = sum(p.out_bytes for p in self.parts)
         self.alg_bytes = sum(p.alg_bytes for p in self.parts)
-        self.batch = batch.Batch(r.lanes, r.dst, r.out_bytes, r.status, r.max_chunk,
-                                 b.dev, b.dst, b.out_bytes, b.status)
-        self.batch.set_order(self.batch.size_order())
-        self.config = {"workload": self.name, "codecs": ["rle_v2", "byte_rle"],
-                       "launch": "persistent work queue (size order)" if batched else "per-codec per-chunk",
+        self.tuned = None
+        if batched and tuned:
+            # launch shape from the autotuner (untimed here; each candidate's
+            # decoded bytes are checked against the first's)
+            self.batch, res = tune.tune_batch(
+                rlev2=r.lanes, rlev2_dst=r.dst, rlev2_dst_bytes=r.out_bytes, rlev2_status=r.status,
+                rlev2_max_chunk=r.max_chunk, brle=b.dev, brle_dst=b.dst, brle_dst_bytes=b.out_bytes,
+                brle_status=b.status, reps=5)
+            ok_ms = [t for _, t in res.candidates if t is not None]
+            self.tuned = {"choice": res.choice, "ms": round(res.ms, 5), "from_db": res.from_db,
+                          "candidates": len(res.candidates),
+                          "slowest_candidate_ms": round(max(ok_ms), 5) if ok_ms else None}
+        else:
+            self.batch = batch.Batch(r.lanes, r.dst, r.out_bytes, r.status, r.max_chunk,
+                                     b.dev, b.dst, b.out_bytes, b.status)
+            self.batch.set_order(self.batch.size_order())
+        self.graph = None
+        if batched and graphed:
+            # the pipeline (cursor reset, persistent kernel, status passes)
+            # captured once and replayed with one launch per step
+            self.gstream = mdz.Stream()
+            with graph.capture(self.gstream) as g:
+                self.batch.submit(self.gstream.ptr)
+            self.graph = g
+        launch = "per-codec per-chunk"
+        if batched:
+            launch = ("hipGraph of the " if self.graph else "") + "persistent work queue" + (
+                ", autotuned shape" if self.tuned else " (size order)")
+        self.config = {"workload": self.name, "codecs": ["rle_v2", "byte_rle"], "launch": launch,
                        "out_bytes": self.out_bytes, "parts": [p.config for p in self.parts]}
+        if self.tuned:
+            self.config["tuned"] = self.tuned
 
     def kernels(self):
         if self.batched:
@@ -272,7 +299,9 @@ class Config2Workload:
             p.step(stream)
 
     def step(self, stream):
-        if self.batched:
+        if self.graph is not None:
+            self.graph.launch(stream)
+        elif self.batched:
             self.batch.submit(stream)
         else:
             self.plain_step(stream)
@@ -315,21 +344,31 @@ class DeflateWorkload:
         pred = self.model.predict(sched.features(self.db), count)
         self.order_model = sched.lpt_order(pred)
         self.order_size = sched.lpt_order(b.src_len.astype(np.float64))
-        self.db.set_order(self.order_model)
-        self.order_db = self.db.order
         self.db.set_order(self.order_size)
         self.size_db = self.db.order
-        self.db.order = self.order_db
+        # order (learned cost / compressed size / index) and grid from the
+        # autotuner; the chosen order stays set on the batch
+        from compression_amd import graph, tune
+        import compression_amd as mdz
+        self.blocks, res = tune.tune_inflate(
+            self.db, {"learned": self.order_model, "size": self.order_size, "index": None}, reps=2)
+        self.order_db = self.db.order
+        self.tuned = {"choice": res.choice, "ms": round(res.ms, 3), "from_db": res.from_db,
+                      "candidates": [[c, None if t is None else round(t, 3)] for c, t in res.candidates]}
+        self.gstream = mdz.Stream()
+        with graph.capture(self.gstream) as g:
+            deflate.inflate_sched(self.db, blocks=self.blocks, stream=self.gstream.ptr)
+        self.graph = g
         self.config = {"workload": self.name, "codec": "deflate", "streams": count,
                        "out_bytes": self.out_bytes, "src_bytes": int(b.src_len.sum()),
                        "modes": {"level6": int((b.mode == 6).sum()), "fixed": int((b.mode == 1).sum()),
                                  "stored": int((b.mode == 0).sum())},
-                       "launch": "persistent work queue, learned-cost LPT order",
-                       "slots": deflate.slots()}
+                       "launch": "hipGraph of the persistent work queue; order and grid autotuned "
+                                 "over learned-cost LPT, compressed-size LPT and index order",
+                       "tuned": self.tuned, "slots": deflate.slots()}
 
     def step(self, stream):
-        from compression_amd import deflate
-        deflate.inflate_sched(self.db, stream=stream)
+        self.graph.launch(stream)
 
     def plain_step(self, stream):
         from compression_amd import deflate
@@ -338,7 +377,7 @@ class DeflateWorkload:
     def size_step(self, stream):
         from compression_amd import deflate
         self.db.order = self.size_db
-        deflate.inflate_sched(self.db, stream=stream)
+        deflate.inflate_sched(self.db, blocks=self.blocks, stream=stream)
         self.db.order = self.order_db
 
     batched = True
@@ -411,6 +450,10 @@ def run(args):
         kw["line"] = args.line
     if args.ring:
         kw["ring"] = args.ring
+    if args.no_tune:
+        kw["tuned"] = False
+    if args.no_graph:
+        kw["graphed"] = False
     wl = WORKLOADS[args.workload](**kw)
     stream = mdz.Stream()
     for _ in range(args.warmup):
@@ -542,6 +585,10 @@ def main():
     ap.add_argument("--ring", type=int, default=0, help="LDS ring words override")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--requests", type=int, default=96, help="request pool size for --workload serve")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="config2: size order and default shape instead of the autotuner's choice")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="config2: submit the batch directly instead of replaying a hipGraph")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "mdz decodes on one GPU; chunks shard across GPUs without "

@@ -41,7 +41,7 @@ class BatchT(C.Structure):
     """mdz_batch_t (include/mdz.h)."""
     _fields_ = [("rlev2", LanesT), ("rlev2_dst", vp), ("rlev2_dst_bytes", u64), ("rlev2_status", vp),
                 ("rlev2_max_chunk", u32), ("brle_max_chunk", u32), ("rlev2_max_part", u32),
-                ("reserved0", u32), ("brle_ptr", vp), ("brle_data", vp),
+                ("rlev2_window", u32), ("brle_ptr", vp), ("brle_data", vp),
                 ("brle_n_chunks", u64), ("brle_chunk_bytes", u64), ("brle_dst", vp),
                 ("brle_dst_bytes", u64), ("brle_status", vp), ("items", vp), ("n_items", u32),
                 ("blocks", u32), ("cursor", vp)]
@@ -49,7 +49,7 @@ class BatchT(C.Structure):
 
 class LanesTuneT(C.Structure):
     _fields_ = [("ring_words", u32), ("max_chunk_bytes", u32), ("max_part_bytes", u32),
-                ("max_lane_tokens", u32), ("reserved", u32 * 4)]
+                ("max_lane_tokens", u32), ("rlev2_window", u32), ("reserved", u32 * 3)]
 
 
 # name -> (restype, argtypes); every symbol declared in include/mdz.h
@@ -57,6 +57,7 @@ SIGNATURES = {
     "mdz_version": (C.c_char_p, []),
     "mdz_last_error": (C.c_char_p, []),
     "mdz_device_count": (C.c_int, [C.POINTER(C.c_int)]),
+    "mdz_device_cus": (C.c_int, [C.POINTER(C.c_int)]),
     "mdz_dev_malloc": (C.c_int, [C.POINTER(vp), u64]),
     "mdz_dev_free": (C.c_int, [vp]),
     "mdz_memcpy_htod": (C.c_int, [vp, vp, u64, vp]),
@@ -99,6 +100,11 @@ SIGNATURES = {
     "mdz_inflate_slots": (C.c_int, [C.POINTER(u32)]),
     "mdz_cost_features": (C.c_int, [vp, C.c_int, C.c_int, vp, vp]),
     "mdz_batch_submit": (C.c_int, [C.POINTER(BatchT), vp]),
+    "mdz_graph_begin": (C.c_int, [vp]),
+    "mdz_graph_end": (C.c_int, [vp, C.POINTER(vp)]),
+    "mdz_graph_launch": (C.c_int, [vp, vp]),
+    "mdz_graph_destroy": (C.c_int, [vp]),
+    "mdz_graph_nodes": (C.c_int, [vp, C.POINTER(u64), C.POINTER(u64)]),
     "mdz_mlp_forward": (C.c_int, [vp, vp, C.c_int, vp, vp]),
     "mdz_mlp_grad": (C.c_int, [vp, vp, vp, C.c_int, vp, vp, vp]),
     "mdz_mlp_adam": (C.c_int, [vp, vp, vp, vp, C.c_float, u32, vp]),
@@ -228,3 +234,10 @@ def device_count() -> int:
     n = C.c_int(0)
     rc = lib().mdz_device_count(C.byref(n))
     return n.value if rc == 0 else 0
+
+
+def device_cus() -> int:
+    """Compute units of the current device."""
+    n = C.c_int(0)
+    check(lib().mdz_device_cus(C.byref(n)), "mdz_device_cus")
+    return n.value

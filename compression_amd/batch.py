@@ -25,11 +25,15 @@ def item(codec: int, chunk: np.ndarray, part=0) -> np.ndarray:
 
 
 class Batch:
-    """Keeps the launch description, the cursor and an optional item order."""
+    """Keeps the launch description, the cursor and an optional item order.
+
+    ``quarters`` decodes RLE v2 in lane quarters (four 4-wave workgroups per
+    chunk instead of one 8-wave workgroup); ``window`` is the RLE v2 consumer
+    window (run records claimed at once, 0 = 16)."""
 
     def __init__(self, rlev2=None, rlev2_dst=None, rlev2_dst_bytes=0, rlev2_status=None,
                  rlev2_max_chunk=0, brle=None, brle_dst=None, brle_dst_bytes=0, brle_status=None,
-                 quarters: bool = False):
+                 quarters: bool = False, window: int = 0):
         self.rlev2, self.brle = rlev2, brle
         self.cursor = DeviceBuffer(4)
         self.items = None
@@ -43,6 +47,7 @@ class Batch:
             b.rlev2_status = as_ptr(rlev2_status)
             b.rlev2_max_chunk = int(rlev2_max_chunk)
             b.rlev2_max_part = rlev2.max_part if quarters else 0
+            b.rlev2_window = int(window)
         self.parts = 4 if (rlev2 is not None and quarters and rlev2.max_part) else 1
         if brle is not None:
             b.brle_ptr = brle.ptr.ptr

@@ -7,6 +7,8 @@
 #include <string.h>
 
 #include <mutex>
+#include <new>
+#include <vector>
 #include <unordered_map>
 
 #include "mdz_internal.h"
@@ -93,6 +95,13 @@ const char* mdz_version(void) { return "mdz 0.1.0 gfx950"; }
 const char* mdz_last_error(void) { return mdz::g_last_error; }
 
 int mdz_device_count(int* count) { return herr(hipGetDeviceCount(count)); }
+int mdz_device_cus(int* cus) {
+    if (!cus) return MDZ_ERR_INVALID;
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return herr(e);
+    return herr(hipDeviceGetAttribute(cus, hipDeviceAttributeMultiprocessorCount, dev));
+}
 int mdz_dev_malloc(void** ptr, uint64_t bytes) {
     return hipMalloc(ptr, bytes ? bytes : 1) == hipSuccess ? MDZ_OK : MDZ_ERR_NOMEM;
 }
@@ -445,6 +454,77 @@ int mdz_mlp_adam(float* params, float* grad, float* m, float* v, float lr, uint3
 // ------------------------------------------------ batched persistent decode
 int mdz_batch_submit(const mdz_batch_t* batch, void* stream) {
     return mdz::batch_submit(batch, S(stream));
+}
+
+// ------------------------------------------------------ captured pipelines
+}  // extern "C"
+
+struct mdz_graph_s {
+    hipGraph_t graph;
+    hipGraphExec_t exec;
+};
+
+extern "C" {
+
+int mdz_graph_begin(void* stream) {
+    if (!stream) return MDZ_ERR_INVALID;
+    return herr(hipStreamBeginCapture(S(stream), hipStreamCaptureModeRelaxed));
+}
+
+int mdz_graph_end(void* stream, mdz_graph_t* graph) {
+    if (!stream || !graph) return MDZ_ERR_INVALID;
+    *graph = nullptr;
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(S(stream), &g);
+    if (e != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        return mdz::record_hip(e, "hipStreamEndCapture");
+    }
+    hipGraphExec_t x = nullptr;
+    e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        (void)hipGraphDestroy(g);
+        return mdz::record_hip(e, "hipGraphInstantiate");
+    }
+    mdz_graph_s* h = new (std::nothrow) mdz_graph_s{g, x};
+    if (!h) {
+        (void)hipGraphExecDestroy(x);
+        (void)hipGraphDestroy(g);
+        return MDZ_ERR_NOMEM;
+    }
+    *graph = h;
+    return MDZ_OK;
+}
+
+int mdz_graph_launch(mdz_graph_t graph, void* stream) {
+    if (!graph) return MDZ_ERR_INVALID;
+    return mdz::record_hip(hipGraphLaunch(graph->exec, S(stream)), "hipGraphLaunch");
+}
+
+int mdz_graph_destroy(mdz_graph_t graph) {
+    if (!graph) return MDZ_OK;
+    const hipError_t e1 = hipGraphExecDestroy(graph->exec);
+    const hipError_t e2 = hipGraphDestroy(graph->graph);
+    delete graph;
+    return herr(e1 != hipSuccess ? e1 : e2);
+}
+
+int mdz_graph_nodes(mdz_graph_t graph, uint64_t* nodes, uint64_t* kernels) {
+    if (!graph || !nodes || !kernels) return MDZ_ERR_INVALID;
+    size_t n = 0;
+    hipError_t e = hipGraphGetNodes(graph->graph, nullptr, &n);
+    if (e != hipSuccess) return herr(e);
+    std::vector<hipGraphNode_t> v(n);
+    if (n && (e = hipGraphGetNodes(graph->graph, v.data(), &n)) != hipSuccess) return herr(e);
+    uint64_t k = 0;
+    for (size_t i = 0; i < n; ++i) {
+        hipGraphNodeType t;
+        if ((e = hipGraphNodeGetType(v[i], &t)) != hipSuccess) return herr(e);
+        k += t == hipGraphNodeTypeKernel;
+    }
+    *nodes = n;
+    *kernels = k;
+    return MDZ_OK;
 }
 
 }  // extern "C"

@@ -122,8 +122,11 @@ int batch_submit(const mdz_batch_t* bt, hipStream_t s) {
                                     ? ((uint64_t)bt->rlev2_max_chunk + 32 * 12 + 255) & ~255ull
                                     : (uint64_t)lds_max;
         if (want < cap) cap = (uint32_t)want;
+        const uint32_t win = bt->rlev2_window ? bt->rlev2_window : kWinDefault;
+        if (!rlev2_window_ok(win)) return MDZ_ERR_INVALID;
         r = Rlev2Args{(const uint8_t*)L.payload, L.blk_off, L.col_len, L.n_chunks, chunk, ru, cap,
-                      L.payload_bytes, (uint8_t*)bt->rlev2_dst, bt->rlev2_dst_bytes, bt->rlev2_status};
+                      L.payload_bytes, (uint8_t*)bt->rlev2_dst, bt->rlev2_dst_bytes, bt->rlev2_status,
+                      win};
         region = (size_t)cap + 16 + tables;
     }
     if (bt->brle_n_chunks) {

@@ -56,10 +56,10 @@ __global__ __launch_bounds__(kQuarterThreads) void rlev2_quarter_kernel(Rlev2Arg
 
 template <int EB>
 hipError_t launch(const mdz_lanes_t& in, void* dst, uint64_t dst_bytes, int32_t* status,
-                  uint32_t data_cap, bool quarters, hipStream_t s) {
+                  uint32_t data_cap, bool quarters, uint32_t win, hipStream_t s) {
     const Rlev2Args a{(const uint8_t*)in.payload, in.blk_off, in.col_len, in.n_chunks,
                       in.chunk_bytes, in.lane_line, data_cap, in.payload_bytes, (uint8_t*)dst,
-                      dst_bytes, status};
+                      dst_bytes, status, win};
     hipError_t e;
     if (!quarters) {
         const size_t smem = (size_t)data_cap + 16 + sizeof(Rlev2Tables<PartPool<32>::value>);
@@ -97,6 +97,8 @@ int rlev2_decompress(const mdz_lanes_t* in, void* dst, uint64_t dst_bytes, int32
     if (in->n_chunks > 0x1fffffffull) return MDZ_ERR_UNSUPPORTED;
     if (((uintptr_t)in->payload & 15) != 0 || ((uintptr_t)dst % eb) != 0) return MDZ_ERR_INVALID;
     const bool quarters = tune && tune->max_part_bytes;
+    const uint32_t win = tune && tune->rlev2_window ? tune->rlev2_window : kWinDefault;
+    if (!rlev2_window_ok(win)) return MDZ_ERR_INVALID;
     const uint32_t tables = quarters ? (uint32_t)sizeof(Rlev2Tables<PartPool<kQuarterLanes>::value>)
                                      : (uint32_t)sizeof(Rlev2Tables<PartPool<32>::value>);
     const uint32_t lds_max = (160 * 1024 - 16 - tables - 256) & ~255u;
@@ -108,10 +110,10 @@ int rlev2_decompress(const mdz_lanes_t* in, void* dst, uint64_t dst_bytes, int32
     if (want < cap) cap = (uint32_t)want;
     hipError_t e;
     switch (eb) {
-    case 1: e = launch<1>(*in, dst, dst_bytes, status, cap, quarters, s); break;
-    case 2: e = launch<2>(*in, dst, dst_bytes, status, cap, quarters, s); break;
-    case 4: e = launch<4>(*in, dst, dst_bytes, status, cap, quarters, s); break;
-    default: e = launch<8>(*in, dst, dst_bytes, status, cap, quarters, s); break;
+    case 1: e = launch<1>(*in, dst, dst_bytes, status, cap, quarters, win, s); break;
+    case 2: e = launch<2>(*in, dst, dst_bytes, status, cap, quarters, win, s); break;
+    case 4: e = launch<4>(*in, dst, dst_bytes, status, cap, quarters, win, s); break;
+    default: e = launch<8>(*in, dst, dst_bytes, status, cap, quarters, win, s); break;
     }
     return record_hip(e, "rlev2_decode_kernel launch");
 }

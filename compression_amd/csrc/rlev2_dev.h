@@ -108,7 +108,9 @@ MDZ_DEV void put_val(uint8_t* out, uint64_t limit, uint32_t q, const LanePlace& 
     else out[off] = (uint8_t)v;
 }
 
-constexpr uint32_t kWin = 32;                   // records a consumer wave claims at once
+constexpr uint32_t kWinDefault = 16;            // records a consumer wave claims at once
+// Consumer windows a launch may use: they divide both record pools.
+inline bool rlev2_window_ok(uint32_t w) { return w == 8 || w == 16 || w == 32 || w == 64; }
 constexpr uint32_t kOpen = 0xffffffffu;         // final_ while the walk runs
 constexpr uint32_t kSpinMax = 1u << 22;         // consumer wait bound (about 0.2 s)
 
@@ -440,7 +442,7 @@ struct Rlev2Tables {
     uint32_t fits;
     uint32_t extent_ok;
     uint32_t pub;            // records the walk has published this epoch
-    uint32_t claimed;        // records claimed by consumer waves (windows of kWin)
+    uint32_t claimed;        // records claimed by consumer waves (windows of Rlev2Args::win)
     uint32_t final_;         // records of this epoch once the walk has ended, else kOpen
     uint32_t more;           // a lane stream has runs left after this epoch
     uint32_t total;          // words of all lanes
@@ -469,6 +471,7 @@ struct Rlev2Args {
     uint8_t* dst;
     uint64_t dst_bytes;
     int32_t* status;
+    uint32_t win;            // records per consumer window: 8, 16, 32 or 64 (divides the pool)
 };
 
 // Decodes lanes [part*NL, part*NL + NL) of chunk c with one workgroup of NT
@@ -488,6 +491,7 @@ MDZ_DEV void rlev2_part(const Rlev2Args& a, uint64_t c, uint32_t part, uint32_t*
     const uint64_t* __restrict__ col_len = a.col_len;
     const uint64_t n_chunks = a.n_chunks, chunk_bytes = a.chunk_bytes, payload_bytes = a.payload_bytes;
     const uint32_t ru = a.ru, data_cap = a.data_cap;
+    const uint32_t win = a.win;
     uint8_t* __restrict__ dst = a.dst;
     const uint64_t dst_bytes = a.dst_bytes;
     int32_t* __restrict__ status = a.status;
@@ -531,7 +535,7 @@ MDZ_DEV void rlev2_part(const Rlev2Args& a, uint64_t c, uint32_t part, uint32_t*
     const uint32_t sh = (ru & (ru - 1)) ? 32u : (uint32_t)__builtin_ctz(ru);
     uint8_t* const out = dst + out_base;
     // Steps 2-4 overlap: wave 0 walks the headers and publishes records as
-    // it goes; every wave (wave 0 once its walk ends) claims windows of kWin
+    // it goes; every wave (wave 0 once its walk ends) claims windows of win
     // published records, checks them one per lane and decodes them.  Only
     // consumers wait, and only for records the walk will publish, so every
     // wait ends; a bound on the wait turns a broken invariant into a status.
@@ -569,7 +573,7 @@ MDZ_DEV void rlev2_part(const Rlev2Args& a, uint64_t c, uint32_t part, uint32_t*
                 n = min(n + (uint32_t)__popcll(want), kRunPool);
                 // publish when a consumer window has filled (each publish
                 // waits for the records' LDS writes)
-                if (n >= pubd + kWin) {
+                if (n >= pubd + win) {
                     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
                     if (l == 0) __hip_atomic_store(&tb.pub, n, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
                     pubd = n;
@@ -587,20 +591,20 @@ MDZ_DEV void rlev2_part(const Rlev2Args& a, uint64_t c, uint32_t part, uint32_t*
         }
         for (;;) {
             uint32_t w0 = 0;
-            if (l == 0) w0 = atomicAdd(&tb.claimed, kWin);
+            if (l == 0) w0 = atomicAdd(&tb.claimed, win);
             w0 = __builtin_amdgcn_readfirstlane(w0);
             uint32_t avail = 0;
             for (uint32_t it = 0;; ++it) {
                 const uint32_t fin = __hip_atomic_load(&tb.final_, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP);
                 if (fin != kOpen) { avail = fin; break; }
                 const uint32_t pv = __hip_atomic_load(&tb.pub, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP);
-                if (pv >= w0 + kWin) { avail = w0 + kWin; break; }
+                if (pv >= w0 + win) { avail = w0 + win; break; }
                 if (it >= kSpinMax) { if (l == 0) tb.stuck = 1; break; }
                 __builtin_amdgcn_s_sleep(2);
             }
             avail = __builtin_amdgcn_readfirstlane(avail);
             if (w0 >= avail) break;
-            const uint32_t cnt = min(kWin, avail - w0);
+            const uint32_t cnt = min(win, avail - w0);
             // check, one record per lane: the oracle's checks in its order
             // (patch lists included) complete the descriptor in registers,
             // and the run's parameters are decoded here, one run per lane

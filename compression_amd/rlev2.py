@@ -16,19 +16,20 @@ from .lanes import DeviceLanes, as_ptr
 
 
 def decompress(lanes: DeviceLanes, dst, dst_bytes: int | None = None, status=None, stream=None,
-               max_chunk_bytes: int = 0, quarters: bool = False) -> None:
+               max_chunk_bytes: int = 0, quarters: bool = False, window: int = 0) -> None:
     """Enqueue the decode of every chunk; ``lanes.lane_line`` is READ_UNIT.
 
     ``quarters`` decodes each chunk's lane quarters with their own workgroups
     (LDS sized by the largest quarter, which the host knows from col_len);
     by default one workgroup decodes each whole chunk, which measures faster
     on the 1 GiB column (0.79 ms against 1.08 ms: a quarter's staging and
-    setup cost nearly what a whole chunk's do)."""
+    setup cost nearly what a whole chunk's do).  ``window`` is the number of
+    run records a consumer wavefront claims at once (8, 16, 32 or 64; 0 = 16)."""
     s = lanes.struct()
     n = lanes.out_bytes if dst_bytes is None else int(dst_bytes)
     part = lanes.max_part if quarters else 0
-    if max_chunk_bytes or part:
-        tune = LanesTuneT(0, max_chunk_bytes, part)
+    if max_chunk_bytes or part or window:
+        tune = LanesTuneT(0, max_chunk_bytes, part, 0, window)
         rc = lib().mdz_rlev2_decompress_tuned(C.byref(s), as_ptr(dst), n, as_ptr(status), C.byref(tune),
                                               as_ptr(stream))
     else:

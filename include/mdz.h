@@ -113,7 +113,9 @@ typedef struct {
     uint32_t max_lane_tokens;    /* RLE v1: at most this many tokens listed per lane stream
                                     (0 = as many as fit in LDS); a chunk with a longer lane
                                     takes the checkpoint decode.  For testing that path */
-    uint32_t reserved[4];
+    uint32_t rlev2_window;       /* RLE v2: run records a consumer wavefront claims at once
+                                    from the header walk's ring: 8, 16, 32 or 64 (0 = 16) */
+    uint32_t reserved[3];
 } mdz_lanes_tune_t;
 
 /* RLE v1 decode -- replaces rle_v1::decompress_gpu<READ_COL_TYPE, DATA_TYPE,
@@ -270,6 +272,8 @@ int mdz_inflate_batch_sched(const mdz_inflate_in_t* in, mdz_inflate_status_t* ou
                             int parse_hdr, const uint32_t* order, uint32_t blocks,
                             uint32_t* cursor, mdz_inflate_cost_t* cost, void* stream);
 
+/* Compute units of the current device (sizes persistent grids). */
+int mdz_device_cus(int* cus);
 /* Workgroups of the persistent inflate kernel that fit on the device at once. */
 int mdz_inflate_slots(uint32_t* slots);
 
@@ -326,7 +330,7 @@ typedef struct {
     uint32_t brle_max_chunk;        /* 0 = the byte-RLE worst case */
     uint32_t rlev2_max_part;        /* as mdz_lanes_tune_t.max_part_bytes: nonzero decodes RLE v2
                                        in lane quarters (256-thread workgroups throughout) */
-    uint32_t reserved0;
+    uint32_t rlev2_window;          /* as mdz_lanes_tune_t.rlev2_window (0 = 16) */
     const uint64_t* brle_ptr;
     const void* brle_data;
     uint64_t brle_n_chunks;         /* 0: no byte-RLE part */
@@ -341,6 +345,29 @@ typedef struct {
 } mdz_batch_t;
 
 int mdz_batch_submit(const mdz_batch_t* batch, void* stream);
+
+/* ==================================================== captured pipelines ===== *
+ * The mdz calls a caller issues on one stream between mdz_graph_begin and
+ * mdz_graph_end are captured into a hipGraph and instantiated; each
+ * mdz_graph_launch replays the whole pipeline (e.g. a batch's status fill,
+ * cursor reset, persistent kernel and status finish) with one launch.  No
+ * counterpart in the reference, which issues each launch from the host; it
+ * is for serving loops that replay the same batch shape.
+ *  - `stream` must be a created stream (the legacy default stream cannot be
+ *    captured); capture uses hipStreamCaptureModeRelaxed.
+ *  - The graph keeps the pointers and sizes the captured calls were given:
+ *    every buffer they name must stay allocated while the graph is used.
+ *  - mdz decode pipelines record only kernel nodes (no memset nodes), so a
+ *    graph may be replayed any number of times.
+ *  - If a call fails during capture, end the capture anyway; mdz_graph_end
+ *    then returns the error and no graph. */
+typedef struct mdz_graph_s* mdz_graph_t;
+int mdz_graph_begin(void* stream);
+int mdz_graph_end(void* stream, mdz_graph_t* graph);
+int mdz_graph_launch(mdz_graph_t graph, void* stream);
+int mdz_graph_destroy(mdz_graph_t graph);
+/* Node counts of a captured graph: all nodes, and those that are kernels. */
+int mdz_graph_nodes(mdz_graph_t graph, uint64_t* nodes, uint64_t* kernels);
 
 #ifdef __cplusplus
 }

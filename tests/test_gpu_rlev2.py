@@ -7,7 +7,8 @@ from oracle import pyoracle as O
 pytestmark = pytest.mark.gpu
 
 
-def _run(mdz, c, dst_bytes=None, payload=None, col_len=None, max_chunk=0, quarters=False):
+def _run(mdz, c, dst_bytes=None, payload=None, col_len=None, max_chunk=0, quarters=False,
+         window=0):
     from compression_amd import rlev2
     from compression_amd.lanes import DeviceLanes
     pay = c.payload if payload is None else payload
@@ -17,7 +18,8 @@ def _run(mdz, c, dst_bytes=None, payload=None, col_len=None, max_chunk=0, quarte
     dst = mdz.DeviceBuffer(max(n, 1))
     dst.fill(0xCD)
     st = mdz.DeviceBuffer(4 * c.n_chunks)
-    rlev2.decompress(lanes, dst, n, st, max_chunk_bytes=max_chunk, quarters=quarters)
+    rlev2.decompress(lanes, dst, n, st, max_chunk_bytes=max_chunk, quarters=quarters,
+                     window=window)
     return dst.download(n), st.download(4 * c.n_chunks, np.int32)
 
 
@@ -90,9 +92,28 @@ def test_host_wrapper(gpu):
     assert out == x.tobytes()
 
 
+@pytest.mark.parametrize("window", [8, 16, 32, 64])
+@pytest.mark.parametrize("quarters", [True, False])
+def test_consumer_windows(gpu, window, quarters):
+    # every consumer window gives the same bytes: enough runs per chunk for
+    # several record epochs (short segments at a small READ_UNIT)
+    x = O.gen_rlev2_i64(1 << 18, 0x4D445A02)
+    c = O.rlev2_encode(x, 131072, 8, threads=8)
+    got, st = _run(gpu, c, quarters=quarters, window=window)
+    assert (st == 0).all() and np.array_equal(got, x.view(np.uint8))
+
+
+def test_bad_window_is_rejected(gpu):
+    x = O.gen_rlev2_i64(1 << 14, 3)
+    c = O.rlev2_encode(x, 32768, 64, threads=2)
+    with pytest.raises(gpu.MdzError):
+        _run(gpu, c, window=24)
+
+
+@pytest.mark.parametrize("window", [8, 32])
 @pytest.mark.parametrize("quarters", [True, False])
 @pytest.mark.parametrize("seed", [1, 2, 3])
-def test_random_corruption_matches_oracle(gpu, seed, quarters):
+def test_random_corruption_matches_oracle(gpu, seed, quarters, window):
     # one random byte overwritten in most chunks (header or data) and a few
     # lane lengths changed: every chunk's status matches the oracle, and
     # chunks the oracle decodes cleanly are bit-exact
@@ -109,7 +130,7 @@ def test_random_corruption_matches_oracle(gpu, seed, quarters):
         if rng.random() < 0.2:
             t = ch * 32 + int(rng.integers(0, 32))
             col[t] = max(0, int(col[t]) - int(rng.integers(1, 40)))
-    got, st = _run(gpu, c, payload=pay, col_len=col, quarters=quarters)
+    got, st = _run(gpu, c, payload=pay, col_len=col, quarters=quarters, window=window)
     ref = O.RleV2Container(pay, col, c.blk_off, c.n_chunks, c.chunk, c.elem_bytes, c.read_unit,
                            c.raw_bytes, c.payload_bytes)
     _, ref_out, ref_st = O.rlev2_decode(ref, threads=4)
