@@ -21,15 +21,17 @@
 //      the i-th terminator (16-bit entries);
 //   3. walk: 32 threads of wave 0, one per lane, follow the token chain in
 //      pseudo-varint steps (one table read and one data read per token),
-//      check each header as the oracle does, and list every token (first
-//      pseudo-varint, first value, header) in LDS after the cut table;
-//   4. one thread per listed token decodes its run base and marks the token
-//      covering the first value of every unit of 32 values;
-//   5. one thread per unit loads its tokens at once and decodes all 32
-//      values without branches: a literal straight from its pseudo-varint
-//      (start and length from the cut table), a run value as base + i*delta;
-//      every load of a unit is independent of the others, so they overlap.
-//      The unit's bytes are stored as P-byte pieces.
+//      check each header as the oracle does, list every token (first
+//      pseudo-varint, first value, header) in LDS after the cut table, and
+//      publish per lane how many values the listed tokens cover;
+//   4. decode, overlapped with the walk: the other waves (and wave 0 once it
+//      is done) claim units of 32 values of a lane in value order, wait until
+//      the lane's walk has passed the unit, find the unit's first token by
+//      binary search, load its tokens (run bases decoded here) and decode all
+//      32 values without branches: a literal straight from its pseudo-varint
+//      (start and length from the cut table), a run value as base + i*delta.
+//      Every load of a unit is independent of the others, so they overlap.
+//      The unit's bytes are stored as P-byte pieces, eight at a time.
 // If a lane has more tokens than its list holds, the chunk is walked again
 // with a checkpoint per value group and decoded in P-byte pieces in output
 // order, each stepping from its group's checkpoint to its token.
@@ -37,6 +39,9 @@
 // and value errors (decode) carry the key (value index, step) and are
 // combined with an atomic minimum; the chunk reports its lowest failing lane,
 // as the oracle does.  Values after a lane's first error are unspecified.
+// Decode waits end when the lane's walk passes the unit or ends, which every
+// walk does; the wait is bounded anyway, and an expired wait reports
+// MDZ_ST_INTERNAL instead of hanging.
 // A chunk whose lanes do not fit the LDS budget, with a lane stream over
 // 65535 bytes, or whose cut table does not fit, is decoded by a single-wave
 // decoder that streams the payload through an LDS window.
@@ -55,7 +60,7 @@ constexpr uint32_t kGroups = 64;         // value groups (walk checkpoints) per 
 constexpr uint32_t kCutThreads = kNT / 32;   // threads per lane building the cut table
 constexpr uint32_t kNoErr = 0xffffffffu;
 constexpr uint32_t kUnit = 32;           // values per decode unit (token path)
-constexpr uint32_t kUnitMax = 128;       // units per lane on the token path
+constexpr uint32_t kSpinMax = 1u << 22;  // decode-unit wait bound (about 0.2 s)
 constexpr uint32_t kMT = 4;              // tokens a unit loads at once
 
 template <int EB> struct VarintSpec;
@@ -213,6 +218,8 @@ struct V1Tables {
     uint32_t cut0[32];       // first cut-table entry of each lane
     uint32_t vstop[32];      // values the walk covered (later values are not decoded)
     uint32_t ntok[32];       // tokens listed per lane (token path)
+    uint32_t vpub[32];       // values the lane's listed tokens cover so far (token path)
+    uint32_t done[32];       // the lane's walk has ended: vstop and ntok are final
     uint32_t total;          // words of all lanes
     uint32_t fits;           // lane regions fit the data budget
     uint32_t extent_ok;      // lanes lie inside the chunk's payload extent
@@ -221,13 +228,15 @@ struct V1Tables {
     uint32_t tok_off;        // byte offset of the token lists in LDS
     uint32_t cap;            // tokens each lane's list holds
     uint32_t overflow;       // a lane has more tokens than its list holds
+    uint32_t claimed;        // decode units claimed, all lanes (token path)
+    uint32_t stuck;          // a decode wait expired (never expected)
+    uint32_t pad2[6];
     union {
         struct {                                 // staging
             uint32_t dest[32][32];
             uint8_t segof[kSegBlocks];
         } st;
         uint32_t ck[32][kGroups];                // checkpoint path: token k | v << 16
-        uint16_t ui[32][kUnitMax];               // token path: token covering value 32u
         LaneTable lt;                            // windowed decoder
     };
 };
@@ -420,19 +429,28 @@ template <int EB> struct TokVal {
     BaseT<EB> A, D;
     uint32_t tv;
     uint32_t lit;
+    uint32_t berr;           // a run's base: its LEB128 status (0 for a literal group)
 };
 
+// Token idx of a lane's list (idx >= nt: none; its first value reads as vs).
+// A run's base is decoded here from its pseudo-varint.
 template <int EB>
-MDZ_DEV TokVal<EB> tok_val(const uint32_t* kvs, const uint16_t* hds, const BaseT<EB>* bss,
-                           uint32_t nt, uint32_t idx, uint32_t vs) {
+MDZ_DEV TokVal<EB> tok_val(const uint8_t* base, const uint16_t* cut, uint32_t nterm, uint32_t cl,
+                           const uint32_t* kvs, const uint16_t* hds, uint32_t nt, uint32_t idx,
+                           uint32_t vs) {
     typedef BaseT<EB> VT;
     const uint32_t ic = min(idx, nt - 1);
-    const uint32_t kv = kvs[ic], h = hds[ic];
+    const uint32_t kv = kvs[ic], h = hds[ic], k = kv & 0xffffu;
     TokVal<EB> r;
     r.tv = idx < nt ? kv >> 16 : vs;
     r.lit = h & 0x80u;
     const VT d = (VT)(int64_t)(int8_t)(h >> 8);
-    r.A = r.lit ? (VT)(r.tv - (kv & 0xffffu)) : (VT)(bss[ic] - (VT)r.tv * d);
+    // run: the delta byte is its own pseudo-varint (bit 7 clear) or joins the base's
+    const uint32_t joined = (h >> 15) & 1u;
+    uint64_t b = 0;
+    const int32_t be = pseudo_value<EB>(base, cut, k + 2 - joined, nterm, cl, joined, b);
+    r.berr = r.lit ? (uint32_t)MDZ_ST_OK : (uint32_t)be;
+    r.A = r.lit ? (VT)(r.tv - k) : (VT)((VT)b - (VT)r.tv * d);
     r.D = d;
     return r;
 }
@@ -444,26 +462,31 @@ MDZ_DEV TokVal<EB> pick(bool c, TokVal<EB> x, TokVal<EB> y) {
     r.D = c ? x.D : y.D;
     r.tv = c ? x.tv : y.tv;
     r.lit = c ? x.lit : y.lit;
+    r.berr = c ? x.berr : y.berr;
     return r;
 }
 
 // Value vi of a lane from its token m (selected by the caller): a literal
 // read from its pseudo-varint, or a run value.  `ok` is false for a literal
-// past the lane's unterminated tail (an earlier value reports that error); a
-// literal's LEB128 status is returned in `le`.
+// past the lane's unterminated tail (an earlier value reports that error).
+// `ek` is the error key the value reports: a literal's LEB128 status at vi,
+// or a run base's status at the run's first value; kNoErr if none.
 template <int EB>
 MDZ_DEV BaseT<EB> unit_value(TokVal<EB> m, uint32_t vi, const uint8_t* base, const uint16_t* cut,
-                             uint32_t nterm, uint32_t cl, bool& ok, int32_t& le) {
+                             uint32_t nterm, uint32_t cl, bool& ok, uint32_t& ek) {
     typedef BaseT<EB> VT;
-    le = MDZ_ST_OK;
     ok = true;
-    if (!m.lit) return m.A + (VT)vi * m.D;
+    if (!m.lit) {
+        ek = m.berr != (uint32_t)MDZ_ST_OK ? err_key(m.tv, 2, (int32_t)m.berr) : kNoErr;
+        return m.A + (VT)vi * m.D;
+    }
     // pseudo-varint j; reads clamped to the lane's table
     const uint32_t j = vi - (uint32_t)m.A;
     const uint32_t a0 = cut[min(j, nterm)] + (m.tv == vi);
     const uint32_t len = cut[min(j + 1, nterm)] - a0;
     ok = j <= nterm;
-    le = lit_status<EB>(j, nterm, cl, a0, len);
+    const int32_t le = lit_status<EB>(j, nterm, cl, a0, len);
+    ek = ok && le != MDZ_ST_OK ? err_key(vi, 2, le) : kNoErr;
     return (VT)leb_at<EB>(base, a0, len);
 }
 
@@ -475,29 +498,28 @@ MDZ_DEV BaseT<EB> unit_value(TokVal<EB> m, uint32_t vi, const uint8_t* base, con
 // it (rare) goes value by value, loading batches as needed.
 template <int EB, int P>
 MDZ_DEV void decode_unit(const uint8_t* base, const uint16_t* cut, const uint32_t* kvs,
-                         const uint16_t* hds, const BaseT<EB>* bss, uint32_t nt, uint32_t i0,
-                         uint32_t nterm, uint32_t cl, uint32_t V, uint32_t n, uint32_t vs,
-                         uint32_t* lerr, uint8_t* out, uint64_t limit, uint32_t t,
-                         uint32_t line) {
+                         const uint16_t* hds, uint32_t nt, uint32_t i0, uint32_t nterm,
+                         uint32_t cl, uint32_t V, uint32_t n, uint32_t vs, uint32_t* lerr,
+                         uint8_t* out, uint64_t limit, uint32_t t, uint32_t line) {
     static_assert(kMT == 4, "four tokens per batch");
     typedef BaseT<EB> VT;
     constexpr uint32_t NV = P / EB;                 // values per piece
-    constexpr VT vmask = EB == 8 ? ~(VT)0 : (VT)((1ull << (8 * EB)) - 1);
+    constexpr VT vmask = (VT)(~0ull >> (64 - 8 * EB));
     const uint32_t vend = min(V + n, vs);
     uint32_t ekey = kNoErr, ib = i0;
     TokVal<EB> T0, T1, T2, T3;
     uint32_t e;
     auto load = [&]() {
-        T0 = tok_val<EB>(kvs, hds, bss, nt, ib, vs);
-        T1 = tok_val<EB>(kvs, hds, bss, nt, ib + 1, vs);
-        T2 = tok_val<EB>(kvs, hds, bss, nt, ib + 2, vs);
-        T3 = tok_val<EB>(kvs, hds, bss, nt, ib + 3, vs);
+        T0 = tok_val<EB>(base, cut, nterm, cl, kvs, hds, nt, ib, vs);
+        T1 = tok_val<EB>(base, cut, nterm, cl, kvs, hds, nt, ib + 1, vs);
+        T2 = tok_val<EB>(base, cut, nterm, cl, kvs, hds, nt, ib + 2, vs);
+        T3 = tok_val<EB>(base, cut, nterm, cl, kvs, hds, nt, ib + 3, vs);
         e = ib + 4 < nt ? kvs[ib + 4] >> 16 : vs;
     };
-    auto value = [&](uint32_t vi, bool& ok, int32_t& le) {
+    auto value = [&](uint32_t vi, bool& ok, uint32_t& ek) {
         const TokVal<EB> m = pick<EB>(vi >= T3.tv, T3, pick<EB>(vi >= T2.tv, T2,
                                       pick<EB>(vi >= T1.tv, T1, T0)));
-        return unit_value<EB>(m, vi, base, cut, nterm, cl, ok, le) & vmask;
+        return unit_value<EB>(m, vi, base, cut, nterm, cl, ok, ek) & vmask;
     };
     load();
     // Groups of up to 8 pieces (one 128-byte line of 16-byte pieces) are kept
@@ -521,10 +543,10 @@ MDZ_DEV void decode_unit(const uint8_t* base, const uint16_t* cut, const uint32_
                 for (uint32_t i = 0; i < NV; ++i) {
                     const uint32_t vi = v0 + i;
                     bool ok;
-                    int32_t le;
-                    const VT val = value(vi, ok, le);
+                    uint32_t ek;
+                    const VT val = value(vi, ok, ek);
                     const bool act = vi < vend;
-                    if (act && le != MDZ_ST_OK) ekey = min(ekey, err_key(vi, 2, le));
+                    ekey = act ? min(ekey, ek) : ekey;
                     put_value<EB>(w, i, act && ok ? val : (VT)0);
                 }
             } else {
@@ -536,10 +558,10 @@ MDZ_DEV void decode_unit(const uint8_t* base, const uint16_t* cut, const uint32_
                         load();
                     }
                     bool ok;
-                    int32_t le;
-                    const VT val = value(vi, ok, le);
+                    uint32_t ek;
+                    const VT val = value(vi, ok, ek);
                     const bool act = vi < vend;
-                    if (act && le != MDZ_ST_OK) ekey = min(ekey, err_key(vi, 2, le));
+                    ekey = act ? min(ekey, ek) : ekey;
                     const VT x = act && ok ? val : (VT)0;
                     if constexpr (EB == 8) {
                         w[0] = w[2]; w[1] = w[3];
@@ -649,14 +671,20 @@ __global__ __launch_bounds__(kNT) void rlev1_decode_kernel(
             const uint32_t nt = l < 32 ? tb.nterm[l] + 1 : 0;
             const uint32_t ex = wave_incl_scan_u32(nt) - nt;
             if (l < 32) tb.cut0[l] = ex;
+            if (l < 32) {
+                tb.vpub[l] = 0;
+                tb.done[l] = 0;
+            }
             if (l == 31) {
                 tb.fast = ex + nt <= cut_cap;
                 // token lists after the cut table: as many tokens per lane as fit
                 const uint32_t off = (uint32_t)sizeof(V1Tables) + used + 16 + ((2 * (ex + nt) + 15) & ~15u);
-                constexpr uint32_t rec = sizeof(BaseT<EB>) + 4 + 2;
+                constexpr uint32_t rec = 4 + 2;
                 tb.tok_off = off;
                 tb.cap = min(off < lds_bytes ? (lds_bytes - off) / (32 * rec) : 0u, tok_limit);
                 tb.overflow = 0;
+                tb.claimed = 0;
+                tb.stuck = 0;
             }
         }
         __syncthreads();
@@ -693,83 +721,110 @@ __global__ __launch_bounds__(kNT) void rlev1_decode_kernel(
         return;
     }
 
-    // ---- 3. walk: one thread per lane stream.  The token path lists every
-    //      token (tkv = first pseudo-varint | first value << 16, thd = header |
-    //      delta << 8, tbs = run base, filled in step 4) after the cut table,
-    //      as many per lane as fit; if a lane has more, the chunk is walked
-    //      again with a checkpoint per value group (checkpoint path).
+    // ---- 3-4. Token path: wave 0 walks, one thread per lane stream, and
+    //      lists every token after the cut table (tkv = first pseudo-varint |
+    //      first value << 16, thd = header | delta << 8), as many per lane as
+    //      fit, publishing per lane the values its listed tokens cover.  The
+    //      other waves (and wave 0 once its walk ends) claim units of kUnit
+    //      values in value order, wait until their lane's walk has passed the
+    //      unit, and decode it.  If a lane has more tokens than its list
+    //      holds, the walk flags the chunk, every wait ends, and the chunk is
+    //      walked again with a checkpoint per value group (checkpoint path).
     const uint32_t upl = (nvals + kUnit - 1) / kUnit;      // decode units per lane
     const uint32_t cap = tb.cap;
-    BaseT<EB>* const tbs = reinterpret_cast<BaseT<EB>*>(reinterpret_cast<uint8_t*>(smem) + tb.tok_off);
-    uint32_t* const tkv = reinterpret_cast<uint32_t*>(tbs + 32 * cap);
+    uint32_t* const tkv = reinterpret_cast<uint32_t*>(reinterpret_cast<uint8_t*>(smem) + tb.tok_off);
     uint16_t* const thd = reinterpret_cast<uint16_t*>(tkv + 32 * cap);
-    bool tokens = upl <= kUnitMax && cap >= kMT;
-    if (tokens) {
-        if (wave == 0 && l < 32) {
-            const uint32_t t = l;
-            const uint8_t* base = data + tb.lbase[t];
-            const uint16_t* ct = cut + tb.cut0[t];
-            uint32_t* kvl = tkv + t * cap;
-            uint16_t* hdl = thd + t * cap;
-            const uint32_t nterm = tb.nterm[t], cl = tb.cl[t];
-            uint32_t k = 0, v = 0, i = 0, err = kNoErr;
-            while (v < nvals && k <= nterm) {
-                const uint32_t s = ct[k];
-                if (s >= cl) { err = err_key(v, 0, MDZ_ST_INPUT_OVERRUN); break; }
-                Token tk;
-                tk.k = k;
-                tk.v = v;
-                tk.hd = lds_u16_at(base, s);
-                const uint32_t cnt = tk.count();
-                if (cnt > nvals - v) { err = err_key(v, 0, MDZ_ST_OUTPUT_OVERRUN); break; }
-                if (!tk.lit() && s + 1 >= cl) { err = err_key(v, 1, MDZ_ST_INPUT_OVERRUN); break; }
-                if (i == cap) { tb.overflow = 1; break; }
-                kvl[i] = k | v << 16;
-                hdl[i] = (uint16_t)tk.hd;
-                ++i;
-                k = tk.next_k();
-                v += cnt;
-            }
-            tb.ntok[t] = i;
-            tb.vstop[t] = v;
-            tb.lerr[t] = err;
-        }
-        __syncthreads();
-        tokens = !tb.overflow;
-    }
-    MDZ_PHASE(3);
     uint8_t* const out = dst + out_base;
+    bool tokens = cap >= kMT;
     if (tokens) {
-        // ---- 4. one thread per listed token: run base (and its status), and the
-        //      token covering the first value of each unit
-        for (uint32_t x = tid; x < 32 * cap; x += kNT) {
-            const uint32_t t = x & 31, i = x >> 5, nt = tb.ntok[t];
-            if (i >= nt) continue;
-            const uint32_t kv = tkv[t * cap + i], k = kv & 0xffffu, v = kv >> 16;
-            const uint32_t vn = i + 1 < nt ? tkv[t * cap + i + 1] >> 16 : tb.vstop[t];
-            const uint32_t hd = thd[t * cap + i];
-            if (!(hd & 0x80u)) {
-                const uint32_t joined = (hd >> 15) & 1u;     // delta byte joined to the base's
-                uint64_t b;
-                const int32_t e = pseudo_value<EB>(data + tb.lbase[t], cut + tb.cut0[t],
-                                                   k + 2 - joined, tb.nterm[t], tb.cl[t], joined, b);
-                if (e != MDZ_ST_OK) atomicMin(&tb.lerr[t], err_key(v, 2, e));
-                tbs[t * cap + i] = (BaseT<EB>)b;
+        if (wave == 0) {
+            if (l < 32) {
+                const uint32_t t = l;
+                const uint8_t* base = data + tb.lbase[t];
+                const uint16_t* ct = cut + tb.cut0[t];
+                uint32_t* kvl = tkv + t * cap;
+                uint16_t* hdl = thd + t * cap;
+                const uint32_t nterm = tb.nterm[t], cl = tb.cl[t];
+                uint32_t k = 0, v = 0, i = 0, err = kNoErr;
+                while (v < nvals && k <= nterm) {
+                    const uint32_t s = ct[k];
+                    if (s >= cl) { err = err_key(v, 0, MDZ_ST_INPUT_OVERRUN); break; }
+                    Token tk;
+                    tk.k = k;
+                    tk.v = v;
+                    tk.hd = lds_u16_at(base, s);
+                    const uint32_t cnt = tk.count();
+                    if (cnt > nvals - v) { err = err_key(v, 0, MDZ_ST_OUTPUT_OVERRUN); break; }
+                    if (!tk.lit() && s + 1 >= cl) { err = err_key(v, 1, MDZ_ST_INPUT_OVERRUN); break; }
+                    if (i == cap) { tb.overflow = 1; break; }
+                    kvl[i] = k | v << 16;
+                    hdl[i] = (uint16_t)tk.hd;
+                    ++i;
+                    k = tk.next_k();
+                    v += cnt;
+                    // publish: the token's LDS writes land before the count
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+                    __hip_atomic_store(&tb.ntok[t], i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                    __hip_atomic_store(&tb.vpub[t], v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                }
+                if (err != kNoErr) atomicMin(&tb.lerr[t], err);
+                tb.ntok[t] = i;
+                tb.vstop[t] = v;
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+                __hip_atomic_store(&tb.done[t], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
             }
-            for (uint32_t g = (v + kUnit - 1) / kUnit; g * kUnit < vn; ++g) tb.ui[t][g] = (uint16_t)i;
+            wave_sync();
+        }
+        MDZ_PHASE(3);
+        // Consumers: a wave claims 64 units at a time in value order (unit u
+        // = lane u % 32, values kUnit * (u / 32) ..), each thread waits until
+        // its lane's walk has covered its unit, then decodes it.
+        const uint32_t units = 32 * upl;
+        for (;;) {
+            uint32_t u0 = 0;
+            if (l == 0) u0 = atomicAdd(&tb.claimed, 64u);
+            u0 = __builtin_amdgcn_readfirstlane(u0);
+            if (u0 >= units) break;
+            const uint32_t u = u0 + l;
+            if (u >= units) continue;
+            const uint32_t t = u & 31, V = (u >> 5) * kUnit, n = min(kUnit, nvals - V);
+            bool fin = false;
+            for (uint32_t it = 0;; ++it) {
+                if (__hip_atomic_load(&tb.done[t], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP)) {
+                    fin = true;
+                    break;
+                }
+                if (__hip_atomic_load(&tb.vpub[t], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) >= V + n)
+                    break;
+                if (it >= kSpinMax) { tb.stuck = 1; break; }
+                __builtin_amdgcn_s_sleep(2);
+            }
+            if (tb.overflow || tb.stuck) continue;
+            // values the lane's listed tokens cover; ntok read after vpub
+            const uint32_t vs = fin ? tb.vstop[t] : __hip_atomic_load(&tb.vpub[t], __ATOMIC_ACQUIRE,
+                                                                       __HIP_MEMORY_SCOPE_WORKGROUP);
+            const uint32_t nt = __hip_atomic_load(&tb.ntok[t], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP);
+            if (V >= vs || nt == 0) continue;
+            const uint32_t* kvs = tkv + t * cap;
+            // the token covering V: the last with first value <= V
+            uint32_t lo = 0, hi = nt - 1;
+            while (lo < hi) {
+                const uint32_t mid = (lo + hi + 1) >> 1;
+                if ((kvs[mid] >> 16) <= V) lo = mid;
+                else hi = mid - 1;
+            }
+            decode_unit<EB, P>(data + tb.lbase[t], cut + tb.cut0[t], kvs, thd + t * cap, nt, lo,
+                               tb.nterm[t], tb.cl[t], V, n, vs, &tb.lerr[t], out, limit, t, line);
         }
         __syncthreads();
         MDZ_PHASE(4);
-        // ---- 5. one thread per unit of kUnit values of a lane
-        for (uint32_t u = tid; u < 32 * upl; u += kNT) {
-            const uint32_t t = u & 31, V = (u >> 5) * kUnit, vs = tb.vstop[t];
-            if (V >= vs) continue;
-            decode_unit<EB, P>(data + tb.lbase[t], cut + tb.cut0[t], tkv + t * cap, thd + t * cap,
-                               tbs + t * cap, tb.ntok[t], tb.ui[t][u >> 5], tb.nterm[t],
-                               tb.cl[t], V, min(kUnit, nvals - V), vs, &tb.lerr[t], out, limit, t,
-                               line);
+        tokens = !tb.overflow;
+        if (tb.overflow) {                        // the checkpoint path redoes the chunk
+            if (tid < 32) tb.lerr[tid] = kNoErr;
+            __syncthreads();
         }
-    } else {
+    }
+    if (!tokens) {
         uint32_t gsh = 0;                            // groups of G = 2^gsh values, at most kGroups
         while (((nvals + (1u << gsh) - 1) >> gsh) > kGroups) ++gsh;
         if (wave == 0 && l < 32) {
@@ -825,7 +880,7 @@ __global__ __launch_bounds__(kNT) void rlev1_decode_kernel(
             const uint32_t e = tb.lerr[t];
             if (e != kNoErr) { st = (int32_t)(int8_t)(e & 0xffu); break; }
         }
-        status[c] = st;
+        status[c] = tb.stuck ? MDZ_ST_INTERNAL : st;
     }
 }
 
