@@ -697,12 +697,11 @@ __global__ __launch_bounds__(kNT) void rlev1_decode_kernel(
                 const u32x4 x = blk[b];
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {
-                    uint32_t m = term_mask(b, i, x[i]);
-                    while (m) {
-                        const uint32_t bit = __builtin_ctz(m);
-                        m &= m - 1;
-                        ct[idx++] = (uint16_t)(16 * b + 4 * i + (bit >> 3) + 1);
-                    }
+                    const uint32_t m = term_mask(b, i, x[i]);
+                    const uint32_t pos = 16 * b + 4 * i + 1;
+#pragma unroll
+                    for (int j = 0; j < 4; ++j)            // predicated, no data-dependent loop
+                        if (m & (0x80u << (8 * j))) ct[idx++] = (uint16_t)(pos + j);
                 }
             }
         }
