@@ -98,6 +98,7 @@ SIGNATURES = {
     "mdz_inflate_batch": (C.c_int, [vp, vp, C.c_int, C.c_int, vp]),
     "mdz_inflate_batch_sched": (C.c_int, [vp, vp, C.c_int, C.c_int, vp, u32, vp, vp, vp]),
     "mdz_inflate_slots": (C.c_int, [C.POINTER(u32)]),
+    "mdz_copy_uncompressed": (C.c_int, [vp, C.c_int, vp]),
     "mdz_cost_features": (C.c_int, [vp, C.c_int, C.c_int, vp, vp]),
     "mdz_batch_submit": (C.c_int, [C.POINTER(BatchT), vp]),
     "mdz_graph_begin": (C.c_int, [vp]),

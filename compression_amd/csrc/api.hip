@@ -371,6 +371,11 @@ int mdz_inflate_batch_sched(const mdz_inflate_in_t* in, mdz_inflate_status_t* ou
                               S(stream));
 }
 
+int mdz_copy_uncompressed(const mdz_inflate_in_t* in, int count, void* stream) {
+    if (count < 0) return MDZ_ERR_INVALID;
+    return mdz::copy_uncompressed(in, (uint32_t)count, S(stream));
+}
+
 int mdz_inflate_slots(uint32_t* slots) {
     if (!slots) return MDZ_ERR_INVALID;
     return mdz::inflate_slots(slots);

@@ -26,8 +26,12 @@
 //   * finished output is flushed to HBM in dword stores after each batch.
 // Errors follow zlib's checks and order (invalid block type, stored length,
 // code-set completeness, repeat codes, missing end-of-block, invalid codes,
-// distance too far back) with cuDF's status codes; the zlib header is checked
-// when parse_hdr is set and the Adler-32 trailer is not verified.
+// distance too far back) with cuDF's status codes.  parse_hdr selects the
+// framing: raw Deflate, zlib (2-byte header checked) or gzip (cuDF's header
+// rules, 8-byte footer excluded from the Deflate data); with MDZ_HDR_VERIFY
+// the wave then checks the trailer (Adler-32, or CRC-32 and ISIZE) against
+// the output it wrote.
+#include "deflate_hdr_dev.h"
 #include "mdz_device.h"
 #include "mdz_internal.h"
 
@@ -235,6 +239,85 @@ MDZ_DEV int slow_decode(const Code& c, Reader& r) {
 
 MDZ_DEV uint32_t ring_at(uint64_t p) { return (uint32_t)(p % kRing); }
 
+// Trailer checksums over the output, 1 KiB blocks per wave step: lane l holds
+// bytes 16l..16l+15 of a block.  A final block of t < 1024 bytes is read as
+// 1024 - t zero bytes followed by its t bytes; leading zeros change neither
+// the CRC residue nor the Adler-32 sums, and the lane weights stay fixed.
+constexpr uint32_t kCkBlock = 1024;
+
+struct CrcTables {
+    uint32_t lane[64];       // x^(8 * 16 * (63 - l)): bytes of the block after lane l's
+    uint32_t pow2[11];       // x^(8 * 2^j)
+};
+
+constexpr CrcTables make_crc_tables() {
+    CrcTables t{};
+    for (uint32_t l = 0; l < 64; ++l) t.lane[l] = crc_xpow(8ull * 16 * (63 - l));
+    for (uint32_t j = 0; j < 11; ++j) t.pow2[j] = crc_xpow(8ull << j);
+    return t;
+}
+
+__constant__ CrcTables kCrcT = make_crc_tables();
+
+// CRC-32 (crc) or Adler-32 of p[0, n); every lane returns it.
+MDZ_DEV uint32_t stream_checksum(const uint8_t* p, uint64_t n, bool crc) {
+    const uint32_t l = lane_id();
+    const uint32_t lane_x = kCrcT.lane[l];
+    const bool aligned = ((uintptr_t)p & 3) == 0;
+    uint32_t A = 1, B = 0, R = 0xFFFFFFFFu;
+    for (uint64_t b0 = 0; b0 < n; b0 += kCkBlock) {
+        const uint32_t t = (uint32_t)min<uint64_t>(n - b0, kCkBlock);
+        const uint8_t* q = p + b0 - (kCkBlock - t);     // virtual block start
+        uint32_t w[4];
+        if (t == kCkBlock && aligned) {
+            const uint32_t* q4 = reinterpret_cast<const uint32_t*>(q + 16 * l);
+#pragma unroll
+            for (int k = 0; k < 4; ++k) w[k] = q4[k];
+        } else {
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                w[k] = 0;
+                for (uint32_t j = 0; j < 4; ++j) {
+                    const uint32_t v = 16 * l + 4 * k + j;
+                    if (v >= kCkBlock - t) w[k] |= (uint32_t)q[v] << (8 * j);
+                }
+            }
+        }
+        if (crc) {
+            uint32_t r = 0;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) r = crc_dword(r, w[k]);
+            r = crc_mulmod(r, lane_x);
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) r ^= __shfl_xor(r, o, kWave);
+            uint32_t shift = kCrcT.pow2[10];
+            if (t != kCkBlock) {
+                shift = 0x80000000u;
+                for (uint32_t j = 0; j < 10; ++j)
+                    if (t & (1u << j)) shift = crc_mulmod(shift, kCrcT.pow2[j]);
+            }
+            R = crc_mulmod(R, shift) ^ r;
+        } else {
+            // s2 weights each byte by its distance to the lane's segment end;
+            // 16 (63 - l) more bytes of the block follow the segment
+            uint32_t s1 = 0, s2 = 0;
+#pragma unroll
+            for (int k = 0; k < 4; ++k)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const uint32_t b = (w[k] >> (8 * j)) & 0xFF;
+                    s1 += b;
+                    s2 += (16 - (4 * k + j)) * b;
+                }
+            const uint32_t bs1 = wave_sum_u32(s1);
+            const uint32_t bs2 = wave_sum_u32(s2 + s1 * 16 * (63 - l));   // < 2^29
+            B = (B + t * A + bs2 % kAdlerMod) % kAdlerMod;
+            A = (A + bs1) % kAdlerMod;
+        }
+    }
+    return crc ? ~R : (B << 16) | A;
+}
+
 // Entry idx (wave-uniform) of a lookup table held in registers, entry i in
 // lane i % 64 of v[i / 64].
 template <uint32_t N>
@@ -273,17 +356,30 @@ __global__ __launch_bounds__(64) void inflate_kernel(const mdz_inflate_in_t* __r
         uint8_t* const dst = reinterpret_cast<uint8_t*>(in.dst);
         const uint64_t cap = in.dst_bytes;
 
+        const int hdr = parse_hdr & 3;
         Reader r;
         r.src = reinterpret_cast<const uint8_t*>(in.src);
         r.n = in.src_bytes;
         int st = MDZ_ST_OK;
+        if (hdr == MDZ_HDR_GZIP) {
+            // as cuDF: the footer is cut off first, and a rejected header
+            // reports -1 with nothing written
+            const int64_t h = gzip_header_len(r.src, r.n);
+            r.n = r.n >= 8 ? r.n - 8 : 0;
+            if (h < 0) {
+                st = MDZ_ST_BAD_BLOCK_TYPE;
+            } else {
+                r.src += h;
+                r.n -= (uint64_t)h;
+            }
+        }
         uint64_t pos = 0, flushed = 0;             // output produced / flushed to dst
         uint32_t fixed_built = 0;
         uint32_t n_sym = 0, n_match = 0, n_stored = 0, n_fixed = 0, n_dyn = 0;
         const uint64_t t0 = clock64();
 
         r.seek(0);
-        if (parse_hdr) {
+        if (hdr == MDZ_HDR_ZLIB) {
             r.refill();
             const uint32_t cmf = r.bits(8), flg = r.bits(8);
             if (r.over()) st = MDZ_ST_INPUT_OVERRUN;
@@ -291,29 +387,42 @@ __global__ __launch_bounds__(64) void inflate_kernel(const mdz_inflate_in_t* __r
                 st = MDZ_ST_BAD_HEADER;
         }
 
-        // flushes whole dwords (or bytes for an unaligned dst) of [flushed, upto)
+        // flushes [flushed, upto) in dword stores at dword-aligned dst
+        // addresses; before the last flush it stops at such an address, so
+        // only the first flush has head bytes and only the last tail bytes
+        const uint32_t dmis = (uint32_t)((uintptr_t)dst & 3);
         auto flush = [&](uint64_t upto, bool all) {
             uint64_t lim = min(upto, cap);
-            const bool aligned = (((uintptr_t)dst + flushed) & 3) == 0;
-            if (aligned && !all) lim &= ~3ull;
+            if (!all) {
+                const uint64_t cut = (lim + dmis) & 3;
+                lim = lim >= cut ? lim - cut : 0;
+            }
             if (lim <= flushed) return;
             // at most one batch (< kRing bytes) is pending: ring indices are
-            // the flush start's plus an offset, wrapped once
+            // the flush start's plus an offset, wrapped once; kRing % 4 == 0,
+            // so ring index and output offset agree mod 4
             const uint32_t rf = ring_at(flushed);
             auto at = [&](uint64_t p) {
                 const uint32_t x = rf + (uint32_t)(p - flushed);
                 return x >= kRing ? x - kRing : x;
             };
-            if (aligned) {
-                const uint64_t nw = (lim - flushed) / 4;
+            const uint64_t h = min(lim, flushed + ((4 - ((dmis + flushed) & 3)) & 3));
+            for (uint64_t p = flushed + l; p < h; p += 64) dst[p] = S.ring[at(p)];
+            const uint64_t nw = (lim - h) / 4;
+            if (dmis == 0) {                       // h % 4 == 0: aligned ring dwords
                 for (uint64_t w = l; w < nw; w += 64) {
-                    const uint64_t p = flushed + 4 * w;
+                    const uint64_t p = h + 4 * w;
                     *reinterpret_cast<uint32_t*>(dst + p) = *reinterpret_cast<const uint32_t*>(&S.ring[at(p)]);
                 }
-                for (uint64_t p = flushed + 4 * nw + l; p < lim; p += 64) dst[p] = S.ring[at(p)];
-            } else {
-                for (uint64_t p = flushed + l; p < lim; p += 64) dst[p] = S.ring[at(p)];
+            } else {                               // ring bytes gathered into aligned dst dwords
+                for (uint64_t w = l; w < nw; w += 64) {
+                    const uint64_t p = h + 4 * w;
+                    *reinterpret_cast<uint32_t*>(dst + p) =
+                        (uint32_t)S.ring[at(p)] | ((uint32_t)S.ring[at(p + 1)] << 8) |
+                        ((uint32_t)S.ring[at(p + 2)] << 16) | ((uint32_t)S.ring[at(p + 3)] << 24);
+                }
             }
+            for (uint64_t p = h + 4 * nw + l; p < lim; p += 64) dst[p] = S.ring[at(p)];
             flushed = lim;
         };
 
@@ -514,6 +623,25 @@ __global__ __launch_bounds__(64) void inflate_kernel(const mdz_inflate_in_t* __r
             }
         }
         flush(pos, true);
+        if ((parse_hdr & MDZ_HDR_VERIFY) && hdr && st == MDZ_ST_OK) {
+            __threadfence();                       // the wave reads back what it stored
+            const uint8_t* s0 = reinterpret_cast<const uint8_t*>(in.src);
+            if (hdr == MDZ_HDR_ZLIB) {
+                const uint64_t tb = (r.cbits + 7) / 8;       // Adler-32, big-endian, after the last block
+                if (tb + 4 > in.src_bytes) {
+                    st = MDZ_ST_INPUT_OVERRUN;
+                } else {
+                    const uint32_t want = ((uint32_t)s0[tb] << 24) | ((uint32_t)s0[tb + 1] << 16) |
+                                          ((uint32_t)s0[tb + 2] << 8) | s0[tb + 3];
+                    if (stream_checksum(dst, pos, false) != want) st = MDZ_ST_BAD_CHECKSUM;
+                }
+            } else {                                          // CRC-32 then ISIZE, little-endian
+                const uint8_t* f = s0 + in.src_bytes - 8;
+                const uint32_t crc = f[0] | ((uint32_t)f[1] << 8) | ((uint32_t)f[2] << 16) | ((uint32_t)f[3] << 24);
+                const uint32_t isz = f[4] | ((uint32_t)f[5] << 8) | ((uint32_t)f[6] << 16) | ((uint32_t)f[7] << 24);
+                if (isz != (uint32_t)pos || stream_checksum(dst, pos, true) != crc) st = MDZ_ST_BAD_CHECKSUM;
+            }
+        }
         if (l == 0) {
             mdz_inflate_status_t o;
             o.bytes_written = pos;
@@ -539,6 +667,50 @@ __global__ __launch_bounds__(64) void inflate_kernel(const mdz_inflate_in_t* __r
 
 __global__ void reset_cursor_kernel(uint32_t* c) { *c = 0; }
 
+// Stored streams: the first min(src_bytes, dst_bytes) bytes of each stream
+// go to its output slot (cuDF's copy_uncompressed_kernel, reference
+// include/cudf_deflate.cuh:1349-1397).  kCopySlices workgroups share a
+// stream; each copies a contiguous range of 16-byte (or dword, or byte)
+// units, by the widest unit at which src and dst are co-aligned.
+constexpr uint32_t kCopySlices = 16, kCopyThreads = 256;
+
+MDZ_DEV void copy_one(const mdz_inflate_in_t in, uint32_t slice) {
+    const uint64_t n = min(in.src_bytes, in.dst_bytes);
+    const uint8_t* src = reinterpret_cast<const uint8_t*>(in.src);
+    uint8_t* dst = reinterpret_cast<uint8_t*>(in.dst);
+    const uintptr_t mis = (uintptr_t)src ^ (uintptr_t)dst;
+    const uint32_t unit = (mis & 15) == 0 ? 16 : (mis & 3) == 0 ? 4 : 1;
+    const uint64_t head = min<uint64_t>((unit - ((uintptr_t)dst & (unit - 1))) & (unit - 1), n);
+    const uint64_t units = (n - head) / unit;
+    const uint64_t per = (units + kCopySlices - 1) / kCopySlices;
+    const uint64_t u0 = min(units, per * slice), u1 = min(units, u0 + per);
+    const uint32_t t = threadIdx.x;
+    if (slice == 0)
+        for (uint64_t i = t; i < head; i += kCopyThreads) dst[i] = src[i];
+    if (slice == kCopySlices - 1)
+        for (uint64_t i = head + units * unit + t; i < n; i += kCopyThreads) dst[i] = src[i];
+    if (unit == 16) {
+        const uint4* s4 = reinterpret_cast<const uint4*>(src + head);
+        uint4* d4 = reinterpret_cast<uint4*>(dst + head);
+        for (uint64_t u = u0 + t; u < u1; u += kCopyThreads) d4[u] = s4[u];
+    } else if (unit == 4) {
+        const uint32_t* s4 = reinterpret_cast<const uint32_t*>(src + head);
+        uint32_t* d4 = reinterpret_cast<uint32_t*>(dst + head);
+        for (uint64_t u = u0 + t; u < u1; u += kCopyThreads) d4[u] = s4[u];
+    } else {
+        for (uint64_t u = u0 + t; u < u1; u += kCopyThreads) dst[head + u] = src[head + u];
+    }
+}
+
+constexpr uint32_t kCopyMaxStreamsPerLaunch = 65536;
+
+__global__ __launch_bounds__(kCopyThreads) void copy_uncompressed_kernel(const mdz_inflate_in_t* __restrict__ ins,
+                                                                          uint32_t count) {
+    const uint32_t slice = blockIdx.x % kCopySlices;
+    for (uint32_t si = blockIdx.x / kCopySlices; si < count; si += gridDim.x / kCopySlices)
+        copy_one(ins[si], slice);
+}
+
 }  // namespace
 
 int inflate_slots(uint32_t* slots) {
@@ -559,6 +731,7 @@ int inflate_batch(const mdz_inflate_in_t* in, mdz_inflate_status_t* out, uint32_
                   int parse_hdr, const uint32_t* order, uint32_t persistent_blocks,
                   uint32_t* cursor, mdz_inflate_cost_t* cost, hipStream_t s) {
     if ((!in || !out) && count) return MDZ_ERR_INVALID;
+    if (!inflate_hdr_ok(parse_hdr)) return MDZ_ERR_INVALID;
     if (count == 0) return MDZ_OK;
     const bool persistent = cursor != nullptr;
     const size_t smem = sizeof(InflateLds);
@@ -577,6 +750,14 @@ int inflate_batch(const mdz_inflate_in_t* in, mdz_inflate_status_t* out, uint32_
     hipLaunchKernelGGL(inflate_kernel, dim3(grid), dim3(64), smem, s, in, out, order, count,
                        parse_hdr, persistent ? cursor : nullptr, cost);
     return record_hip(hipGetLastError(), "inflate_kernel launch");
+}
+
+int copy_uncompressed(const mdz_inflate_in_t* in, uint32_t count, hipStream_t s) {
+    if (!in && count) return MDZ_ERR_INVALID;
+    if (count == 0) return MDZ_OK;
+    const uint32_t grid = min(count, kCopyMaxStreamsPerLaunch) * kCopySlices;
+    hipLaunchKernelGGL(copy_uncompressed_kernel, dim3(grid), dim3(kCopyThreads), 0, s, in, count);
+    return record_hip(hipGetLastError(), "copy_uncompressed_kernel launch");
 }
 
 }  // namespace mdz

@@ -33,6 +33,13 @@ int inflate_batch(const mdz_inflate_in_t* in, mdz_inflate_status_t* out, uint32_
                   int parse_hdr, const uint32_t* order, uint32_t persistent_blocks,
                   uint32_t* cursor, mdz_inflate_cost_t* cost, hipStream_t s);
 int inflate_slots(uint32_t* slots);
+int copy_uncompressed(const mdz_inflate_in_t* in, uint32_t count, hipStream_t s);
+
+// parse_hdr: MDZ_HDR_RAW / ZLIB / GZIP, optionally | MDZ_HDR_VERIFY (framed streams only)
+inline bool inflate_hdr_ok(int h) {
+    const int f = h & 3;
+    return (h & ~(3 | MDZ_HDR_VERIFY)) == 0 && f != 3 && (f != 0 || !(h & MDZ_HDR_VERIFY));
+}
 
 int mlp_features(const mdz_inflate_in_t* in, uint32_t n, int parse_hdr, float* x, hipStream_t s);
 int mlp_forward(const float* params, const float* x, uint32_t n, float* y, hipStream_t s);

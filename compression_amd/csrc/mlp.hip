@@ -11,6 +11,7 @@
 // deltas in LDS, sums every parameter's outer-product terms over the
 // workgroup's samples (one thread per parameter) and adds them with one
 // atomic per parameter and block; Adam then updates in place.
+#include "deflate_hdr_dev.h"
 #include "mdz_device.h"
 #include "mdz_internal.h"
 
@@ -36,9 +37,11 @@ __global__ void features_kernel(const mdz_inflate_in_t* __restrict__ in, uint32_
     xi[0] = ls;
     xi[1] = ld;
     xi[2] = ld - ls;
-    const uint64_t at = parse_hdr ? 2 : 0;
+    const uint8_t* src = reinterpret_cast<const uint8_t*>(d.src);
+    const int hdr = parse_hdr & 3;
+    const int64_t at = hdr == MDZ_HDR_ZLIB ? 2 : hdr == MDZ_HDR_GZIP ? gzip_header_len(src, d.src_bytes) : 0;
     int type = -1;
-    if (d.src_bytes > at) type = (reinterpret_cast<const uint8_t*>(d.src)[at] >> 1) & 3;
+    if (at >= 0 && d.src_bytes > (uint64_t)at) type = (src[at] >> 1) & 3;
     xi[3] = type == 0;
     xi[4] = type == 1;
     xi[5] = type == 2;
@@ -169,6 +172,7 @@ __global__ void zero_kernel(float* a, int n) {
 }  // namespace
 
 int mlp_features(const mdz_inflate_in_t* in, uint32_t n, int parse_hdr, float* x, hipStream_t s) {
+    if (!inflate_hdr_ok(parse_hdr)) return MDZ_ERR_INVALID;
     if (!n) return MDZ_OK;
     if (!in || !x) return MDZ_ERR_INVALID;
     hipLaunchKernelGGL(features_kernel, dim3((n + 255) / 256), dim3(256), 0, s, in, n, parse_hdr, x);

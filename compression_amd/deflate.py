@@ -67,16 +67,30 @@ class DeviceBatch:
         return self.dst.download()
 
 
-def inflate(b: DeviceBatch, parse_hdr: bool = True, stream=None) -> None:
+# parse_hdr values (the framing around each stream's Deflate data)
+HDR_RAW, HDR_ZLIB, HDR_GZIP, HDR_VERIFY = 0, 1, 2, 0x100
+ST_BAD_CHECKSUM = -13
+
+
+def inflate(b: DeviceBatch, parse_hdr: int = HDR_ZLIB, stream=None) -> None:
+    """cuDF ``gpuinflate`` over the batch.  ``parse_hdr`` is HDR_RAW, HDR_ZLIB
+    or HDR_GZIP (cuDF's nonzero parse_hdr), optionally | HDR_VERIFY to check
+    the Adler-32 / CRC-32 trailer; True means HDR_ZLIB."""
     check(lib().mdz_inflate_batch(b.desc.ptr, b.status.ptr, b.count, int(parse_hdr), as_ptr(stream)),
           "mdz_inflate_batch")
 
 
-def inflate_sched(b: DeviceBatch, parse_hdr: bool = True, blocks: int = 0, stream=None) -> None:
+def inflate_sched(b: DeviceBatch, parse_hdr: int = HDR_ZLIB, blocks: int = 0, stream=None) -> None:
     check(lib().mdz_inflate_batch_sched(b.desc.ptr, b.status.ptr, b.count, int(parse_hdr),
                                         b.order.ptr if b.order is not None else None, blocks,
                                         b.cursor.ptr, b.cost.ptr if b.cost is not None else None,
                                         as_ptr(stream)), "mdz_inflate_batch_sched")
+
+
+def copy_uncompressed(b: DeviceBatch, stream=None) -> None:
+    """cuDF ``gpu_copy_uncompressed_blocks``: the first min(src, dst) bytes of
+    each stream go to its output slot unchanged."""
+    check(lib().mdz_copy_uncompressed(b.desc.ptr, b.count, as_ptr(stream)), "mdz_copy_uncompressed")
 
 
 def slots() -> int:

@@ -47,6 +47,7 @@ extern "C" {
 #define MDZ_ST_BAD_SYMBOL     -10
 #define MDZ_ST_BAD_HEADER     -11
 #define MDZ_ST_INTERNAL       -12   /* a decoder's bounded internal wait expired (never expected) */
+#define MDZ_ST_BAD_CHECKSUM   -13   /* Deflate with MDZ_HDR_VERIFY: trailer checksum or size differs */
 
 /* --------------------------------------------------------------- version */
 /* Returns "mdz <version> gfx950". */
@@ -242,12 +243,38 @@ typedef struct {
     uint32_t reserved;
 } mdz_inflate_status_t;
 
-/* in/out: device arrays of `count` entries.  parse_hdr != 0: each stream
- * starts with a 2-byte zlib header (checked; the Adler-32 trailer is not
- * verified), else raw Deflate.  One stream per workgroup, launched in index
- * order (the plain per-stream decoder). */
+/* parse_hdr: the framing around each stream's Deflate data.
+ *   MDZ_HDR_RAW   raw Deflate;
+ *   MDZ_HDR_ZLIB  2-byte zlib header (checked: MDZ_ST_BAD_HEADER), then
+ *                 Deflate; the 4-byte Adler-32 trailer follows the last block;
+ *   MDZ_HDR_GZIP  gzip member (cuDF's parse_hdr != 0, reference
+ *                 include/cudf_deflate.cuh:1179-1217,1244-1253): header
+ *                 skipped by cuDF's rules (a rejected header reports -1, as
+ *                 cuDF does), the last 8 bytes (CRC-32, ISIZE) are the footer
+ *                 and are not Deflate input.  FEXTRA skips XLEN's own 2 bytes
+ *                 too (RFC 1952; cuDF omits them).
+ * MDZ_HDR_VERIFY may be or'ed onto ZLIB or GZIP: a stream that decoded with
+ * status 0 is then checked against its trailer (Adler-32; or CRC-32 and
+ * ISIZE) and reports MDZ_ST_BAD_CHECKSUM on a mismatch, or
+ * MDZ_ST_INPUT_OVERRUN for a zlib stream whose trailer is cut off.  cuDF
+ * verifies neither; this is the check zlib's own inflate makes. */
+#define MDZ_HDR_RAW    0
+#define MDZ_HDR_ZLIB   1
+#define MDZ_HDR_GZIP   2
+#define MDZ_HDR_VERIFY 0x100
+
+/* in/out: device arrays of `count` entries; parse_hdr as above (other values:
+ * MDZ_ERR_INVALID).  One stream per workgroup, launched in index order (the
+ * plain per-stream decoder). */
 int mdz_inflate_batch(const mdz_inflate_in_t* in, mdz_inflate_status_t* out, int count,
                       int parse_hdr, void* stream);
+
+/* Stored streams: copies the first min(src_bytes, dst_bytes) bytes of each
+ * entry of in[count] (device array) to its dst.  Replaces cuDF's
+ * gpu_copy_uncompressed_blocks / copy_uncompressed_kernel (reference
+ * include/cudf_deflate.cuh:1349-1397,1409-1415); sizes stay 64-bit where
+ * cuDF truncates them to 32 bits. */
+int mdz_copy_uncompressed(const mdz_inflate_in_t* in, int count, void* stream);
 
 /* Per-stream decode-cost sample the decoder logs for the learned scheduler:
  * shader-clock cycles from the stream's first to last instruction and what

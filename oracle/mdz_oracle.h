@@ -40,7 +40,8 @@ enum {
     MDZ_ST_NO_EOB = -8,          /* Deflate: missing end-of-block code */
     MDZ_ST_BAD_DISTANCE = -9,    /* back-reference before the start of output / bad dist symbol */
     MDZ_ST_BAD_SYMBOL = -10,     /* undecodable symbol / invalid token */
-    MDZ_ST_BAD_HEADER = -11      /* zlib header check failed / bad container metadata */
+    MDZ_ST_BAD_HEADER = -11,     /* zlib header check failed / bad container metadata */
+    MDZ_ST_BAD_CHECKSUM = -13    /* Deflate trailer check: Adler-32 / CRC-32 / ISIZE differs */
 };
 
 /* ============================================================ RLE v1 ===== *
@@ -83,8 +84,15 @@ int mdz_ref_lzss_decode(const uint8_t* payload, uint64_t payload_bytes, const ui
                         int threads);
 
 /* =========================================================== Deflate ===== *
- * zlib is the oracle (RFC 1950/1951).  zlib_hdr: check the 2-byte zlib
- * header, then inflate raw (the Adler-32 trailer is not verified).  Writes
+ * zlib is the oracle (RFC 1950/1951/1952).  hdr = framing | verify:
+ *   0 raw Deflate; 1 zlib: check the 2-byte header, then inflate raw;
+ *   2 gzip: header length by cuDF's parse_gzip_header rules
+ *     (include/cudf_deflate.cuh:1179-1217; FEXTRA per RFC 1952), rejected
+ *     header -> -1, the last 8 bytes excluded from the Deflate input
+ *     (:1244-1253);
+ *   | 0x100 verify: after status 0, compare zlib's adler32() (zlib) or
+ *     crc32() and the length (gzip) with the trailer -> -13 on a mismatch, 2
+ *     for a cut-off zlib trailer.  Writes
  * at most cap bytes; *written = bytes produced.  Status per cuDF's codes:
  * 1 output full before the end, 2 input ended early, negatives per zlib's
  * error (invalid block type -1, stored lengths -2, code lengths set -3,
@@ -92,11 +100,13 @@ int mdz_ref_lzss_decode(const uint8_t* payload, uint64_t payload_bytes, const ui
  * symbols -7, missing end-of-block -8, invalid distance code / too far back
  * -9, invalid literal/length code -10, header -11). */
 int mdz_ref_inflate(const uint8_t* src, uint64_t n, uint8_t* dst, uint64_t cap, uint64_t* written,
-                    int zlib_hdr);
+                    int hdr);
 int mdz_ref_inflate_batch(const uint8_t* src, const uint64_t* src_off, const uint64_t* src_len,
                           uint8_t* dst, const uint64_t* dst_off, const uint64_t* dst_cap,
-                          uint64_t count, int zlib_hdr, uint64_t* written, int32_t* status,
+                          uint64_t count, int hdr, uint64_t* written, int32_t* status,
                           int threads);
+/* gzip member header length or -1 (rules above). */
+int64_t mdz_ref_gzip_header_len(const uint8_t* src, uint64_t n);
 const char* mdz_zlib_version(void);
 
 #ifdef __cplusplus

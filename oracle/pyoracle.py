@@ -73,6 +73,8 @@ def lib():
                                           C.c_int]
         L.mdz_ref_inflate.restype = C.c_int
         L.mdz_ref_inflate.argtypes = [C.c_void_p, C.c_uint64, C.c_void_p, C.c_uint64, u64p, C.c_int]
+        L.mdz_ref_gzip_header_len.restype = C.c_int64
+        L.mdz_ref_gzip_header_len.argtypes = [C.c_void_p, C.c_uint64]
         L.mdz_ref_inflate_batch.restype = C.c_int
         L.mdz_ref_inflate_batch.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                             C.c_void_p, C.c_uint64, C.c_int, C.c_void_p, C.c_void_p,
@@ -163,12 +165,18 @@ def lzss_decode(c: LzssContainer, out_bytes: int | None = None, threads: int | N
 def zlib_version() -> str:
     return lib().mdz_zlib_version().decode()
 
-def inflate(stream: bytes, cap: int, zlib_hdr: bool = True) -> tuple[int, bytes]:
+def inflate(stream: bytes, cap: int, zlib_hdr: bool = True, hdr: int | None = None) -> tuple[int, bytes]:
+    """hdr: 0 raw, 1 zlib, 2 gzip, | 0x100 trailer check (default: zlib_hdr)."""
     a = np.frombuffer(bytes(stream) + b"\0", np.uint8)
     out = np.zeros(max(cap, 1), np.uint8)
     w = C.c_uint64()
-    st = lib().mdz_ref_inflate(_ptr(a), len(stream), _ptr(out), cap, C.byref(w), int(zlib_hdr))
+    mode = int(zlib_hdr) if hdr is None else int(hdr)
+    st = lib().mdz_ref_inflate(_ptr(a), len(stream), _ptr(out), cap, C.byref(w), mode)
     return st, out[:w.value].tobytes()
+
+def gzip_header_len(stream: bytes) -> int:
+    a = np.frombuffer(bytes(stream) + b"\0", np.uint8)
+    return lib().mdz_ref_gzip_header_len(_ptr(a), len(stream))
 
 def inflate_batch(b: DeflateBatch, caps=None, zlib_hdr: bool = True, threads: int | None = None):
     caps = b.raw_len.copy() if caps is None else np.ascontiguousarray(caps, np.uint64)

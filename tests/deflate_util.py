@@ -206,3 +206,36 @@ def crafted_faults() -> list[tuple[str, bytes, int, int]]:
     write_symbols(w, litc, distc, [("L", 97), ("D", 3, 0)])
     cases.append(("dist_single_unused", zlib_wrap(w.tobytes()), 16, -9))
     return cases
+
+
+def gzip_member(data: bytes, level: int = 6, extra: bytes | None = None, name: bytes | None = None,
+                comment: bytes | None = None, fhcrc: bool = False) -> tuple[bytes, int]:
+    """An RFC 1952 member built field by field: (stream, header length).  The
+    Deflate data comes from zlib (raw), the trailer from zlib's crc32."""
+    import struct
+    import zlib
+    flg = (0x02 if fhcrc else 0) | (0x04 if extra is not None else 0) | \
+          (0x08 if name is not None else 0) | (0x10 if comment is not None else 0)
+    h = bytearray(b"\x1f\x8b\x08" + bytes([flg]) + struct.pack("<I", 0x5EED5EED) + b"\x00\x03")
+    if extra is not None:
+        h += struct.pack("<H", len(extra)) + extra
+    if name is not None:
+        h += name + b"\x00"
+    if comment is not None:
+        h += comment + b"\x00"
+    if fhcrc:
+        h += struct.pack("<H", zlib.crc32(bytes(h)) & 0xFFFF)
+    c = zlib.compressobj(level, zlib.DEFLATED, -15)
+    body = c.compress(data) + c.flush()
+    return bytes(h) + body + struct.pack("<II", zlib.crc32(data), len(data) & 0xFFFFFFFF), len(h)
+
+
+GZIP_FIELD_CASES = [
+    {},
+    {"name": b"column.bin"},
+    {"extra": b"AB\x04\x00xyzw"},
+    {"comment": b"a comment"},
+    {"fhcrc": True},
+    {"extra": b"", "name": b"n", "comment": b"", "fhcrc": True},
+    {"extra": bytes(300), "name": b"long-name" * 20},
+]

@@ -39,3 +39,14 @@ def test_python_binding_covers_header():
     lib = L.lib()       # dlopen only; no HIP call is made
     for n in declared():
         assert getattr(lib, n) is not None
+
+
+def test_inflate_framing_checked_on_host():
+    """Bad parse_hdr values are refused before any device call."""
+    import compression_amd._lib as L
+    lib = L.lib()
+    for bad in (3, 0x100, 0x200, 0x103, -1):
+        assert lib.mdz_inflate_batch(None, None, 0, bad, None) == -1
+        assert lib.mdz_cost_features(None, 0, bad, None, None) == -1
+    assert lib.mdz_copy_uncompressed(None, -1, None) == -1
+    assert lib.mdz_copy_uncompressed(None, 0, None) == 0
