@@ -432,11 +432,63 @@ class CopyWorkload:
         return None
 
 
+class FileWorkload:
+    """The RLE v2 column of the rlev2 workload (1 GiB int64, 128 KiB chunks)
+    decoded from a container file: each step reads the payload from the file
+    through pinned staging pieces, copies them over PCIe and decodes each piece
+    once it has landed.  The rate includes the file read and PCIe; the file
+    sits in the page cache after the first step."""
+    name = "rlev2_i64_1GiB_from_file"
+    dtype = "u64"
+
+    def __init__(self, line=512, n_vals=1 << 27, seed=0x4D445A02, stage=64 << 20, **_):
+        import os
+        import tempfile
+        from compression_amd import container
+        from compression_amd import host as H
+        import compression_amd as mdz
+        self.data = H.gen_rlev2_i64(n_vals, seed)
+        c = H.rlev2_encode(self.data, 131072, line)
+        fd, self.path = tempfile.mkstemp(suffix=".mdz")
+        os.close(fd)
+        container.save(self.path, c)
+        self.file = container.File(self.path)
+        self.dst = mdz.DeviceBuffer(c.raw_bytes)
+        self.status = mdz.DeviceBuffer(4 * c.n_chunks)
+        self.stage = stage
+        self.out_bytes = c.raw_bytes
+        n = c.n_chunks
+        self.alg_bytes = c.payload_bytes + 256 * n + 8 * (n + 1) + c.raw_bytes
+        self.config = {"workload": self.name, "codec": "rle_v2", "elem": "u64",
+                       "out_bytes": c.raw_bytes, "file_payload_bytes": c.payload_bytes,
+                       "n_chunks": n, "chunk_bytes": 131072, "read_unit": line,
+                       "stage_bytes": stage, "includes": "file read + PCIe H2D + decode"}
+
+    def step(self, stream):
+        self.file.decode(self.dst, self.status, stage_bytes=self.stage, stream=stream)
+
+    def verify(self):
+        got = self.dst.download(self.out_bytes)
+        st = self.status.download(dtype=np.int32)
+        return bool((st == 0).all() and np.array_equal(got, self.data.view(np.uint8)))
+
+    def cpu(self, threads):
+        return None
+
+    def __del__(self):
+        import os
+        try:
+            self.file.close()
+            os.unlink(self.path)
+        except Exception:
+            pass
+
+
 WORKLOADS = {"config2": Config2Workload,
              "config2-plain": lambda **kw: Config2Workload(batched=False, **kw),
              "rlev1": RleV1Workload, "rlev2": RleV2Workload,
              "brle": BrleWorkload, "lzss": LzssWorkload, "deflate": DeflateWorkload,
-             "copy": CopyWorkload}
+             "copy": CopyWorkload, "file": FileWorkload}
 DEFAULT = "config2"
 
 

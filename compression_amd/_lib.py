@@ -99,6 +99,12 @@ SIGNATURES = {
     "mdz_inflate_batch_sched": (C.c_int, [vp, vp, C.c_int, C.c_int, vp, u32, vp, vp, vp]),
     "mdz_inflate_slots": (C.c_int, [C.POINTER(u32)]),
     "mdz_copy_uncompressed": (C.c_int, [vp, C.c_int, vp]),
+    "mdz_file_write": (C.c_int, [C.c_char_p, vp, vp, vp, vp, vp, vp]),
+    "mdz_file_open": (C.c_int, [C.c_char_p, C.POINTER(vp)]),
+    "mdz_file_info": (C.c_int, [vp, vp]),
+    "mdz_file_raw_len": (C.c_int, [vp, vp]),
+    "mdz_file_decode": (C.c_int, [vp, vp, u64, vp, u64, vp]),
+    "mdz_file_close": (C.c_int, [vp]),
     "mdz_cost_features": (C.c_int, [vp, C.c_int, C.c_int, vp, vp]),
     "mdz_batch_submit": (C.c_int, [C.POINTER(BatchT), vp]),
     "mdz_graph_begin": (C.c_int, [vp]),

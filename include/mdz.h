@@ -316,6 +316,62 @@ int mdz_deflate_decompress_gpu(const uint8_t* in, uint8_t** out, uint64_t in_n_b
                                uint64_t col_n_bytes, const uint64_t* blk_offset_f,
                                uint64_t blk_n_bytes, uint64_t chunk_size);
 
+/* ====================================================== container file ===== *
+ * A self-describing file (MDZF) holding one container of any codec with all
+ * its metadata: header (codec, parameters, counts, section offsets, CRC-32),
+ * blk_off, col_len, col_map, Deflate stream sizes and the payload, each on a
+ * 4 KiB boundary.  It replaces the reference's side files in the working
+ * directory (include/rle_v1/rlev1.h:81-87,110-173; include/brle/
+ * brle_trans.h:646-657) and the LZSS .meta blob (include/utils.h:46-118).
+ * No counterpart in the reference otherwise. */
+#define MDZ_CODEC_RLEV1   1
+#define MDZ_CODEC_RLEV2   2
+#define MDZ_CODEC_BRLE    3
+#define MDZ_CODEC_LZSS    4
+#define MDZ_CODEC_DEFLATE 5
+
+typedef struct {
+    uint32_t codec;          /* MDZ_CODEC_* */
+    uint32_t elem_bytes;     /* RLE v1 DATA_TYPE / RLE v2 COMP_TYPE size */
+    uint32_t lane_line;      /* RLE v1 COMP_COL_LEN / RLE v2 READ_UNIT */
+    uint32_t framing;        /* Deflate: parse_hdr (MDZ_HDR_*) */
+    uint64_t n_chunks;       /* chunks, or Deflate streams (< 2^32) */
+    uint64_t chunk_bytes;    /* uncompressed bytes per chunk (unused for Deflate) */
+    uint64_t raw_bytes;      /* uncompressed bytes in all */
+    uint64_t payload_bytes;
+} mdz_file_info_t;
+
+typedef struct mdz_file_s* mdz_file_t;
+
+/* Writes a file from host arrays: blk_off[n+1] (payload offsets, the last
+ * entry as the codec defines it: RLE v2 holds the uncompressed size there),
+ * col_len[32n] (RLE v1, RLE v2, LZSS), col_map[32n] (LZSS), raw_len[n]
+ * (Deflate: output bytes per stream).  Tables are checked first. */
+int mdz_file_write(const char* path, const mdz_file_info_t* info, const void* payload,
+                   const uint64_t* blk_off, const uint64_t* col_len, const uint8_t* col_map,
+                   const uint64_t* raw_len);
+/* Maps a file read-only and checks header CRC, parameters and tables
+ * (MDZ_ERR_INVALID, or MDZ_ERR_CORRUPT for a header CRC mismatch).  No
+ * device call. */
+int mdz_file_open(const char* path, mdz_file_t* file);
+int mdz_file_info(mdz_file_t file, mdz_file_info_t* info);
+/* Deflate files: raw_len[n] (host) = each stream's output bytes. */
+int mdz_file_raw_len(mdz_file_t file, uint64_t* raw_len);
+/* Decodes the whole file into dst (device, dst_bytes).  Chunked codecs write
+ * chunk c at dst + c * chunk_bytes and status (device int32[n], or NULL);
+ * Deflate writes stream i after streams 0..i-1 and needs status (device
+ * mdz_inflate_status_t[n]) and dst_bytes >= raw_bytes.  The payload moves to
+ * HBM in pieces of about stage_bytes (0 = 64 MiB) through two pinned buffers;
+ * the host reads one from the file while the other is copied on a separate
+ * stream (the call returns once the last piece is read), and each
+ * piece's chunks decode on `stream` once their bytes have landed.  The first
+ * call allocates the device copies of payload and tables and the staging
+ * buffers; they stay with the handle for later calls. */
+int mdz_file_decode(mdz_file_t file, void* dst, uint64_t dst_bytes, void* status,
+                    uint64_t stage_bytes, void* stream);
+/* Waits for the handle's copies, frees its device and host memory, unmaps. */
+int mdz_file_close(mdz_file_t file);
+
 /* ================================================ learned cost model ===== *
  * Scheduler-side MLP (features -> 32 -> 32 -> 1, ReLU, fp32) predicting
  * log2(decode cycles) of a Deflate stream, trained on the decoder's own cost
