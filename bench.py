@@ -432,6 +432,49 @@ class CopyWorkload:
         return None
 
 
+class BrleEncodeWorkload:
+    """Byte RLE encode on the GPU (the byte-RLE workload's 256 MiB u8 stream,
+    256 KiB chunks as brle::compress_gpu uses): encode, scan, pack."""
+    name = "brle_u8_256MiB_encode"
+    dtype = "u8"
+
+    def __init__(self, n=1 << 28, chunk=262144, seed=0x4D445A03, **_):
+        from compression_amd import brle
+        from compression_amd import host as H
+        import compression_amd as mdz
+        self.brle, self.H, self.chunk = brle, H, chunk
+        self.data = H.gen_brle_u8(n, seed)
+        self.n, wb, cap = brle.compress_bound(n, chunk)
+        self.src = mdz.DeviceBuffer.from_array(self.data)
+        self.work, self.dst = mdz.DeviceBuffer(wb), mdz.DeviceBuffer(cap)
+        self.ptr = mdz.DeviceBuffer(8 * (self.n + 1))
+        self.out_bytes = n                      # input bytes encoded per step
+        self.alg_bytes = None
+        self.config = {"workload": self.name, "codec": "byte_rle", "direction": "encode",
+                       "in_bytes": n, "chunk_bytes": chunk, "n_chunks": self.n}
+
+    def step(self, stream):
+        self.brle.compress(self.src, self.out_bytes, self.dst, self.ptr, self.work, self.chunk, stream)
+
+    def verify(self):
+        ptr = self.ptr.download(8 * (self.n + 1)).view(np.uint64)
+        ref = self.H.brle_encode(self.data, self.chunk)
+        self.alg_bytes = int(self.out_bytes + 2 * ptr[-1] + 16 * self.n)
+        self.config["out_bytes"] = int(ptr[-1])
+        return bool(np.array_equal(ptr, ref.ptr) and
+                    self.dst.download(int(ptr[-1])).tobytes() == ref.data.tobytes())
+
+    def kernels(self):
+        return [(self.name, self.step, self.alg_bytes)]
+
+    def cpu(self, threads):
+        t = _cpu_time(lambda: self.H.brle_encode(self.data, self.chunk, threads=threads))
+        return {"value": round(self.out_bytes / t / 1e9, 3), "unit": "GB/s", "cores": threads,
+                "kind": "port",
+                "sample": f"whole workload ({self.out_bytes >> 20} MiB in, {self.n} chunks), host "
+                          f"C encoder (the reference encoder's algorithm), median of >=3 runs"}
+
+
 class FileWorkload:
     """The RLE v2 column of the rlev2 workload (1 GiB int64, 128 KiB chunks)
     decoded from a container file: each step reads the payload from the file
@@ -488,7 +531,7 @@ WORKLOADS = {"config2": Config2Workload,
              "config2-plain": lambda **kw: Config2Workload(batched=False, **kw),
              "rlev1": RleV1Workload, "rlev2": RleV2Workload,
              "brle": BrleWorkload, "lzss": LzssWorkload, "deflate": DeflateWorkload,
-             "copy": CopyWorkload, "file": FileWorkload}
+             "copy": CopyWorkload, "file": FileWorkload, "brle-encode": BrleEncodeWorkload}
 DEFAULT = "config2"
 
 

@@ -99,6 +99,9 @@ SIGNATURES = {
     "mdz_inflate_batch_sched": (C.c_int, [vp, vp, C.c_int, C.c_int, vp, u32, vp, vp, vp]),
     "mdz_inflate_slots": (C.c_int, [C.POINTER(u32)]),
     "mdz_copy_uncompressed": (C.c_int, [vp, C.c_int, vp]),
+    "mdz_brle_compress_bound": (C.c_int, [u64, u64, C.POINTER(u64), C.POINTER(u64), C.POINTER(u64)]),
+    "mdz_brle_compress": (C.c_int, [vp, u64, u64, vp, vp, vp, vp]),
+    "mdz_brle_compress_gpu": (C.c_int, [vp, C.POINTER(vp), u64, C.POINTER(u64)]),
     "mdz_file_write": (C.c_int, [C.c_char_p, vp, vp, vp, vp, vp, vp]),
     "mdz_file_open": (C.c_int, [C.c_char_p, C.POINTER(vp)]),
     "mdz_file_info": (C.c_int, [vp, vp]),

@@ -50,3 +50,30 @@ def decompress_gpu(container: bytes, chunk_bytes: int = 256 * 1024) -> bytes:
     if rc != 0:
         raise MdzError(rc, "mdz_brle_decompress_host (corrupt chunk)")
     return data
+
+
+def compress_bound(in_bytes: int, chunk_bytes: int = 256 * 1024) -> tuple[int, int, int]:
+    """(n_chunks, work_bytes, data_cap) for ``compress``."""
+    n, w, d = C.c_uint64(), C.c_uint64(), C.c_uint64()
+    check(lib().mdz_brle_compress_bound(in_bytes, chunk_bytes, C.byref(n), C.byref(w), C.byref(d)),
+          "mdz_brle_compress_bound")
+    return n.value, w.value, d.value
+
+
+def compress(src, in_bytes: int, data, ptr, work, chunk_bytes: int = 256 * 1024, stream=None) -> None:
+    """Device-resident encode: ``src`` (in_bytes) -> ``data`` + ``ptr`` (u64[n+1]),
+    with ``work`` sized by ``compress_bound``."""
+    check(lib().mdz_brle_compress(as_ptr(src), in_bytes, chunk_bytes, as_ptr(work), as_ptr(data),
+                                  as_ptr(ptr), as_ptr(stream)), "mdz_brle_compress")
+
+
+def compress_gpu(data: bytes) -> bytes:
+    """brle::compress_gpu: host bytes -> the reference container (256 KiB chunks)."""
+    src = np.frombuffer(data, np.uint8)
+    out = C.c_void_p()
+    n = C.c_uint64()
+    check(lib().mdz_brle_compress_gpu(src.ctypes.data, C.byref(out), src.size, C.byref(n)),
+          "mdz_brle_compress_gpu")
+    blob = C.string_at(out.value, n.value)
+    lib().mdz_host_free(out)
+    return blob

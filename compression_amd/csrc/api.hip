@@ -299,6 +299,58 @@ int mdz_brle_decompress_gpu(const uint8_t* in, uint8_t** out, uint64_t in_n_byte
     return mdz_brle_decompress_host(in, out, in_n_bytes, out_n_bytes, 256 * 1024);
 }
 
+int mdz_brle_compress_bound(uint64_t in_bytes, uint64_t chunk_bytes, uint64_t* n_chunks,
+                            uint64_t* work_bytes, uint64_t* data_cap) {
+    return mdz::brle_compress_bound(in_bytes, chunk_bytes, n_chunks, work_bytes, data_cap);
+}
+
+int mdz_brle_compress(const void* in, uint64_t in_bytes, uint64_t chunk_bytes, void* work, void* data,
+                      uint64_t* ptr, void* stream) {
+    return mdz::brle_compress(in, in_bytes, chunk_bytes, work, data, ptr, S(stream));
+}
+
+int mdz_brle_compress_gpu(const uint8_t* in, uint8_t** out, uint64_t in_n_bytes, uint64_t* out_n_bytes) {
+    if (!in || !out || !out_n_bytes) return MDZ_ERR_INVALID;
+    const uint64_t chunk = 256 * 1024;
+    uint64_t n = 0, wb = 0, cap = 0;
+    int rc = mdz::brle_compress_bound(in_n_bytes, chunk, &n, &wb, &cap);
+    if (rc) return rc;
+    if (n + 1 > 0xFFFFFFFFull) return MDZ_ERR_INVALID;
+    DevBuf d_in, d_work, d_data, d_ptr;
+    if ((rc = d_in.alloc(in_n_bytes)) || (rc = d_work.alloc(wb)) || (rc = d_data.alloc(cap)) ||
+        (rc = d_ptr.alloc(8 * (n + 1))))
+        return rc;
+    if (hipMemcpy(d_in.p, in, in_n_bytes, hipMemcpyHostToDevice) != hipSuccess) return MDZ_ERR_HIP;
+    if ((rc = mdz::brle_compress(d_in.p, in_n_bytes, chunk, d_work.p, d_data.p, (uint64_t*)d_ptr.p, nullptr)))
+        return rc;
+    // [u32 n + 1][u64 ptr[n + 1]][u64 in_n_bytes][data], as brle.h:278-292 lays it out
+    uint64_t* ptr = (uint64_t*)malloc(8 * (n + 1));
+    if (!ptr) return MDZ_ERR_NOMEM;
+    if (hipMemcpy(ptr, d_ptr.p, 8 * (n + 1), hipMemcpyDeviceToHost) != hipSuccess) {
+        free(ptr);
+        return MDZ_ERR_HIP;
+    }
+    const uint64_t hdr = 4 + 8 * (n + 1) + 8, total = hdr + ptr[n];
+    uint8_t* o = (uint8_t*)malloc(total);
+    if (!o) {
+        free(ptr);
+        return MDZ_ERR_NOMEM;
+    }
+    const uint32_t n_ptr = (uint32_t)(n + 1);
+    memcpy(o, &n_ptr, 4);
+    memcpy(o + 4, ptr, 8 * (n + 1));
+    memcpy(o + 4 + 8 * (n + 1), &in_n_bytes, 8);
+    const hipError_t e = ptr[n] ? hipMemcpy(o + hdr, d_data.p, ptr[n], hipMemcpyDeviceToHost) : hipSuccess;
+    free(ptr);
+    if (e != hipSuccess) {
+        free(o);
+        return MDZ_ERR_HIP;
+    }
+    *out = o;
+    *out_n_bytes = total;
+    return MDZ_OK;
+}
+
 // -------------------------------------------------------------------- LZSS
 int mdz_lzss_decompress(const mdz_lanes_t* in, void* dst, uint64_t dst_bytes, int32_t* status,
                         void* stream) {

@@ -29,6 +29,11 @@ int brle_decompress(const uint64_t* ptr, uint64_t n_chunks, uint64_t chunk_bytes
                     void* dst, uint64_t dst_bytes, int32_t* status, uint32_t max_chunk_bytes,
                     hipStream_t s);
 
+int brle_compress_bound(uint64_t in_bytes, uint64_t chunk, uint64_t* n_chunks, uint64_t* work_bytes,
+                        uint64_t* data_cap);
+int brle_compress(const void* in, uint64_t in_bytes, uint64_t chunk, void* work, void* data, uint64_t* ptr,
+                  hipStream_t s);
+
 int inflate_batch(const mdz_inflate_in_t* in, mdz_inflate_status_t* out, uint32_t count,
                   int parse_hdr, const uint32_t* order, uint32_t persistent_blocks,
                   uint32_t* cursor, mdz_inflate_cost_t* cost, hipStream_t s);

@@ -192,6 +192,21 @@ int mdz_brle_decompress_host(const uint8_t* in, uint8_t** out, uint64_t in_n_byt
 int mdz_brle_decompress_gpu(const uint8_t* in, uint8_t** out, uint64_t in_n_bytes,
                             uint64_t* out_n_bytes);
 
+/* Byte RLE encode -- replaces brle::compress_gpu and its kernels (reference
+ * include/brle/brle.h:232-303; the encoder :129-230).  The bytes equal that
+ * encoder's.  Device form: in (in_bytes, device) is cut into chunk_bytes
+ * chunks; work (work_bytes from mdz_brle_compress_bound) holds per-chunk
+ * slots and sizes; data (data_cap bytes) receives the packed chunk data and
+ * ptr (device u64[n + 1]) its offsets, ptr[0] = 0 and ptr[n] = data bytes. */
+int mdz_brle_compress_bound(uint64_t in_bytes, uint64_t chunk_bytes, uint64_t* n_chunks,
+                            uint64_t* work_bytes, uint64_t* data_cap);
+int mdz_brle_compress(const void* in, uint64_t in_bytes, uint64_t chunk_bytes, void* work, void* data,
+                      uint64_t* ptr, void* stream);
+/* brle::compress_gpu's host form: in (host) -> *out = the reference container
+ * [u32 n + 1][u64 ptr[n + 1]][u64 in_n_bytes][data] with 256 KiB chunks
+ * (brle.h:8,278-292), allocated here; free with mdz_host_free. */
+int mdz_brle_compress_gpu(const uint8_t* in, uint8_t** out, uint64_t in_n_bytes, uint64_t* out_n_bytes);
+
 /* ============================================================== LZSS ===== *
  * LZSS decode -- replaces lzss::kernel_decompress / decompress_func
  * (reference include/lzss/lzss_gpu2.h:1898-1907,114-325) and the device part
