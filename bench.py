@@ -475,6 +475,52 @@ class BrleEncodeWorkload:
                           f"C encoder (the reference encoder's algorithm), median of >=3 runs"}
 
 
+class RleV1EncodeWorkload:
+    """RLE v1 encode on the GPU: the rlev1 workload's 64 MiB u32 column,
+    128 KiB chunks, 128-byte lines (lane encode, scan, interleave)."""
+    name = "rlev1_u32_64MiB_encode"
+    dtype = "u32"
+
+    def __init__(self, n_vals=1 << 24, chunk=131072, line=128, seed=0x4D445A01, **_):
+        from compression_amd import host as H
+        from compression_amd import rle_v1
+        import compression_amd as mdz
+        self.rle_v1, self.H, self.chunk, self.line = rle_v1, H, chunk, line
+        self.data = H.gen_rlev1_u32(n_vals, seed)
+        raw = self.data.view(np.uint8)
+        self.n, wb, cap = rle_v1.compress_bound(raw.size, chunk, 4, line)
+        self.src = mdz.DeviceBuffer.from_array(raw)
+        self.work, self.pay = mdz.DeviceBuffer(wb), mdz.DeviceBuffer(cap)
+        self.col, self.off = mdz.DeviceBuffer(256 * self.n), mdz.DeviceBuffer(8 * (self.n + 1))
+        self.out_bytes = raw.size               # input bytes encoded per step
+        self.alg_bytes = None
+        self.config = {"workload": self.name, "codec": "rle_v1", "direction": "encode", "elem": "u32",
+                       "in_bytes": raw.size, "chunk_bytes": chunk, "line": line, "n_chunks": self.n}
+
+    def step(self, stream):
+        self.rle_v1.compress(self.src, self.out_bytes, self.chunk, 4, self.line, self.work, self.pay,
+                             self.col, self.off, stream)
+
+    def verify(self):
+        off = self.off.download(8 * (self.n + 1)).view(np.uint64)
+        ref = self.H.rlev1_encode(self.data, self.chunk, self.line)
+        self.alg_bytes = int(self.out_bytes + 2 * off[-1] + 256 * self.n + 8 * (self.n + 1))
+        self.config["payload_bytes"] = int(off[-1])
+        return bool(np.array_equal(off, ref.blk_off) and
+                    np.array_equal(self.col.download(256 * self.n).view(np.uint64), ref.col_len) and
+                    self.pay.download(int(off[-1])).tobytes() == ref.payload.tobytes())
+
+    def kernels(self):
+        return [(self.name, self.step, self.alg_bytes)]
+
+    def cpu(self, threads):
+        t = _cpu_time(lambda: self.H.rlev1_encode(self.data, self.chunk, self.line, threads=threads))
+        return {"value": round(self.out_bytes / t / 1e9, 3), "unit": "GB/s", "cores": threads,
+                "kind": "port",
+                "sample": f"whole workload ({self.out_bytes >> 20} MiB in, {self.n} chunks), host C "
+                          f"encoder (the reference encoder's algorithm), median of >=3 runs"}
+
+
 class FileWorkload:
     """The RLE v2 column of the rlev2 workload (1 GiB int64, 128 KiB chunks)
     decoded from a container file: each step reads the payload from the file
@@ -531,7 +577,8 @@ WORKLOADS = {"config2": Config2Workload,
              "config2-plain": lambda **kw: Config2Workload(batched=False, **kw),
              "rlev1": RleV1Workload, "rlev2": RleV2Workload,
              "brle": BrleWorkload, "lzss": LzssWorkload, "deflate": DeflateWorkload,
-             "copy": CopyWorkload, "file": FileWorkload, "brle-encode": BrleEncodeWorkload}
+             "copy": CopyWorkload, "file": FileWorkload, "brle-encode": BrleEncodeWorkload,
+             "rlev1-encode": RleV1EncodeWorkload}
 DEFAULT = "config2"
 
 

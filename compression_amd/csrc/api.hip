@@ -205,6 +205,67 @@ int mdz_rle_v1_decompress_gpu(const uint8_t* in, uint8_t** out, uint64_t in_n_by
     return rc;
 }
 
+int mdz_rlev1_compress_bound(uint64_t in_bytes, uint64_t chunk_bytes, uint32_t elem_bytes, uint32_t comp_col_len,
+                             uint64_t* n_chunks, uint64_t* work_bytes, uint64_t* payload_cap) {
+    return mdz::rlev1_compress_bound(in_bytes, chunk_bytes, elem_bytes, comp_col_len, n_chunks, work_bytes,
+                                     payload_cap);
+}
+
+int mdz_rlev1_compress(const void* in, uint64_t in_bytes, uint64_t chunk_bytes, uint32_t elem_bytes,
+                       uint32_t comp_col_len, void* work, void* payload, uint64_t* col_len, uint64_t* blk_off,
+                       void* stream) {
+    return mdz::rlev1_compress(in, in_bytes, chunk_bytes, elem_bytes, comp_col_len, work, payload, col_len,
+                               blk_off, S(stream));
+}
+
+int mdz_rle_v1_compress_gpu(const uint8_t* in, uint8_t** out, uint64_t in_n_bytes, uint64_t* out_n_bytes,
+                            int comp_col_len, uint64_t chunk_size, uint32_t elem_bytes) {
+    if (!in || !out || !out_n_bytes || comp_col_len <= 0) return MDZ_ERR_INVALID;
+    uint64_t n = 0, wb = 0, cap = 0;
+    int rc = mdz::rlev1_compress_bound(in_n_bytes, chunk_size, elem_bytes, (uint32_t)comp_col_len, &n, &wb, &cap);
+    if (rc) return rc;
+    DevBuf d_in, d_work, d_pay, d_col, d_off;
+    if ((rc = d_in.alloc(in_n_bytes)) || (rc = d_work.alloc(wb)) || (rc = d_pay.alloc(cap)) ||
+        (rc = d_col.alloc(32 * 8 * n)) || (rc = d_off.alloc(8 * (n + 1))))
+        return rc;
+    if (hipMemcpy(d_in.p, in, in_n_bytes, hipMemcpyHostToDevice) != hipSuccess) return MDZ_ERR_HIP;
+    if ((rc = mdz::rlev1_compress(d_in.p, in_n_bytes, chunk_size, elem_bytes, (uint32_t)comp_col_len, d_work.p,
+                                  d_pay.p, (uint64_t*)d_col.p, (uint64_t*)d_off.p, nullptr)))
+        return rc;
+    uint64_t* col = (uint64_t*)malloc(32 * 8 * n);
+    uint64_t* off = (uint64_t*)malloc(8 * (n + 1));
+    uint8_t* o = nullptr;
+    rc = (col && off) ? MDZ_OK : MDZ_ERR_NOMEM;
+    if (rc == MDZ_OK && (hipMemcpy(col, d_col.p, 32 * 8 * n, hipMemcpyDeviceToHost) != hipSuccess ||
+                         hipMemcpy(off, d_off.p, 8 * (n + 1), hipMemcpyDeviceToHost) != hipSuccess))
+        rc = MDZ_ERR_HIP;
+    // [u64 in_n_bytes][payload]; col_len and blk_off go to the side files the
+    // reference writes (rlev1.h:85-93) and mdz_rle_v1_decompress_gpu reads
+    if (rc == MDZ_OK && !(o = (uint8_t*)malloc(off[n] + 8))) rc = MDZ_ERR_NOMEM;
+    if (rc == MDZ_OK) {
+        memcpy(o, &in_n_bytes, 8);
+        if (off[n] && hipMemcpy(o + 8, d_pay.p, off[n], hipMemcpyDeviceToHost) != hipSuccess) rc = MDZ_ERR_HIP;
+    }
+    if (rc == MDZ_OK) {
+        FILE* fc = fopen("./input_data/rle_v1_col_len.bin", "wb");
+        FILE* fo = fopen("./input_data/rle_v1_blk_offset.bin", "wb");
+        if (!fc || !fo || fwrite(col, 8, 32 * n, fc) != 32 * n || fwrite(off, 8, n + 1, fo) != n + 1)
+            rc = MDZ_ERR_INVALID;
+        if (fc) fclose(fc);
+        if (fo) fclose(fo);
+    }
+    const uint64_t total = rc == MDZ_OK ? off[n] + 8 : 0;
+    free(col);
+    free(off);
+    if (rc != MDZ_OK) {
+        free(o);
+        return rc;
+    }
+    *out = o;
+    *out_n_bytes = total;
+    return MDZ_OK;
+}
+
 // ------------------------------------------------------------------ RLE v2
 int mdz_rlev2_decompress(const mdz_lanes_t* in, void* dst, uint64_t dst_bytes, int32_t* status,
                          void* stream) {

@@ -52,3 +52,33 @@ def decompress_gpu(file_bytes: bytes, comp_col_len: int, chunk_size: int, elem_b
     if rc != 0:
         raise MdzError(rc, "mdz_rlev1_decompress_host (corrupt chunk)")
     return data
+
+
+def compress_bound(in_bytes: int, chunk_bytes: int, elem_bytes: int, comp_col_len: int) -> tuple[int, int, int]:
+    """(n_chunks, work_bytes, payload_cap) for ``compress``."""
+    n, w, p = C.c_uint64(), C.c_uint64(), C.c_uint64()
+    check(lib().mdz_rlev1_compress_bound(in_bytes, chunk_bytes, elem_bytes, comp_col_len, C.byref(n),
+                                         C.byref(w), C.byref(p)), "mdz_rlev1_compress_bound")
+    return n.value, w.value, p.value
+
+
+def compress(src, in_bytes: int, chunk_bytes: int, elem_bytes: int, comp_col_len: int, work, payload,
+             col_len, blk_off, stream=None) -> None:
+    """Device-resident encode (rle_v1::compress_gpu): ``src`` -> ``payload``,
+    ``col_len`` (u64[32n]) and ``blk_off`` (u64[n+1])."""
+    check(lib().mdz_rlev1_compress(as_ptr(src), in_bytes, chunk_bytes, elem_bytes, comp_col_len, as_ptr(work),
+                                   as_ptr(payload), as_ptr(col_len), as_ptr(blk_off), as_ptr(stream)),
+          "mdz_rlev1_compress")
+
+
+def compress_gpu(data: bytes, comp_col_len: int, chunk_size: int, elem_bytes: int) -> bytes:
+    """rle_v1::compress_gpu: returns [u64 size][payload] and writes col_len /
+    blk_off to ./input_data/rle_v1_col_len.bin and rle_v1_blk_offset.bin."""
+    src = np.frombuffer(data, np.uint8)
+    out = C.c_void_p()
+    n = C.c_uint64()
+    check(lib().mdz_rle_v1_compress_gpu(src.ctypes.data, C.byref(out), src.size, C.byref(n), comp_col_len,
+                                        chunk_size, elem_bytes), "mdz_rle_v1_compress_gpu")
+    blob = C.string_at(out.value, n.value)
+    lib().mdz_host_free(out)
+    return blob

@@ -146,6 +146,26 @@ int mdz_rle_v1_decompress_gpu(const uint8_t* in, uint8_t** out, uint64_t in_n_by
                               uint64_t* out_n_bytes, int comp_col_len, uint64_t chunk_size,
                               uint32_t elem_bytes);
 
+/* RLE v1 encode -- replaces rle_v1::compress_gpu and its kernels (reference
+ * include/rle_v1/rlev1.h:24-97; encoder include/rle_v1/encoder.h:607-1018).
+ * payload, col_len and blk_off equal that encoder's byte for byte.  Device
+ * form: in (device, in_bytes) in chunks of chunk_bytes, elem_bytes-wide
+ * values in lines of comp_col_len bytes; work (work_bytes) and payload
+ * (payload_cap) sized by mdz_rlev1_compress_bound; col_len (device u64[32n])
+ * and blk_off (device u64[n + 1]) receive the metadata, blk_off[n] = payload
+ * bytes. */
+int mdz_rlev1_compress_bound(uint64_t in_bytes, uint64_t chunk_bytes, uint32_t elem_bytes, uint32_t comp_col_len,
+                             uint64_t* n_chunks, uint64_t* work_bytes, uint64_t* payload_cap);
+int mdz_rlev1_compress(const void* in, uint64_t in_bytes, uint64_t chunk_bytes, uint32_t elem_bytes,
+                       uint32_t comp_col_len, void* work, void* payload, uint64_t* col_len, uint64_t* blk_off,
+                       void* stream);
+/* rle_v1::compress_gpu's host form: *out = [u64 in_n_bytes][payload]
+ * (allocated here; mdz_host_free), and col_len / blk_off written to
+ * ./input_data/rle_v1_col_len.bin and rle_v1_blk_offset.bin as the reference
+ * writes them (rlev1.h:85-93), where mdz_rle_v1_decompress_gpu reads them. */
+int mdz_rle_v1_compress_gpu(const uint8_t* in, uint8_t** out, uint64_t in_n_bytes, uint64_t* out_n_bytes,
+                            int comp_col_len, uint64_t chunk_size, uint32_t elem_bytes);
+
 /* RLE v2 decode -- replaces rlev2::decompress_gpu<READ_UNIT, queue_size,
  * COMP_TYPE> (reference include/rlev2/decoder_trans.h:615-681) and its
  * inflate kernel (include/rlev2/decoder_new.h:618-657).  elem_bytes =
