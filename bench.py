@@ -521,6 +521,54 @@ class RleV1EncodeWorkload:
                           f"encoder (the reference encoder's algorithm), median of >=3 runs"}
 
 
+class RleV2EncodeWorkload:
+    """RLE v2 encode on the GPU: the rlev2 workload's 1 GiB int64 column,
+    128 KiB chunks, READ_UNIT 512 (lane encode, scan, interleave)."""
+    name = "rlev2_i64_1GiB_encode"
+    dtype = "u64"
+
+    def __init__(self, line=512, n_vals=1 << 27, chunk=131072, seed=0x4D445A02, **_):
+        from compression_amd import host as H
+        from compression_amd import rlev2
+        import compression_amd as mdz
+        self.rlev2, self.H, self.chunk, self.ru = rlev2, H, chunk, line
+        self.data = H.gen_rlev2_i64(n_vals, seed)
+        raw = self.data.view(np.uint8)
+        self.n, wb, cap = rlev2.compress_bound(raw.size, chunk, 8, line)
+        self.src = mdz.DeviceBuffer.from_array(raw)
+        self.work, self.pay, self.pb = mdz.DeviceBuffer(wb), mdz.DeviceBuffer(cap), mdz.DeviceBuffer(8)
+        self.col, self.off = mdz.DeviceBuffer(256 * self.n), mdz.DeviceBuffer(8 * (self.n + 1))
+        self.out_bytes = raw.size               # input bytes encoded per step
+        self.alg_bytes = None
+        self.config = {"workload": self.name, "codec": "rle_v2", "direction": "encode", "elem": "u64",
+                       "in_bytes": raw.size, "chunk_bytes": chunk, "read_unit": line, "n_chunks": self.n}
+
+    def step(self, stream):
+        self.rlev2.compress(self.src, self.out_bytes, self.chunk, 8, self.ru, self.work, self.pay, self.col,
+                            self.off, self.pb, stream)
+
+    def verify(self):
+        nbytes = int(self.pb.download(8).view(np.uint64)[0])
+        ref = self.H.rlev2_encode(self.data, self.chunk, self.ru)
+        self.alg_bytes = int(self.out_bytes + 2 * nbytes + 256 * self.n + 8 * (self.n + 1))
+        self.config["payload_bytes"] = nbytes
+        return bool(nbytes == ref.payload_bytes and
+                    np.array_equal(self.off.download(8 * (self.n + 1)).view(np.uint64), ref.blk_off) and
+                    np.array_equal(self.col.download(256 * self.n).view(np.uint64), ref.col_len) and
+                    self.pay.download(nbytes).tobytes() == ref.payload[:nbytes].tobytes())
+
+    def kernels(self):
+        return [(self.name, self.step, self.alg_bytes)]
+
+    def cpu(self, threads):
+        t = _cpu_time(lambda: self.H.rlev2_encode(self.data, self.chunk, self.ru, threads=threads),
+                      min_reps=2, max_reps=3)
+        return {"value": round(self.out_bytes / t / 1e9, 3), "unit": "GB/s", "cores": threads,
+                "kind": "port",
+                "sample": f"whole workload ({self.out_bytes >> 20} MiB in, {self.n} chunks), host C "
+                          f"encoder (the reference encoder's algorithm), median of 2-3 runs"}
+
+
 class FileWorkload:
     """The RLE v2 column of the rlev2 workload (1 GiB int64, 128 KiB chunks)
     decoded from a container file: each step reads the payload from the file
@@ -578,7 +626,7 @@ WORKLOADS = {"config2": Config2Workload,
              "rlev1": RleV1Workload, "rlev2": RleV2Workload,
              "brle": BrleWorkload, "lzss": LzssWorkload, "deflate": DeflateWorkload,
              "copy": CopyWorkload, "file": FileWorkload, "brle-encode": BrleEncodeWorkload,
-             "rlev1-encode": RleV1EncodeWorkload}
+             "rlev1-encode": RleV1EncodeWorkload, "rlev2-encode": RleV2EncodeWorkload}
 DEFAULT = "config2"
 
 

@@ -103,6 +103,9 @@ SIGNATURES = {
                                            C.POINTER(u64)]),
     "mdz_rlev1_compress": (C.c_int, [vp, u64, u64, C.c_uint32, C.c_uint32, vp, vp, vp, vp, vp]),
     "mdz_rle_v1_compress_gpu": (C.c_int, [vp, C.POINTER(vp), u64, C.POINTER(u64), C.c_int, u64, C.c_uint32]),
+    "mdz_rlev2_compress_bound": (C.c_int, [u64, u64, C.c_uint32, C.c_uint32, C.POINTER(u64), C.POINTER(u64),
+                                           C.POINTER(u64)]),
+    "mdz_rlev2_compress": (C.c_int, [vp, u64, u64, C.c_uint32, C.c_uint32, vp, vp, vp, vp, vp, vp]),
     "mdz_brle_compress_bound": (C.c_int, [u64, u64, C.POINTER(u64), C.POINTER(u64), C.POINTER(u64)]),
     "mdz_brle_compress": (C.c_int, [vp, u64, u64, vp, vp, vp, vp]),
     "mdz_brle_compress_gpu": (C.c_int, [vp, C.POINTER(vp), u64, C.POINTER(u64)]),

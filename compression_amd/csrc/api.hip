@@ -301,6 +301,19 @@ int mdz_rlev2_decompress_host(const uint8_t* in, uint64_t in_n_bytes, uint64_t n
     return finish_host(d_out.p, raw, (const int32_t*)d_st.p, n_chunks, out, out_n_bytes);
 }
 
+int mdz_rlev2_compress_bound(uint64_t in_bytes, uint64_t chunk_bytes, uint32_t elem_bytes, uint32_t read_unit,
+                             uint64_t* n_chunks, uint64_t* work_bytes, uint64_t* payload_cap) {
+    return mdz::rlev2_compress_bound(in_bytes, chunk_bytes, elem_bytes, read_unit, n_chunks, work_bytes,
+                                     payload_cap);
+}
+
+int mdz_rlev2_compress(const void* in, uint64_t in_bytes, uint64_t chunk_bytes, uint32_t elem_bytes,
+                       uint32_t read_unit, void* work, void* payload, uint64_t* col_len, uint64_t* blk_off,
+                       uint64_t* payload_bytes, void* stream) {
+    return mdz::rlev2_compress(in, in_bytes, chunk_bytes, elem_bytes, read_unit, work, payload, col_len, blk_off,
+                               payload_bytes, S(stream));
+}
+
 // ---------------------------------------------------------------- byte RLE
 int mdz_brle_decompress(const uint64_t* ptr, uint64_t n_chunks, uint64_t chunk_bytes,
                         const void* data, void* dst, uint64_t dst_bytes, int32_t* status,

@@ -12,10 +12,9 @@
 //      input) and writing the stream to its own slot of the work buffer; the
 //      wave then sums the rounded lane lengths into the chunk's length;
 //   2. one workgroup scans the chunk lengths into blk_off;
-//   3. one workgroup per chunk interleaves: lane t's 4-byte word r lands at
-//      word sum_t' min(words_t', r) + #{t' < t : words_t' > r} of the chunk,
-//      the round-robin order of encoder.h:564-588, and the chunk's tail up
-//      to its 128-byte length is zeroed.
+//   3. one workgroup per chunk interleaves the lane streams in the
+//      round-robin order of encoder.h:564-588 (lanes_enc_dev.h).
+#include "lanes_enc_dev.h"
 #include "mdz_device.h"
 #include "mdz_internal.h"
 
@@ -245,67 +244,6 @@ __global__ __launch_bounds__(kWave) void rlev1_encode_kernel(const uint8_t* __re
     if (l == 0) chunk_len[c] = ((uint64_t)total * 4 + 127) / 128 * 128;
 }
 
-constexpr uint32_t kScanThreads = 1024;
-
-__global__ __launch_bounds__(kScanThreads) void rlev1_scan_kernel(const uint64_t* __restrict__ sizes, uint64_t n,
-                                                                  uint64_t* __restrict__ off) {
-    __shared__ uint64_t part[kScanThreads / kWave];
-    const uint32_t t = threadIdx.x;
-    const uint64_t per = (n + kScanThreads - 1) / kScanThreads;
-    const uint64_t b = min(n, per * t), e = min(n, b + per);
-    uint64_t s = 0;
-    for (uint64_t i = b; i < e; ++i) s += sizes[i];
-    const uint64_t incl = wave_incl_scan_u64(s);
-    if (lane_id() == kWave - 1) part[t / kWave] = incl;
-    __syncthreads();
-    uint64_t run = 0;
-    for (uint32_t w = 0; w < t / kWave; ++w) run += part[w];
-    run += incl - s;
-    if (t == 0) off[0] = 0;
-    for (uint64_t i = b; i < e; ++i) {
-        run += sizes[i];
-        off[i + 1] = run;
-    }
-}
-
-constexpr uint32_t kIlvThreads = 256;
-
-__global__ __launch_bounds__(kIlvThreads) void rlev1_interleave_kernel(const uint8_t* __restrict__ work,
-                                                                       uint64_t lane_slot,
-                                                                       const uint64_t* __restrict__ col_len,
-                                                                       const uint64_t* __restrict__ blk_off,
-                                                                       uint8_t* __restrict__ payload) {
-    __shared__ uint32_t words[32];
-    const uint32_t tid = threadIdx.x;
-    const uint64_t c = blockIdx.x;
-    if (tid < 32) words[tid] = (uint32_t)((col_len[c * 32 + tid] + 3) / 4);
-    __syncthreads();
-    uint32_t* dst = reinterpret_cast<uint32_t*>(payload + blk_off[c]);   // blk_off is a multiple of 128
-    uint32_t total = 0;
-    for (uint32_t t = 0; t < 32; ++t) total += words[t];
-    // word (t, r) for t = lane of this thread's turn: threads stride over
-    // (t, r) pairs lane by lane
-    for (uint32_t t = 0; t < 32; ++t) {
-        const uint32_t wt = words[t];
-        const uint64_t len = col_len[c * 32 + t];
-        const uint8_t* src = work + (c * 32 + t) * lane_slot;
-        for (uint32_t r = tid; r < wt; r += kIlvThreads) {
-            uint32_t pos = 0;
-            for (uint32_t u = 0; u < 32; ++u) {
-                const uint32_t wu = words[u];
-                pos += wu < r ? wu : r;                 // rounds before r
-                pos += (u < t && wu > r) ? 1u : 0u;     // earlier lanes in round r
-            }
-            uint32_t v = *reinterpret_cast<const uint32_t*>(src + 4ull * r);
-            const uint64_t have = len - 4ull * r;       // bytes of this word inside the stream
-            if (have < 4) v &= (1u << (8 * have)) - 1u;
-            dst[pos] = v;
-        }
-    }
-    const uint64_t clen = blk_off[c + 1] - blk_off[c];
-    for (uint64_t q = total + tid; q < clen / 4; q += kIlvThreads) dst[q] = 0;
-}
-
 uint64_t lane_slot_bytes(uint64_t chunk, uint32_t eb) {
     const uint64_t n = chunk / 32 / eb;
     return (n * 11 + 16 + 15) & ~15ull;          // LEB128 <= 10 bytes per value, plus headers
@@ -338,11 +276,12 @@ int rlev1_compress(const void* in, uint64_t in_bytes, uint64_t chunk, uint32_t e
                        in_bytes, chunk, eb, line, slot, static_cast<uint8_t*>(work), col_len, chunk_len);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return record_hip(e, "rlev1_encode_kernel launch");
-    hipLaunchKernelGGL(rlev1_scan_kernel, dim3(1), dim3(kScanThreads), 0, s, chunk_len, n, blk_off);
-    if ((e = hipGetLastError()) != hipSuccess) return record_hip(e, "rlev1_scan_kernel launch");
-    hipLaunchKernelGGL(rlev1_interleave_kernel, dim3((uint32_t)n), dim3(kIlvThreads), 0, s,
-                       static_cast<const uint8_t*>(work), slot, col_len, blk_off, static_cast<uint8_t*>(payload));
-    return record_hip(hipGetLastError(), "rlev1_interleave_kernel launch");
+    hipLaunchKernelGGL(lanes_scan_kernel, dim3(1), dim3(kScanThreads), 0, s, chunk_len, n, blk_off, nullptr, ~0ull);
+    if ((e = hipGetLastError()) != hipSuccess) return record_hip(e, "lanes_scan_kernel launch");
+    hipLaunchKernelGGL(lanes_interleave_kernel, dim3((uint32_t)n), dim3(kIlvThreads), 0, s,
+                       static_cast<const uint8_t*>(work), slot, col_len, blk_off, chunk_len,
+                       static_cast<uint8_t*>(payload));
+    return record_hip(hipGetLastError(), "lanes_interleave_kernel launch");
 }
 
 }  // namespace mdz

@@ -54,3 +54,21 @@ def decompress_gpu(payload: bytes, n_chunks: int, blk_off: np.ndarray, col_len: 
     if rc != 0:
         raise MdzError(rc, "mdz_rlev2_decompress_host (corrupt chunk)")
     return data
+
+
+def compress_bound(in_bytes: int, chunk_bytes: int, elem_bytes: int, read_unit: int) -> tuple[int, int, int]:
+    """(n_chunks, work_bytes, payload_cap) for ``compress``."""
+    n, w, p = C.c_uint64(), C.c_uint64(), C.c_uint64()
+    check(lib().mdz_rlev2_compress_bound(in_bytes, chunk_bytes, elem_bytes, read_unit, C.byref(n), C.byref(w),
+                                         C.byref(p)), "mdz_rlev2_compress_bound")
+    return n.value, w.value, p.value
+
+
+def compress(src, in_bytes: int, chunk_bytes: int, elem_bytes: int, read_unit: int, work, payload, col_len,
+             blk_off, payload_bytes=None, stream=None) -> None:
+    """Device-resident encode (rlev2::compress_gpu_transpose): ``src`` -> ``payload``,
+    ``col_len`` (u64[32n]), ``blk_off`` (u64[n+1], last = in_bytes) and the
+    payload length in ``payload_bytes`` (device u64)."""
+    check(lib().mdz_rlev2_compress(as_ptr(src), in_bytes, chunk_bytes, elem_bytes, read_unit, as_ptr(work),
+                                   as_ptr(payload), as_ptr(col_len), as_ptr(blk_off), as_ptr(payload_bytes),
+                                   as_ptr(stream)), "mdz_rlev2_compress")

@@ -188,6 +188,18 @@ int mdz_rlev2_decompress_host(const uint8_t* in, uint64_t in_n_bytes, uint64_t n
                               uint64_t* out_n_bytes, uint64_t chunk_size, uint32_t read_unit,
                               uint32_t elem_bytes);
 
+/* RLE v2 encode -- replaces rlev2::compress_gpu_transpose (reference
+ * include/rlev2/encoder_trans.h:428-537; block_encode :14-289; writers
+ * include/rlev2/encoder.h:124-785).  payload, col_len and blk_off equal that
+ * encoder's byte for byte; blk_off[n] holds in_bytes (the uncompressed size,
+ * as the reference leaves it) and *payload_bytes (device u64, may be NULL)
+ * the payload's length.  work / payload sized by mdz_rlev2_compress_bound. */
+int mdz_rlev2_compress_bound(uint64_t in_bytes, uint64_t chunk_bytes, uint32_t elem_bytes, uint32_t read_unit,
+                             uint64_t* n_chunks, uint64_t* work_bytes, uint64_t* payload_cap);
+int mdz_rlev2_compress(const void* in, uint64_t in_bytes, uint64_t chunk_bytes, uint32_t elem_bytes,
+                       uint32_t read_unit, void* work, void* payload, uint64_t* col_len, uint64_t* blk_off,
+                       uint64_t* payload_bytes, void* stream);
+
 /* ========================================================== byte RLE ===== *
  * Byte RLE decode -- replaces brle::decompress_gpu / kernel_decompress
  * (reference include/brle/brle.h:26-109,305-342).  Chunk c's compressed bytes
