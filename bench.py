@@ -569,6 +569,53 @@ class RleV2EncodeWorkload:
                           f"encoder (the reference encoder's algorithm), median of 2-3 runs"}
 
 
+class LzssEncodeWorkload:
+    """LZSS encode on the GPU: the lzss workload's 1 GiB Markov+copy stream,
+    64 KiB chunks (lane matchers, lane sort, scan, interleave)."""
+    name = "lzss_1GiB_encode"
+    dtype = "u8"
+
+    def __init__(self, n=1 << 30, chunk=65536, seed=0x4D445A03, **_):
+        from compression_amd import host as H
+        from compression_amd import lzss
+        import compression_amd as mdz
+        self.lzss, self.H, self.chunk = lzss, H, chunk
+        self.data = H.gen_lzss_markov(n, seed)
+        self.n, wb, cap = lzss.compress_bound(n, chunk)
+        self.src = mdz.DeviceBuffer.from_array(self.data)
+        self.work, self.pay = mdz.DeviceBuffer(wb), mdz.DeviceBuffer(cap)
+        self.col, self.cmap = mdz.DeviceBuffer(256 * self.n), mdz.DeviceBuffer(32 * self.n)
+        self.off = mdz.DeviceBuffer(8 * (self.n + 1))
+        self.out_bytes = n                      # input bytes encoded per step
+        self.alg_bytes = None
+        self.config = {"workload": self.name, "codec": "lzss", "direction": "encode", "in_bytes": n,
+                       "chunk_bytes": chunk, "n_chunks": self.n}
+
+    def step(self, stream):
+        self.lzss.compress(self.src, self.out_bytes, self.chunk, self.work, self.pay, self.col, self.cmap,
+                           self.off, stream)
+
+    def verify(self):
+        off = self.off.download(8 * (self.n + 1)).view(np.uint64)
+        ref = self.H.lzss_encode(self.data, self.chunk)
+        self.alg_bytes = int(self.out_bytes + 2 * off[-1] + 288 * self.n)
+        self.config["payload_bytes"] = int(off[-1])
+        return bool(np.array_equal(off, ref.blk_off) and
+                    np.array_equal(self.col.download(256 * self.n).view(np.uint64), ref.col_len) and
+                    np.array_equal(self.cmap.download(32 * self.n), ref.col_map) and
+                    self.pay.download(int(off[-1])).tobytes() == ref.payload[:int(off[-1])].tobytes())
+
+    def kernels(self):
+        return [(self.name, self.step, self.alg_bytes)]
+
+    def cpu(self, threads):
+        t = _cpu_time(lambda: self.H.lzss_encode(self.data, self.chunk, threads=threads), min_reps=2, max_reps=3)
+        return {"value": round(self.out_bytes / t / 1e9, 3), "unit": "GB/s", "cores": threads,
+                "kind": "port",
+                "sample": f"whole workload ({self.out_bytes >> 20} MiB in, {self.n} chunks), host C "
+                          f"encoder (hash-chain matcher, same choices as the reference's scan), median of 2-3 runs"}
+
+
 class FileWorkload:
     """The RLE v2 column of the rlev2 workload (1 GiB int64, 128 KiB chunks)
     decoded from a container file: each step reads the payload from the file
@@ -626,7 +673,8 @@ WORKLOADS = {"config2": Config2Workload,
              "rlev1": RleV1Workload, "rlev2": RleV2Workload,
              "brle": BrleWorkload, "lzss": LzssWorkload, "deflate": DeflateWorkload,
              "copy": CopyWorkload, "file": FileWorkload, "brle-encode": BrleEncodeWorkload,
-             "rlev1-encode": RleV1EncodeWorkload, "rlev2-encode": RleV2EncodeWorkload}
+             "rlev1-encode": RleV1EncodeWorkload, "rlev2-encode": RleV2EncodeWorkload,
+             "lzss-encode": LzssEncodeWorkload}
 DEFAULT = "config2"
 
 

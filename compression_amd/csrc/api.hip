@@ -461,6 +461,75 @@ int mdz_lzss_decompress_host(const uint8_t* in, uint64_t in_n_bytes,
     return finish_host(d_out.p, raw, (const int32_t*)d_st.p, n, out, out_n_bytes);
 }
 
+int mdz_lzss_compress_bound(uint64_t in_bytes, uint64_t chunk_bytes, uint64_t* n_chunks, uint64_t* work_bytes,
+                            uint64_t* payload_cap) {
+    return mdz::lzss_compress_bound(in_bytes, chunk_bytes, n_chunks, work_bytes, payload_cap);
+}
+
+int mdz_lzss_compress(const void* in, uint64_t in_bytes, uint64_t chunk_bytes, void* work, void* payload,
+                      uint64_t* col_len, uint8_t* col_map, uint64_t* blk_off, void* stream) {
+    return mdz::lzss_compress(in, in_bytes, chunk_bytes, work, payload, col_len, col_map, blk_off, S(stream));
+}
+
+int mdz_lzss_compress_gpu(const uint8_t* in, uint64_t in_n_bytes, uint8_t** out, uint64_t* out_n_bytes,
+                          uint8_t*** out_metadata, uint64_t** out_metadata_lens, uint64_t* out_metadata_num) {
+    if (!in || !out || !out_n_bytes || !out_metadata || !out_metadata_lens || !out_metadata_num)
+        return MDZ_ERR_INVALID;
+    const uint64_t chunk = 128 * 1024;
+    uint64_t n = 0, wb = 0, cap = 0;
+    int rc = mdz::lzss_compress_bound(in_n_bytes, chunk, &n, &wb, &cap);
+    if (rc) return rc;
+    DevBuf d_in, d_work, d_pay, d_col, d_map, d_off;
+    if ((rc = d_in.alloc(in_n_bytes)) || (rc = d_work.alloc(wb)) || (rc = d_pay.alloc(cap)) ||
+        (rc = d_col.alloc(256 * n)) || (rc = d_map.alloc(32 * n)) || (rc = d_off.alloc(8 * (n + 1))))
+        return rc;
+    if (hipMemcpy(d_in.p, in, in_n_bytes, hipMemcpyHostToDevice) != hipSuccess) return MDZ_ERR_HIP;
+    if ((rc = mdz::lzss_compress(d_in.p, in_n_bytes, chunk, d_work.p, d_pay.p, (uint64_t*)d_col.p,
+                                 (uint8_t*)d_map.p, (uint64_t*)d_off.p, nullptr)))
+        return rc;
+    // metadata blobs as the reference returns them: [u64 chunk][blk_off][col_len][col_map]
+    uint8_t** md = (uint8_t**)calloc(4, sizeof(uint8_t*));
+    uint64_t* lens = (uint64_t*)malloc(4 * sizeof(uint64_t));
+    if (!md || !lens) {
+        free(md);
+        free(lens);
+        return MDZ_ERR_NOMEM;
+    }
+    lens[0] = 8;
+    lens[1] = 8 * (n + 1);
+    lens[2] = 256 * n;
+    lens[3] = 32 * n;
+    for (int k = 0; k < 4 && rc == MDZ_OK; ++k)
+        if (!(md[k] = (uint8_t*)malloc(lens[k]))) rc = MDZ_ERR_NOMEM;
+    if (rc == MDZ_OK) {
+        memcpy(md[0], &chunk, 8);
+        if (hipMemcpy(md[1], d_off.p, lens[1], hipMemcpyDeviceToHost) != hipSuccess ||
+            hipMemcpy(md[2], d_col.p, lens[2], hipMemcpyDeviceToHost) != hipSuccess ||
+            hipMemcpy(md[3], d_map.p, lens[3], hipMemcpyDeviceToHost) != hipSuccess)
+            rc = MDZ_ERR_HIP;
+    }
+    uint8_t* o = nullptr;
+    uint64_t total = 0;
+    if (rc == MDZ_OK) {
+        memcpy(&total, md[1] + 8 * n, 8);
+        if (!(o = (uint8_t*)malloc(total ? total : 1))) rc = MDZ_ERR_NOMEM;
+        else if (total && hipMemcpy(o, d_pay.p, total, hipMemcpyDeviceToHost) != hipSuccess) rc = MDZ_ERR_HIP;
+    }
+    if (rc != MDZ_OK) {
+        for (int k = 0; k < 4; ++k) free(md[k]);
+        free(md);
+        free(lens);
+        free(o);
+        return rc;
+    }
+    *out = o;
+    *out_n_bytes = total;
+    *out_metadata = md;
+    *out_metadata_lens = lens;
+    *out_metadata_num = 4;
+    return MDZ_OK;
+}
+
 int mdz_meta_split(const uint8_t* meta, uint64_t meta_bytes, const uint8_t** blobs, uint64_t* lens,
                    uint64_t* num) {
     if (!meta || !num || meta_bytes < 8) return MDZ_ERR_INVALID;

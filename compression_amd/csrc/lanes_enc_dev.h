@@ -1,6 +1,6 @@
 // Scan and interleave kernels shared by the lane-transposed encoders (RLE v1,
-// RLE v2): chunk lengths -> offsets, and the 4-byte round-robin transpose of
-// 32 lane streams into a chunk (lane t's word r lands at word
+// RLE v2, LZSS): chunk lengths -> offsets, and the 4-byte round-robin
+// transpose of 32 lane streams (LZSS: in its sorted order) into a chunk (lane t's word r lands at word
 // sum_t' min(words_t', r) + #{t' < t : words_t' > r}), tail zeroed to the
 // chunk's 128-byte length.
 #pragma once
@@ -42,6 +42,7 @@ __global__ __launch_bounds__(kIlvThreads) void lanes_interleave_kernel(const uin
                                                                        const uint64_t* __restrict__ col_len,
                                                                        const uint64_t* __restrict__ blk_off,
                                                                        const uint64_t* __restrict__ chunk_len,
+                                                                       const uint8_t* __restrict__ lane_map,
                                                                        uint8_t* __restrict__ payload) {
     __shared__ uint32_t words[32];
     const uint32_t tid = threadIdx.x;
@@ -56,7 +57,8 @@ __global__ __launch_bounds__(kIlvThreads) void lanes_interleave_kernel(const uin
     for (uint32_t t = 0; t < 32; ++t) {
         const uint32_t wt = words[t];
         const uint64_t len = col_len[c * 32 + t];
-        const uint8_t* src = work + (c * 32 + t) * lane_slot;
+        // slot t of the chunk holds lane_map[t]'s stream (identity without a map)
+        const uint8_t* src = work + (c * 32 + (lane_map ? lane_map[c * 32 + t] : t)) * lane_slot;
         for (uint32_t r = tid; r < wt; r += kIlvThreads) {
             uint32_t pos = 0;
             for (uint32_t u = 0; u < 32; ++u) {

@@ -37,6 +37,10 @@ int rlev2_compress_bound(uint64_t in_bytes, uint64_t chunk, uint32_t eb, uint32_
                          uint64_t* work_bytes, uint64_t* payload_cap);
 int rlev2_compress(const void* in, uint64_t in_bytes, uint64_t chunk, uint32_t eb, uint32_t ru, void* work,
                    void* payload, uint64_t* col_len, uint64_t* blk_off, uint64_t* payload_bytes, hipStream_t s);
+int lzss_compress_bound(uint64_t in_bytes, uint64_t chunk, uint64_t* n_chunks, uint64_t* work_bytes,
+                        uint64_t* payload_cap);
+int lzss_compress(const void* in, uint64_t in_bytes, uint64_t chunk, void* work, void* payload, uint64_t* col_len,
+                  uint8_t* col_map, uint64_t* blk_off, hipStream_t s);
 int brle_compress_bound(uint64_t in_bytes, uint64_t chunk, uint64_t* n_chunks, uint64_t* work_bytes,
                         uint64_t* data_cap);
 int brle_compress(const void* in, uint64_t in_bytes, uint64_t chunk, void* work, void* data, uint64_t* ptr,

@@ -279,7 +279,7 @@ int rlev1_compress(const void* in, uint64_t in_bytes, uint64_t chunk, uint32_t e
     hipLaunchKernelGGL(lanes_scan_kernel, dim3(1), dim3(kScanThreads), 0, s, chunk_len, n, blk_off, nullptr, ~0ull);
     if ((e = hipGetLastError()) != hipSuccess) return record_hip(e, "lanes_scan_kernel launch");
     hipLaunchKernelGGL(lanes_interleave_kernel, dim3((uint32_t)n), dim3(kIlvThreads), 0, s,
-                       static_cast<const uint8_t*>(work), slot, col_len, blk_off, chunk_len,
+                       static_cast<const uint8_t*>(work), slot, col_len, blk_off, chunk_len, nullptr,
                        static_cast<uint8_t*>(payload));
     return record_hip(hipGetLastError(), "lanes_interleave_kernel launch");
 }

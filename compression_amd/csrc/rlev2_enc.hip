@@ -446,7 +446,7 @@ int rlev2_compress(const void* in, uint64_t in_bytes, uint64_t chunk, uint32_t e
                        in_bytes);
     if ((e = hipGetLastError()) != hipSuccess) return record_hip(e, "lanes_scan_kernel launch");
     hipLaunchKernelGGL(lanes_interleave_kernel, dim3((uint32_t)n), dim3(kIlvThreads), 0, s,
-                       static_cast<const uint8_t*>(work), slot, col_len, blk_off, chunk_len,
+                       static_cast<const uint8_t*>(work), slot, col_len, blk_off, chunk_len, nullptr,
                        static_cast<uint8_t*>(payload));
     return record_hip(hipGetLastError(), "lanes_interleave_kernel launch");
 }

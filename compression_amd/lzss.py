@@ -58,3 +58,39 @@ def decompress_gpu(payload: bytes, metadata: list[bytes]) -> bytes:
     if rc != 0:
         raise MdzError(rc, "mdz_lzss_decompress_host (corrupt chunk)")
     return data
+
+
+def compress_bound(in_bytes: int, chunk_bytes: int) -> tuple[int, int, int]:
+    """(n_chunks, work_bytes, payload_cap) for ``compress``."""
+    n, w, p = C.c_uint64(), C.c_uint64(), C.c_uint64()
+    check(lib().mdz_lzss_compress_bound(in_bytes, chunk_bytes, C.byref(n), C.byref(w), C.byref(p)),
+          "mdz_lzss_compress_bound")
+    return n.value, w.value, p.value
+
+
+def compress(src, in_bytes: int, chunk_bytes: int, work, payload, col_len, col_map, blk_off, stream=None) -> None:
+    """Device-resident encode (lzss::compress_gpu): ``src`` -> ``payload``,
+    sorted ``col_len`` (u64[32n]), ``col_map`` (u8[32n]) and ``blk_off`` (u64[n+1])."""
+    check(lib().mdz_lzss_compress(as_ptr(src), in_bytes, chunk_bytes, as_ptr(work), as_ptr(payload),
+                                  as_ptr(col_len), as_ptr(col_map), as_ptr(blk_off), as_ptr(stream)),
+          "mdz_lzss_compress")
+
+
+def compress_gpu(data: bytes) -> tuple[bytes, list[bytes]]:
+    """lzss::compress_gpu: (payload, [chunk, blk_off, col_len, col_map] blobs)."""
+    src = np.frombuffer(data, np.uint8)
+    out, md, lens = C.c_void_p(), C.c_void_p(), C.c_void_p()
+    n, num = C.c_uint64(), C.c_uint64()
+    check(lib().mdz_lzss_compress_gpu(src.ctypes.data, src.size, C.byref(out), C.byref(n), C.byref(md),
+                                      C.byref(lens), C.byref(num)), "mdz_lzss_compress_gpu")
+    payload = C.string_at(out.value, n.value) if n.value else b""
+    lib().mdz_host_free(out)
+    blobs_p = C.cast(md, C.POINTER(C.c_void_p))
+    lens_p = C.cast(lens, C.POINTER(C.c_uint64))
+    blobs = []
+    for k in range(num.value):
+        blobs.append(C.string_at(blobs_p[k], lens_p[k]))
+        lib().mdz_host_free(C.c_void_p(blobs_p[k]))
+    lib().mdz_host_free(md)
+    lib().mdz_host_free(lens)
+    return payload, blobs

@@ -265,6 +265,24 @@ int mdz_lzss_decompress_host(const uint8_t* in, uint64_t in_n_bytes,
                              const uint8_t* const* in_metadata, const uint64_t* in_metadata_lens,
                              uint64_t in_metadata_num, uint8_t** out, uint64_t* out_n_bytes);
 
+/* LZSS encode -- replaces lzss::compress_gpu (reference include/lzss/
+ * lzss_gpu2.h:1952-2123: find_match :70-109, lane encoder :332-494,511-707,
+ * lane sort :1063-1085, interleave :497-509).  payload, col_len (sorted),
+ * col_map and blk_off equal that encoder's byte for byte.  Whole chunks only
+ * (in_bytes / chunk_bytes of them; a trailing partial chunk is dropped, as
+ * the reference encoder drops it); chunk_bytes a multiple of 128, at most
+ * 1 MiB.  work / payload sized by mdz_lzss_compress_bound. */
+int mdz_lzss_compress_bound(uint64_t in_bytes, uint64_t chunk_bytes, uint64_t* n_chunks, uint64_t* work_bytes,
+                            uint64_t* payload_cap);
+int mdz_lzss_compress(const void* in, uint64_t in_bytes, uint64_t chunk_bytes, void* work, void* payload,
+                      uint64_t* col_len, uint8_t* col_map, uint64_t* blk_off, void* stream);
+/* lzss::compress_gpu's host form (128 KiB chunks): *out = payload, and the
+ * metadata blobs [u64 chunk][blk_off][col_len][col_map] with their lengths,
+ * the shape mdz_lzss_decompress_host takes.  Free *out, each blob, and the
+ * two arrays with mdz_host_free. */
+int mdz_lzss_compress_gpu(const uint8_t* in, uint64_t in_n_bytes, uint8_t** out, uint64_t* out_n_bytes,
+                          uint8_t*** out_metadata, uint64_t** out_metadata_lens, uint64_t* out_metadata_num);
+
 /* Splits a .meta side file image [u64 num][u64 len[num]][blob...] into blob
  * pointers (into `meta`) and lengths -- the reader in src/lzss/lzss_gpu.cu:
  * 109-133 (format: include/utils.h:46-118).  *num is in/out: capacity of
