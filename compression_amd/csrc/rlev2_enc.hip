@@ -9,9 +9,10 @@
 //
 // One wave per chunk; lane t < 32 encodes lane stream t, whose q-th value is
 // chunk element (q / RU) * 32 * RU + t * RU + q % RU, read straight from the
-// input.  A lane's pending values and deltas (up to 128 each) live in LDS
-// ([i][lane], 64 KiB per wave); the patched-base writer builds its patch
-// list on the fly in a second pass instead of storing it.  The scan and the
+// input.  A lane's pending values (up to 128) live in LDS ([i][lane], 32 KiB
+// per wave); deltas and patched-base offsets are recomputed from them, and
+// the patched-base writer builds its patch list in a second pass instead of
+// storing it.  The scan and the
 // interleave are the shared lane-encoder kernels (lanes_enc_dev.h).
 #include "lanes_enc_dev.h"
 #include "mdz_device.h"
@@ -24,7 +25,6 @@ constexpr uint32_t kMinRepeat = 3, kMaxLit = 128, kMaxShortRepeat = 10;
 
 struct V2EncLds {
     uint64_t lit[kMaxLit][32];
-    uint64_t delta[kMaxLit][32];
 };
 
 // The 5-bit width code of a bit width, and the widths each writer rounds to.
@@ -69,7 +69,10 @@ struct LaneEnc {
     bool fixed_delta = false;
 
     MDZ_DEV uint64_t& lit(uint32_t i) { return S.lit[i][t]; }
-    MDZ_DEV uint64_t& dl(uint32_t i) { return S.delta[i][t]; }
+    // The reference keeps a delta array; every delta it writes out equals
+    // lit(i + 1) - lit(i) (runs of equal values have delta 0), so it is
+    // recomputed instead of stored.
+    MDZ_DEV uint64_t dl(uint32_t i) { return lit(i + 1) - lit(i); }
     MDZ_DEV void put(uint32_t b) { out[pos++] = (uint8_t)b; }
 
     // Big-endian byte-aligned fields (widths >= 8) or MSB-first sub-byte fields.
@@ -189,15 +192,15 @@ struct LaneEnc {
         put(0xC0 | (code << 1) | ((len & 0x100) >> 8));
         put(len & 0xFF);
         varint(lit(0));
-        const int64_t d0 = (int64_t)dl(0);
+        const int64_t d0 = n > 1 ? (int64_t)dl(0) : 0;
         varint((uint64_t)(d0 << 1) ^ (uint64_t)(d0 >> 63));          // zigzag
         if (!fixed_delta) write_aligned([&](uint32_t i) { return dl(i + 1); }, n - 2, bits);
-        for (uint32_t i = 0; i < n; ++i) dl(i) = 0;
         n = 0;
         var = 0;
     }
-    // red(i) = lit(i) - base, kept in the delta array.
+    // red(i) = lit(i) - base, recomputed where needed.
     MDZ_DEV void patched(uint64_t base, uint32_t b95, uint32_t b100) {
+        auto red = [&](uint32_t i) { return lit(i) - base; };
         uint64_t mask = (1ull << b95) - 1;
         uint32_t pw = fixed_width(b100 - b95);
         if (pw == 64) {
@@ -208,7 +211,7 @@ struct LaneEnc {
         // pass 1: patch count and largest gap (gaps stay below 128)
         uint32_t np = 0, prev = 0, max_gap = 0;
         for (uint32_t i = 0; i < n; ++i)
-            if (dl(i) > mask) {
+            if (red(i) > mask) {
                 max_gap = max(max_gap, i - prev);
                 prev = i;
                 ++np;
@@ -224,7 +227,7 @@ struct LaneEnc {
         {
             BitWriter bw{*this};
             const uint32_t w = fixed_width(b95);
-            for (uint32_t i = 0; i < n; ++i) bw.field(dl(i) & mask, w);
+            for (uint32_t i = 0; i < n; ++i) bw.field(red(i) & mask, w);
             bw.flush();
         }
         {   // pass 2: the patch list, gap << pw | high bits, in order
@@ -232,8 +235,8 @@ struct LaneEnc {
             const uint32_t w = fixed_width(pgw + pw);
             prev = 0;
             for (uint32_t i = 0; i < n; ++i)
-                if (dl(i) > mask) {
-                    bw.field((uint64_t)(i - prev) << pw | (dl(i) >> b95), w);
+                if (red(i) > mask) {
+                    bw.field((uint64_t)(i - prev) << pw | (red(i) >> b95), w);
                     prev = i;
                 }
             bw.flush();
@@ -257,10 +260,8 @@ struct LaneEnc {
             inc = inc && l0 <= l1;
             dec = dec && l0 >= l1;
             fixed_delta = fixed_delta && cd == d_init;
-            dl(i - 1) = cd;
             max_delta = cd > max_delta ? cd : max_delta;
         }
-        dl(0) = lit(1) - lit(0);
         // the reference's signed-subtract safety test with INT8_MIN fails
         // only when min == 0 and max <= 128
         if (!(mn > 0 || mx > mn + 128)) {
@@ -281,8 +282,7 @@ struct LaneEnc {
         auto L = [&](uint32_t i) { return lit(i); };
         const uint32_t b100 = percentile(L, n, 100), b90 = percentile(L, n, 90);
         if (b100 - b90 > 1) {
-            for (uint32_t i = 0; i < n; ++i) dl(i) = lit(i) - mn;
-            auto R = [&](uint32_t i) { return dl(i); };
+            auto R = [&](uint32_t i) { return lit(i) - mn; };
             const uint32_t r95 = percentile(R, n, 95), r100 = percentile(R, n, 100);
             if (r100 != r95) {
                 patched(mn, r95, r100);
@@ -326,13 +326,8 @@ struct LaneEnc {
                     var -= kMinRepeat - 1;
                     determine();
                     prev_delta = 0;
-                    for (uint32_t i = 0; i < kMinRepeat; ++i) {
-                        lit(i) = val;
-                        dl(i) = 0;
-                    }
+                    for (uint32_t i = 0; i < kMinRepeat; ++i) lit(i) = val;
                     n = kMinRepeat;
-                } else if (fix >= kMinRepeat) {
-                    dl(0) = 0;
                 }
                 if (fix == kMaxLit) {
                     if (var == 0) var = n;          // the run's literal count is its length
