@@ -616,6 +616,86 @@ class LzssEncodeWorkload:
                           f"encoder (hash-chain matcher, same choices as the reference's scan), median of 2-3 runs"}
 
 
+class InflateVerifyWorkload:
+    """Trailer checks on the uneven Deflate batch (4,096 zlib streams, 4 KiB to
+    4 MiB): the persistent launch in longest-first order with the Adler-32
+    check (MDZ_HDR_VERIFY), timed beside the same launch without it."""
+    name = "deflate_4096_streams_verify"
+    dtype = "u8"
+
+    def __init__(self, count=4096, **_):
+        from compression_amd import deflate
+        from compression_amd import host as H
+        self.deflate = deflate
+        self.b = b = H.gen_deflate_batch(count, 0x4D445A04, 4096, 4 << 20, threads=16)
+        self.db = deflate.DeviceBatch(b.src, b.src_off, b.src_len, b.raw_len)
+        self.db.set_order(np.argsort(-b.src_len.astype(np.int64), kind="stable").astype(np.uint32))
+        self.out_bytes = int(b.raw_len.sum())
+        self.alg_bytes = int(b.src_len.sum()) + 2 * self.out_bytes + 48 * count
+        self.config = {"workload": self.name, "codec": "deflate", "streams": count, "framing": "zlib+verify",
+                       "out_bytes": self.out_bytes, "order": "compressed size, longest first"}
+
+    def step(self, stream):
+        self.deflate.inflate_sched(self.db, parse_hdr=self.deflate.HDR_ZLIB | self.deflate.HDR_VERIFY,
+                                   stream=stream)
+
+    def plain_framing_step(self, stream):       # the same launch without the check
+        self.deflate.inflate_sched(self.db, parse_hdr=self.deflate.HDR_ZLIB, stream=stream)
+
+    def kernels(self):
+        return [("inflate_kernel, zlib + Adler-32 check", self.step, self.alg_bytes),
+                ("inflate_kernel, zlib, unchecked", self.plain_framing_step, self.alg_bytes)]
+
+    def verify(self):
+        st = self.db.statuses()
+        return bool((st["status"] == 0).all() and (st["bytes_written"] == self.b.raw_len).all())
+
+    def cpu(self, threads):
+        return None
+
+
+class CopyUncompressedWorkload:
+    """cuDF's stored-stream copy: 4,096 streams of 4 KiB to 1 MiB (log-uniform,
+    about 1 GiB), source offsets 1 byte off the destination's alignment in a
+    quarter of them; one launch of mdz_copy_uncompressed."""
+    name = "copy_uncompressed_4096_streams"
+    dtype = "u8"
+
+    def __init__(self, count=4096, **_):
+        from compression_amd import deflate
+        self.deflate = deflate
+        rng = np.random.default_rng(0x4D445A06)
+        sizes = np.exp(rng.uniform(np.log(4096), np.log(1 << 20), count)).astype(np.int64)
+        skew = np.where(rng.random(count) < 0.25, 1, 0)
+        offs = np.zeros(count, np.int64)
+        pos = 0
+        for i in range(count):
+            pos += (-pos) % 16 + int(skew[i])
+            offs[i] = pos
+            pos += int(sizes[i])
+        src = rng.integers(0, 256, pos + 16).astype(np.uint8)
+        self.db = deflate.DeviceBatch(src, offs, sizes, sizes)
+        self.src, self.offs, self.sizes = src, offs, sizes
+        self.out_bytes = int(sizes.sum())
+        self.alg_bytes = 2 * self.out_bytes + 32 * count
+        self.config = {"workload": self.name, "streams": count, "bytes": self.out_bytes,
+                       "misaligned_streams": int(skew.sum())}
+
+    def step(self, stream):
+        self.deflate.copy_uncompressed(self.db, stream=stream)
+
+    def verify(self):
+        out = self.db.outputs()
+        for i in range(0, len(self.sizes), 97):
+            o, n = int(self.db.dst_off[i]), int(self.sizes[i])
+            if out[o:o + n].tobytes() != self.src[self.offs[i]:self.offs[i] + n].tobytes():
+                return False
+        return True
+
+    def cpu(self, threads):
+        return None
+
+
 class FileWorkload:
     """The RLE v2 column of the rlev2 workload (1 GiB int64, 128 KiB chunks)
     decoded from a container file: each step reads the payload from the file
@@ -674,7 +754,8 @@ WORKLOADS = {"config2": Config2Workload,
              "brle": BrleWorkload, "lzss": LzssWorkload, "deflate": DeflateWorkload,
              "copy": CopyWorkload, "file": FileWorkload, "brle-encode": BrleEncodeWorkload,
              "rlev1-encode": RleV1EncodeWorkload, "rlev2-encode": RleV2EncodeWorkload,
-             "lzss-encode": LzssEncodeWorkload}
+             "lzss-encode": LzssEncodeWorkload, "inflate-verify": InflateVerifyWorkload,
+             "copy-uncompressed": CopyUncompressedWorkload}
 DEFAULT = "config2"
 
 

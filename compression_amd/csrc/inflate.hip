@@ -28,9 +28,9 @@
 // code-set completeness, repeat codes, missing end-of-block, invalid codes,
 // distance too far back) with cuDF's status codes.  parse_hdr selects the
 // framing: raw Deflate, zlib (2-byte header checked) or gzip (cuDF's header
-// rules, 8-byte footer excluded from the Deflate data); with MDZ_HDR_VERIFY
-// the wave then checks the trailer (Adler-32, or CRC-32 and ISIZE) against
-// the output it wrote.
+// rules, applied by a prepass kernel; the 8-byte footer is excluded from the
+// Deflate data); with MDZ_HDR_VERIFY a kernel after the decode checks each
+// trailer (Adler-32, or CRC-32 and ISIZE) against the output written.
 #include "deflate_hdr_dev.h"
 #include "mdz_device.h"
 #include "mdz_internal.h"
@@ -259,6 +259,7 @@ constexpr CrcTables make_crc_tables() {
 
 __constant__ CrcTables kCrcT = make_crc_tables();
 
+
 // CRC-32 (crc) or Adler-32 of p[0, n); every lane returns it.
 MDZ_DEV uint32_t stream_checksum(const uint8_t* p, uint64_t n, bool crc) {
     const uint32_t l = lane_id();
@@ -362,15 +363,16 @@ __global__ __launch_bounds__(64) void inflate_kernel(const mdz_inflate_in_t* __r
         r.n = in.src_bytes;
         int st = MDZ_ST_OK;
         if (hdr == MDZ_HDR_GZIP) {
-            // as cuDF: the footer is cut off first, and a rejected header
+            // the gzip prepass left the header length (or -1) in the status
+            // slot; as cuDF, the footer is cut off and a rejected header
             // reports -1 with nothing written
-            const int64_t h = gzip_header_len(r.src, r.n);
+            const mdz_inflate_status_t pre = outs[si];
             r.n = r.n >= 8 ? r.n - 8 : 0;
-            if (h < 0) {
+            if (pre.status != MDZ_ST_OK) {
                 st = MDZ_ST_BAD_BLOCK_TYPE;
             } else {
-                r.src += h;
-                r.n -= (uint64_t)h;
+                r.src += pre.bytes_written;
+                r.n -= pre.bytes_written;
             }
         }
         uint64_t pos = 0, flushed = 0;             // output produced / flushed to dst
@@ -623,25 +625,6 @@ __global__ __launch_bounds__(64) void inflate_kernel(const mdz_inflate_in_t* __r
             }
         }
         flush(pos, true);
-        if ((parse_hdr & MDZ_HDR_VERIFY) && hdr && st == MDZ_ST_OK) {
-            __threadfence();                       // the wave reads back what it stored
-            const uint8_t* s0 = reinterpret_cast<const uint8_t*>(in.src);
-            if (hdr == MDZ_HDR_ZLIB) {
-                const uint64_t tb = (r.cbits + 7) / 8;       // Adler-32, big-endian, after the last block
-                if (tb + 4 > in.src_bytes) {
-                    st = MDZ_ST_INPUT_OVERRUN;
-                } else {
-                    const uint32_t want = ((uint32_t)s0[tb] << 24) | ((uint32_t)s0[tb + 1] << 16) |
-                                          ((uint32_t)s0[tb + 2] << 8) | s0[tb + 3];
-                    if (stream_checksum(dst, pos, false) != want) st = MDZ_ST_BAD_CHECKSUM;
-                }
-            } else {                                          // CRC-32 then ISIZE, little-endian
-                const uint8_t* f = s0 + in.src_bytes - 8;
-                const uint32_t crc = f[0] | ((uint32_t)f[1] << 8) | ((uint32_t)f[2] << 16) | ((uint32_t)f[3] << 24);
-                const uint32_t isz = f[4] | ((uint32_t)f[5] << 8) | ((uint32_t)f[6] << 16) | ((uint32_t)f[7] << 24);
-                if (isz != (uint32_t)pos || stream_checksum(dst, pos, true) != crc) st = MDZ_ST_BAD_CHECKSUM;
-            }
-        }
         if (l == 0) {
             mdz_inflate_status_t o;
             o.bytes_written = pos;
@@ -657,7 +640,9 @@ __global__ __launch_bounds__(64) void inflate_kernel(const mdz_inflate_in_t* __r
                 cost[si] = c;
             }
             o.status = (uint32_t)st;
-            o.reserved = 0;
+            // Deflate input consumed through the last block (from the start
+            // of the Deflate data), for the trailer check; saturates
+            o.reserved = (uint32_t)min<uint64_t>((r.cbits + 7) / 8, 0xFFFFFFFFull);
             outs[si] = o;
         }
         if (!cursor) return;
@@ -666,6 +651,55 @@ __global__ __launch_bounds__(64) void inflate_kernel(const mdz_inflate_in_t* __r
 }
 
 __global__ void reset_cursor_kernel(uint32_t* c) { *c = 0; }
+
+// gzip framing, before the decode: each stream's header length by cuDF's
+// rules, left in its status slot (bytes_written = length, status = 0, or
+// status = -1 for a rejected header) for inflate_kernel to read.  Kept out of
+// the decode kernel: any framing code there slowed its symbol loop.
+__global__ __launch_bounds__(256) void gzip_prepass_kernel(const mdz_inflate_in_t* __restrict__ ins,
+                                                           mdz_inflate_status_t* __restrict__ outs, uint32_t count) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    const mdz_inflate_in_t in = ins[i];
+    const int64_t h = gzip_header_len(reinterpret_cast<const uint8_t*>(in.src), in.src_bytes);
+    mdz_inflate_status_t o;
+    o.bytes_written = h < 0 ? 0 : (uint64_t)h;
+    o.status = h < 0 ? (uint32_t)MDZ_ST_BAD_BLOCK_TYPE : (uint32_t)MDZ_ST_OK;
+    o.reserved = 0;
+    outs[i] = o;
+}
+
+// Trailer check, after the decode, one wave per stream that decoded with
+// status 0: the zlib Adler-32 (big-endian, right after the Deflate data,
+// whose length inflate_kernel left in `reserved`) or the gzip CRC-32 and
+// ISIZE (the member's last 8 bytes) against the output written.
+__global__ __launch_bounds__(64) void trailer_check_kernel(const mdz_inflate_in_t* __restrict__ ins,
+                                                           mdz_inflate_status_t* __restrict__ outs, int hdr) {
+    const uint32_t si = blockIdx.x;
+    const mdz_inflate_status_t o = outs[si];
+    if (o.status != (uint32_t)MDZ_ST_OK) return;
+    const mdz_inflate_in_t in = ins[si];
+    const uint8_t* s0 = reinterpret_cast<const uint8_t*>(in.src);
+    const uint8_t* dst = reinterpret_cast<const uint8_t*>(in.dst);
+    int st = MDZ_ST_OK;
+    if (hdr == MDZ_HDR_ZLIB) {
+        const uint64_t tb = (uint64_t)o.reserved;   // the 2-byte header counts in the Deflate reader
+        if (o.reserved == 0xFFFFFFFFu || tb + 4 > in.src_bytes) {
+            st = MDZ_ST_INPUT_OVERRUN;
+        } else {
+            const uint32_t want = ((uint32_t)s0[tb] << 24) | ((uint32_t)s0[tb + 1] << 16) |
+                                  ((uint32_t)s0[tb + 2] << 8) | s0[tb + 3];
+            if (stream_checksum(dst, o.bytes_written, false) != want) st = MDZ_ST_BAD_CHECKSUM;
+        }
+    } else {
+        const uint8_t* f = s0 + in.src_bytes - 8;
+        const uint32_t crc = f[0] | ((uint32_t)f[1] << 8) | ((uint32_t)f[2] << 16) | ((uint32_t)f[3] << 24);
+        const uint32_t isz = f[4] | ((uint32_t)f[5] << 8) | ((uint32_t)f[6] << 16) | ((uint32_t)f[7] << 24);
+        if (isz != (uint32_t)o.bytes_written || stream_checksum(dst, o.bytes_written, true) != crc)
+            st = MDZ_ST_BAD_CHECKSUM;
+    }
+    if (lane_id() == 0 && st != MDZ_ST_OK) outs[si].status = (uint32_t)st;
+}
 
 // Stored streams: the first min(src_bytes, dst_bytes) bytes of each stream
 // go to its output slot (cuDF's copy_uncompressed_kernel, reference
@@ -747,9 +781,19 @@ int inflate_batch(const mdz_inflate_in_t* in, mdz_inflate_status_t* out, uint32_
         hipLaunchKernelGGL(reset_cursor_kernel, dim3(1), dim3(1), 0, s, cursor);
         if ((e = hipGetLastError()) != hipSuccess) return record_hip(e, "inflate cursor reset");
     }
+    const int hdr = parse_hdr & 3;
+    if (hdr == MDZ_HDR_GZIP) {
+        hipLaunchKernelGGL(gzip_prepass_kernel, dim3((count + 255) / 256), dim3(256), 0, s, in, out, count);
+        if ((e = hipGetLastError()) != hipSuccess) return record_hip(e, "gzip_prepass_kernel launch");
+    }
     hipLaunchKernelGGL(inflate_kernel, dim3(grid), dim3(64), smem, s, in, out, order, count,
                        parse_hdr, persistent ? cursor : nullptr, cost);
-    return record_hip(hipGetLastError(), "inflate_kernel launch");
+    if ((e = hipGetLastError()) != hipSuccess) return record_hip(e, "inflate_kernel launch");
+    if ((parse_hdr & MDZ_HDR_VERIFY) && hdr) {
+        hipLaunchKernelGGL(trailer_check_kernel, dim3(count), dim3(64), 0, s, in, out, hdr);
+        if ((e = hipGetLastError()) != hipSuccess) return record_hip(e, "trailer_check_kernel launch");
+    }
+    return MDZ_OK;
 }
 
 int copy_uncompressed(const mdz_inflate_in_t* in, uint32_t count, hipStream_t s) {

@@ -305,7 +305,9 @@ typedef struct {
 typedef struct {
     uint64_t bytes_written;
     uint32_t status;       /* MDZ_ST_* as uint32 (negative codes wrap) */
-    uint32_t reserved;
+    uint32_t reserved;     /* input bytes consumed through the last block (from the
+                              stream's start; gzip: from the Deflate data's start),
+                              saturating at 2^32 - 1.  cuDF leaves a debug count here */
 } mdz_inflate_status_t;
 
 /* parse_hdr: the framing around each stream's Deflate data.
