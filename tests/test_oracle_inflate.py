@@ -14,7 +14,7 @@ FIX = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "deflate_
 
 
 def test_kats_c10_c12():
-    # hand-checked against RFC 1950/1951 in the survey; the compressor bytes
+    # hand-checked against RFC 1950/1951; the compressor bytes
     # below are zlib 1.2.11's
     kat = {"C10_stored_abc": "78010103 00fcff616263024d0127",
            "C11_fixed_abcabcabc": "78014b4c4a4e042300113d0373",

@@ -102,7 +102,7 @@ def py_decode(s: bytes, n: int, bits=64, modes=None):
     return out, p
 
 
-# ---- known-answer vectors (hand-derived) ----
+# ---- known-answer vectors (hand-derived from the RLE v2 header grammar) ----
 @pytest.mark.parametrize("vals,expect", [
     ([7, 7, 7, 7, 7], "0207"),                                          # SHORT_REPEAT
     (list(range(0, 21, 2)), "4e0a00020406080a0c0e101214"),             # DIRECT (safe-subtract quirk)
