@@ -656,7 +656,7 @@ class InflateVerifyWorkload:
 
 class CopyUncompressedWorkload:
     """cuDF's stored-stream copy: 4,096 streams of 4 KiB to 1 MiB (log-uniform,
-    about 1 GiB), source offsets 1 byte off the destination's alignment in a
+    about 744 MB), source offsets 1 byte off the destination's alignment in a
     quarter of them; one launch of mdz_copy_uncompressed."""
     name = "copy_uncompressed_4096_streams"
     dtype = "u8"

@@ -436,7 +436,9 @@ int mdz_file_raw_len(mdz_file_t file, uint64_t* raw_len);
  * buffers; they stay with the handle for later calls. */
 int mdz_file_decode(mdz_file_t file, void* dst, uint64_t dst_bytes, void* status,
                     uint64_t stage_bytes, void* stream);
-/* Waits for the handle's copies, frees its device and host memory, unmaps. */
+/* Waits for the handle's copies, frees its device and host memory, unmaps.
+ * A handle's staging buffers and device copies are its own: calls on one
+ * handle must not overlap (use one handle per thread). */
 int mdz_file_close(mdz_file_t file);
 
 /* ================================================ learned cost model ===== *
