@@ -33,7 +33,17 @@ struct LzEncLds {
 struct LaneSrc {
     const uint8_t* chunk;
     uint32_t t;
+    uint32_t nd;                                 // lane dwords (lane bytes / 4)
     MDZ_DEV uint32_t at(uint32_t i) const { return chunk[128u * (i >> 2) + 4u * t + (i & 3u)]; }
+    // lane bytes i..i+3 (zero past the lane's end): two aligned dwords of the
+    // lane's rows, shifted together
+    MDZ_DEV uint32_t dword(uint32_t k) const {
+        return k < nd ? *reinterpret_cast<const uint32_t*>(chunk + 128u * k + 4u * t) : 0u;
+    }
+    MDZ_DEV uint32_t word_at(uint32_t i) const {
+        const uint32_t k = i >> 2, sh = i & 3u, lo = dword(k);
+        return sh ? alignbyte(dword(k + 1), lo, sh) : lo;
+    }
     MDZ_DEV uint32_t hash3(uint32_t i) const {
         const uint32_t v = at(i) | (at(i + 1) << 8) | (at(i + 2) << 16);
         return (v * 2654435761u) >> (32 - kHashBits);
@@ -61,8 +71,16 @@ MDZ_DEV uint64_t encode_lzss_lane(const LaneSrc& s, uint32_t n, uint16_t* head, 
             for (uint32_t q = head[s.hash3(cur)]; q != kNoPos && q >= lo; q = prev[q]) {
                 uint32_t cap = min(cur - q, look);
                 cap = cap < kMaxLen ? cap : kMaxLen;
-                uint32_t f = 0;
-                while (f < cap && s.at(q + f) == s.at(cur + f)) ++f;
+                uint32_t f = 0;                  // common prefix, four bytes at a time
+                while (f < cap) {
+                    const uint32_t x = s.word_at(q + f) ^ s.word_at(cur + f);
+                    if (x) {
+                        f += (uint32_t)__builtin_ctz(x) >> 3;
+                        break;
+                    }
+                    f += 4;
+                }
+                f = f < cap ? f : cap;
                 if (f >= best) {
                     best = f;
                     best_q = q;
@@ -105,7 +123,7 @@ __global__ __launch_bounds__(kWave) void lzss_encode_kernel(const uint8_t* __res
     const uint64_t c = blockIdx.x;
     const uint32_t lb = (uint32_t)(chunk / 32);
     if (l < 32) {
-        const LaneSrc src{in + c * chunk, l};
+        const LaneSrc src{in + c * chunk, l, lb / 4};
         S.len[l] = encode_lzss_lane(src, lb, S.head[l], links + (c * 32 + l) * lb,
                                     work + (c * 32 + l) * lane_slot);
     }
@@ -153,6 +171,7 @@ int lzss_compress(const void* in, uint64_t in_bytes, uint64_t chunk, void* work,
     int rc = lzss_compress_bound(in_bytes, chunk, &n, nullptr, nullptr);
     if (rc) return rc;
     if (!in || !work || !payload || !col_len || !col_map || !blk_off || n > (1ull << 24)) return MDZ_ERR_INVALID;
+    if ((uintptr_t)in & 3) return MDZ_ERR_INVALID;   // lanes are read a dword at a time
     const uint64_t slot = lz_lane_slot(chunk);
     uint8_t* w = static_cast<uint8_t*>(work);
     uint16_t* links = reinterpret_cast<uint16_t*>(w + n * 32 * slot);

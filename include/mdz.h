@@ -271,7 +271,7 @@ int mdz_lzss_decompress_host(const uint8_t* in, uint64_t in_n_bytes,
  * col_map and blk_off equal that encoder's byte for byte.  Whole chunks only
  * (in_bytes / chunk_bytes of them; a trailing partial chunk is dropped, as
  * the reference encoder drops it); chunk_bytes a multiple of 128, at most
- * 1 MiB.  work / payload sized by mdz_lzss_compress_bound. */
+ * 1 MiB; in 4-byte aligned.  work / payload sized by mdz_lzss_compress_bound. */
 int mdz_lzss_compress_bound(uint64_t in_bytes, uint64_t chunk_bytes, uint64_t* n_chunks, uint64_t* work_bytes,
                             uint64_t* payload_cap);
 int mdz_lzss_compress(const void* in, uint64_t in_bytes, uint64_t chunk_bytes, void* work, void* payload,
