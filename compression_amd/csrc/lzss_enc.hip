@@ -44,8 +44,8 @@ struct LaneSrc {
         const uint32_t k = i >> 2, sh = i & 3u, lo = dword(k);
         return sh ? alignbyte(dword(k + 1), lo, sh) : lo;
     }
-    MDZ_DEV uint32_t hash3(uint32_t i) const {
-        const uint32_t v = at(i) | (at(i + 1) << 8) | (at(i + 2) << 16);
+    MDZ_DEV uint32_t hash3(uint32_t i) const {          // lane bytes i, i+1, i+2
+        const uint32_t v = word_at(i) & 0xFFFFFFu;
         return (v * 2654435761u) >> (32 - kHashBits);
     }
 };
