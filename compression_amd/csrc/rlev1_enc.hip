@@ -21,15 +21,28 @@
 namespace mdz {
 namespace {
 
+// eb little-endian bytes at `at` (bytes past the input read as 0): one load
+// of the element's width when it is whole and aligned, else bytewise.
+MDZ_DEV uint64_t load_elem(const uint8_t* in, uint64_t in_bytes, uint64_t at, uint32_t eb) {
+    const uint8_t* p = in + at;
+    if (at + eb <= in_bytes && ((uintptr_t)p & (eb - 1)) == 0) {
+        if (eb == 8) return *reinterpret_cast<const uint64_t*>(p);
+        if (eb == 4) return *reinterpret_cast<const uint32_t*>(p);
+        if (eb == 2) return *reinterpret_cast<const uint16_t*>(p);
+        return *p;
+    }
+    uint64_t v = 0;
+    for (uint32_t b = 0; b < eb; ++b)
+        if (at + b < in_bytes) v |= (uint64_t)in[at + b] << (8 * b);
+    return v;
+}
+
 // Value i of lane t: lane byte i*eb sits in line t + 32*(i*eb / line).
 MDZ_DEV uint64_t lane_value(const uint8_t* in, uint64_t in_bytes, uint64_t cbase, uint32_t line, uint32_t t,
                             uint64_t i, uint32_t eb) {
     const uint64_t o = i * eb, k = o / line, j = o - k * line;
     const uint64_t at = cbase + (t + 32 * k) * line + j;
-    uint64_t v = 0;
-    for (uint32_t b = 0; b < eb; ++b)
-        if (at + b < in_bytes) v |= (uint64_t)in[at + b] << (8 * b);
-    return v;
+    return load_elem(in, in_bytes, at, eb);
 }
 
 // `read - prev` and `prev + delta == read` in the element type's arithmetic

@@ -392,9 +392,16 @@ __global__ __launch_bounds__(kWave) void rlev2_encode_kernel(const uint8_t* __re
         auto val_at = [&](uint64_t q) -> uint64_t {
             const uint64_t el = (q / ru) * 32 * ru + (uint64_t)l * ru + q % ru;
             const uint64_t off = base + el * eb;
+            if (off + eb > in_bytes) return 0;           // a partial last value reads as 0
+            const uint8_t* p = in + off;
+            if (((uintptr_t)p & (eb - 1)) == 0) {         // one load of the element's width
+                if (eb == 8) return *reinterpret_cast<const uint64_t*>(p);
+                if (eb == 4) return *reinterpret_cast<const uint32_t*>(p);
+                if (eb == 2) return *reinterpret_cast<const uint16_t*>(p);
+                return *p;
+            }
             uint64_t v = 0;
-            if (off + eb <= in_bytes)
-                for (uint32_t b = 0; b < eb; ++b) v |= (uint64_t)in[off + b] << (8 * b);
+            for (uint32_t b = 0; b < eb; ++b) v |= (uint64_t)p[b] << (8 * b);
             return v;
         };
         LaneEnc e{S, l, work + (c * 32 + l) * lane_slot};
