@@ -35,7 +35,7 @@ TRAFFIC: dict = {
     "batch_kernel<8> (rle_v2+byte_rle)": 1792273658,   # profiles/config2_rlev2_brle_batched.md
     "rlev2_i64_1GiB_plain": 1430079078,
     "brle_u8_256MiB_plain": 547122419,
-    "rlev1_u32_64MiB_plain": 106355686,                # profiles/rlev1_u32_64MiB_plain.md
+    "rlev1_u32_64MiB_plain": 105750425,                # profiles/rlev1_u32_64MiB_plain.md
     "lzss_markov_1GiB_64KiB": 1861163514,              # profiles/lzss_markov_1GiB_64KiB.md
 }
 
