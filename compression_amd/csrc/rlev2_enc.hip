@@ -23,8 +23,11 @@ namespace {
 
 constexpr uint32_t kMinRepeat = 3, kMaxLit = 128, kMaxShortRepeat = 10;
 
+// Pending values; u32 storage when elements are 4 bytes or less (16 KiB per
+// wave instead of 32), arithmetic always in 64 bits.
+template <class LT>
 struct V2EncLds {
-    uint64_t lit[kMaxLit][32];
+    LT lit[kMaxLit][32];
 };
 
 // The 5-bit width code of a bit width, and the widths each writer rounds to.
@@ -60,19 +63,20 @@ MDZ_DEV uint32_t value_width(uint64_t v) {               // fixed width of the b
     return fixed_width(v ? 64 - __builtin_clzll(v) : 0);
 }
 
+template <class LT>
 struct LaneEnc {
-    V2EncLds& S;
+    V2EncLds<LT>& S;
     uint32_t t;
     uint8_t* out;
     uint64_t pos = 0;
     uint32_t n = 0, fix = 0, var = 0, delta_bits = 0;
     bool fixed_delta = false;
 
-    MDZ_DEV uint64_t& lit(uint32_t i) { return S.lit[i][t]; }
+    MDZ_DEV LT& lit(uint32_t i) { return S.lit[i][t]; }
     // The reference keeps a delta array; every delta it writes out equals
     // lit(i + 1) - lit(i) (runs of equal values have delta 0), so it is
     // recomputed instead of stored.
-    MDZ_DEV uint64_t dl(uint32_t i) { return lit(i + 1) - lit(i); }
+    MDZ_DEV uint64_t dl(uint32_t i) { return (uint64_t)lit(i + 1) - (uint64_t)lit(i); }
     MDZ_DEV void put(uint32_t b) { out[pos++] = (uint8_t)b; }
 
     // Big-endian byte-aligned fields (widths >= 8) or MSB-first sub-byte fields.
@@ -164,7 +168,7 @@ struct LaneEnc {
         var = 0;
     }
     MDZ_DEV void direct() {
-        auto L = [&](uint32_t i) { return lit(i); };
+        auto L = [&](uint32_t i) { return (uint64_t)lit(i); };
         const uint32_t bits = aligned_width(percentile(L, n, 100));
         const uint32_t len = var - 1;
         put(0x40 | (width_code(bits) << 1) | ((len & 0x100) >> 8));
@@ -200,7 +204,7 @@ struct LaneEnc {
     }
     // red(i) = lit(i) - base, recomputed where needed.
     MDZ_DEV void patched(uint64_t base, uint32_t b95, uint32_t b100) {
-        auto red = [&](uint32_t i) { return lit(i) - base; };
+        auto red = [&](uint32_t i) { return (uint64_t)lit(i) - base; };
         uint64_t mask = (1ull << b95) - 1;
         uint32_t pw = fixed_width(b100 - b95);
         if (pw == 64) {
@@ -252,7 +256,7 @@ struct LaneEnc {
         bool inc = true, dec = true;
         fixed_delta = true;
         uint64_t mn = lit(0), mx = lit(0), max_delta = 0;
-        const uint64_t d_init = lit(1) - lit(0);
+        const uint64_t d_init = (uint64_t)lit(1) - (uint64_t)lit(0);
         for (uint32_t i = 1; i < n; ++i) {
             const uint64_t l1 = lit(i), l0 = lit(i - 1), cd = l1 - l0;
             mn = l1 < mn ? l1 : mn;
@@ -279,10 +283,10 @@ struct LaneEnc {
                 return;
             }
         }
-        auto L = [&](uint32_t i) { return lit(i); };
+        auto L = [&](uint32_t i) { return (uint64_t)lit(i); };
         const uint32_t b100 = percentile(L, n, 100), b90 = percentile(L, n, 90);
         if (b100 - b90 > 1) {
-            auto R = [&](uint32_t i) { return lit(i) - mn; };
+            auto R = [&](uint32_t i) { return (uint64_t)lit(i) - mn; };
             const uint32_t r95 = percentile(R, n, 95), r100 = percentile(R, n, 100);
             if (r100 != r95) {
                 patched(mn, r95, r100);
@@ -306,8 +310,8 @@ struct LaneEnc {
             }
             if (n == 1) {
                 lit(n++) = val;
-                prev_delta = val - lit(0);
-                if (val == lit(0)) {
+                prev_delta = val - (uint64_t)lit(0);
+                if (val == (uint64_t)lit(0)) {
                     fix = 2;
                     var = 0;
                 } else {
@@ -316,7 +320,7 @@ struct LaneEnc {
                 }
                 continue;
             }
-            const uint64_t cur_delta = val - lit(n - 1);
+            const uint64_t cur_delta = val - (uint64_t)lit(n - 1);
             if (prev_delta == 0 && cur_delta == 0) {
                 lit(n++) = val;
                 if (var > 0) fix = 2;
@@ -343,7 +347,7 @@ struct LaneEnc {
                     delta();
                 }
             }
-            if (fix > 0 && fix < kMinRepeat && val != lit(n - 1)) {
+            if (fix > 0 && fix < kMinRepeat && val != (uint64_t)lit(n - 1)) {
                 var = fix;
                 fix = 0;
             }
@@ -352,8 +356,8 @@ struct LaneEnc {
                 fix = 1;
                 var = 1;
             } else {
-                prev_delta = val - lit(n - 1);
-                lit(n++) = val;
+                prev_delta = val - (uint64_t)lit(n - 1);
+                lit(n++) = (LT)val;
                 ++var;
                 if (var == kMaxLit) determine();
             }
@@ -378,12 +382,13 @@ struct LaneEnc {
     }
 };
 
+template <class LT>
 __global__ __launch_bounds__(kWave) void rlev2_encode_kernel(const uint8_t* __restrict__ in, uint64_t in_bytes,
                                                              uint64_t chunk, uint32_t eb, uint32_t ru,
                                                              uint64_t lane_slot, uint8_t* __restrict__ work,
                                                              uint64_t* __restrict__ col_len,
                                                              uint64_t* __restrict__ chunk_len) {
-    __shared__ V2EncLds S;
+    __shared__ V2EncLds<LT> S;
     const uint32_t l = lane_id();
     const uint64_t c = blockIdx.x;
     uint64_t len = 0;
@@ -404,7 +409,7 @@ __global__ __launch_bounds__(kWave) void rlev2_encode_kernel(const uint8_t* __re
             for (uint32_t b = 0; b < eb; ++b) v |= (uint64_t)p[b] << (8 * b);
             return v;
         };
-        LaneEnc e{S, l, work + (c * 32 + l) * lane_slot};
+        LaneEnc<LT> e{S, l, work + (c * 32 + l) * lane_slot};
         len = e.encode(val_at, nv);
         col_len[c * 32 + l] = len;
     }
@@ -439,8 +444,14 @@ int rlev2_compress(const void* in, uint64_t in_bytes, uint64_t chunk, uint32_t e
     if (!in || !work || !payload || !col_len || !blk_off || n > (1ull << 24)) return MDZ_ERR_INVALID;
     const uint64_t slot = v2_lane_slot(chunk, eb);
     uint64_t* chunk_len = reinterpret_cast<uint64_t*>(static_cast<uint8_t*>(work) + n * 32 * slot);
-    hipLaunchKernelGGL(rlev2_encode_kernel, dim3((uint32_t)n), dim3(kWave), 0, s, static_cast<const uint8_t*>(in),
-                       in_bytes, chunk, eb, ru, slot, static_cast<uint8_t*>(work), col_len, chunk_len);
+    if (eb <= 4)
+        hipLaunchKernelGGL(rlev2_encode_kernel<uint32_t>, dim3((uint32_t)n), dim3(kWave), 0, s,
+                           static_cast<const uint8_t*>(in), in_bytes, chunk, eb, ru, slot,
+                           static_cast<uint8_t*>(work), col_len, chunk_len);
+    else
+        hipLaunchKernelGGL(rlev2_encode_kernel<uint64_t>, dim3((uint32_t)n), dim3(kWave), 0, s,
+                           static_cast<const uint8_t*>(in), in_bytes, chunk, eb, ru, slot,
+                           static_cast<uint8_t*>(work), col_len, chunk_len);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return record_hip(e, "rlev2_encode_kernel launch");
     // blk_off[n] holds the uncompressed size; the true payload end goes to *payload_bytes
