@@ -131,6 +131,8 @@ SIGNATURES = {
     "mdz_mlp_adam": (C.c_int, [vp, vp, vp, vp, C.c_float, u32, vp]),
     "mdz_deflate_decompress_gpu": (C.c_int, [vp, C.POINTER(vp), u64, C.POINTER(u64), vp, u64, vp, u64,
                                              u64]),
+    "mdz_cudf_uncompress_host": (C.c_int, [vp, u64, C.POINTER(vp), C.POINTER(u64)]),
+    "mdz_cudf_uncompress": (C.c_int, [C.c_char_p, C.c_char_p]),
 }
 
 _lib = None

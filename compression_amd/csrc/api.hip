@@ -7,6 +7,7 @@
 #include <string.h>
 
 #include <mutex>
+#include <stdarg.h>
 #include <new>
 #include <vector>
 #include <unordered_map>
@@ -19,6 +20,14 @@ int record_hip(hipError_t e, const char* what) {
     if (e == hipSuccess) return MDZ_OK;
     snprintf(g_last_error, sizeof g_last_error, "%s: %s (%d)", what, hipGetErrorString(e), (int)e);
     return MDZ_ERR_HIP;
+}
+
+int set_error(int rc, const char* fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_last_error, sizeof g_last_error, fmt, ap);
+    va_end(ap);
+    return rc;
 }
 
 hipError_t set_dynamic_lds(const void* kernel, size_t bytes) {
@@ -80,7 +89,11 @@ int finish_host(const void* d_out, uint64_t raw, const int32_t* d_st, uint64_t n
     }
     int rc = MDZ_OK;
     for (uint64_t c = 0; c < n; ++c)
-        if (st[c] != MDZ_ST_OK) { rc = MDZ_ERR_CORRUPT; break; }
+        if (st[c] != MDZ_ST_OK) {
+            rc = mdz::set_error(MDZ_ERR_CORRUPT, "chunk %llu: status %d", (unsigned long long)c,
+                                (int)st[c]);
+            break;
+        }
     free(st);
     *out = o;
     *out_n_bytes = raw;
@@ -590,20 +603,24 @@ int mdz_deflate_decompress_gpu(const uint8_t* in, uint8_t** out, uint64_t in_n_b
     if (data_size > n * chunk_size) return MDZ_ERR_INVALID;
     for (uint64_t i = 0; i < n; ++i)
         if (off[i] > in_n_bytes || col_len_f[i] > in_n_bytes - off[i]) return MDZ_ERR_INVALID;
+    // Every stream gets a whole chunk_size slot, as the reference's reader
+    // allocates chunk_size * num_blk (zlib.h:992-996): its writer zero-pads the
+    // input to a chunk multiple before compressing (orig_compressor.py:62-66),
+    // so the last stream inflates to a full chunk and only data_size bytes of
+    // the slots are returned.
     DevBuf d_in, d_out, d_desc, d_st;
     int rc;
-    if ((rc = d_in.alloc(in_n_bytes)) || (rc = d_out.alloc(data_size)) ||
+    if ((rc = d_in.alloc(in_n_bytes)) || (rc = d_out.alloc(n * chunk_size)) ||
         (rc = d_desc.alloc(n * sizeof(mdz_inflate_in_t))) ||
         (rc = d_st.alloc(n * sizeof(mdz_inflate_status_t))))
         return rc;
     mdz_inflate_in_t* desc = (mdz_inflate_in_t*)malloc(n * sizeof(mdz_inflate_in_t) + 1);
     if (!desc) return MDZ_ERR_NOMEM;
     for (uint64_t i = 0; i < n; ++i) {
-        const uint64_t o = i * chunk_size;
         desc[i].src = (const uint8_t*)d_in.p + off[i];
         desc[i].src_bytes = col_len_f[i];
-        desc[i].dst = (uint8_t*)d_out.p + o;
-        desc[i].dst_bytes = o < data_size ? (data_size - o < chunk_size ? data_size - o : chunk_size) : 0;
+        desc[i].dst = (uint8_t*)d_out.p + i * chunk_size;
+        desc[i].dst_bytes = chunk_size;
     }
     const bool ok = hipMemcpy(d_in.p, in, in_n_bytes, hipMemcpyHostToDevice) == hipSuccess &&
                     hipMemcpy(d_desc.p, desc, n * sizeof(mdz_inflate_in_t), hipMemcpyHostToDevice) ==
@@ -622,12 +639,108 @@ int mdz_deflate_decompress_gpu(const uint8_t* in, uint8_t** out, uint64_t in_n_b
         free(st); free(o);
         return MDZ_ERR_HIP;
     }
+    // A stream is good when it decodes cleanly and covers its share of the
+    // first data_size bytes: a full chunk, or the remainder for the last one.
+    // The reference's writer warp waits for exactly chunk_size bytes per
+    // stream (zlib.h:741-743); a short stream inside data_size is corrupt.
     rc = MDZ_OK;
-    for (uint64_t i = 0; i < n; ++i)
-        if (st[i].status != 0) { rc = MDZ_ERR_CORRUPT; break; }
+    for (uint64_t i = 0; i < n; ++i) {
+        const uint64_t o = i * chunk_size;
+        const uint64_t need = o < data_size ? (data_size - o < chunk_size ? data_size - o : chunk_size) : 0;
+        if (st[i].status != 0 || st[i].bytes_written < need) {
+            rc = mdz::set_error(MDZ_ERR_CORRUPT, "deflate stream %llu: status %d, %llu of %llu bytes",
+                                (unsigned long long)i, (int)st[i].status,
+                                (unsigned long long)st[i].bytes_written, (unsigned long long)need);
+            break;
+        }
+    }
     free(st);
     *out = o;
     *out_n_bytes = data_size;
+    return rc;
+}
+
+int mdz_cudf_uncompress_host(const uint8_t* file, uint64_t file_bytes, uint8_t** out,
+                             uint64_t* out_n_bytes) {
+    if (!file || !out || !out_n_bytes || file_bytes < 16) return MDZ_ERR_INVALID;
+    uint64_t chunk, n;
+    memcpy(&chunk, file, 8);
+    memcpy(&n, file + 8, 8);
+    // [u64 chunk][u64 n][u64 size[n]] must fit, and so must the streams.
+    if (!chunk || !n || n > (file_bytes - 16) / 8 || n > 0x7fffffffull) return MDZ_ERR_INVALID;
+    const uint64_t data_off = 16 + 8 * n, data_size = file_bytes - data_off;
+    std::vector<uint64_t> sz(n);
+    memcpy(sz.data(), file + 16, 8 * n);
+    uint64_t cur = 0;
+    for (uint64_t i = 0; i < n; ++i) {
+        // each stream carries a 2-byte zlib header that the reader skips
+        if (sz[i] < 2 || sz[i] > data_size - cur) return MDZ_ERR_INVALID;
+        cur += sz[i];
+    }
+    if (n > UINT64_MAX / chunk) return MDZ_ERR_INVALID;
+    DevBuf d_in, d_out, d_desc, d_st;
+    int rc;
+    if ((rc = d_in.alloc(data_size)) || (rc = d_out.alloc(n * chunk)) ||
+        (rc = d_desc.alloc(n * sizeof(mdz_inflate_in_t))) ||
+        (rc = d_st.alloc(n * sizeof(mdz_inflate_status_t))))
+        return rc;
+    std::vector<mdz_inflate_in_t> desc(n);
+    cur = 0;
+    for (uint64_t i = 0; i < n; ++i) {
+        desc[i].src = (const uint8_t*)d_in.p + cur + 2;
+        desc[i].src_bytes = sz[i] - 2;
+        desc[i].dst = (uint8_t*)d_out.p + i * chunk;
+        desc[i].dst_bytes = chunk;
+        cur += sz[i];
+    }
+    if (hipMemcpy(d_in.p, file + data_off, data_size, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(d_desc.p, desc.data(), n * sizeof(mdz_inflate_in_t), hipMemcpyHostToDevice) !=
+            hipSuccess)
+        return MDZ_ERR_HIP;
+    rc = mdz::inflate_batch((const mdz_inflate_in_t*)d_desc.p, (mdz_inflate_status_t*)d_st.p,
+                            (uint32_t)n, MDZ_HDR_RAW, nullptr, 0, nullptr, nullptr, nullptr);
+    if (rc) return rc;
+    if (hipDeviceSynchronize() != hipSuccess) return MDZ_ERR_HIP;
+    std::vector<mdz_inflate_status_t> st(n);
+    if (hipMemcpy(st.data(), d_st.p, n * sizeof(mdz_inflate_status_t), hipMemcpyDeviceToHost) !=
+        hipSuccess)
+        return MDZ_ERR_HIP;
+    const uint64_t out_size = (n - 1) * chunk + st[n - 1].bytes_written;
+    uint8_t* o = (uint8_t*)malloc(out_size ? out_size : 1);
+    if (!o) return MDZ_ERR_NOMEM;
+    if (out_size && hipMemcpy(o, d_out.p, out_size, hipMemcpyDeviceToHost) != hipSuccess) {
+        free(o);
+        return MDZ_ERR_HIP;
+    }
+    rc = MDZ_OK;
+    for (uint64_t i = 0; i < n; ++i)
+        if (st[i].status != 0 || (i + 1 < n && st[i].bytes_written != chunk)) {
+            rc = mdz::set_error(MDZ_ERR_CORRUPT, "cuDF container stream %llu: status %d, %llu bytes",
+                                (unsigned long long)i, (int)st[i].status,
+                                (unsigned long long)st[i].bytes_written);
+            break;
+        }
+    *out = o;
+    *out_n_bytes = out_size;
+    return rc;
+}
+
+int mdz_cudf_uncompress(const char* in_file, const char* out_file) {
+    if (!in_file || !out_file) return MDZ_ERR_INVALID;
+    uint8_t* in = nullptr;
+    uint64_t n = 0;
+    int rc = read_file(in_file, &in, &n);
+    if (rc) return mdz::set_error(rc, "cannot read %s", in_file);
+    uint8_t* o = nullptr;
+    uint64_t on = 0;
+    rc = mdz_cudf_uncompress_host(in, n, &o, &on);
+    free(in);
+    if (rc != MDZ_OK && rc != MDZ_ERR_CORRUPT) return rc;
+    FILE* f = fopen(out_file, "wb");
+    const bool ok = f && (on == 0 || fwrite(o, 1, on, f) == on);
+    if (f && fclose(f) != 0) rc = mdz::set_error(MDZ_ERR_INVALID, "cannot write %s", out_file);
+    free(o);
+    if (!ok) return mdz::set_error(MDZ_ERR_INVALID, "cannot write %s", out_file);
     return rc;
 }
 

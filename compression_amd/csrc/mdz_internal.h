@@ -11,6 +11,8 @@ namespace mdz {
 
 // Records a failed HIP call for mdz_last_error(); returns MDZ_OK/MDZ_ERR_HIP.
 int record_hip(hipError_t e, const char* what);
+// Formats a message for mdz_last_error() and returns rc.
+int set_error(int rc, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
 
 // Raises a kernel's dynamic-LDS limit to at least `bytes` (gfx950 allows
 // 160 KiB per workgroup, static LDS included).  Thread-safe; each

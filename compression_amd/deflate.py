@@ -116,3 +116,21 @@ def decompress_gpu(data: bytes, col_len: np.ndarray, blk_offset: np.ndarray, chu
     if rc != 0:
         raise MdzError(rc, "mdz_deflate_decompress_gpu (corrupt stream)")
     return res
+
+
+def cudf_uncompress(data: bytes) -> bytes:
+    """The cuDF container (``[u64 chunk][u64 n][u64 size[n]][zlib streams]``,
+    ``rapids_compressor.py:222-275``) read as ``base_deflate.h:20-130`` reads
+    it: 2-byte zlib header skipped, raw inflate into chunk slots, output
+    (n-1)*chunk + the last stream's bytes."""
+    src = np.frombuffer(data, np.uint8) if len(data) else np.zeros(1, np.uint8)
+    out = C.c_void_p()
+    n = C.c_uint64()
+    rc = lib().mdz_cudf_uncompress_host(src.ctypes.data, len(data), C.byref(out), C.byref(n))
+    if rc not in (0, -5):
+        raise MdzError(rc, "mdz_cudf_uncompress_host")
+    res = C.string_at(out.value, n.value) if n.value else b""
+    lib().mdz_host_free(out)
+    if rc != 0:
+        raise MdzError(rc, "mdz_cudf_uncompress_host (corrupt stream)")
+    return res

@@ -52,7 +52,7 @@ extern "C" {
 /* --------------------------------------------------------------- version */
 /* Returns "mdz <version> gfx950". */
 const char* mdz_version(void);
-/* Text of the last HIP error an mdz call on this host thread saw ("" if none). */
+/* Text of the last error an mdz call on this host thread recorded ("" if none). */
 const char* mdz_last_error(void);
 
 /* -------------------------------------------- device plumbing (thin HIP) */
@@ -375,13 +375,34 @@ int mdz_inflate_slots(uint32_t* slots);
  * repo container (orig_compressor.py:44-125): in = concatenated zlib streams,
  * col_len_f = u64 padded stream sizes [n], blk_offset_f = [u64 data_size]
  * [u64 0][u64 cumulative padded offsets...] (n+2 entries), chunk_size = output
- * bytes per stream (the last one is shorter).  *out (data_size bytes) is
- * allocated here; free with mdz_host_free.  Stored blocks are written (the
- * reference consumes them without writing, B15). */
+ * bytes per stream.  The writer zero-pads the input to a chunk multiple, so
+ * every stream (the last included) inflates to chunk_size bytes; each gets a
+ * full chunk_size slot on the device, as the reference reader allocates
+ * chunk_size * num_blk (zlib.h:992-996), and the first data_size bytes come
+ * back.  A last stream that holds only the remainder is accepted too.
+ * *out (data_size bytes) is allocated here; free with mdz_host_free.
+ * MDZ_ERR_CORRUPT (with *out set) when a stream fails or falls short of its
+ * share of data_size; mdz_last_error() names the stream.  Stored blocks are
+ * written (the reference consumes them without writing, B15). */
 int mdz_deflate_decompress_gpu(const uint8_t* in, uint8_t** out, uint64_t in_n_bytes,
                                uint64_t* out_n_bytes, const uint64_t* col_len_f,
                                uint64_t col_n_bytes, const uint64_t* blk_offset_f,
                                uint64_t blk_n_bytes, uint64_t chunk_size);
+
+/* The cuDF container's reader, include/base_deflate.h:20-130 (`uncompress`),
+ * over a buffer: file = [u64 chunk_size][u64 n][u64 size[n]][zlib streams back
+ * to back] as rapids_compressor.py:222-275 writes it.  Each stream's 2-byte
+ * zlib header is skipped and the rest inflated raw (base_deflate.h:66-70,
+ * gpuinflate with parse_hdr 0) into a chunk_size slot.  The output is
+ * (n-1)*chunk_size + the last stream's bytes_written (:97-98).  *out is
+ * allocated here (free with mdz_host_free).  Returns MDZ_ERR_INVALID for a
+ * header or size table that does not fit the buffer, MDZ_ERR_CORRUPT (with
+ * *out set) when a stream fails or a stream before the last is short. */
+int mdz_cudf_uncompress_host(const uint8_t* file, uint64_t file_bytes, uint8_t** out,
+                             uint64_t* out_n_bytes);
+/* The same reader from a file to a file, as `uncompress(in_file, out_file)`
+ * (base_deflate.h:20); the output file is created or truncated. */
+int mdz_cudf_uncompress(const char* in_file, const char* out_file);
 
 /* ====================================================== container file ===== *
  * A self-describing file (MDZF) holding one container of any codec with all

@@ -239,3 +239,35 @@ GZIP_FIELD_CASES = [
     {"extra": b"", "name": b"n", "comment": b"", "fhcrc": True},
     {"extra": bytes(300), "name": b"long-name" * 20},
 ]
+
+
+def orig_container(data: bytes, chunk: int = 128 * 1024):
+    """The repo Deflate container as orig_compressor.py:44-125 writes it:
+    the input zero-padded to a chunk multiple (:62-66), each chunk zlib
+    level 9 (:87), each stream zero-padded to 8 bytes (:102-109),
+    col_len = padded sizes (:91,122), blk_offset = [len(data), 0, cumsum]
+    (:57-58,115-121).  Returns (payload, col_len u64, blk_offset u64)."""
+    import zlib
+
+    import numpy as np
+    n = -(-len(data) // chunk)
+    padded = data + bytes(n * chunk - len(data))
+    streams = []
+    for p in range(n):
+        z = zlib.compress(padded[p * chunk:(p + 1) * chunk], 9)
+        streams.append(z + bytes(-len(z) % 8))
+    sizes = [len(s) for s in streams]
+    blk = np.array([len(data), 0] + list(np.cumsum(sizes, dtype=np.uint64)), np.uint64)
+    return b"".join(streams), np.array(sizes, np.uint64), blk
+
+
+def cudf_container(data: bytes, chunk: int = 4096 * 2 * 8) -> bytes:
+    """The cuDF container as rapids_compressor.py:222-275 writes it:
+    [u64 chunk][u64 n][u64 len(zlib stream)[n]][streams back to back], the
+    input zero-padded to a chunk multiple, level 9, no per-stream padding."""
+    import struct
+    import zlib
+    n = -(-len(data) // chunk)
+    padded = data + bytes(n * chunk - len(data))
+    streams = [zlib.compress(padded[p * chunk:(p + 1) * chunk], 9) for p in range(n)]
+    return struct.pack(f"<QQ{n}Q", chunk, n, *[len(s) for s in streams]) + b"".join(streams)

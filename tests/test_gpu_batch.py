@@ -80,3 +80,29 @@ def test_batch_statuses_match_plain(gpu):
     brle.decompress(db_bad, pd, cb.raw_bytes, ps)
     assert np.array_equal(s2.download(dtype=np.int32), ps.download(dtype=np.int32))
     assert (s1.download(dtype=np.int32) == 0).all()
+
+
+def test_full_config2_batched(gpu):
+    """Config 2 at full size through the one persistent launch, size order:
+    the 1 GiB int64 column (8,192 chunks, READ_UNIT 512) and the 256 MiB
+    byte-RLE stream (2,048 chunks), both bit-exact with the data and with the
+    oracle's decode, every status 0."""
+    from compression_amd import batch, brle
+    from compression_amd.lanes import DeviceLanes
+    x = O.gen_rlev2_i64(1 << 27, 0x4D445A02)
+    c2 = O.rlev2_encode(x, 131072, 512, threads=16)
+    y = O.gen_brle_u8(1 << 28, 0x4D445A03)
+    cb = O.brle_encode(y, 131072, threads=16)
+    lanes = DeviceLanes(c2.payload, c2.blk_off, c2.col_len, c2.n_chunks, 131072, 8, 512)
+    db = brle.DeviceBrle(cb.data, cb.ptr, cb.n_chunks, 131072)
+    assert c2.n_chunks == 8192 and cb.n_chunks == 2048
+    b, (d1, s1, d2, s2) = _batch(gpu, lanes, c2, db, cb)
+    b.set_order(b.size_order())
+    b.submit()
+    assert (s1.download(dtype=np.int32) == 0).all() and (s2.download(dtype=np.int32) == 0).all()
+    got1 = d1.download(c2.raw_bytes)
+    assert np.array_equal(got1, x.view(np.uint8))
+    got2 = d2.download(cb.raw_bytes)
+    assert np.array_equal(got2, y)
+    rc, ref2, _ = O.brle_decode(cb)
+    assert rc == 0 and np.array_equal(got2, ref2)

@@ -8,6 +8,7 @@ import numpy as np
 import pytest
 
 from oracle import pyoracle as O
+from tests import deflate_util as DU
 
 pytestmark = pytest.mark.gpu
 FIX = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "deflate_fixtures.json")))
@@ -127,21 +128,27 @@ def test_full_config4(gpu):
     deflate.inflate(db)
     st = db.statuses()
     assert (st["status"] == 0).all() and (st["bytes_written"] == b.raw_len).all()
-    out = db.outputs()
-    for i in range(0, b.count, 97):
-        o = int(db.dst_off[i])
-        assert out[o:o + int(b.raw_len[i])].tobytes() == b.raw_stream(i)
+    _check_all_streams(b, db, db.outputs())
     order = np.argsort(-b.src_len.astype(np.int64)).astype(np.uint32)
     db.set_order(order)
     db.dst.fill(0)
     deflate.inflate_sched(db)
     st = db.statuses()
-    assert (st["status"] == 0).all()
-    out2 = db.outputs()
-    assert np.array_equal(out, out2)
+    assert (st["status"] == 0).all() and (st["bytes_written"] == b.raw_len).all()
+    _check_all_streams(b, db, db.outputs())
 
 
-def test_repo_container_host_wrapper(gpu):
+def _check_all_streams(b, db, out):
+    """Every stream's output slot equals zlib's data, byte for byte."""
+    bad = [i for i in range(b.count)
+           if not np.array_equal(out[int(db.dst_off[i]):int(db.dst_off[i]) + int(b.raw_len[i])],
+                                 b.raw[int(b.raw_off[i]):int(b.raw_off[i]) + int(b.raw_len[i])])]
+    assert not bad, f"{len(bad)} streams differ, first {bad[:8]}"
+
+
+def test_repo_container_unpadded_tail(gpu):
+    """A variant container whose last stream holds only the remainder (not
+    what the reference writes) decodes as well."""
     from compression_amd import deflate
     x = O.gen_lzss_markov(5 * 131072 + 777, 21)
     chunk = 131072
@@ -153,6 +160,64 @@ def test_repo_container_host_wrapper(gpu):
         sizes.append(len(z))
     blk = np.array([x.size, 0] + list(np.cumsum(sizes)), np.uint64)
     assert deflate.decompress_gpu(comp, np.array(sizes, np.uint64), blk, chunk) == x.tobytes()
+
+
+@pytest.mark.parametrize("size", [1, 131071, 131072, 3 * 131072 + 5, 7 * 131072 + 65536])
+def test_repo_container_as_written(gpu, size):
+    """The container exactly as orig_compressor.py:44-125 writes it: input
+    zero-padded to a 128 KiB multiple before compressing, so the last stream
+    inflates to a whole chunk; the reader returns the first len(data) bytes."""
+    from compression_amd import deflate
+    x = O.gen_lzss_markov(size, 0x51 + size).tobytes()
+    comp, col_len, blk = DU.orig_container(x)
+    # the last stream inflates to a full chunk (the writer's zero padding)
+    last = comp[int(blk[-2] - blk[1]):]
+    assert len(zlib.decompress(last)) == 131072
+    assert deflate.decompress_gpu(comp, col_len, blk, 131072) == x
+
+
+def test_repo_container_short_stream_is_corrupt(gpu):
+    """A stream inside data_size that inflates short is reported, not hidden."""
+    from compression_amd import deflate
+    from compression_amd._lib import MdzError
+    x = O.gen_lzss_markov(3 * 131072, 0x77).tobytes()
+    comp, col_len, blk = DU.orig_container(x)
+    z = zlib.compress(x[:1000], 9)
+    z += bytes(-len(z) % 8)
+    comp = z + comp[int(col_len[0]):]
+    col_len = col_len.copy()
+    col_len[0] = len(z)
+    blk = np.array([len(x), 0] + list(np.cumsum(col_len, dtype=np.uint64)), np.uint64)
+    with pytest.raises(MdzError) as e:
+        deflate.decompress_gpu(comp, col_len, blk, 131072)
+    assert e.value.rc == -5
+
+
+@pytest.mark.parametrize("size,chunk", [(1, 65536), (65536, 65536), (5 * 65536 + 123, 65536),
+                                        (300_000, 131072)])
+def test_cudf_container(gpu, size, chunk):
+    """The cuDF container (rapids_compressor.py:222-275) read as
+    base_deflate.h:20-130 reads it: (n-1)*chunk + the last stream's bytes,
+    which include the writer's zero padding."""
+    from compression_amd import deflate
+    x = O.gen_lzss_markov(size, 0x91 + size).tobytes()
+    f = DU.cudf_container(x, chunk)
+    n = -(-size // chunk)
+    got = deflate.cudf_uncompress(f)
+    assert len(got) == n * chunk
+    assert got == x + bytes(n * chunk - size)
+
+
+def test_cudf_container_bad_tables(gpu):
+    from compression_amd import deflate
+    from compression_amd._lib import MdzError
+    import struct
+    f = DU.cudf_container(O.gen_lzss_markov(70_000, 5).tobytes(), 65536)
+    for bad in (f[:16], struct.pack("<QQ", 65536, 1 << 40) + f[16:],
+                f[:16] + struct.pack("<Q", len(f)) + f[24:]):
+        with pytest.raises(MdzError) as e:
+            deflate.cudf_uncompress(bad)
+        assert e.value.rc == -1
 
 
 def test_cost_log(gpu):
