@@ -3,14 +3,22 @@
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
 
-A step is one decode pass of the workload's hot path over its whole
-synthetic input, which is resident in HBM before the timed region starts.
-K steps are enqueued back-to-back on one stream between two HIP events; the
-device is synchronised on both sides.  Prints ONE JSON line.
+The default workload, ``all``, measures configs 1-4 in one process (RLE v1
+64 MiB; RLE v2 1 GiB + byte RLE 256 MiB through the batched persistent
+kernel; LZSS 1 GiB; the 4,096-stream uneven Deflate batch) and then config 5
+(mixed-codec serving, p50/p99 at 10-80% of saturation).  A step is one
+decode pass over each workload's whole synthetic input, which is resident in
+HBM before the timed region starts.  K steps are enqueued back-to-back on
+one stream between two HIP events; the device is synchronised on both
+sides.  Prints ONE JSON line: the headline over configs 1-4 and one record
+per config, each with the roofline of its dominant kernel and its CPU
+baseline (all of the process's host cores up to 16, and one core).
 
 Algorithmic bytes (each byte touched once):
-  lane-transposed codecs: payload + 256*n (col_len) + 8*(n+1) (blk_off) + out
+  lane-transposed codecs: payload + 256*n (col_len) + 8*(n+1) (blk_off)
+                          [+ 32*n col_map for LZSS] + out
   byte RLE:               payload + 8*(n+1) + 12 + out
+  Deflate batch:          sum(src) + sum(out) + 48*count
 """
 from __future__ import annotations
 
@@ -55,7 +63,8 @@ def _cpu_time(fn, min_reps=3, max_reps=7, budget_s=20.0):
 # ------------------------------------------------------------- workloads --
 class RleV1Workload:
     """RLE v1, plain method: 64 MiB u32 column, 128 KiB chunks, one chunk per
-    workgroup (one wave64), COMP_COL_LEN = line bytes."""
+    1024-thread workgroup (16 waves: a lane walker and unit decoders),
+    COMP_COL_LEN = line bytes."""
     name = "rlev1_u32_64MiB_plain"
     dtype = "u32"
 
@@ -387,10 +396,10 @@ class DeflateWorkload:
         st = self.db.statuses()
         if not ((st["status"] == 0).all() and (st["bytes_written"] == self.b.raw_len).all()):
             return False
-        out = self.db.outputs()
-        for i in range(0, self.b.count, 41):
-            o = int(self.db.dst_off[i])
-            if out[o:o + int(self.b.raw_len[i])].tobytes() != self.b.raw_stream(i):
+        out, b = self.db.outputs(), self.b
+        for i in range(b.count):                      # every stream, byte for byte
+            o, ro, n = int(self.db.dst_off[i]), int(b.raw_off[i]), int(b.raw_len[i])
+            if not np.array_equal(out[o:o + n], b.raw[ro:ro + n]):
                 return False
         return True
 
@@ -756,12 +765,110 @@ WORKLOADS = {"config2": Config2Workload,
              "rlev1-encode": RleV1EncodeWorkload, "rlev2-encode": RleV2EncodeWorkload,
              "lzss-encode": LzssEncodeWorkload, "inflate-verify": InflateVerifyWorkload,
              "copy-uncompressed": CopyUncompressedWorkload}
-DEFAULT = "config2"
+DEFAULT = "all"
+
+
+def _log(msg: str) -> None:
+    print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
+
+
+def _cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    return ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_protocol(wl, threads: int) -> dict | None:
+    """The CPU baseline: the workload's host decoder over `threads` cores (one
+    chunk or stream per task, dynamic scheduling), then over one core, with
+    the host's affinity core count, CPU model and zlib version."""
+    from compression_amd import host as H
+    from oracle import pyoracle as O   # the CPU baseline leg
+    cb = wl.cpu(threads)
+    if cb is None:
+        return None
+    one = wl.cpu(1)
+    cb["one_core"] = {"value": one["value"], "unit": one["unit"], "sample": one["sample"]}
+    cb["affinity_cores"] = H.host_cores()
+    cb["cpu_model"] = _cpu_model()
+    cb["zlib_version"] = O.zlib_version()
+    return cb
+
+
+def _time_steps(fn, stream, steps):
+    import compression_amd as mdz
+    e0, e1 = mdz.Event(), mdz.Event()
+    mdz.lib().mdz_device_sync()
+    e0.record(stream)
+    for _ in range(steps):
+        fn(stream.ptr)
+    e1.record(stream)
+    e1.sync()
+    mdz.lib().mdz_device_sync()
+    return e0.elapsed_ms(e1) / steps
+
+
+def measure(wl, args, stream, threads: int) -> dict:
+    """One workload: W warm-up steps, verification, K timed steps between HIP
+    events on `stream`, each kernel's own launch time, batched vs plain where
+    the workload has both, and the CPU baseline."""
+    for _ in range(args.warmup):
+        wl.step(stream.ptr)
+    stream.sync()
+    ok = wl.verify()
+    ms = _time_steps(wl.step, stream, args.steps)
+    gbs = wl.out_bytes / (ms * 1e-3) / 1e9
+    # dominant kernel: each codec's launch timed alone with HIP events on the
+    # stream it is enqueued on
+    kernels = wl.kernels() if hasattr(wl, "kernels") else [(wl.name, wl.step, wl.alg_bytes)]
+    per_kernel = []
+    for kname, fn, kbytes in kernels:
+        kms = _time_steps(fn, stream, args.steps)
+        per_kernel.append({"kernel": kname, "ms": round(kms, 5), "alg_bytes": kbytes,
+                           "GBps": round(kbytes / (kms * 1e-3) / 1e9, 1) if kbytes else None})
+    dom = max(per_kernel, key=lambda k: k["ms"])
+    ach = dom["GBps"]
+    rec = {"value": round(gbs, 2), "unit": "GB/s", "ms_per_step": round(ms, 5), "dtype": wl.dtype,
+           "config": wl.config}
+    if ach is not None:
+        rec["roofline"] = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                           "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": TRAFFIC.get(dom["kernel"]),
+                           "kernel": dom["kernel"], "alg_bytes_per_launch": dom["alg_bytes"]}
+    rec["kernels"] = per_kernel
+    if getattr(wl, "batched", False):               # batched vs plain, same data, same run
+        plain_ms = _time_steps(wl.plain_step, stream, args.steps)
+        bvp = {"batched_ms": round(ms, 5), "plain_ms": round(plain_ms, 5),
+               "speedup": round(plain_ms / ms, 3)}
+        if hasattr(wl, "size_step"):                # compressed-size order, same launch
+            bvp["size_order_ms"] = round(_time_steps(wl.size_step, stream, args.steps), 5)
+        rec["batched_vs_plain"] = bvp
+    rec["bit_exact"] = ok
+    if not args.no_cpu:
+        cb = cpu_protocol(wl, threads)
+        if cb is not None:
+            rec["cpu_baseline"] = cb
+    return rec
+
+
+def _header(args, rec: dict) -> dict:
+    return {"metric": METRIC, "value": rec["value"], "unit": rec["unit"], "n_gpus": 1,
+            "steps": args.steps, "warmup": args.warmup, "ms_per_step": rec.get("ms_per_step"),
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": rec["dtype"],
+            "data": "synthetic (seeded generators, encoded by the host library)"}
+
+
+def _threads() -> int:
+    from compression_amd import host as H
+    return H.default_threads()
 
 
 def run(args):
     import compression_amd as mdz
-    from compression_amd import host as H
 
     if mdz.device_count() < 1:
         raise SystemExit("bench.py needs a GPU")
@@ -776,89 +883,111 @@ def run(args):
         kw["graphed"] = False
     wl = WORKLOADS[args.workload](**kw)
     stream = mdz.Stream()
-    for _ in range(args.warmup):
-        wl.step(stream.ptr)
-    stream.sync()
-    ok = wl.verify()
-    e0, e1 = mdz.Event(), mdz.Event()
-    mdz.lib().mdz_device_sync()
-    e0.record(stream)
-    for _ in range(args.steps):
-        wl.step(stream.ptr)
-    e1.record(stream)
-    e1.sync()
-    mdz.lib().mdz_device_sync()
-    ms = e0.elapsed_ms(e1) / args.steps
-    gbs = wl.out_bytes / (ms * 1e-3) / 1e9
-    # dominant kernel: each codec's launch timed alone with HIP events on the
-    # stream it is enqueued on
-    kernels = wl.kernels() if hasattr(wl, "kernels") else [(wl.name, wl.step, wl.alg_bytes)]
-    per_kernel = []
-    for kname, fn, kbytes in kernels:
-        k0, k1 = mdz.Event(), mdz.Event()
-        k0.record(stream)
-        for _ in range(args.steps):
-            fn(stream.ptr)
-        k1.record(stream)
-        k1.sync()
-        kms = k0.elapsed_ms(k1) / args.steps
-        per_kernel.append({"kernel": kname, "ms": round(kms, 5), "alg_bytes": kbytes,
-                           "GBps": round(kbytes / (kms * 1e-3) / 1e9, 1)})
-    dom = max(per_kernel, key=lambda k: k["ms"])
-    ach = dom["GBps"]
-    plain_ms = None
-    if getattr(wl, "batched", False):               # batched vs plain, same data, same run
-        p0, p1 = mdz.Event(), mdz.Event()
-        p0.record(stream)
-        for _ in range(args.steps):
-            wl.plain_step(stream.ptr)
-        p1.record(stream)
-        p1.sync()
-        plain_ms = p0.elapsed_ms(p1) / args.steps
-    size_ms = None
-    if hasattr(wl, "size_step"):                    # compressed-size order, same launch
-        q0, q1 = mdz.Event(), mdz.Event()
-        q0.record(stream)
-        for _ in range(args.steps):
-            wl.size_step(stream.ptr)
-        q1.record(stream)
-        q1.sync()
-        size_ms = q0.elapsed_ms(q1) / args.steps
-    threads = max(1, min(H.host_cores(), 16))
-    line = {
-        "metric": METRIC, "value": round(gbs, 2), "unit": "GB/s", "n_gpus": 1,
-        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 5),
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": wl.dtype,
-        "data": "synthetic (seeded generators, encoded by the host library)", "config": wl.config,
-        "roofline": {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": TRAFFIC.get(dom["kernel"]),
-                     "kernel": dom["kernel"], "alg_bytes_per_launch": dom["alg_bytes"]},
-        "kernels": per_kernel,
-        "batched_vs_plain": ({"batched_ms": round(ms, 5), "plain_ms": round(plain_ms, 5),
-                              "speedup": round(plain_ms / ms, 3),
-                              **({"size_order_ms": round(size_ms, 5)} if size_ms else {})}
-                             if plain_ms else None),
-        "bit_exact": ok,
-    }
-    if not args.no_cpu:
-        cb = wl.cpu(threads)
-        if cb is not None:
-            line["cpu_baseline"] = cb
+    rec = measure(wl, args, stream, _threads())
+    line = _header(args, rec)
+    line.update({k: v for k, v in rec.items() if k not in line})
     print(json.dumps(line), flush=True)
 
 
-def run_serve(args):
+def rlev1_line_sweep(args, stream, lines=(32, 128, 512)) -> list:
+    """Config 1 at COMP_COL_LEN 32, 128 and 512: kernel time per launch and
+    bit-exactness at each line length."""
+    out = []
+    for ln in lines:
+        wl = RleV1Workload(line=ln)
+        wl.step(stream.ptr)
+        stream.sync()
+        ok = wl.verify()
+        ms = _time_steps(wl.step, stream, args.steps)
+        out.append({"comp_col_len": ln, "ms": round(ms, 5), "payload_bytes": wl.config["payload_bytes"],
+                    "alg_GBps": round(wl.alg_bytes / (ms * 1e-3) / 1e9, 1),
+                    "frac": round(wl.alg_bytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4), "bit_exact": ok})
+        del wl
+    return out
+
+
+ALL_CONFIGS = [("config1", "rlev1", "RLE v1, 64 MiB u32 column, plain"),
+               ("config2", "config2", "RLE v2 1 GiB int64 + byte RLE 256 MiB, batched persistent"),
+               ("config3", "lzss", "LZSS, 1 GiB Markov+copy, 64 KiB chunks"),
+               ("config4", "deflate", "Deflate, 4,096 uneven zlib streams, scheduled persistent")]
+
+
+def run_all(args):
+    """Default: every config in one process.  Configs 1-4 are decoded with
+    their inputs resident in HBM; a step is one decode pass over each of the
+    four workloads, so the headline value is their total decompressed bytes
+    over the sum of their per-step times.  Config 5 (serving) follows with
+    its p50/p99 at 10-80% of saturation.  Each config keeps its own record
+    (value, roofline of its dominant kernel, CPU baseline)."""
+    import gc
+
+    import compression_amd as mdz
+
+    if mdz.device_count() < 1:
+        raise SystemExit("bench.py needs a GPU")
+    threads = _threads()
+    stream = mdz.Stream()
+    recs = {}
+    for key, wname, desc in ALL_CONFIGS:
+        _log(f"{key}: building {wname} ({desc})")
+        wl = WORKLOADS[wname]()
+        _log(f"{key}: measuring")
+        rec = measure(wl, args, stream, threads)
+        rec["description"] = desc
+        recs[key] = rec
+        _log(f"{key}: {rec['ms_per_step']} ms/step, {rec['value']} GB/s, bit_exact={rec['bit_exact']}")
+        del wl
+        gc.collect()
+    if not args.no_sweep:
+        _log("config1: COMP_COL_LEN sweep 32/128/512")
+        recs["config1"]["comp_col_len_sweep"] = rlev1_line_sweep(args, stream)
+    if not args.no_serve:
+        _log(f"config5: serving, {args.requests} requests")
+        recs["config5"] = serve_record(args)
+        _log(f"config5: p99 at 80% load {recs['config5']['p99_ms_at_80']} ms")
+    parts = [recs[k] for k, _, _ in ALL_CONFIGS]
+    out_bytes = sum(r["config"]["out_bytes"] for r in parts)
+    ms = sum(r["ms_per_step"] for r in parts)
+    # headline roofline: the kernel with the most time per step over all configs
+    dom = max((k for r in parts for k in r["kernels"]), key=lambda k: k["ms"])
+    line = {"metric": METRIC, "value": round(out_bytes / (ms * 1e-3) / 1e9, 2), "unit": "GB/s",
+            "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 5),
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "integer (u8/u32/u64 decode)",
+            "data": "synthetic (seeded generators, encoded by the host library)",
+            "config": {"workload": "configs 1-4 decoded once each per step (RLE v1 64 MiB; RLE v2 1 GiB + "
+                                   "byte RLE 256 MiB batched; LZSS 1 GiB; Deflate 4,096 streams), "
+                                   "config 5 serving reported beside",
+                       "out_bytes": out_bytes, "parts": [k for k, _, _ in ALL_CONFIGS]},
+            "roofline": {"bound": "hbm", "achieved": dom["GBps"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": round(dom["GBps"] / HBM_PEAK_GBS, 4), "traffic": TRAFFIC.get(dom["kernel"]),
+                         "kernel": dom["kernel"], "alg_bytes_per_launch": dom["alg_bytes"]},
+            "bit_exact": all(r["bit_exact"] for r in parts)}
+    if all("cpu_baseline" in r for r in parts):
+        cpu_s = sum(r["config"]["out_bytes"] / (r["cpu_baseline"]["value"] * 1e9) for r in parts)
+        cpu1_s = sum(r["config"]["out_bytes"] / (r["cpu_baseline"]["one_core"]["value"] * 1e9)
+                     for r in parts)
+        c0 = parts[0]["cpu_baseline"]
+        line["cpu_baseline"] = {
+            "value": round(out_bytes / cpu_s / 1e9, 3), "unit": "GB/s", "cores": threads,
+            "kind": "port", "one_core": {"value": round(out_bytes / cpu1_s / 1e9, 3), "unit": "GB/s"},
+            "affinity_cores": c0["affinity_cores"], "cpu_model": c0["cpu_model"],
+            "zlib_version": c0["zlib_version"],
+            "sample": "each config's host decoders (the C restatements of the reference decoders; zlib "
+                      "inflate for Deflate) over its whole workload, total bytes over total time"}
+    line["configs"] = recs
+    print(json.dumps(line), flush=True)
+
+
+def serve_record(args) -> dict:
     """Config 5: mixed-codec serving.  A pool of requests (codec uniform over
     RLE v1 / RLE v2 / LZSS / Deflate, sizes log-uniform 16 KiB..64 MiB) is
     generated once; saturation is measured with every request arriving at
     t = 0; then open-loop Poisson arrivals at 10..80% of saturation are
     replayed with the same requests.  Latency = arrival -> output ready on
     the device; the 64 GB HBM budget bounds what is admitted."""
-    import compression_amd as mdz
     from compression_amd import serve
 
-    if mdz.device_count() < 1:
-        raise SystemExit("bench.py needs a GPU")
     n = args.requests
     sizes = serve.request_sizes(n, 0x4D445A05)
     codecs = [serve.CODECS[i] for i in np.random.default_rng(0x4D445A05).integers(0, 4, n)]
@@ -884,27 +1013,39 @@ def run_serve(args):
         r.arrival = 0.0
     ok = chk.run(pool, time_limit=180)["bit_exact"]
     at80 = sweep[-1]
-    print(json.dumps({
-        "metric": METRIC, "value": at80["GBps"], "unit": "GB/s", "n_gpus": 1,
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "mixed",
-        "data": "synthetic (seeded generators, host encoders)",
-        "config": {"workload": "serve_mixed_codecs", "requests": n, "out_bytes": total,
-                   "codecs": {c: codecs.count(c) for c in serve.CODECS}, "hbm_budget_GiB": 64,
-                   "streams": 4},
-        "saturation_GBps": round(sat["GBps"], 2), "p50_ms_at_80": at80["p50_ms"],
-        "p99_ms_at_80": at80["p99_ms"], "sweep": sweep, "bit_exact": ok}), flush=True)
+    return {"value": at80["GBps"], "unit": "GB/s", "dtype": "mixed",
+            "config": {"workload": "serve_mixed_codecs", "requests": n, "out_bytes": total,
+                       "codecs": {c: codecs.count(c) for c in serve.CODECS}, "hbm_budget_GiB": 64,
+                       "streams": 4},
+            "saturation_GBps": round(sat["GBps"], 2), "p50_ms_at_80": at80["p50_ms"],
+            "p99_ms_at_80": at80["p99_ms"], "sweep": sweep, "bit_exact": ok}
+
+
+def run_serve(args):
+    import compression_amd as mdz
+
+    if mdz.device_count() < 1:
+        raise SystemExit("bench.py needs a GPU")
+    rec = serve_record(args)
+    line = {"metric": METRIC, "value": rec["value"], "unit": "GB/s", "n_gpus": 1,
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "mixed",
+            "data": "synthetic (seeded generators, host encoders)"}
+    line.update({k: v for k, v in rec.items() if k not in line})
+    print(json.dumps(line), flush=True)
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
-    ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--workload", default=DEFAULT, choices=sorted(WORKLOADS) + ["serve"])
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default=DEFAULT, choices=sorted(WORKLOADS) + ["serve", "all"])
     ap.add_argument("--line", type=int, default=0, help="RLE v1 COMP_COL_LEN override")
     ap.add_argument("--ring", type=int, default=0, help="LDS ring words override")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--requests", type=int, default=96, help="request pool size for --workload serve")
+    ap.add_argument("--no-serve", action="store_true", help="all: skip config 5")
+    ap.add_argument("--no-sweep", action="store_true", help="all: skip the RLE v1 line sweep")
+    ap.add_argument("--requests", type=int, default=64, help="request pool size for serving")
     ap.add_argument("--no-tune", action="store_true",
                     help="config2: size order and default shape instead of the autotuner's choice")
     ap.add_argument("--no-graph", action="store_true",
@@ -916,6 +1057,8 @@ def main():
         raise SystemExit(2)
     if args.workload == "serve":
         run_serve(args)
+    elif args.workload == "all":
+        run_all(args)
     else:
         run(args)
 
