@@ -97,6 +97,7 @@ SIGNATURES = {
     "mdz_meta_split": (C.c_int, [vp, u64, C.POINTER(vp), C.POINTER(u64), C.POINTER(u64)]),
     "mdz_inflate_batch": (C.c_int, [vp, vp, C.c_int, C.c_int, vp]),
     "mdz_inflate_batch_sched": (C.c_int, [vp, vp, C.c_int, C.c_int, vp, u32, vp, vp, vp]),
+    "mdz_inflate_batch_sched_noalloc": (C.c_int, [vp, vp, C.c_int, C.c_int, vp, u32, vp, vp]),
     "mdz_inflate_slots": (C.c_int, [C.POINTER(u32)]),
     "mdz_copy_uncompressed": (C.c_int, [vp, C.c_int, vp]),
     "mdz_rlev1_compress_bound": (C.c_int, [u64, u64, C.c_uint32, C.c_uint32, C.POINTER(u64), C.POINTER(u64),

@@ -31,9 +31,17 @@
 // rules, applied by a prepass kernel; the 8-byte footer is excluded from the
 // Deflate data); with MDZ_HDR_VERIFY a kernel after the decode checks each
 // trailer (Adler-32, or CRC-32 and ISIZE) against the output written.
+#include <mutex>
+
 #include "deflate_hdr_dev.h"
 #include "mdz_device.h"
 #include "mdz_internal.h"
+
+// Device memory of mdz_inflate_batch_sched_noalloc: word 0 is its work-queue
+// cursor.  Every device the library runs on holds one copy until the process
+// ends; calls that overlap on one device share it (and race).  Host code uses
+// the per-device address from hipGetSymbolAddress, never this name.
+static __device__ __align__(256) uint32_t scratch[64];
 
 namespace mdz {
 namespace {
@@ -796,6 +804,32 @@ int inflate_batch(const mdz_inflate_in_t* in, mdz_inflate_status_t* out, uint32_
     return MDZ_OK;
 }
 
+namespace {
+constexpr int kScratchDevices = 64;
+std::mutex g_scratch_mu;
+uint32_t* g_scratch[kScratchDevices];   // scratch's address on each device
+}  // namespace
+
+// Looks up scratch's address on the current device, once per device (the
+// device is current during the lookup, so the address is that device's copy).
+int scratch_address(uint32_t** p) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return record_hip(e, "hipGetDevice");
+    if (dev < 0 || dev >= kScratchDevices)
+        return set_error(MDZ_ERR_UNSUPPORTED, "device %d: no-allocation entry points cover devices 0..%d",
+                         dev, kScratchDevices - 1);
+    std::lock_guard<std::mutex> g(g_scratch_mu);
+    if (!g_scratch[dev]) {
+        void* q = nullptr;
+        e = hipGetSymbolAddress(&q, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return record_hip(e, "hipGetSymbolAddress(scratch)");
+        g_scratch[dev] = static_cast<uint32_t*>(q);
+    }
+    *p = g_scratch[dev];
+    return MDZ_OK;
+}
+
 int copy_uncompressed(const mdz_inflate_in_t* in, uint32_t count, hipStream_t s) {
     if (!in && count) return MDZ_ERR_INVALID;
     if (count == 0) return MDZ_OK;
@@ -805,3 +839,16 @@ int copy_uncompressed(const mdz_inflate_in_t* in, uint32_t count, hipStream_t s)
 }
 
 }  // namespace mdz
+
+extern "C" int mdz_inflate_batch_sched_noalloc(const mdz_inflate_in_t* in, mdz_inflate_status_t* out,
+                                               int count, int parse_hdr, const uint32_t* order,
+                                               uint32_t blocks, mdz_inflate_cost_t* cost,
+                                               void* stream) {
+    if (count < 0 || !mdz::inflate_hdr_ok(parse_hdr)) return MDZ_ERR_INVALID;
+    uint32_t* cursor = nullptr;
+    const int rc = mdz::scratch_address(&cursor);
+    if (rc) return rc;
+    // the cursor is cleared by inflate_batch's reset kernel before the launch
+    return mdz::inflate_batch(in, out, (uint32_t)count, parse_hdr, order, blocks, cursor, cost,
+                              reinterpret_cast<hipStream_t>(stream));
+}

@@ -366,6 +366,19 @@ int mdz_inflate_batch_sched(const mdz_inflate_in_t* in, mdz_inflate_status_t* ou
                             int parse_hdr, const uint32_t* order, uint32_t blocks,
                             uint32_t* cursor, mdz_inflate_cost_t* cost, void* stream);
 
+/* mdz_inflate_batch_sched without the cursor argument, for callers that
+ * cannot allocate device memory (code under hipGraph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  No counterpart in the reference.
+ * The work-queue cursor lives in the library's own device memory (a 256-byte
+ * file-scope array per device, of which the cursor uses 4 bytes), so the call
+ * allocates nothing; count is limited only as for mdz_inflate_batch_sched
+ * (0..INT32_MAX).  The first call on a device looks up that array's address.
+ * All calls on one device share the cursor: calls that overlap on one device
+ * (on different streams) race and must be serialised by the caller. */
+int mdz_inflate_batch_sched_noalloc(const mdz_inflate_in_t* in, mdz_inflate_status_t* out,
+                                    int count, int parse_hdr, const uint32_t* order,
+                                    uint32_t blocks, mdz_inflate_cost_t* cost, void* stream);
+
 /* Compute units of the current device (sizes persistent grids). */
 int mdz_device_cus(int* cus);
 /* Workgroups of the persistent inflate kernel that fit on the device at once. */
