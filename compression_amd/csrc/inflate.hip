@@ -46,13 +46,22 @@ static __device__ __align__(256) uint32_t scratch[64];
 namespace mdz {
 namespace {
 
-constexpr uint32_t kRing = 32768 + 4608;    // history + one batch of output
-constexpr uint32_t kBatchOut = 4096;        // a batch closes once this much output is queued
-constexpr uint32_t kLitBits = 10, kDistBits = 8;
+constexpr uint32_t kBatchOut = 2048;                    // a batch closes once this much output is queued
+constexpr uint32_t kRing = 32768 + kBatchOut + 512;     // history + one batch (< kBatchOut + 258 bytes)
+constexpr uint32_t kLitRoot = 10, kLitCap = 1024 + 512; // two-level tables: root bits, root + subtables
+constexpr uint32_t kDistRoot = 8, kDistCap = 256 + 384;
+constexpr uint32_t kIndirect = 0x400;                   // root entry pointing at a subtable
+constexpr uint32_t kSpanLog = 9, kSpan = 1u << kSpanLog; // input bits each lane takes per round
+constexpr uint32_t kReach = 3;                          // spans a hand-off may run ahead
+constexpr uint32_t kStageWords = 64 * kSpan / 32 + 8;   // one round's input, plus a symbol's overhang
+constexpr uint32_t kSymCap = 2048;                      // true-path symbols kept per round
 
-// LUT entry: [3:0] code length (0: take the canonical path), [7:4] extra
-// bits, [9:8] kind (0 literal, 1 end of block, 2 length/distance, 3 invalid),
-// [31:16] value (literal byte, base length or base distance).
+// Table entry: [3:0] code length (0: no entry, take the canonical path),
+// [7:4] extra bits, [9:8] kind (0 literal, 1 end of block, 2 length/distance,
+// 3 invalid), [31:16] value (literal byte, base length or base distance).  A
+// root entry with kIndirect set points at a subtable instead: [14:11] its
+// index bits, [31:16] its first entry; subtable entries hold the full code
+// length.
 enum : uint32_t { kLit = 0, kEob = 1, kRef = 2, kBad = 3 };
 
 __constant__ uint16_t kLenBase[29] = {3, 4, 5, 6, 7, 8, 9, 10, 11, 13, 15, 17, 19, 23, 27, 31,
@@ -84,14 +93,27 @@ struct Code {
     uint16_t sym[288];
 };
 
-struct InflateLds {
-    uint32_t lit_lut[1u << kLitBits];
-    uint32_t dist_lut[1u << kDistBits];
-    uint32_t cl_lut[128];                      // code-length code: 7-bit table, complete
-    Code lit, dist, cl;
+// Block-header scratch; it shares its space with the round's symbol list.
+struct HdrScratch {
     uint8_t lens[320];                         // literal/length then distance code lengths
+    uint32_t cl_lut[128];                      // code-length code: 7-bit table, complete
+    Code cl;
+};
+
+struct InflateLds {
+    uint32_t lit[kLitCap];                     // literal/length table, 10-bit root
+    uint32_t dist[kDistCap];                   // distance table, 8-bit root
+    Code lit_c, dist_c;                        // canonical tables of the same codes
+    uint32_t in[kStageWords];                  // the round's input bits
+    union {
+        HdrScratch h;
+        uint32_t bm[64 * (kSpan / 32)];        // per lane: symbol starts met in its span
+        uint32_t sym[kSymCap];                 // the round's symbols, in stream order
+    } u;
     uint8_t ring[kRing];
 };
+static_assert(sizeof(HdrScratch) <= sizeof(uint32_t) * kSymCap, "header scratch fits");
+static_assert(sizeof(InflateLds) <= 80 * 1024, "two workgroups per CU");
 
 MDZ_DEV uint32_t bitrev(uint32_t v, uint32_t n) { return __builtin_bitreverse32(v) >> (32 - n); }
 
@@ -166,6 +188,195 @@ MDZ_DEV int build_code(Code& c, uint32_t* lut, uint32_t lut_bits, const uint8_t*
     return 0;
 }
 
+MDZ_DEV bool is_marker(uint32_t v, uint32_t rb) { return v > rb && v < 16; }
+
+// Builds the two-level table T (root bits rb, cap entries) and the canonical
+// tables c of the literal/length (kind 0) or distance (kind 1) code with
+// lengths lens[0..n), whole wave.  Codes of at most rb bits fill root
+// entries; each root prefix of longer codes gets a subtable of 2^(m - rb)
+// entries, m its longest code.  If the subtables do not fit in cap, those
+// root entries stay empty and long codes decode through c.  Returns 0, or -1
+// for a set zlib rejects (over-subscribed, or incomplete unless its longest
+// code has length 1).  Empty entries of incomplete sets mark patterns no code
+// defines.
+MDZ_DEV int build_table2(Code& c, uint32_t* T, uint32_t rb, uint32_t cap, const uint8_t* lens,
+                         uint32_t n, uint32_t kind) {
+    const uint32_t l = lane_id();
+    uint32_t cnt = 0;                          // lane k < 16 counts length k
+    for (uint32_t s0 = 0; s0 < n; s0 += 64) {
+        const uint32_t s = s0 + l;
+        const uint32_t len = s < n ? lens[s] : 0;
+#pragma unroll
+        for (uint32_t k = 1; k < 16; ++k) {
+            const uint32_t b = __popcll(__ballot(len == k));
+            if (l == k) cnt += b;
+        }
+    }
+    if (l < 16) c.cnt[l] = (uint16_t)(l == 0 ? 0 : cnt);
+    wave_sync();
+    int left = 1, maxlen = 0;
+    uint32_t offs[17], code = 0, first[16];
+    offs[0] = offs[1] = 0;
+#pragma unroll
+    for (uint32_t k = 1; k < 16; ++k) {
+        const uint32_t ck = c.cnt[k];
+        left = (left << 1) - (int)ck;
+        if (left < 0) return -1;
+        if (ck) maxlen = (int)k;
+        code = (code + (k > 1 ? c.cnt[k - 1] : 0)) << 1;
+        first[k] = code;
+        offs[k + 1] = offs[k] + ck;
+    }
+    if (left > 0 && maxlen != 1 && maxlen != 0) return -1;
+    const uint32_t R = 1u << rb;
+    for (uint32_t i = l; i < cap; i += 64) T[i] = 0;
+    wave_sync();
+    // sorted symbols, root entries of short codes, longest code per long prefix
+    uint32_t run[16];
+#pragma unroll
+    for (uint32_t k = 0; k < 16; ++k) run[k] = 0;
+    for (uint32_t s0 = 0; s0 < n; s0 += 64) {
+        const uint32_t s = s0 + l;
+        const uint32_t len = s < n ? lens[s] : 0;
+        uint32_t rank = 0;
+#pragma unroll
+        for (uint32_t k = 1; k < 16; ++k) {
+            const uint64_t b = __ballot(len == k);
+            if (len == k) rank = run[k] + __popcll(b & ((1ull << l) - 1));
+            run[k] += __popcll(b);
+        }
+        if (len) {
+            uint32_t off = 0, fc = 0;
+#pragma unroll
+            for (uint32_t k = 1; k < 16; ++k)
+                if (k == len) { off = offs[k]; fc = first[k]; }
+            c.sym[off + rank] = (uint16_t)s;
+            const uint32_t rev = bitrev(fc + rank, len);
+            if (len <= rb) {
+                const uint32_t e = len | (kind == 0 ? lit_entry_bits(s) : dist_entry_bits(s));
+                for (uint32_t j = rev; j < R; j += 1u << len) T[j] = e;
+            } else {
+                atomicMax(&T[rev & (R - 1)], len);
+            }
+        }
+    }
+    wave_sync();
+    // subtables: lane l places the long prefixes among root entries
+    // [l * R / 64, (l + 1) * R / 64)
+    const uint32_t per = R / 64;
+    uint32_t mine = 0;
+    for (uint32_t j = 0; j < per; ++j) {
+        const uint32_t v = T[l * per + j];
+        if (is_marker(v, rb)) mine += 1u << (v - rb);
+    }
+    const uint32_t incl = wave_incl_scan_u32(mine);
+    const uint32_t total = __builtin_amdgcn_readlane(incl, 63);
+    const bool fits = R + total <= cap;
+    uint32_t base = R + incl - mine;
+    for (uint32_t j = 0; j < per; ++j) {
+        const uint32_t v = T[l * per + j];
+        if (is_marker(v, rb)) {
+            T[l * per + j] = fits ? (kIndirect | ((v - rb) << 11) | (base << 16)) : 0;
+            base += 1u << (v - rb);
+        }
+    }
+    wave_sync();
+    if (!fits) return 0;
+    const uint32_t ntot = offs[16];
+    for (uint32_t k = offs[rb + 1] + l; k < ntot; k += 64) {
+        uint32_t len = rb + 1, o = 0, fc = 0;
+#pragma unroll
+        for (uint32_t q = 1; q < 16; ++q)
+            if (q > rb && offs[q] <= k) { len = q; o = offs[q]; fc = first[q]; }
+        const uint32_t s = c.sym[k];
+        const uint32_t rev = bitrev(fc + (k - o), len);
+        const uint32_t root = T[rev & (R - 1)];
+        const uint32_t sb = (root >> 11) & 15, sbase = root >> 16;
+        const uint32_t e = len | (kind == 0 ? lit_entry_bits(s) : dist_entry_bits(s));
+        for (uint32_t j = rev >> rb; j < (1u << sb); j += 1u << (len - rb)) T[sbase + j] = e;
+    }
+    wave_sync();
+    return 0;
+}
+
+// One lane's table lookup on the bit pattern b (LSB first): the entry, with
+// its code length in [3:0]; 0 where no code is defined.
+MDZ_DEV uint32_t lookup2(const uint32_t* T, uint32_t rb, const Code& c, uint32_t kind, uint64_t b) {
+    uint32_t e = T[(uint32_t)b & ((1u << rb) - 1)];
+    if (e & kIndirect) {
+        const uint32_t sb = (e >> 11) & 15;
+        e = T[(e >> 16) + ((uint32_t)(b >> rb) & ((1u << sb) - 1))];
+    }
+    if ((e & 15) == 0) {                        // canonical decode (RFC 1951 3.2.2 order)
+        int code = 0, fst = 0, index = 0;
+        e = 0;
+        for (uint32_t len = 1; len < 16; ++len) {
+            code |= (int)(b & 1);
+            b >>= 1;
+            const int count = c.cnt[len];
+            if (code - count < fst) {
+                const uint32_t s = c.sym[index + (code - fst)];
+                e = len | (kind == 0 ? lit_entry_bits(s) : dist_entry_bits(s));
+                break;
+            }
+            index += count;
+            fst += count;
+            fst <<= 1;
+            code <<= 1;
+        }
+    }
+    return e;
+}
+
+// 64 input bits from relative bit position p of the staged round input.
+MDZ_DEV uint64_t peek64(const uint32_t* in, uint32_t p) {
+    const uint32_t w = p >> 5, sh = p & 31;
+    const uint64_t lo = (uint64_t)in[w] | ((uint64_t)in[w + 1] << 32);
+    const uint64_t hi = in[w + 2];
+    return sh ? (lo >> sh) | (hi << (64 - sh)) : lo;
+}
+
+enum : uint32_t { kSymOk = 0, kSymEob = 1, kSymErr = 2 };
+
+// One lane decodes the literal/length (and distance) symbol at relative bit
+// p; `lim` is the stream's end in the same coordinates.  Returns kSymOk with
+// *sym packed (literal: the byte; match: 1<<31 | (len-3)<<15 | (dist-1)) and
+// *used its bits, kSymEob with *used the code's bits, or kSymErr with *sym
+// the status, by zlib's checks in zlib's order.
+MDZ_DEV uint32_t decode_sym(const InflateLds& S, uint32_t p, uint32_t lim, uint32_t* sym,
+                            uint32_t* used) {
+    uint64_t b = peek64(S.in, p);
+    const uint32_t e = lookup2(S.lit, kLitRoot, S.lit_c, 0, b);
+    const uint32_t el = e & 15;
+    const uint32_t kind = el ? (e >> 8) & 3 : (uint32_t)kBad;
+    uint32_t q = p + el;
+    *used = el;
+    if (q > lim || (kind == kBad && q + 1 > lim)) { *sym = MDZ_ST_INPUT_OVERRUN; return kSymErr; }
+    if (kind == kBad) { *sym = (uint32_t)MDZ_ST_BAD_SYMBOL; return kSymErr; }
+    if (kind == kEob) return kSymEob;
+    if (kind == kLit) { *sym = e >> 16; return kSymOk; }
+    b >>= el;
+    const uint32_t xb = (e >> 4) & 15;
+    const uint32_t len = (e >> 16) + ((uint32_t)b & ((1u << xb) - 1));
+    b >>= xb;
+    q += xb;
+    const uint32_t d = lookup2(S.dist, kDistRoot, S.dist_c, 1, b);
+    const uint32_t dl = d & 15;
+    q += dl;
+    if (!dl || ((d >> 8) & 3) == kBad) {
+        *sym = q + 1 > lim ? (uint32_t)MDZ_ST_INPUT_OVERRUN : (uint32_t)MDZ_ST_BAD_DISTANCE;
+        return kSymErr;
+    }
+    b >>= dl;
+    const uint32_t dx = (d >> 4) & 15;
+    const uint32_t dist = (d >> 16) + ((uint32_t)b & ((1u << dx) - 1));
+    q += dx;
+    if (q > lim) { *sym = MDZ_ST_INPUT_OVERRUN; return kSymErr; }
+    *used = q - p;
+    *sym = 0x80000000u | ((len - 3) << 15) | (dist - 1);
+    return kSymOk;
+}
+
 struct Reader {
     const uint8_t* src;
     uint64_t n;          // stream bytes
@@ -202,6 +413,11 @@ struct Reader {
         wpos = 1;
         cbits = 8 * q;
     }
+    // positions the reader at stream bit q
+    MDZ_DEV void seek_bit(uint64_t q) {
+        seek(q >> 3);
+        drop((uint32_t)(q & 7));
+    }
     MDZ_DEV void refill() {
         if (nb > 32) return;
         const uint32_t w = __builtin_amdgcn_readlane(win, wpos);
@@ -223,27 +439,6 @@ struct Reader {
     }
     MDZ_DEV bool over() const { return cbits > 8 * n; }
 };
-
-// Canonical decode for codes longer than the LUT (or undefined patterns).
-// Returns the symbol or -1; consumes the code's bits.
-MDZ_DEV int slow_decode(const Code& c, Reader& r) {
-    int code = 0, first = 0, index = 0;
-    uint64_t b = r.bb;
-    for (uint32_t len = 1; len < 16; ++len) {
-        code |= (int)(b & 1);
-        b >>= 1;
-        const int count = c.cnt[len];
-        if (code - count < first) {
-            r.drop(len);
-            return c.sym[index + (code - first)];
-        }
-        index += count;
-        first += count;
-        first <<= 1;
-        code <<= 1;
-    }
-    return -1;
-}
 
 MDZ_DEV uint32_t ring_at(uint64_t p) { return (uint32_t)(p % kRing); }
 
@@ -327,14 +522,6 @@ MDZ_DEV uint32_t stream_checksum(const uint8_t* p, uint64_t n, bool crc) {
     return crc ? ~R : (B << 16) | A;
 }
 
-// Entry idx (wave-uniform) of a lookup table held in registers, entry i in
-// lane i % 64 of v[i / 64].
-template <uint32_t N>
-MDZ_DEV uint32_t lut_pick(const uint32_t (&v)[N], uint32_t idx) {
-    const uint32_t e = __builtin_amdgcn_readfirstlane(idx >> 6);
-    return (uint32_t)__builtin_amdgcn_readlane(v[e], idx & 63);
-}
-
 __global__ __launch_bounds__(64) void inflate_kernel(const mdz_inflate_in_t* __restrict__ ins,
                                                      mdz_inflate_status_t* __restrict__ outs,
                                                      const uint32_t* __restrict__ order,
@@ -343,6 +530,7 @@ __global__ __launch_bounds__(64) void inflate_kernel(const mdz_inflate_in_t* __r
                                                      mdz_inflate_cost_t* __restrict__ cost) {
     extern __shared__ __align__(16) uint8_t smem[];
     InflateLds& S = *reinterpret_cast<InflateLds*>(smem);
+    HdrScratch& H = S.u.h;
     const uint32_t l = lane_id();
 
     for (;;) {
@@ -437,6 +625,7 @@ __global__ __launch_bounds__(64) void inflate_kernel(const mdz_inflate_in_t* __r
         };
 
         bool final = false;
+        MDZ_PHASE_START();
         while (st == MDZ_ST_OK && !final) {
             r.refill();
             final = r.bits(1);
@@ -466,16 +655,17 @@ __global__ __launch_bounds__(64) void inflate_kernel(const mdz_inflate_in_t* __r
                 if (st != MDZ_ST_OK) break;
                 if (take < len) { st = MDZ_ST_INPUT_OVERRUN; break; }
                 r.seek(q + len);
+                MDZ_PHASE(4);
                 continue;
             }
             if (type == 1) {                                    // fixed tables
                 ++n_fixed;
                 if (!fixed_built) {
                     for (uint32_t s = l; s < 320; s += 64)
-                        S.lens[s] = s < 144 ? 8 : s < 256 ? 9 : s < 280 ? 7 : s < 288 ? 8 : 5;
+                        H.lens[s] = s < 144 ? 8 : s < 256 ? 9 : s < 280 ? 7 : s < 288 ? 8 : 5;
                     wave_sync();
-                    build_code(S.lit, S.lit_lut, kLitBits, S.lens, 288, 0);
-                    build_code(S.dist, S.dist_lut, kDistBits, S.lens + 288, 32, 1);
+                    build_table2(S.lit_c, S.lit, kLitRoot, kLitCap, H.lens, 288, 0);
+                    build_table2(S.dist_c, S.dist, kDistRoot, kDistCap, H.lens + 288, 32, 1);
                     fixed_built = 1;
                 }
             } else {                                            // dynamic tables
@@ -485,21 +675,21 @@ __global__ __launch_bounds__(64) void inflate_kernel(const mdz_inflate_in_t* __r
                 const uint32_t nlen = r.bits(5) + 257, ndist = r.bits(5) + 1, ncode = r.bits(4) + 4;
                 if (r.over()) { st = MDZ_ST_INPUT_OVERRUN; break; }
                 if (nlen > 286 || ndist > 30) { st = MDZ_ST_BAD_COUNTS; break; }
-                for (uint32_t s = l; s < 19; s += 64) S.lens[s] = 0;
+                for (uint32_t s = l; s < 19; s += 64) H.lens[s] = 0;
                 wave_sync();
                 for (uint32_t i = 0; i < ncode; ++i) {
                     r.refill();
                     const uint32_t v = r.bits(3);
-                    if (l == 0) S.lens[kClOrder[i]] = (uint8_t)v;
+                    if (l == 0) H.lens[kClOrder[i]] = (uint8_t)v;
                 }
                 wave_sync();
                 if (r.over()) { st = MDZ_ST_INPUT_OVERRUN; break; }
-                if (build_code(S.cl, S.cl_lut, 7, S.lens, 19, 2)) { st = MDZ_ST_BAD_CODE_LENS; break; }
+                if (build_code(H.cl, H.cl_lut, 7, H.lens, 19, 2)) { st = MDZ_ST_BAD_CODE_LENS; break; }
                 // code lengths for nlen + ndist symbols (lens[] reused from 0)
                 uint32_t idx = 0, prev = 0;
                 while (idx < nlen + ndist) {
                     r.refill();
-                    const uint32_t e = __builtin_amdgcn_readfirstlane(S.cl_lut[r.peek(7)]);
+                    const uint32_t e = __builtin_amdgcn_readfirstlane(H.cl_lut[r.peek(7)]);
                     r.drop(e & 15);
                     const uint32_t sym = e >> 16;
                     uint32_t rep = 0, val = 0;
@@ -512,125 +702,244 @@ __global__ __launch_bounds__(64) void inflate_kernel(const mdz_inflate_in_t* __r
                     if (idx + rep > nlen + ndist) { st = MDZ_ST_BAD_REPEAT; break; }
                     for (uint32_t j = l; j < rep; j += 64) {
                         const uint32_t t = idx + j;
-                        S.lens[t < nlen ? t : 288 + (t - nlen)] = (uint8_t)val;
+                        H.lens[t < nlen ? t : 288 + (t - nlen)] = (uint8_t)val;
                     }
                     idx += rep;
                 }
                 if (st != MDZ_ST_OK) break;
-                for (uint32_t s = nlen + l; s < 288; s += 64) S.lens[s] = 0;
-                for (uint32_t s = 288 + ndist + l; s < 320; s += 64) S.lens[s] = 0;
+                for (uint32_t s = nlen + l; s < 288; s += 64) H.lens[s] = 0;
+                for (uint32_t s = 288 + ndist + l; s < 320; s += 64) H.lens[s] = 0;
                 wave_sync();
-                if (S.lens[256] == 0) { st = MDZ_ST_NO_EOB; break; }
-                if (build_code(S.lit, S.lit_lut, kLitBits, S.lens, nlen, 0)) { st = MDZ_ST_BAD_LITLEN; break; }
-                if (build_code(S.dist, S.dist_lut, kDistBits, S.lens + 288, ndist, 1)) {
+                if (H.lens[256] == 0) { st = MDZ_ST_NO_EOB; break; }
+                if (build_table2(S.lit_c, S.lit, kLitRoot, kLitCap, H.lens, nlen, 0)) {
+                    st = MDZ_ST_BAD_LITLEN;
+                    break;
+                }
+                if (build_table2(S.dist_c, S.dist, kDistRoot, kDistCap, H.lens + 288, ndist, 1)) {
                     st = MDZ_ST_BAD_DIST;
                     break;
                 }
             }
 
-            // ---- symbols, in batches.  The two lookup tables are copied into
-            // registers (entry i in lane i % 64 of register i / 64): a lookup
-            // with the wave-uniform bit pattern is then an indexed register
-            // move and a readlane, with no LDS round trip on the decode chain.
-            uint32_t litv[(1u << kLitBits) / 64], distv[(1u << kDistBits) / 64];
-#pragma unroll
-            for (uint32_t e = 0; e < (1u << kLitBits) / 64; ++e) litv[e] = S.lit_lut[e * 64 + l];
-#pragma unroll
-            for (uint32_t e = 0; e < (1u << kDistBits) / 64; ++e) distv[e] = S.dist_lut[e * 64 + l];
+            // ---- symbols, in rounds.  Lane t takes the kSpan input bits from
+            // R + t*kSpan (R: the round's first true symbol) and decodes from
+            // there as if a symbol started at its first bit, noting each symbol
+            // start it meets (an end-of-block or invalid code restarts its
+            // decode one bit later).  Huffman decoding resynchronises: the true
+            // symbol sequence, followed past the end of lane t's span, soon meets
+            // a start lane t+1 noted, and from there lane t+1's sequence is the
+            // true one.  Lane t extends its decode into span t+1 until that
+            // happens (its hand-off point h_t).  The true path of the round is
+            // then lane 0 from R to h_0, lane t from h_{t-1} to h_t, ...; each
+            // lane decodes its piece once to count it and once to list its
+            // symbols in order, and the list is emitted in batches.
+            MDZ_PHASE(0);
+            uint64_t P = r.cbits;                  // the block's next symbol (stream bit)
+            const uintptr_t src_addr = (uintptr_t)r.src;
             bool eob = false;
             while (!eob && st == MDZ_ST_OK) {
-                uint32_t n = 0, bout = 0;
-                uint32_t rec_off = 0, rec_len = 0, rec_val = 0;  // lane k: symbol k of the batch
-                while (n < 64 && bout < kBatchOut) {
-                    r.refill();
-                    uint32_t e = lut_pick(litv, r.peek(kLitBits));
-                    if ((e & 15) == 0) {
-                        const int sym = slow_decode(S.lit, r);
-                        e = sym < 0 ? (kBad << 8) : lit_entry_bits((uint32_t)sym);
-                    } else {
-                        r.drop(e & 15);
-                    }
-                    const uint32_t kind = (e >> 8) & 3;
-                    if (r.over() || (kind == kBad && r.cbits + 1 > 8 * r.n)) {
-                        st = MDZ_ST_INPUT_OVERRUN;
-                        break;
-                    }
-                    if (kind == kBad) { st = MDZ_ST_BAD_SYMBOL; break; }
-                    if (kind == kEob) { eob = true; break; }
-                    uint32_t len = 1, val = e >> 16;
-                    if (kind == kRef) {
-                        r.refill();
-                        len = val + r.bits((e >> 4) & 15);
-                        uint32_t d = lut_pick(distv, r.peek(kDistBits));
-                        if ((d & 15) == 0) {
-                            const int sym = slow_decode(S.dist, r);
-                            d = sym < 0 ? (kBad << 8) : dist_entry_bits((uint32_t)sym);
-                        } else {
-                            r.drop(d & 15);
-                        }
-                        if (((d >> 8) & 3) == kBad) {
-                            st = r.over() || r.cbits + 1 > 8 * r.n ? MDZ_ST_INPUT_OVERRUN
-                                                                    : MDZ_ST_BAD_DISTANCE;
-                            break;
-                        }
-                        r.refill();
-                        val = (d >> 16) + r.bits((d >> 4) & 15);
-                        if (r.over()) { st = MDZ_ST_INPUT_OVERRUN; break; }
-                        if (val > pos + bout) { st = MDZ_ST_BAD_DISTANCE; break; }
-                    }
-                    if (l == n) { rec_off = bout; rec_len = len; rec_val = val; }
-                    ++n;
-                    n_match += len > 1;
-                    bout += len;
-                    if (pos + bout > cap) break;       // overrun: nothing after it is decoded
+                // stage the round's input: dwords from an aligned address A
+                const uint64_t byte = P >> 3;
+                const uintptr_t A = (src_addr + byte) & ~(uintptr_t)3;
+                const uint32_t rel = (uint32_t)(8 * (src_addr + byte - A) + (P & 7));
+                const uint64_t limb = 8 * (uint64_t)(src_addr + r.n - A);
+                const uint32_t lim = limb > 0xF0000000ull ? 0xF0000000u : (uint32_t)limb;
+                for (uint32_t k = l; k < kStageWords; k += 64) S.in[k] = r.load_dword(A + 4 * k);
+                wave_sync();
+                MDZ_PHASE(5);
+
+                // phase 1: each lane's own span (positions past the stream end
+                // start no symbol; the true path meets them only as an overrun)
+                const uint32_t s_me = rel + kSpan * l, s_nx = s_me + kSpan;
+                uint32_t bm[kSpan / 32];
+#pragma unroll
+                for (uint32_t k = 0; k < kSpan / 32; ++k) bm[k] = 0;
+                uint32_t p = s_me, sym = 0, used = 0;
+                while (p < s_nx) {
+                    if (p >= lim) break;
+                    const uint32_t k = p - s_me;
+#pragma unroll
+                    for (uint32_t w = 0; w < kSpan / 32; ++w)
+                        if ((k >> 5) == w) bm[w] |= 1u << (k & 31);
+                    const uint32_t rc = decode_sym(S, p, lim, &sym, &used);
+                    p += rc == kSymOk ? used : 1;
                 }
-                n_sym += n;
-                // ---- emit the batch: literals, then matches in order.  The
+                if (p < s_nx) p = s_nx;
+#pragma unroll
+                for (uint32_t k = 0; k < kSpan / 32; ++k) S.u.bm[(kSpan / 32) * l + k] = bm[k];
+                wave_sync();
+                MDZ_PHASE(6);
+
+                // phase 2: hand-off.  From its span's end the lane follows its
+                // sequence until it meets a start noted by a later lane g (in
+                // the span it has reached, at most kReach spans ahead).  stop:
+                // the sequence has its end of block or an error at h; lost: no
+                // noted start met in reach.  Lane 63 hands off to the next round.
+                uint32_t h = p, g = 64;
+                bool stop = false, lost = false;
+                if (l < 63) {
+                    for (;;) {
+                        const uint32_t j = (h - rel) >> kSpanLog;
+                        if (j > l + kReach || j > 63) { lost = true; break; }
+                        const uint32_t k = h - (rel + kSpan * j);
+                        if ((S.u.bm[(kSpan / 32) * j + (k >> 5)] >> (k & 31)) & 1) { g = j; break; }
+                        const uint32_t rc = decode_sym(S, h, lim, &sym, &used);
+                        if (rc != kSymOk) { stop = true; break; }
+                        h += used;
+                    }
+                }
+                MDZ_PHASE(7);
+                // the true chain: lane 0, then g(0), g(g(0)), ...; each member's
+                // piece starts where its predecessor handed off
+                uint32_t a = rel;
+                uint64_t chain = 0;
+                for (uint32_t cur = 0;;) {
+                    chain |= 1ull << cur;
+                    const uint32_t nx = __builtin_amdgcn_readlane(g, cur);
+                    if (nx > 63) break;
+                    const uint32_t hc = __builtin_amdgcn_readlane(h, cur);
+                    if (l == nx) a = hc;
+                    cur = nx;
+                }
+                const bool member = (chain >> l) & 1;
+
+                // phase 3: count each member's piece [a, h) (and the event at h)
+                uint32_t cnt = 0, ev = kSymOk, evst = 0, evpos = 0, evlen = 0;
+                if (member) {
+                    p = a;
+                    while (p < h) {
+                        const uint32_t rc = decode_sym(S, p, lim, &sym, &used);
+                        if (rc != kSymOk) { ev = rc; evst = sym; evpos = p; evlen = used; break; }
+                        ++cnt;
+                        p += used;
+                    }
+                    if (ev == kSymOk && stop) {
+                        ev = decode_sym(S, h, lim, &sym, &used);
+                        evst = sym;
+                        evpos = h;
+                        evlen = used;
+                    }
+                    if (ev == kSymOk && p != h) { ev = kSymErr; evst = (uint32_t)MDZ_ST_INTERNAL; evpos = p; }
+                }
+                // the round ends at the first member with an event or a lost hand-off
+                const uint64_t endm = __ballot(member && (ev != kSymOk || lost));
+                const uint32_t top = 63 - (uint32_t)__builtin_clzll(chain);
+                uint32_t last = endm ? (uint32_t)__builtin_ctzll(endm) : top;
+                const uint32_t incl = wave_incl_scan_u32(member && l <= last ? cnt : 0);
+                // keep whole pieces within the symbol list's capacity
+                const uint64_t over = __ballot(member && l <= last && incl > kSymCap);
+                const bool cut = over != 0;
+                if (cut) {   // the last member below the first that overflows (lane 0's piece always fits)
+                    const uint64_t below = chain & ((1ull << __builtin_ctzll(over)) - 1);
+                    last = 63 - (uint32_t)__builtin_clzll(below);
+                }
+                const uint32_t nsym = __builtin_amdgcn_readlane(incl, last);
+                const uint32_t r_ev = cut ? kSymOk : __builtin_amdgcn_readlane(ev, last);
+                const uint32_t r_evst = __builtin_amdgcn_readlane(evst, last);
+                const uint32_t r_evpos = __builtin_amdgcn_readlane(evpos, last);
+                const uint32_t r_evlen = __builtin_amdgcn_readlane(evlen, last);
+                const uint32_t r_h = __builtin_amdgcn_readlane(h, last);
+                wave_sync();      // the hand-off bits are dead: the list takes their space
+                MDZ_PHASE(8);
+
+                // phase 4: list the true symbols in order
+                if (member && l <= last) {
+                    uint32_t o = incl - cnt;
+                    p = a;
+                    for (uint32_t i = 0; i < cnt; ++i) {
+                        decode_sym(S, p, lim, &sym, &used);
+                        S.u.sym[o++] = sym;
+                        p += used;
+                    }
+                }
+                wave_sync();
+                MDZ_PHASE(1);
+
+                // ---- emit the list in batches (at most 64 symbols or about
+                // kBatchOut bytes each): literals, then matches in order.  The
                 // workgroup is this one wave and its LDS operations execute in
                 // order, so a match reads what earlier literals and matches of
-                // the batch stored without a wait in between; ring positions
-                // are kept below 2 * kRing and wrapped with one compare.
-                const uint64_t room = pos < cap ? cap - pos : 0;
-                const uint32_t rb = ring_at(pos);          // ring index of the batch start
-                auto wrap = [](uint32_t x) { return x >= kRing ? x - kRing : x; };
-                // matches are at least 3 long, so rec_len == 1 marks a literal
-                if (l < n && rec_len == 1) S.ring[wrap(rb + rec_off)] = (uint8_t)rec_val;
-                __builtin_amdgcn_wave_barrier();
-                uint64_t mm = __ballot(l < n && rec_len > 1);
-                while (mm) {
-                    const uint32_t m = __builtin_ctzll(mm);
-                    mm &= mm - 1;
-                    const uint32_t off = __builtin_amdgcn_readlane(rec_off, m);
-                    const uint32_t len = __builtin_amdgcn_readlane(rec_len, m);
-                    const uint32_t dist = __builtin_amdgcn_readlane(rec_val, m);
-                    const uint32_t d0 = wrap(rb + off);                  // destination start
-                    const uint32_t s0 = wrap(d0 + kRing - dist);         // source start (dist <= 32768)
-                    if (dist >= len) {
-                        for (uint32_t j = l; j < len; j += 64) S.ring[wrap(d0 + j)] = S.ring[wrap(s0 + j)];
-                    } else {
-                        // overlapping copy: byte j repeats source byte j mod dist;
-                        // j < 258, so a float reciprocal and one correction step
-                        // give the exact quotient
-                        const float rd = __builtin_amdgcn_rcpf((float)dist);
-                        for (uint32_t j = l; j < len; j += 64) {
-                            uint32_t qj = (uint32_t)((float)j * rd);
-                            if (qj * dist > j) --qj;
-                            else if ((qj + 1) * dist <= j) ++qj;
-                            S.ring[wrap(d0 + j)] = S.ring[wrap(s0 + (j - qj * dist))];
-                        }
+                // the batch stored without a wait in between; ring positions are
+                // kept below 2 * kRing and wrapped with one compare.
+                for (uint32_t b0 = 0; b0 < nsym && st == MDZ_ST_OK;) {
+                    const uint32_t i = b0 + l;
+                    const uint32_t x = i < nsym ? S.u.sym[i] : 0;
+                    const bool have = i < nsym, ref = have && (x >> 31);
+                    const uint32_t rec_len = !have ? 0 : ref ? ((x >> 15) & 255) + 3 : 1;
+                    const uint32_t rec_val = ref ? (x & 32767) + 1 : (x & 255);
+                    const uint32_t oin = wave_incl_scan_u32(rec_len);
+                    const uint32_t rec_off = oin - rec_len;
+                    // batch rules, in stream order: a symbol joins while fewer than
+                    // kBatchOut bytes are queued; a distance past the output start
+                    // is an error before the symbol; an overrun ends the batch with
+                    // the overrunning symbol included
+                    const uint64_t full = __ballot(have && rec_off >= kBatchOut);
+                    const uint64_t bad = __ballot(ref && rec_val > pos + rec_off);
+                    const uint64_t ovr = __ballot(have && pos + oin > cap);
+                    uint32_t n = min(nsym - b0, 64u);
+                    if (full) n = min(n, (uint32_t)__builtin_ctzll(full));
+                    if (ovr) n = min(n, (uint32_t)__builtin_ctzll(ovr) + 1);
+                    if (bad && (uint32_t)__builtin_ctzll(bad) < n) {
+                        n = (uint32_t)__builtin_ctzll(bad);
+                        st = MDZ_ST_BAD_DISTANCE;
                     }
+                    const uint32_t bout = n ? __builtin_amdgcn_readlane(oin, n - 1) : 0;
+                    n_sym += n;
+                    n_match += __popcll(__ballot(ref && l < n));
+                    const uint64_t room = pos < cap ? cap - pos : 0;
+                    const uint32_t rb = ring_at(pos);          // ring index of the batch start
+                    auto wrap = [](uint32_t v) { return v >= kRing ? v - kRing : v; };
+                    if (l < n && !ref) S.ring[wrap(rb + rec_off)] = (uint8_t)rec_val;
                     __builtin_amdgcn_wave_barrier();
+                    uint64_t mm = __ballot(l < n && ref);
+                    while (mm) {
+                        const uint32_t m = __builtin_ctzll(mm);
+                        mm &= mm - 1;
+                        const uint32_t off = __builtin_amdgcn_readlane(rec_off, m);
+                        const uint32_t len = __builtin_amdgcn_readlane(rec_len, m);
+                        const uint32_t dist = __builtin_amdgcn_readlane(rec_val, m);
+                        const uint32_t d0 = wrap(rb + off);                  // destination start
+                        const uint32_t s0 = wrap(d0 + kRing - dist);         // source start (dist <= 32768)
+                        if (dist >= len) {
+                            for (uint32_t j = l; j < len; j += 64) S.ring[wrap(d0 + j)] = S.ring[wrap(s0 + j)];
+                        } else {
+                            // overlapping copy: byte j repeats source byte j mod dist;
+                            // j < 258, so a float reciprocal and one correction step
+                            // give the exact quotient
+                            const float rd = __builtin_amdgcn_rcpf((float)dist);
+                            for (uint32_t j = l; j < len; j += 64) {
+                                uint32_t qj = (uint32_t)((float)j * rd);
+                                if (qj * dist > j) --qj;
+                                else if ((qj + 1) * dist <= j) ++qj;
+                                S.ring[wrap(d0 + j)] = S.ring[wrap(s0 + (j - qj * dist))];
+                            }
+                        }
+                        __builtin_amdgcn_wave_barrier();
+                    }
+                    if (bout > room) {
+                        pos += room;
+                        flush(pos, true);
+                        st = MDZ_ST_OUTPUT_OVERRUN;
+                        break;
+                    }
+                    pos += bout;
+                    flush(pos, false);
+                    wave_sync();
+                    b0 += n;
                 }
-                if (bout > room) {
-                    pos += room;
-                    flush(pos, true);
-                    st = MDZ_ST_OUTPUT_OVERRUN;
-                    break;
+                MDZ_PHASE(2);
+                if (st != MDZ_ST_OK) break;
+                if (r_ev == kSymEob) {
+                    eob = true;
+                    P += (r_evpos + r_evlen) - rel;
+                } else if (r_ev == kSymErr) {
+                    st = (int)r_evst;
+                    P += r_evpos - rel;
+                } else {
+                    P += r_h - rel;
                 }
-                pos += bout;
-                flush(pos, false);
-                wave_sync();
             }
+            r.seek_bit(P);
         }
         flush(pos, true);
         if (l == 0) {
