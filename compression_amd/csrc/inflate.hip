@@ -755,13 +755,20 @@ __global__ __launch_bounds__(64) void inflate_kernel(const mdz_inflate_in_t* __r
 #pragma unroll
                 for (uint32_t k = 0; k < kSpan / 32; ++k) bm[k] = 0;
                 uint32_t p = s_me, sym = 0, used = 0;
+                uint32_t ev1 = ~0u, ev2 = ~0u, nev = 0;   // first two events (restarts)
+                bool at_end = false;
                 while (p < s_nx) {
-                    if (p >= lim) break;
+                    if (p >= lim) { at_end = true; break; }
                     const uint32_t k = p - s_me;
 #pragma unroll
                     for (uint32_t w = 0; w < kSpan / 32; ++w)
                         if ((k >> 5) == w) bm[w] |= 1u << (k & 31);
                     const uint32_t rc = decode_sym(S, p, lim, &sym, &used);
+                    if (rc != kSymOk) {
+                        if (nev == 0) ev1 = p;
+                        else if (nev == 1) ev2 = p;
+                        ++nev;
+                    }
                     p += rc == kSymOk ? used : 1;
                 }
                 if (p < s_nx) p = s_nx;
@@ -775,7 +782,7 @@ __global__ __launch_bounds__(64) void inflate_kernel(const mdz_inflate_in_t* __r
                 // the span it has reached, at most kReach spans ahead).  stop:
                 // the sequence has its end of block or an error at h; lost: no
                 // noted start met in reach.  Lane 63 hands off to the next round.
-                uint32_t h = p, g = 64;
+                uint32_t h = p, g = 64, steps = 0;
                 bool stop = false, lost = false;
                 if (l < 63) {
                     for (;;) {
@@ -786,6 +793,7 @@ __global__ __launch_bounds__(64) void inflate_kernel(const mdz_inflate_in_t* __r
                         const uint32_t rc = decode_sym(S, h, lim, &sym, &used);
                         if (rc != kSymOk) { stop = true; break; }
                         h += used;
+                        ++steps;
                     }
                 }
                 MDZ_PHASE(7);
@@ -803,23 +811,50 @@ __global__ __launch_bounds__(64) void inflate_kernel(const mdz_inflate_in_t* __r
                 }
                 const bool member = (chain >> l) & 1;
 
-                // phase 3: count each member's piece [a, h) (and the event at h)
+                // phase 3: count each member's piece [a, h) and find the event
+                // that ends it, if any.  From a (one of its noted starts) the
+                // lane's own sequence is the true one, so up to the first event
+                // at or after a its noted starts are the piece's symbols, and past
+                // its span the hand-off steps are.  A lane that met more than two
+                // events, or the stream's end, decodes its piece to count it.
                 uint32_t cnt = 0, ev = kSymOk, evst = 0, evpos = 0, evlen = 0;
                 if (member) {
-                    p = a;
-                    while (p < h) {
-                        const uint32_t rc = decode_sym(S, p, lim, &sym, &used);
-                        if (rc != kSymOk) { ev = rc; evst = sym; evpos = p; evlen = used; break; }
-                        ++cnt;
-                        p += used;
+                    const uint32_t x = ev1 >= a ? ev1 : ev2 >= a ? ev2 : ~0u;
+                    bool need_ev = false;
+                    if (nev > 2 || at_end) {
+                        p = a;
+                        while (p < h) {
+                            const uint32_t rc = decode_sym(S, p, lim, &sym, &used);
+                            if (rc != kSymOk) { ev = rc; evst = sym; evpos = p; evlen = used; break; }
+                            ++cnt;
+                            p += used;
+                        }
+                        if (ev == kSymOk && p != h) { ev = kSymErr; evst = (uint32_t)MDZ_ST_INTERNAL; evpos = p; }
+                        if (ev == kSymOk && stop) { evpos = h; need_ev = true; }
+                    } else {
+                        // noted starts in [a, min(x, span end))
+                        const uint32_t lo = a - s_me, hi = x < s_nx ? x - s_me : kSpan;
+#pragma unroll
+                        for (uint32_t w = 0; w < kSpan / 32; ++w) {
+                            const uint32_t b0 = 32 * w;
+                            uint32_t m = bm[w];
+                            if (lo > b0) m = lo - b0 >= 32 ? 0u : m & (~0u << (lo - b0));
+                            if (hi < b0 + 32) m = hi <= b0 ? 0u : m & (~0u >> (b0 + 32 - hi));
+                            cnt += __popc(m);
+                        }
+                        if (x < s_nx) {
+                            evpos = x;
+                            need_ev = true;
+                        } else {
+                            cnt += steps;
+                            if (stop) { evpos = h; need_ev = true; }
+                        }
                     }
-                    if (ev == kSymOk && stop) {
-                        ev = decode_sym(S, h, lim, &sym, &used);
+                    if (need_ev) {
+                        ev = decode_sym(S, evpos, lim, &sym, &used);
                         evst = sym;
-                        evpos = h;
                         evlen = used;
                     }
-                    if (ev == kSymOk && p != h) { ev = kSymErr; evst = (uint32_t)MDZ_ST_INTERNAL; evpos = p; }
                 }
                 // the round ends at the first member with an event or a lost hand-off
                 const uint64_t endm = __ballot(member && (ev != kSymOk || lost));
@@ -843,6 +878,7 @@ __global__ __launch_bounds__(64) void inflate_kernel(const mdz_inflate_in_t* __r
                 MDZ_PHASE(8);
 
                 // phase 4: list the true symbols in order
+                bool astray = false;
                 if (member && l <= last) {
                     uint32_t o = incl - cnt;
                     p = a;
@@ -851,8 +887,11 @@ __global__ __launch_bounds__(64) void inflate_kernel(const mdz_inflate_in_t* __r
                         S.u.sym[o++] = sym;
                         p += used;
                     }
+                    // the listed symbols must end where the piece does
+                    astray = p != (ev != kSymOk ? evpos : h);
                 }
                 wave_sync();
+                if (__ballot(astray)) { st = MDZ_ST_INTERNAL; break; }   // never expected
                 MDZ_PHASE(1);
 
                 // ---- emit the list in batches (at most 64 symbols or about
