@@ -43,6 +43,21 @@
 // the per-device address from hipGetSymbolAddress, never this name.
 static __device__ __align__(256) uint32_t scratch[64];
 
+// Phase timing of the two roles (profiling builds only): lane 0 of each wave
+// adds the cycles since its previous mark to mdz_phase_cycles[k].
+#ifdef MDZ_PHASE_PROF
+#define INF_PHASE_START() uint64_t inf_phase_t_ = clock64()
+#define INF_PHASE(k)                                                                 \
+    do {                                                                             \
+        const uint64_t inf_now_ = clock64();                                         \
+        if (lane_id() == 0) atomicAdd(&mdz_phase_cycles[k], inf_now_ - inf_phase_t_); \
+        inf_phase_t_ = inf_now_;                                                     \
+    } while (0)
+#else
+#define INF_PHASE_START() ((void)0)
+#define INF_PHASE(k) ((void)0)
+#endif
+
 namespace mdz {
 namespace {
 
@@ -100,6 +115,26 @@ struct HdrScratch {
     Code cl;
 };
 
+// One entry of the queue between the producer wave (block headers and
+// symbol decode) and the consumer wave (LZ77 output and stores).
+enum : uint32_t { kItemRound = 0, kItemStored = 1, kItemNone = 2 };
+struct Item {
+    uint32_t kind;        // a round's symbols, a stored block, or nothing
+    uint32_t nsym;        // round: symbols listed in the item's buffer
+    int32_t status;       // after the item's output: the stream's status (0: none)
+    uint32_t last;        // the stream ends with this item
+    uint64_t src;         // stored: stream byte offset of the data
+    uint64_t take;        // stored: bytes present (take < LEN ends in an input overrun)
+};
+
+struct Ctrl {
+    uint32_t qi;          // queue position of the stream in hand
+    uint32_t halt;        // the consumer has stopped on an error
+    uint32_t done;        // the consumer has finished the stream
+    uint32_t n_stored, n_fixed, n_dyn;
+    uint64_t consumed;    // Deflate input bits through the last item
+};
+
 struct InflateLds {
     uint32_t lit[kLitCap];                     // literal/length table, 10-bit root
     uint32_t dist[kDistCap];                   // distance table, 8-bit root
@@ -108,11 +143,12 @@ struct InflateLds {
     union {
         HdrScratch h;
         uint32_t bm[64 * (kSpan / 32)];        // per lane: symbol starts met in its span
-        uint32_t sym[kSymCap];                 // the round's symbols, in stream order
     } u;
-    uint8_t ring[kRing];
+    uint32_t sym[2][kSymCap];                  // rounds' symbols, in stream order (double buffer)
+    Item item[2];
+    Ctrl ctrl;
+    uint8_t ring[kRing];                       // the consumer's output window
 };
-static_assert(sizeof(HdrScratch) <= sizeof(uint32_t) * kSymCap, "header scratch fits");
 static_assert(sizeof(InflateLds) <= 80 * 1024, "two workgroups per CU");
 
 MDZ_DEV uint32_t bitrev(uint32_t v, uint32_t n) { return __builtin_bitreverse32(v) >> (32 - n); }
@@ -522,487 +558,561 @@ MDZ_DEV uint32_t stream_checksum(const uint8_t* p, uint64_t n, bool crc) {
     return crc ? ~R : (B << 16) | A;
 }
 
-__global__ __launch_bounds__(64) void inflate_kernel(const mdz_inflate_in_t* __restrict__ ins,
-                                                     mdz_inflate_status_t* __restrict__ outs,
-                                                     const uint32_t* __restrict__ order,
-                                                     uint32_t count, int parse_hdr,
-                                                     uint32_t* __restrict__ cursor,
-                                                     mdz_inflate_cost_t* __restrict__ cost) {
+// Two waves per stream.  Wave 1, the producer, parses block headers, builds
+// the tables and decodes the symbols of one round at a time into one of two
+// LDS lists; wave 0, the consumer, turns the previous round's list (or a
+// stored block) into output in the LDS window and stores it to HBM.  Both
+// meet at one workgroup barrier per step, so round k+1 decodes while round k
+// is written out.
+__global__ __launch_bounds__(128) void inflate_kernel(const mdz_inflate_in_t* __restrict__ ins,
+                                                      mdz_inflate_status_t* __restrict__ outs,
+                                                      const uint32_t* __restrict__ order,
+                                                      uint32_t count, int parse_hdr,
+                                                      uint32_t* __restrict__ cursor,
+                                                      mdz_inflate_cost_t* __restrict__ cost) {
     extern __shared__ __align__(16) uint8_t smem[];
     InflateLds& S = *reinterpret_cast<InflateLds*>(smem);
     HdrScratch& H = S.u.h;
     const uint32_t l = lane_id();
+    const bool producer = threadIdx.x >= 64;
 
     for (;;) {
         // next stream: one per block (plain) or from the shared cursor (persistent)
-        uint32_t qi;
-        if (cursor) {
-            uint32_t t = 0;
-            if (l == 0) t = atomicAdd(cursor, 1u);
-            qi = __builtin_amdgcn_readfirstlane(t);
-        } else {
-            qi = blockIdx.x;
+        if (threadIdx.x == 0) {
+            S.ctrl.qi = cursor ? atomicAdd(cursor, 1u) : blockIdx.x;
+            S.ctrl.halt = 0;
+            S.ctrl.done = 0;
         }
+        __syncthreads();
+        const uint32_t qi = S.ctrl.qi;
         if (qi >= count) return;
         const uint32_t si = order ? order[qi] : qi;
         if (si >= count) {                         // a bad order entry names no stream
             if (!cursor) return;
+            __syncthreads();
             continue;
         }
         const mdz_inflate_in_t in = ins[si];
-        uint8_t* const dst = reinterpret_cast<uint8_t*>(in.dst);
-        const uint64_t cap = in.dst_bytes;
-
         const int hdr = parse_hdr & 3;
-        Reader r;
-        r.src = reinterpret_cast<const uint8_t*>(in.src);
-        r.n = in.src_bytes;
-        int st = MDZ_ST_OK;
-        if (hdr == MDZ_HDR_GZIP) {
-            // the gzip prepass left the header length (or -1) in the status
-            // slot; as cuDF, the footer is cut off and a rejected header
-            // reports -1 with nothing written
-            const mdz_inflate_status_t pre = outs[si];
-            r.n = r.n >= 8 ? r.n - 8 : 0;
-            if (pre.status != MDZ_ST_OK) {
-                st = MDZ_ST_BAD_BLOCK_TYPE;
-            } else {
-                r.src += pre.bytes_written;
-                r.n -= pre.bytes_written;
-            }
-        }
-        uint64_t pos = 0, flushed = 0;             // output produced / flushed to dst
-        uint32_t fixed_built = 0;
-        uint32_t n_sym = 0, n_match = 0, n_stored = 0, n_fixed = 0, n_dyn = 0;
-        const uint64_t t0 = clock64();
 
-        r.seek(0);
-        if (hdr == MDZ_HDR_ZLIB) {
-            r.refill();
-            const uint32_t cmf = r.bits(8), flg = r.bits(8);
-            if (r.over()) st = MDZ_ST_INPUT_OVERRUN;
-            else if ((cmf & 15) != 8 || (cmf >> 4) > 7 || (flg & 0x20) || ((cmf << 8) | flg) % 31)
-                st = MDZ_ST_BAD_HEADER;
-        }
-
-        // flushes [flushed, upto) in dword stores at dword-aligned dst
-        // addresses; before the last flush it stops at such an address, so
-        // only the first flush has head bytes and only the last tail bytes
-        const uint32_t dmis = (uint32_t)((uintptr_t)dst & 3);
-        auto flush = [&](uint64_t upto, bool all) {
-            uint64_t lim = min(upto, cap);
-            if (!all) {
-                const uint64_t cut = (lim + dmis) & 3;
-                lim = lim >= cut ? lim - cut : 0;
-            }
-            if (lim <= flushed) return;
-            // at most one batch (< kRing bytes) is pending: ring indices are
-            // the flush start's plus an offset, wrapped once; kRing % 4 == 0,
-            // so ring index and output offset agree mod 4
-            const uint32_t rf = ring_at(flushed);
-            auto at = [&](uint64_t p) {
-                const uint32_t x = rf + (uint32_t)(p - flushed);
-                return x >= kRing ? x - kRing : x;
-            };
-            const uint64_t h = min(lim, flushed + ((4 - ((dmis + flushed) & 3)) & 3));
-            for (uint64_t p = flushed + l; p < h; p += 64) dst[p] = S.ring[at(p)];
-            const uint64_t nw = (lim - h) / 4;
-            if (dmis == 0) {                       // h % 4 == 0: aligned ring dwords
-                for (uint64_t w = l; w < nw; w += 64) {
-                    const uint64_t p = h + 4 * w;
-                    *reinterpret_cast<uint32_t*>(dst + p) = *reinterpret_cast<const uint32_t*>(&S.ring[at(p)]);
-                }
-            } else {                               // ring bytes gathered into aligned dst dwords
-                for (uint64_t w = l; w < nw; w += 64) {
-                    const uint64_t p = h + 4 * w;
-                    *reinterpret_cast<uint32_t*>(dst + p) =
-                        (uint32_t)S.ring[at(p)] | ((uint32_t)S.ring[at(p + 1)] << 8) |
-                        ((uint32_t)S.ring[at(p + 2)] << 16) | ((uint32_t)S.ring[at(p + 3)] << 24);
+        if (producer) {
+            // ------------------------------------------------------ producer
+            Reader r;
+            r.src = reinterpret_cast<const uint8_t*>(in.src);
+            r.n = in.src_bytes;
+            int st = MDZ_ST_OK;
+            if (hdr == MDZ_HDR_GZIP) {
+                // the gzip prepass left the header length (or -1) in the status
+                // slot; as cuDF, the footer is cut off and a rejected header
+                // reports -1 with nothing written
+                const mdz_inflate_status_t pre = outs[si];
+                r.n = r.n >= 8 ? r.n - 8 : 0;
+                if (pre.status != MDZ_ST_OK) {
+                    st = MDZ_ST_BAD_BLOCK_TYPE;
+                } else {
+                    r.src += pre.bytes_written;
+                    r.n -= pre.bytes_written;
                 }
             }
-            for (uint64_t p = h + 4 * nw + l; p < lim; p += 64) dst[p] = S.ring[at(p)];
-            flushed = lim;
-        };
-
-        bool final = false;
-        MDZ_PHASE_START();
-        while (st == MDZ_ST_OK && !final) {
-            r.refill();
-            final = r.bits(1);
-            const uint32_t type = r.bits(2);
-            if (r.over()) { st = MDZ_ST_INPUT_OVERRUN; break; }
-            if (type == 3) { st = MDZ_ST_BAD_BLOCK_TYPE; break; }
-            if (type == 0) {                                    // stored
-                ++n_stored;
-                r.drop(r.nb & 7);
+            r.seek(0);
+            if (st == MDZ_ST_OK && hdr == MDZ_HDR_ZLIB) {
                 r.refill();
-                const uint32_t len = r.bits(16), nlen = r.bits(16);
-                if (r.over()) { st = MDZ_ST_INPUT_OVERRUN; break; }
-                if (len != (~nlen & 0xffffu)) { st = MDZ_ST_STORED_LEN; break; }
-                const uint64_t q = r.cbits / 8;
-                uint64_t avail = q <= r.n ? r.n - q : 0, take = min<uint64_t>(len, avail);
-                for (uint64_t done = 0; done < take;) {
-                    const uint64_t k = min<uint64_t>(take - done, kBatchOut);
-                    for (uint64_t j = l; j < k; j += 64) S.ring[ring_at(pos + j)] = r.src[q + done + j];
-                    wave_sync();
-                    const uint64_t room = pos < cap ? cap - pos : 0;
-                    pos += min(k, room);
-                    flush(pos, false);
-                    wave_sync();
-                    done += k;
-                    if (k > room) { st = MDZ_ST_OUTPUT_OVERRUN; break; }
-                }
-                if (st != MDZ_ST_OK) break;
-                if (take < len) { st = MDZ_ST_INPUT_OVERRUN; break; }
-                r.seek(q + len);
-                MDZ_PHASE(4);
-                continue;
+                const uint32_t cmf = r.bits(8), flg = r.bits(8);
+                if (r.over()) st = MDZ_ST_INPUT_OVERRUN;
+                else if ((cmf & 15) != 8 || (cmf >> 4) > 7 || (flg & 0x20) || ((cmf << 8) | flg) % 31)
+                    st = MDZ_ST_BAD_HEADER;
             }
-            if (type == 1) {                                    // fixed tables
-                ++n_fixed;
-                if (!fixed_built) {
-                    for (uint32_t s = l; s < 320; s += 64)
-                        H.lens[s] = s < 144 ? 8 : s < 256 ? 9 : s < 280 ? 7 : s < 288 ? 8 : 5;
-                    wave_sync();
-                    build_table2(S.lit_c, S.lit, kLitRoot, kLitCap, H.lens, 288, 0);
-                    build_table2(S.dist_c, S.dist, kDistRoot, kDistCap, H.lens + 288, 32, 1);
-                    fixed_built = 1;
-                }
-            } else {                                            // dynamic tables
-                ++n_dyn;
-                fixed_built = 0;
-                r.refill();
-                const uint32_t nlen = r.bits(5) + 257, ndist = r.bits(5) + 1, ncode = r.bits(4) + 4;
-                if (r.over()) { st = MDZ_ST_INPUT_OVERRUN; break; }
-                if (nlen > 286 || ndist > 30) { st = MDZ_ST_BAD_COUNTS; break; }
-                for (uint32_t s = l; s < 19; s += 64) H.lens[s] = 0;
-                wave_sync();
-                for (uint32_t i = 0; i < ncode; ++i) {
-                    r.refill();
-                    const uint32_t v = r.bits(3);
-                    if (l == 0) H.lens[kClOrder[i]] = (uint8_t)v;
-                }
-                wave_sync();
-                if (r.over()) { st = MDZ_ST_INPUT_OVERRUN; break; }
-                if (build_code(H.cl, H.cl_lut, 7, H.lens, 19, 2)) { st = MDZ_ST_BAD_CODE_LENS; break; }
-                // code lengths for nlen + ndist symbols (lens[] reused from 0)
-                uint32_t idx = 0, prev = 0;
-                while (idx < nlen + ndist) {
-                    r.refill();
-                    const uint32_t e = __builtin_amdgcn_readfirstlane(H.cl_lut[r.peek(7)]);
-                    r.drop(e & 15);
-                    const uint32_t sym = e >> 16;
-                    uint32_t rep = 0, val = 0;
-                    if (sym < 16) { val = sym; rep = 1; prev = sym; }
-                    else if (sym == 16) { val = prev; rep = 3 + r.bits(2); }
-                    else if (sym == 17) { rep = 3 + r.bits(3); prev = 0; }
-                    else { rep = 11 + r.bits(7); prev = 0; }
-                    if (r.over()) { st = MDZ_ST_INPUT_OVERRUN; break; }
-                    if (sym == 16 && idx == 0) { st = MDZ_ST_BAD_REPEAT; break; }
-                    if (idx + rep > nlen + ndist) { st = MDZ_ST_BAD_REPEAT; break; }
-                    for (uint32_t j = l; j < rep; j += 64) {
-                        const uint32_t t = idx + j;
-                        H.lens[t < nlen ? t : 288 + (t - nlen)] = (uint8_t)val;
-                    }
-                    idx += rep;
-                }
-                if (st != MDZ_ST_OK) break;
-                for (uint32_t s = nlen + l; s < 288; s += 64) H.lens[s] = 0;
-                for (uint32_t s = 288 + ndist + l; s < 320; s += 64) H.lens[s] = 0;
-                wave_sync();
-                if (H.lens[256] == 0) { st = MDZ_ST_NO_EOB; break; }
-                if (build_table2(S.lit_c, S.lit, kLitRoot, kLitCap, H.lens, nlen, 0)) {
-                    st = MDZ_ST_BAD_LITLEN;
-                    break;
-                }
-                if (build_table2(S.dist_c, S.dist, kDistRoot, kDistCap, H.lens + 288, ndist, 1)) {
-                    st = MDZ_ST_BAD_DIST;
-                    break;
-                }
-            }
-
-            // ---- symbols, in rounds.  Lane t takes the kSpan input bits from
-            // R + t*kSpan (R: the round's first true symbol) and decodes from
-            // there as if a symbol started at its first bit, noting each symbol
-            // start it meets (an end-of-block or invalid code restarts its
-            // decode one bit later).  Huffman decoding resynchronises: the true
-            // symbol sequence, followed past the end of lane t's span, soon meets
-            // a start lane t+1 noted, and from there lane t+1's sequence is the
-            // true one.  Lane t extends its decode into span t+1 until that
-            // happens (its hand-off point h_t).  The true path of the round is
-            // then lane 0 from R to h_0, lane t from h_{t-1} to h_t, ...; each
-            // lane decodes its piece once to count it and once to list its
-            // symbols in order, and the list is emitted in batches.
-            MDZ_PHASE(0);
-            uint64_t P = r.cbits;                  // the block's next symbol (stream bit)
+            uint64_t P = r.cbits;                  // next bit of the Deflate data
+            INF_PHASE_START();
+            bool need_hdr = true, final = false, pdone = false;
+            uint32_t fixed_built = 0, n_stored = 0, n_fixed = 0, n_dyn = 0;
             const uintptr_t src_addr = (uintptr_t)r.src;
-            bool eob = false;
-            while (!eob && st == MDZ_ST_OK) {
-                // stage the round's input: dwords from an aligned address A
-                const uint64_t byte = P >> 3;
-                const uintptr_t A = (src_addr + byte) & ~(uintptr_t)3;
-                const uint32_t rel = (uint32_t)(8 * (src_addr + byte - A) + (P & 7));
-                const uint64_t limb = 8 * (uint64_t)(src_addr + r.n - A);
-                const uint32_t lim = limb > 0xF0000000ull ? 0xF0000000u : (uint32_t)limb;
-                for (uint32_t k = l; k < kStageWords; k += 64) S.in[k] = r.load_dword(A + 4 * k);
-                wave_sync();
-                MDZ_PHASE(5);
-
-                // phase 1: each lane's own span (positions past the stream end
-                // start no symbol; the true path meets them only as an overrun)
-                const uint32_t s_me = rel + kSpan * l, s_nx = s_me + kSpan;
-                uint32_t bm[kSpan / 32];
-#pragma unroll
-                for (uint32_t k = 0; k < kSpan / 32; ++k) bm[k] = 0;
-                uint32_t p = s_me, sym = 0, used = 0;
-                uint32_t ev1 = ~0u, ev2 = ~0u, nev = 0;   // first two events (restarts)
-                bool at_end = false;
-                while (p < s_nx) {
-                    if (p >= lim) { at_end = true; break; }
-                    const uint32_t k = p - s_me;
-#pragma unroll
-                    for (uint32_t w = 0; w < kSpan / 32; ++w)
-                        if ((k >> 5) == w) bm[w] |= 1u << (k & 31);
-                    const uint32_t rc = decode_sym(S, p, lim, &sym, &used);
-                    if (rc != kSymOk) {
-                        if (nev == 0) ev1 = p;
-                        else if (nev == 1) ev2 = p;
-                        ++nev;
+            for (uint32_t step = 0;; ++step) {
+                if (!pdone && !S.ctrl.halt) {
+                    Item it = {kItemNone, 0, MDZ_ST_OK, 0, 0, 0};
+                    const uint32_t b = step & 1;
+                    if (st != MDZ_ST_OK) {
+                        it.status = st;
+                        it.last = 1;
+                    } else if (need_hdr) {
+                        r.seek_bit(P);
+                        r.refill();
+                        final = r.bits(1);
+                        const uint32_t type = r.bits(2);
+                        if (r.over()) st = MDZ_ST_INPUT_OVERRUN;
+                        else if (type == 3) st = MDZ_ST_BAD_BLOCK_TYPE;
+                        else if (type == 0) {                           // stored
+                            ++n_stored;
+                            r.drop(r.nb & 7);
+                            r.refill();
+                            const uint32_t len = r.bits(16), nlen = r.bits(16);
+                            if (r.over()) st = MDZ_ST_INPUT_OVERRUN;
+                            else if (len != (~nlen & 0xffffu)) st = MDZ_ST_STORED_LEN;
+                            else {
+                                const uint64_t q = r.cbits / 8;
+                                const uint64_t avail = q <= r.n ? r.n - q : 0;
+                                it.kind = kItemStored;
+                                it.src = q;
+                                it.take = min<uint64_t>(len, avail);
+                                if (it.take < len) { it.status = MDZ_ST_INPUT_OVERRUN; it.last = 1; }
+                                else if (final) it.last = 1;
+                                P = 8 * (q + len);
+                            }
+                        } else if (type == 1) {                         // fixed tables
+                            ++n_fixed;
+                            if (!fixed_built) {
+                                for (uint32_t k = l; k < 320; k += 64)
+                                    H.lens[k] = k < 144 ? 8 : k < 256 ? 9 : k < 280 ? 7 : k < 288 ? 8 : 5;
+                                wave_sync();
+                                build_table2(S.lit_c, S.lit, kLitRoot, kLitCap, H.lens, 288, 0);
+                                build_table2(S.dist_c, S.dist, kDistRoot, kDistCap, H.lens + 288, 32, 1);
+                                fixed_built = 1;
+                            }
+                            need_hdr = false;
+                            P = r.cbits;
+                        } else {                                        // dynamic tables
+                            ++n_dyn;
+                            fixed_built = 0;
+                            r.refill();
+                            const uint32_t nlen = r.bits(5) + 257, ndist = r.bits(5) + 1, ncode = r.bits(4) + 4;
+                            if (r.over()) st = MDZ_ST_INPUT_OVERRUN;
+                            else if (nlen > 286 || ndist > 30) st = MDZ_ST_BAD_COUNTS;
+                            if (st == MDZ_ST_OK) {
+                                for (uint32_t k = l; k < 19; k += 64) H.lens[k] = 0;
+                                wave_sync();
+                                for (uint32_t i = 0; i < ncode; ++i) {
+                                    r.refill();
+                                    const uint32_t v = r.bits(3);
+                                    if (l == 0) H.lens[kClOrder[i]] = (uint8_t)v;
+                                }
+                                wave_sync();
+                                if (r.over()) st = MDZ_ST_INPUT_OVERRUN;
+                                else if (build_code(H.cl, H.cl_lut, 7, H.lens, 19, 2)) st = MDZ_ST_BAD_CODE_LENS;
+                            }
+                            if (st == MDZ_ST_OK) {
+                                // code lengths for nlen + ndist symbols (lens[] reused from 0)
+                                uint32_t idx = 0, prev = 0;
+                                while (idx < nlen + ndist) {
+                                    r.refill();
+                                    const uint32_t e = __builtin_amdgcn_readfirstlane(H.cl_lut[r.peek(7)]);
+                                    r.drop(e & 15);
+                                    const uint32_t sym = e >> 16;
+                                    uint32_t rep = 0, val = 0;
+                                    if (sym < 16) { val = sym; rep = 1; prev = sym; }
+                                    else if (sym == 16) { val = prev; rep = 3 + r.bits(2); }
+                                    else if (sym == 17) { rep = 3 + r.bits(3); prev = 0; }
+                                    else { rep = 11 + r.bits(7); prev = 0; }
+                                    if (r.over()) { st = MDZ_ST_INPUT_OVERRUN; break; }
+                                    if (sym == 16 && idx == 0) { st = MDZ_ST_BAD_REPEAT; break; }
+                                    if (idx + rep > nlen + ndist) { st = MDZ_ST_BAD_REPEAT; break; }
+                                    for (uint32_t j = l; j < rep; j += 64) {
+                                        const uint32_t t = idx + j;
+                                        H.lens[t < nlen ? t : 288 + (t - nlen)] = (uint8_t)val;
+                                    }
+                                    idx += rep;
+                                }
+                            }
+                            if (st == MDZ_ST_OK) {
+                                for (uint32_t k = nlen + l; k < 288; k += 64) H.lens[k] = 0;
+                                for (uint32_t k = 288 + ndist + l; k < 320; k += 64) H.lens[k] = 0;
+                                wave_sync();
+                                if (H.lens[256] == 0) st = MDZ_ST_NO_EOB;
+                                else if (build_table2(S.lit_c, S.lit, kLitRoot, kLitCap, H.lens, nlen, 0))
+                                    st = MDZ_ST_BAD_LITLEN;
+                                else if (build_table2(S.dist_c, S.dist, kDistRoot, kDistCap, H.lens + 288, ndist, 1))
+                                    st = MDZ_ST_BAD_DIST;
+                            }
+                            need_hdr = false;
+                            P = r.cbits;
+                        }
+                        if (st != MDZ_ST_OK) { it.status = st; it.last = 1; }
                     }
-                    p += rc == kSymOk ? used : 1;
-                }
-                if (p < s_nx) p = s_nx;
+                    INF_PHASE(0);
+                    if (it.kind == kItemNone && !it.last) {
+                        // ---- one round of symbols.  Lane t takes the kSpan input bits
+                        // from R + t*kSpan (R: the round's first true symbol) and
+                        // decodes from there as if a symbol started at its first bit,
+                        // noting each symbol start it meets (an end-of-block or
+                        // invalid code restarts its decode one bit later).  Huffman
+                        // decoding resynchronises: the true symbol sequence, followed
+                        // past the end of lane t's span, soon meets a start a later
+                        // lane noted, and from there that lane's sequence is the true
+                        // one.  The lanes the true sequence passes through form a
+                        // chain; each lists its piece of the sequence, in order.
+                        const uint64_t byte = P >> 3;
+                        const uintptr_t A = (src_addr + byte) & ~(uintptr_t)3;
+                        const uint32_t rel = (uint32_t)(8 * (src_addr + byte - A) + (P & 7));
+                        const uint64_t limb = 8 * (uint64_t)(src_addr + r.n - A);
+                        const uint32_t lim = limb > 0xF0000000ull ? 0xF0000000u : (uint32_t)limb;
+                        for (uint32_t k = l; k < kStageWords; k += 64) S.in[k] = r.load_dword(A + 4 * k);
+                        wave_sync();
+                        INF_PHASE(5);
+
+                        // phase 1: each lane's own span (positions past the stream end
+                        // start no symbol; the true path meets them only as an overrun)
+                        const uint32_t s_me = rel + kSpan * l, s_nx = s_me + kSpan;
+                        uint32_t bm[kSpan / 32];
 #pragma unroll
-                for (uint32_t k = 0; k < kSpan / 32; ++k) S.u.bm[(kSpan / 32) * l + k] = bm[k];
-                wave_sync();
-                MDZ_PHASE(6);
-
-                // phase 2: hand-off.  From its span's end the lane follows its
-                // sequence until it meets a start noted by a later lane g (in
-                // the span it has reached, at most kReach spans ahead).  stop:
-                // the sequence has its end of block or an error at h; lost: no
-                // noted start met in reach.  Lane 63 hands off to the next round.
-                uint32_t h = p, g = 64, steps = 0;
-                bool stop = false, lost = false;
-                if (l < 63) {
-                    for (;;) {
-                        const uint32_t j = (h - rel) >> kSpanLog;
-                        if (j > l + kReach || j > 63) { lost = true; break; }
-                        const uint32_t k = h - (rel + kSpan * j);
-                        if ((S.u.bm[(kSpan / 32) * j + (k >> 5)] >> (k & 31)) & 1) { g = j; break; }
-                        const uint32_t rc = decode_sym(S, h, lim, &sym, &used);
-                        if (rc != kSymOk) { stop = true; break; }
-                        h += used;
-                        ++steps;
-                    }
-                }
-                MDZ_PHASE(7);
-                // the true chain: lane 0, then g(0), g(g(0)), ...; each member's
-                // piece starts where its predecessor handed off
-                uint32_t a = rel;
-                uint64_t chain = 0;
-                for (uint32_t cur = 0;;) {
-                    chain |= 1ull << cur;
-                    const uint32_t nx = __builtin_amdgcn_readlane(g, cur);
-                    if (nx > 63) break;
-                    const uint32_t hc = __builtin_amdgcn_readlane(h, cur);
-                    if (l == nx) a = hc;
-                    cur = nx;
-                }
-                const bool member = (chain >> l) & 1;
-
-                // phase 3: count each member's piece [a, h) and find the event
-                // that ends it, if any.  From a (one of its noted starts) the
-                // lane's own sequence is the true one, so up to the first event
-                // at or after a its noted starts are the piece's symbols, and past
-                // its span the hand-off steps are.  A lane that met more than two
-                // events, or the stream's end, decodes its piece to count it.
-                uint32_t cnt = 0, ev = kSymOk, evst = 0, evpos = 0, evlen = 0;
-                if (member) {
-                    const uint32_t x = ev1 >= a ? ev1 : ev2 >= a ? ev2 : ~0u;
-                    bool need_ev = false;
-                    if (nev > 2 || at_end) {
-                        p = a;
-                        while (p < h) {
+                        for (uint32_t k = 0; k < kSpan / 32; ++k) bm[k] = 0;
+                        uint32_t p = s_me, sym = 0, used = 0;
+                        uint32_t ev1 = ~0u, ev2 = ~0u, nev = 0;   // first two events (restarts)
+                        bool at_end = false;
+                        while (p < s_nx) {
+                            if (p >= lim) { at_end = true; break; }
+                            const uint32_t k = p - s_me;
+#pragma unroll
+                            for (uint32_t w = 0; w < kSpan / 32; ++w)
+                                if ((k >> 5) == w) bm[w] |= 1u << (k & 31);
                             const uint32_t rc = decode_sym(S, p, lim, &sym, &used);
-                            if (rc != kSymOk) { ev = rc; evst = sym; evpos = p; evlen = used; break; }
-                            ++cnt;
-                            p += used;
+                            if (rc != kSymOk) {
+                                if (nev == 0) ev1 = p;
+                                else if (nev == 1) ev2 = p;
+                                ++nev;
+                            }
+                            p += rc == kSymOk ? used : 1;
                         }
-                        if (ev == kSymOk && p != h) { ev = kSymErr; evst = (uint32_t)MDZ_ST_INTERNAL; evpos = p; }
-                        if (ev == kSymOk && stop) { evpos = h; need_ev = true; }
-                    } else {
-                        // noted starts in [a, min(x, span end))
-                        const uint32_t lo = a - s_me, hi = x < s_nx ? x - s_me : kSpan;
+                        if (p < s_nx) p = s_nx;
 #pragma unroll
-                        for (uint32_t w = 0; w < kSpan / 32; ++w) {
-                            const uint32_t b0 = 32 * w;
-                            uint32_t m = bm[w];
-                            if (lo > b0) m = lo - b0 >= 32 ? 0u : m & (~0u << (lo - b0));
-                            if (hi < b0 + 32) m = hi <= b0 ? 0u : m & (~0u >> (b0 + 32 - hi));
-                            cnt += __popc(m);
-                        }
-                        if (x < s_nx) {
-                            evpos = x;
-                            need_ev = true;
-                        } else {
-                            cnt += steps;
-                            if (stop) { evpos = h; need_ev = true; }
-                        }
-                    }
-                    if (need_ev) {
-                        ev = decode_sym(S, evpos, lim, &sym, &used);
-                        evst = sym;
-                        evlen = used;
-                    }
-                }
-                // the round ends at the first member with an event or a lost hand-off
-                const uint64_t endm = __ballot(member && (ev != kSymOk || lost));
-                const uint32_t top = 63 - (uint32_t)__builtin_clzll(chain);
-                uint32_t last = endm ? (uint32_t)__builtin_ctzll(endm) : top;
-                const uint32_t incl = wave_incl_scan_u32(member && l <= last ? cnt : 0);
-                // keep whole pieces within the symbol list's capacity
-                const uint64_t over = __ballot(member && l <= last && incl > kSymCap);
-                const bool cut = over != 0;
-                if (cut) {   // the last member below the first that overflows (lane 0's piece always fits)
-                    const uint64_t below = chain & ((1ull << __builtin_ctzll(over)) - 1);
-                    last = 63 - (uint32_t)__builtin_clzll(below);
-                }
-                const uint32_t nsym = __builtin_amdgcn_readlane(incl, last);
-                const uint32_t r_ev = cut ? kSymOk : __builtin_amdgcn_readlane(ev, last);
-                const uint32_t r_evst = __builtin_amdgcn_readlane(evst, last);
-                const uint32_t r_evpos = __builtin_amdgcn_readlane(evpos, last);
-                const uint32_t r_evlen = __builtin_amdgcn_readlane(evlen, last);
-                const uint32_t r_h = __builtin_amdgcn_readlane(h, last);
-                wave_sync();      // the hand-off bits are dead: the list takes their space
-                MDZ_PHASE(8);
+                        for (uint32_t k = 0; k < kSpan / 32; ++k) S.u.bm[(kSpan / 32) * l + k] = bm[k];
+                        wave_sync();
+                        INF_PHASE(6);
 
-                // phase 4: list the true symbols in order
-                bool astray = false;
-                if (member && l <= last) {
-                    uint32_t o = incl - cnt;
-                    p = a;
-                    for (uint32_t i = 0; i < cnt; ++i) {
-                        decode_sym(S, p, lim, &sym, &used);
-                        S.u.sym[o++] = sym;
-                        p += used;
-                    }
-                    // the listed symbols must end where the piece does
-                    astray = p != (ev != kSymOk ? evpos : h);
-                }
-                wave_sync();
-                if (__ballot(astray)) { st = MDZ_ST_INTERNAL; break; }   // never expected
-                MDZ_PHASE(1);
-
-                // ---- emit the list in batches (at most 64 symbols or about
-                // kBatchOut bytes each): literals, then matches in order.  The
-                // workgroup is this one wave and its LDS operations execute in
-                // order, so a match reads what earlier literals and matches of
-                // the batch stored without a wait in between; ring positions are
-                // kept below 2 * kRing and wrapped with one compare.
-                for (uint32_t b0 = 0; b0 < nsym && st == MDZ_ST_OK;) {
-                    const uint32_t i = b0 + l;
-                    const uint32_t x = i < nsym ? S.u.sym[i] : 0;
-                    const bool have = i < nsym, ref = have && (x >> 31);
-                    const uint32_t rec_len = !have ? 0 : ref ? ((x >> 15) & 255) + 3 : 1;
-                    const uint32_t rec_val = ref ? (x & 32767) + 1 : (x & 255);
-                    const uint32_t oin = wave_incl_scan_u32(rec_len);
-                    const uint32_t rec_off = oin - rec_len;
-                    // batch rules, in stream order: a symbol joins while fewer than
-                    // kBatchOut bytes are queued; a distance past the output start
-                    // is an error before the symbol; an overrun ends the batch with
-                    // the overrunning symbol included
-                    const uint64_t full = __ballot(have && rec_off >= kBatchOut);
-                    const uint64_t bad = __ballot(ref && rec_val > pos + rec_off);
-                    const uint64_t ovr = __ballot(have && pos + oin > cap);
-                    uint32_t n = min(nsym - b0, 64u);
-                    if (full) n = min(n, (uint32_t)__builtin_ctzll(full));
-                    if (ovr) n = min(n, (uint32_t)__builtin_ctzll(ovr) + 1);
-                    if (bad && (uint32_t)__builtin_ctzll(bad) < n) {
-                        n = (uint32_t)__builtin_ctzll(bad);
-                        st = MDZ_ST_BAD_DISTANCE;
-                    }
-                    const uint32_t bout = n ? __builtin_amdgcn_readlane(oin, n - 1) : 0;
-                    n_sym += n;
-                    n_match += __popcll(__ballot(ref && l < n));
-                    const uint64_t room = pos < cap ? cap - pos : 0;
-                    const uint32_t rb = ring_at(pos);          // ring index of the batch start
-                    auto wrap = [](uint32_t v) { return v >= kRing ? v - kRing : v; };
-                    if (l < n && !ref) S.ring[wrap(rb + rec_off)] = (uint8_t)rec_val;
-                    __builtin_amdgcn_wave_barrier();
-                    uint64_t mm = __ballot(l < n && ref);
-                    while (mm) {
-                        const uint32_t m = __builtin_ctzll(mm);
-                        mm &= mm - 1;
-                        const uint32_t off = __builtin_amdgcn_readlane(rec_off, m);
-                        const uint32_t len = __builtin_amdgcn_readlane(rec_len, m);
-                        const uint32_t dist = __builtin_amdgcn_readlane(rec_val, m);
-                        const uint32_t d0 = wrap(rb + off);                  // destination start
-                        const uint32_t s0 = wrap(d0 + kRing - dist);         // source start (dist <= 32768)
-                        if (dist >= len) {
-                            for (uint32_t j = l; j < len; j += 64) S.ring[wrap(d0 + j)] = S.ring[wrap(s0 + j)];
-                        } else {
-                            // overlapping copy: byte j repeats source byte j mod dist;
-                            // j < 258, so a float reciprocal and one correction step
-                            // give the exact quotient
-                            const float rd = __builtin_amdgcn_rcpf((float)dist);
-                            for (uint32_t j = l; j < len; j += 64) {
-                                uint32_t qj = (uint32_t)((float)j * rd);
-                                if (qj * dist > j) --qj;
-                                else if ((qj + 1) * dist <= j) ++qj;
-                                S.ring[wrap(d0 + j)] = S.ring[wrap(s0 + (j - qj * dist))];
+                        // phase 2: hand-off.  From its span's end the lane follows its
+                        // sequence until it meets a start noted by a later lane g (in
+                        // the span it has reached, at most kReach spans ahead).  stop:
+                        // the sequence has its end of block or an error at h; lost: no
+                        // noted start met in reach.  Lane 63 hands off to the next round.
+                        uint32_t h = p, g = 64, steps = 0;
+                        bool stop = false, lost = false;
+                        if (l < 63) {
+                            for (;;) {
+                                const uint32_t j = (h - rel) >> kSpanLog;
+                                if (j > l + kReach || j > 63) { lost = true; break; }
+                                const uint32_t k = h - (rel + kSpan * j);
+                                if ((S.u.bm[(kSpan / 32) * j + (k >> 5)] >> (k & 31)) & 1) { g = j; break; }
+                                const uint32_t rc = decode_sym(S, h, lim, &sym, &used);
+                                if (rc != kSymOk) { stop = true; break; }
+                                h += used;
+                                ++steps;
                             }
                         }
-                        __builtin_amdgcn_wave_barrier();
+                        INF_PHASE(7);
+                        // the true chain: lane 0, then g(0), g(g(0)), ...; each member's
+                        // piece starts where its predecessor handed off
+                        uint32_t a = rel;
+                        uint64_t chain = 0;
+                        for (uint32_t cur = 0;;) {
+                            chain |= 1ull << cur;
+                            const uint32_t nx = __builtin_amdgcn_readlane(g, cur);
+                            if (nx > 63) break;
+                            const uint32_t hc = __builtin_amdgcn_readlane(h, cur);
+                            if (l == nx) a = hc;
+                            cur = nx;
+                        }
+                        const bool member = (chain >> l) & 1;
+
+                        // phase 3: count each member's piece [a, h) and find the event
+                        // that ends it, if any.  From a (one of its noted starts) the
+                        // lane's own sequence is the true one, so up to the first event
+                        // at or after a its noted starts are the piece's symbols, and
+                        // past its span the hand-off steps are.  A lane that met more
+                        // than two events, or the stream's end, decodes its piece to
+                        // count it.
+                        uint32_t cnt = 0, ev = kSymOk, evst = 0, evpos = 0, evlen = 0;
+                        if (member) {
+                            const uint32_t x = ev1 >= a ? ev1 : ev2 >= a ? ev2 : ~0u;
+                            bool need_ev = false;
+                            if (nev > 2 || at_end) {
+                                p = a;
+                                while (p < h) {
+                                    const uint32_t rc = decode_sym(S, p, lim, &sym, &used);
+                                    if (rc != kSymOk) { ev = rc; evst = sym; evpos = p; evlen = used; break; }
+                                    ++cnt;
+                                    p += used;
+                                }
+                                if (ev == kSymOk && p != h) { ev = kSymErr; evst = (uint32_t)MDZ_ST_INTERNAL; evpos = p; }
+                                if (ev == kSymOk && stop) { evpos = h; need_ev = true; }
+                            } else {
+                                // noted starts in [a, min(x, span end))
+                                const uint32_t lo = a - s_me, hi = x < s_nx ? x - s_me : kSpan;
+#pragma unroll
+                                for (uint32_t w = 0; w < kSpan / 32; ++w) {
+                                    const uint32_t b0 = 32 * w;
+                                    uint32_t m = bm[w];
+                                    if (lo > b0) m = lo - b0 >= 32 ? 0u : m & (~0u << (lo - b0));
+                                    if (hi < b0 + 32) m = hi <= b0 ? 0u : m & (~0u >> (b0 + 32 - hi));
+                                    cnt += __popc(m);
+                                }
+                                if (x < s_nx) {
+                                    evpos = x;
+                                    need_ev = true;
+                                } else {
+                                    cnt += steps;
+                                    if (stop) { evpos = h; need_ev = true; }
+                                }
+                            }
+                            if (need_ev) {
+                                ev = decode_sym(S, evpos, lim, &sym, &used);
+                                evst = sym;
+                                evlen = used;
+                            }
+                        }
+                        // the round ends at the first member with an event or a lost hand-off
+                        const uint64_t endm = __ballot(member && (ev != kSymOk || lost));
+                        const uint32_t top = 63 - (uint32_t)__builtin_clzll(chain);
+                        uint32_t last = endm ? (uint32_t)__builtin_ctzll(endm) : top;
+                        const uint32_t incl = wave_incl_scan_u32(member && l <= last ? cnt : 0);
+                        // keep whole pieces within the symbol list's capacity
+                        const uint64_t over = __ballot(member && l <= last && incl > kSymCap);
+                        const bool cut = over != 0;
+                        if (cut) {   // the last member below the first that overflows (lane 0's piece always fits)
+                            const uint64_t below = chain & ((1ull << __builtin_ctzll(over)) - 1);
+                            last = 63 - (uint32_t)__builtin_clzll(below);
+                        }
+                        const uint32_t nsym = __builtin_amdgcn_readlane(incl, last);
+                        const uint32_t r_ev = cut ? kSymOk : __builtin_amdgcn_readlane(ev, last);
+                        const uint32_t r_evst = __builtin_amdgcn_readlane(evst, last);
+                        const uint32_t r_evpos = __builtin_amdgcn_readlane(evpos, last);
+                        const uint32_t r_evlen = __builtin_amdgcn_readlane(evlen, last);
+                        const uint32_t r_h = __builtin_amdgcn_readlane(h, last);
+                        INF_PHASE(8);
+
+                        // phase 4: list the true symbols in order
+                        bool astray = false;
+                        if (member && l <= last) {
+                            uint32_t o = incl - cnt;
+                            p = a;
+                            for (uint32_t i = 0; i < cnt; ++i) {
+                                decode_sym(S, p, lim, &sym, &used);
+                                S.sym[b][o++] = sym;
+                                p += used;
+                            }
+                            // the listed symbols must end where the piece does
+                            astray = p != (ev != kSymOk ? evpos : h);
+                        }
+                        it.kind = kItemRound;
+                        it.nsym = nsym;
+                        if (__ballot(astray)) {                 // never expected
+                            it.nsym = 0;
+                            st = MDZ_ST_INTERNAL;
+                            it.status = st;
+                            it.last = 1;
+                        } else if (r_ev == kSymEob) {
+                            P += (r_evpos + r_evlen) - rel;
+                            need_hdr = true;
+                            if (final) it.last = 1;
+                        } else if (r_ev == kSymErr) {
+                            st = (int)r_evst;
+                            it.status = st;
+                            it.last = 1;
+                            P += r_evpos - rel;
+                        } else {
+                            P += r_h - rel;
+                        }
+                        INF_PHASE(1);
                     }
-                    if (bout > room) {
-                        pos += room;
-                        flush(pos, true);
-                        st = MDZ_ST_OUTPUT_OVERRUN;
-                        break;
+                    if (l == 0) S.item[b] = it;
+                    if (it.last) {
+                        pdone = true;
+                        if (l == 0) {
+                            S.ctrl.n_stored = n_stored;
+                            S.ctrl.n_fixed = n_fixed;
+                            S.ctrl.n_dyn = n_dyn;
+                            S.ctrl.consumed = P;
+                        }
                     }
-                    pos += bout;
-                    flush(pos, false);
                     wave_sync();
-                    b0 += n;
                 }
-                MDZ_PHASE(2);
-                if (st != MDZ_ST_OK) break;
-                if (r_ev == kSymEob) {
-                    eob = true;
-                    P += (r_evpos + r_evlen) - rel;
-                } else if (r_ev == kSymErr) {
-                    st = (int)r_evst;
-                    P += r_evpos - rel;
-                } else {
-                    P += r_h - rel;
+                INF_PHASE(11);
+                __syncthreads();
+                INF_PHASE(9);                      // producer waiting for the consumer
+                if (S.ctrl.done) break;
+            }
+        } else {
+            // ------------------------------------------------------ consumer
+            uint8_t* const dst = reinterpret_cast<uint8_t*>(in.dst);
+            const uint64_t cap = in.dst_bytes;
+            uint64_t pos = 0, flushed = 0;             // output produced / flushed to dst
+            uint32_t n_sym = 0, n_match = 0;
+            int st = MDZ_ST_OK;
+            const uint64_t t0 = clock64();
+
+            // flushes [flushed, upto) in dword stores at dword-aligned dst
+            // addresses; before the last flush it stops at such an address, so
+            // only the first flush has head bytes and only the last tail bytes
+            const uint32_t dmis = (uint32_t)((uintptr_t)dst & 3);
+            auto flush = [&](uint64_t upto, bool all) {
+                uint64_t lim = min(upto, cap);
+                if (!all) {
+                    const uint64_t cut = (lim + dmis) & 3;
+                    lim = lim >= cut ? lim - cut : 0;
                 }
+                if (lim <= flushed) return;
+                // at most one batch (< kRing bytes) is pending: ring indices are
+                // the flush start's plus an offset, wrapped once; kRing % 4 == 0,
+                // so ring index and output offset agree mod 4
+                const uint32_t rf = ring_at(flushed);
+                auto at = [&](uint64_t p) {
+                    const uint32_t x = rf + (uint32_t)(p - flushed);
+                    return x >= kRing ? x - kRing : x;
+                };
+                const uint64_t h = min(lim, flushed + ((4 - ((dmis + flushed) & 3)) & 3));
+                for (uint64_t p = flushed + l; p < h; p += 64) dst[p] = S.ring[at(p)];
+                const uint64_t nw = (lim - h) / 4;
+                if (dmis == 0) {                       // h % 4 == 0: aligned ring dwords
+                    for (uint64_t w = l; w < nw; w += 64) {
+                        const uint64_t p = h + 4 * w;
+                        *reinterpret_cast<uint32_t*>(dst + p) = *reinterpret_cast<const uint32_t*>(&S.ring[at(p)]);
+                    }
+                } else {                               // ring bytes gathered into aligned dst dwords
+                    for (uint64_t w = l; w < nw; w += 64) {
+                        const uint64_t p = h + 4 * w;
+                        *reinterpret_cast<uint32_t*>(dst + p) =
+                            (uint32_t)S.ring[at(p)] | ((uint32_t)S.ring[at(p + 1)] << 8) |
+                            ((uint32_t)S.ring[at(p + 2)] << 16) | ((uint32_t)S.ring[at(p + 3)] << 24);
+                    }
+                }
+                for (uint64_t p = h + 4 * nw + l; p < lim; p += 64) dst[p] = S.ring[at(p)];
+                flushed = lim;
+            };
+
+            const uint8_t* src = reinterpret_cast<const uint8_t*>(in.src);
+            if (hdr == MDZ_HDR_GZIP && outs[si].status == MDZ_ST_OK) src += outs[si].bytes_written;
+            INF_PHASE_START();
+            bool cdone = false;
+            for (uint32_t step = 0;; ++step) {
+                if (step > 0 && !cdone) {
+                    const uint32_t b = (step - 1) & 1;
+                    const Item it = S.item[b];
+                    if (it.kind == kItemStored) {
+                        const uint64_t q = it.src, take = it.take;
+                        for (uint64_t done = 0; done < take;) {
+                            const uint64_t k = min<uint64_t>(take - done, kBatchOut);
+                            for (uint64_t j = l; j < k; j += 64) S.ring[ring_at(pos + j)] = src[q + done + j];
+                            wave_sync();
+                            const uint64_t room = pos < cap ? cap - pos : 0;
+                            pos += min(k, room);
+                            flush(pos, false);
+                            wave_sync();
+                            done += k;
+                            if (k > room) { st = MDZ_ST_OUTPUT_OVERRUN; break; }
+                        }
+                        INF_PHASE(4);
+                    } else if (it.kind == kItemRound) {
+                        // ---- emit the list in batches (at most 64 symbols or about
+                        // kBatchOut bytes each): literals, then matches in order.  The
+                        // consumer is one wave and its LDS operations execute in order,
+                        // so a match reads what earlier literals and matches of the
+                        // batch stored without a wait in between; ring positions are
+                        // kept below 2 * kRing and wrapped with one compare.
+                        const uint32_t nsym = it.nsym;
+                        for (uint32_t b0 = 0; b0 < nsym && st == MDZ_ST_OK;) {
+                            const uint32_t i = b0 + l;
+                            const uint32_t x = i < nsym ? S.sym[b][i] : 0;
+                            const bool have = i < nsym, ref = have && (x >> 31);
+                            const uint32_t rec_len = !have ? 0 : ref ? ((x >> 15) & 255) + 3 : 1;
+                            const uint32_t rec_val = ref ? (x & 32767) + 1 : (x & 255);
+                            const uint32_t oin = wave_incl_scan_u32(rec_len);
+                            const uint32_t rec_off = oin - rec_len;
+                            // batch rules, in stream order: a symbol joins while fewer than
+                            // kBatchOut bytes are queued; a distance past the output start
+                            // is an error before the symbol; an overrun ends the batch with
+                            // the overrunning symbol included
+                            const uint64_t full = __ballot(have && rec_off >= kBatchOut);
+                            const uint64_t bad = __ballot(ref && rec_val > pos + rec_off);
+                            const uint64_t ovr = __ballot(have && pos + oin > cap);
+                            uint32_t n = min(nsym - b0, 64u);
+                            if (full) n = min(n, (uint32_t)__builtin_ctzll(full));
+                            if (ovr) n = min(n, (uint32_t)__builtin_ctzll(ovr) + 1);
+                            if (bad && (uint32_t)__builtin_ctzll(bad) < n) {
+                                n = (uint32_t)__builtin_ctzll(bad);
+                                st = MDZ_ST_BAD_DISTANCE;
+                            }
+                            const uint32_t bout = n ? __builtin_amdgcn_readlane(oin, n - 1) : 0;
+                            n_sym += n;
+                            n_match += __popcll(__ballot(ref && l < n));
+                            const uint64_t room = pos < cap ? cap - pos : 0;
+                            const uint32_t rb = ring_at(pos);          // ring index of the batch start
+                            auto wrap = [](uint32_t v) { return v >= kRing ? v - kRing : v; };
+                            if (l < n && !ref) S.ring[wrap(rb + rec_off)] = (uint8_t)rec_val;
+                            __builtin_amdgcn_wave_barrier();
+                            uint64_t mm = __ballot(l < n && ref);
+                            while (mm) {
+                                const uint32_t m = __builtin_ctzll(mm);
+                                mm &= mm - 1;
+                                const uint32_t off = __builtin_amdgcn_readlane(rec_off, m);
+                                const uint32_t len = __builtin_amdgcn_readlane(rec_len, m);
+                                const uint32_t dist = __builtin_amdgcn_readlane(rec_val, m);
+                                const uint32_t d0 = wrap(rb + off);                  // destination start
+                                const uint32_t s0 = wrap(d0 + kRing - dist);         // source start (dist <= 32768)
+                                if (dist >= len) {
+                                    for (uint32_t j = l; j < len; j += 64) S.ring[wrap(d0 + j)] = S.ring[wrap(s0 + j)];
+                                } else {
+                                    // overlapping copy: byte j repeats source byte j mod dist;
+                                    // j < 258, so a float reciprocal and one correction step
+                                    // give the exact quotient
+                                    const float rd = __builtin_amdgcn_rcpf((float)dist);
+                                    for (uint32_t j = l; j < len; j += 64) {
+                                        uint32_t qj = (uint32_t)((float)j * rd);
+                                        if (qj * dist > j) --qj;
+                                        else if ((qj + 1) * dist <= j) ++qj;
+                                        S.ring[wrap(d0 + j)] = S.ring[wrap(s0 + (j - qj * dist))];
+                                    }
+                                }
+                                __builtin_amdgcn_wave_barrier();
+                            }
+                            if (bout > room) {
+                                pos += room;
+                                flush(pos, true);
+                                st = MDZ_ST_OUTPUT_OVERRUN;
+                                break;
+                            }
+                            pos += bout;
+                            flush(pos, false);
+                            wave_sync();
+                            b0 += n;
+                        }
+                        INF_PHASE(2);
+                    }
+                    if (st == MDZ_ST_OK && it.status != MDZ_ST_OK) st = it.status;
+                    if (st != MDZ_ST_OK || it.last) {
+                        cdone = true;
+                        if (l == 0) {
+                            S.ctrl.halt = 1;
+                            S.ctrl.done = 1;
+                        }
+                    }
+                    wave_sync();
+                }
+                INF_PHASE(12);
+                __syncthreads();
+                INF_PHASE(10);                     // consumer waiting for the producer
+                if (S.ctrl.done) break;
             }
-            r.seek_bit(P);
-        }
-        flush(pos, true);
-        if (l == 0) {
-            mdz_inflate_status_t o;
-            o.bytes_written = pos;
-            if (cost) {
-                mdz_inflate_cost_t c;
-                c.cycles = clock64() - t0;
-                c.symbols = n_sym;
-                c.matches = n_match;
-                c.stored_blocks = n_stored;
-                c.fixed_blocks = n_fixed;
-                c.dynamic_blocks = n_dyn;
-                c.reserved = 0;
-                cost[si] = c;
+            flush(pos, true);
+            if (l == 0) {
+                mdz_inflate_status_t o;
+                o.bytes_written = pos;
+                if (cost) {
+                    mdz_inflate_cost_t c;
+                    c.cycles = clock64() - t0;
+                    c.symbols = n_sym;
+                    c.matches = n_match;
+                    c.stored_blocks = S.ctrl.n_stored;
+                    c.fixed_blocks = S.ctrl.n_fixed;
+                    c.dynamic_blocks = S.ctrl.n_dyn;
+                    c.reserved = 0;
+                    cost[si] = c;
+                }
+                o.status = (uint32_t)st;
+                // Deflate input consumed through the last block (from the start
+                // of the Deflate data), for the trailer check; saturates
+                o.reserved = (uint32_t)min<uint64_t>((S.ctrl.consumed + 7) / 8, 0xFFFFFFFFull);
+                outs[si] = o;
             }
-            o.status = (uint32_t)st;
-            // Deflate input consumed through the last block (from the start
-            // of the Deflate data), for the trailer check; saturates
-            o.reserved = (uint32_t)min<uint64_t>((r.cbits + 7) / 8, 0xFFFFFFFFull);
-            outs[si] = o;
         }
+        __syncthreads();
         if (!cursor) return;
-        wave_sync();
     }
 }
 
@@ -1110,7 +1220,7 @@ int inflate_slots(uint32_t* slots) {
     int dev = 0, per_cu = 0, cus = 0;
     if ((e = hipGetDevice(&dev)) != hipSuccess ||
         (e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev)) != hipSuccess ||
-        (e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)inflate_kernel, 64,
+        (e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)inflate_kernel, 128,
                                                           smem)) != hipSuccess)
         return record_hip(e, "inflate occupancy");
     *slots = (uint32_t)(per_cu * cus);
@@ -1142,7 +1252,7 @@ int inflate_batch(const mdz_inflate_in_t* in, mdz_inflate_status_t* out, uint32_
         hipLaunchKernelGGL(gzip_prepass_kernel, dim3((count + 255) / 256), dim3(256), 0, s, in, out, count);
         if ((e = hipGetLastError()) != hipSuccess) return record_hip(e, "gzip_prepass_kernel launch");
     }
-    hipLaunchKernelGGL(inflate_kernel, dim3(grid), dim3(64), smem, s, in, out, order, count,
+    hipLaunchKernelGGL(inflate_kernel, dim3(grid), dim3(128), smem, s, in, out, order, count,
                        parse_hdr, persistent ? cursor : nullptr, cost);
     if ((e = hipGetLastError()) != hipSuccess) return record_hip(e, "inflate_kernel launch");
     if ((parse_hdr & MDZ_HDR_VERIFY) && hdr) {
