@@ -148,6 +148,7 @@ struct InflateLds {
     Item item[2];
     Ctrl ctrl;
     uint8_t ring[kRing];                       // the consumer's output window
+    uint32_t sink[64];                         // per-lane target of stores past a match's end
 };
 static_assert(sizeof(InflateLds) <= 80 * 1024, "two workgroups per CU");
 
@@ -988,7 +989,25 @@ __global__ __launch_bounds__(128) void inflate_kernel(const mdz_inflate_in_t* __
                         const uint64_t q = it.src, take = it.take;
                         for (uint64_t done = 0; done < take;) {
                             const uint64_t k = min<uint64_t>(take - done, kBatchOut);
-                            for (uint64_t j = l; j < k; j += 64) S.ring[ring_at(pos + j)] = src[q + done + j];
+                            // lane l moves bytes [32l, 32l + 32) of the piece: all its
+                            // loads are issued before the first store
+                            const uint8_t* const from = src + q + done;
+                            const uint32_t rb = ring_at(pos);
+                            // (a byte past the piece loads byte 0 and is stored to the
+                            // lane's sink: no branch per byte)
+                            uint8_t* const sink = reinterpret_cast<uint8_t*>(&S.sink[l]);
+                            uint8_t v[32];
+#pragma unroll
+                            for (uint32_t j = 0; j < 32; ++j) {
+                                const uint32_t o = 32 * l + j;
+                                v[j] = from[o < k ? o : 0];
+                            }
+#pragma unroll
+                            for (uint32_t j = 0; j < 32; ++j) {
+                                const uint32_t o = 32 * l + j, x = rb + o;
+                                uint8_t* const t = o < k ? &S.ring[x >= kRing ? x - kRing : x] : sink;
+                                *t = v[j];
+                            }
                             wave_sync();
                             const uint64_t room = pos < cap ? cap - pos : 0;
                             pos += min(k, room);
