@@ -2,8 +2,9 @@
 // ReLU) with hand-written forward, backward and Adam kernels, fp32.
 //
 // Features come from what a scheduler knows before decoding a Deflate stream
-// (its descriptor and first block header); the target is log2 of the cycles
-// the decoder logged for it (mdz_inflate_cost_t).  The model only orders
+// (its descriptor and first block header, including what its code lengths
+// imply); the target is log2 of the cycles the decoder logged for it
+// (mdz_inflate_cost_t).  The model only orders
 // work: decoded bytes never pass through it.
 //
 // Parameter layout (floats): W1[F][H] b1[H] W2[H][H] b2[H] w3[H] b3.
@@ -18,15 +19,73 @@
 namespace mdz {
 namespace {
 
+__constant__ uint8_t kOrder[19] = {16, 17, 18, 0, 8, 7, 9, 6, 10, 5, 11, 4, 12, 3, 13, 2, 14, 1, 15};
+
 constexpr int F = MDZ_MLP_FEATURES, H = MDZ_MLP_HIDDEN;
 constexpr int P_W1 = 0, P_B1 = F * H, P_W2 = P_B1 + H, P_B2 = P_W2 + H * H, P_W3 = P_B2 + H,
               P_B3 = P_W3 + H, P_N = P_B3 + 1;
 static_assert(P_N == MDZ_MLP_PARAMS, "parameter count");
 constexpr int kBlock = 256;
 
-// Features of stream i: log2(1 + src bytes), log2(1 + dst bytes),
-// log2((1 + dst) / (1 + src)), and the first block's type one-hot
-// (stored, fixed, dynamic; all zero if unreadable).
+// A scheduler's view of a Deflate stream before decoding it: its sizes and
+// its first block header.  A dynamic header's code lengths imply a symbol
+// distribution (a code of length L stands for probability 2^-L), which gives
+// the share of match symbols and the expected bits per symbol; with the
+// compressed size these estimate how many symbols, matches and blocks the
+// decoder will meet (zlib closes a block every 16,384 symbols at its default
+// memory level).  Fixed blocks use the fixed code's lengths; stored streams
+// have no symbols and a block per 65,535 bytes.
+struct HeaderBits {                     // LSB-first reader over src[0, n)
+    const uint8_t* p;
+    uint64_t n, pos;
+    bool bad;
+    MDZ_DEV uint32_t get(uint32_t k) {
+        uint32_t v = 0;
+        for (uint32_t i = 0; i < k; ++i, ++pos) {
+            const uint64_t b = pos >> 3;
+            if (b >= n) { bad = true; return 0; }
+            v |= (uint32_t)((p[b] >> (pos & 7)) & 1) << i;
+        }
+        return v;
+    }
+};
+
+MDZ_DEV uint32_t len_extra_bits(uint32_t s) { return s < 265 || s == 285 ? 0 : (s - 261) / 4; }
+MDZ_DEV uint32_t dist_extra_bits(uint32_t d) { return d < 4 ? 0 : (d - 2) / 2; }
+
+// Expected bits per literal/length symbol and match share implied by code
+// lengths lit[0..nlen) and dist[0..ndist); false if no code is present.
+MDZ_DEV bool implied_stats(const uint8_t* lit, uint32_t nlen, const uint8_t* dist, uint32_t ndist,
+                           float* bps, float* pmatch) {
+    float w = 0.f, wm = 0.f, eb = 0.f, ex = 0.f, wd = 0.f, ed = 0.f;
+    for (uint32_t s = 0; s < nlen; ++s) {
+        if (!lit[s]) continue;
+        const float q = __builtin_ldexpf(1.0f, -(int)lit[s]);
+        w += q;
+        eb += q * lit[s];
+        if (s >= 257) {
+            wm += q;
+            ex += q * len_extra_bits(s);
+        }
+    }
+    for (uint32_t d = 0; d < ndist; ++d) {
+        if (!dist[d]) continue;
+        const float q = __builtin_ldexpf(1.0f, -(int)dist[d]);
+        wd += q;
+        ed += q * (dist[d] + dist_extra_bits(d));
+    }
+    if (w <= 0.f) return false;
+    *pmatch = wm / w;
+    const float per_match = (wm > 0.f ? ex / wm : 0.f) + (wd > 0.f ? ed / wd : 0.f);
+    *bps = eb / w + *pmatch * per_match;
+    return true;
+}
+
+// Features of stream i (MDZ_MLP_FEATURES = 11): log2(1 + src bytes),
+// log2(1 + dst bytes), their difference, the first block's type one-hot
+// (stored, fixed, dynamic; all zero if unreadable), log2(1 + estimated
+// symbols), the implied match share, log2(1 + estimated matches),
+// log2(1 + estimated blocks), and the implied bits per symbol / 16.
 __global__ void features_kernel(const mdz_inflate_in_t* __restrict__ in, uint32_t n, int parse_hdr,
                                 float* __restrict__ x) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -34,17 +93,73 @@ __global__ void features_kernel(const mdz_inflate_in_t* __restrict__ in, uint32_
     const mdz_inflate_in_t d = in[i];
     const float ls = log2f(1.0f + (float)d.src_bytes), ld = log2f(1.0f + (float)d.dst_bytes);
     float* xi = x + (size_t)i * F;
+    for (int k = 0; k < F; ++k) xi[k] = 0.f;
     xi[0] = ls;
     xi[1] = ld;
     xi[2] = ld - ls;
     const uint8_t* src = reinterpret_cast<const uint8_t*>(d.src);
     const int hdr = parse_hdr & 3;
     const int64_t at = hdr == MDZ_HDR_ZLIB ? 2 : hdr == MDZ_HDR_GZIP ? gzip_header_len(src, d.src_bytes) : 0;
-    int type = -1;
-    if (at >= 0 && d.src_bytes > (uint64_t)at) type = (src[at] >> 1) & 3;
+    if (at < 0 || d.src_bytes <= (uint64_t)at) return;
+    HeaderBits r{src + at, d.src_bytes - (uint64_t)at, 0, false};
+    r.get(1);
+    const uint32_t type = r.get(2);
+    if (r.bad || type == 3) return;
     xi[3] = type == 0;
     xi[4] = type == 1;
     xi[5] = type == 2;
+    const float bytes = (float)(d.src_bytes - (uint64_t)at);
+    if (type == 0) {
+        xi[9] = log2f(1.0f + bytes / 65535.0f);
+        return;
+    }
+    uint8_t lens[320];
+    uint32_t nlen = 288, ndist = 30;
+    if (type == 1) {
+        for (uint32_t s = 0; s < 288; ++s) lens[s] = s < 144 ? 8 : s < 256 ? 9 : s < 280 ? 7 : 8;
+        for (uint32_t s = 0; s < 30; ++s) lens[288 + s] = 5;
+    } else {
+        nlen = r.get(5) + 257;
+        ndist = r.get(5) + 1;
+        const uint32_t ncode = r.get(4) + 4;
+        uint8_t cl[19] = {};
+        for (uint32_t k = 0; k < ncode; ++k) cl[kOrder[k]] = (uint8_t)r.get(3);
+        if (r.bad || nlen > 286 || ndist > 30) return;
+        uint32_t cnt[8] = {}, sym[19], offs[8] = {};
+        for (uint32_t s = 0; s < 19; ++s) ++cnt[cl[s]];
+        cnt[0] = 0;
+        for (uint32_t k = 1; k < 7; ++k) offs[k + 1] = offs[k] + cnt[k];
+        for (uint32_t s = 0; s < 19; ++s)
+            if (cl[s]) sym[offs[cl[s]]++] = s;
+        uint32_t idx = 0, prev = 0;
+        while (idx < nlen + ndist) {
+            int code = 0, first = 0, index = 0, s = -1;      // canonical decode
+            for (uint32_t len = 1; len < 8; ++len) {
+                code |= (int)r.get(1);
+                const int c = (int)cnt[len];
+                if (code - c < first) { s = (int)sym[index + (code - first)]; break; }
+                index += c;
+                first = (first + c) << 1;
+                code <<= 1;
+            }
+            if (s < 0 || r.bad) return;
+            uint32_t rep = 1, val = (uint32_t)s;
+            if (s == 16) { if (idx == 0) return; val = prev; rep = 3 + r.get(2); }
+            else if (s == 17) { val = 0; rep = 3 + r.get(3); }
+            else if (s == 18) { val = 0; rep = 11 + r.get(7); }
+            if (r.bad || idx + rep > nlen + ndist) return;
+            for (uint32_t k = 0; k < rep; ++k, ++idx) lens[idx < nlen ? idx : 288 + (idx - nlen)] = (uint8_t)val;
+            prev = val;
+        }
+    }
+    float bps = 0.f, pm = 0.f;
+    if (!implied_stats(lens, nlen, lens + 288, ndist, &bps, &pm) || bps <= 0.f) return;
+    const float syms = 8.0f * bytes / bps;
+    xi[6] = log2f(1.0f + syms);
+    xi[7] = pm;
+    xi[8] = log2f(1.0f + syms * pm);
+    xi[9] = log2f(1.0f + syms / 16384.0f);
+    xi[10] = bps / 16.0f;
 }
 
 MDZ_DEV void forward_one(const float* __restrict__ p, const float* xi, float* h1, float* h2, float& y) {

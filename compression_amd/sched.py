@@ -18,7 +18,7 @@ import numpy as np
 from ._lib import DeviceBuffer, check, lib
 from .lanes import as_ptr
 
-F, H = 6, 32
+F, H = 11, 32
 P = F * H + H + H * H + H + H + 1
 
 

@@ -480,7 +480,7 @@ int mdz_file_close(mdz_file_t file);
  * log2(decode cycles) of a Deflate stream, trained on the decoder's own cost
  * log.  It only orders work.  No counterpart in the reference.
  * Parameters (floats): W1[F][H] b1[H] W2[H][H] b2[H] w3[H] b3. */
-#define MDZ_MLP_FEATURES 6
+#define MDZ_MLP_FEATURES 11
 #define MDZ_MLP_HIDDEN 32
 #define MDZ_MLP_PARAMS (MDZ_MLP_FEATURES * MDZ_MLP_HIDDEN + MDZ_MLP_HIDDEN + \
                         MDZ_MLP_HIDDEN * MDZ_MLP_HIDDEN + MDZ_MLP_HIDDEN + MDZ_MLP_HIDDEN + 1)

@@ -9,7 +9,7 @@ from __future__ import annotations
 
 import numpy as np
 
-F, H = 6, 32
+F, H = 11, 32
 P = F * H + H + H * H + H + H + 1
 
 
@@ -70,3 +70,123 @@ def init(seed: int, target_mean: float = 0.0) -> np.ndarray:
     p[o:o + H] = rng.normal(0, np.sqrt(1.0 / H), H); o += H
     p[o] = target_mean
     return p
+
+
+# ---------------------------------------------------------------- features --
+_ORDER = [16, 17, 18, 0, 8, 7, 9, 6, 10, 5, 11, 4, 12, 3, 13, 2, 14, 1, 15]
+
+
+def _len_extra(s):
+    return 0 if s < 265 or s == 285 else (s - 261) // 4
+
+
+def _dist_extra(d):
+    return 0 if d < 4 else (d - 2) // 2
+
+
+def features(stream: bytes, dst_bytes: int, zlib_hdr: bool = True) -> np.ndarray:
+    """The 11 scheduler features of one Deflate stream, restated from the
+    format (RFC 1951 3.2.3-3.2.7) for checking the GPU feature kernel: sizes,
+    first block type, and what a first dynamic (or fixed) block's code
+    lengths imply -- match share p = sum over length codes of 2^-L divided by
+    the sum over all codes, bits per symbol = E[L] + p * (E[length extra] +
+    E[distance code + extra]), symbols = 8 * bytes / bits per symbol, matches
+    = symbols * p, blocks = symbols / 16384 (stored: bytes / 65535)."""
+    x = np.zeros(F)
+    n = len(stream)
+    x[0], x[1] = np.log2(1 + n), np.log2(1 + dst_bytes)
+    x[2] = x[1] - x[0]
+    at = 2 if zlib_hdr else 0
+    if n <= at:
+        return x
+    data = stream[at:]
+    pos = [0]
+
+    def get(k):
+        v = 0
+        for i in range(k):
+            b = pos[0] >> 3
+            if b >= len(data):
+                raise EOFError
+            v |= ((data[b] >> (pos[0] & 7)) & 1) << i
+            pos[0] += 1
+        return v
+    try:
+        get(1)
+        btype = get(2)
+    except EOFError:
+        return x
+    if btype == 3:
+        return x
+    x[3 + btype] = 1
+    nbytes = float(len(data))
+    if btype == 0:
+        x[9] = np.log2(1 + nbytes / 65535)
+        return x
+    if btype == 1:
+        lit = [8] * 144 + [9] * 112 + [7] * 24 + [8] * 8
+        dist = [5] * 30
+    else:
+        try:
+            nlen, ndist, ncode = get(5) + 257, get(5) + 1, get(4) + 4
+            cl = [0] * 19
+            for k in range(ncode):
+                cl[_ORDER[k]] = get(3)
+            if nlen > 286 or ndist > 30:
+                return x
+            codes, code, nxt = {}, 0, {}
+            cnt = [0] * 8
+            for v in cl:
+                if v:
+                    cnt[v] += 1
+            for L in range(1, 8):
+                code = (code + cnt[L - 1]) << 1 if L > 1 else 0
+                nxt[L] = code
+            for s, L in enumerate(cl):
+                if L:
+                    codes[(L, nxt[L])] = s
+                    nxt[L] += 1
+            lens, prev = [], 0
+            while len(lens) < nlen + ndist:
+                c, s = 0, None
+                for L in range(1, 8):
+                    c = (c << 1) | get(1)
+                    if (L, c) in codes:
+                        s = codes[(L, c)]
+                        break
+                if s is None:
+                    return x
+                if s < 16:
+                    lens.append(s)
+                    prev = s
+                elif s == 16:
+                    if not lens:
+                        return x
+                    lens += [prev] * (3 + get(2))
+                elif s == 17:
+                    lens += [0] * (3 + get(3))
+                    prev = 0
+                else:
+                    lens += [0] * (11 + get(7))
+                    prev = 0
+            if len(lens) > nlen + ndist:
+                return x
+        except EOFError:
+            return x
+        lit, dist = lens[:nlen], lens[nlen:]
+    w = sum(2.0 ** -L for L in lit if L)
+    if w <= 0:
+        return x
+    wm = sum(2.0 ** -L for s, L in enumerate(lit) if L and s >= 257)
+    eb = sum(2.0 ** -L * L for L in lit if L)
+    ex = sum(2.0 ** -L * _len_extra(s) for s, L in enumerate(lit) if L and s >= 257)
+    wd = sum(2.0 ** -L for L in dist if L)
+    ed = sum(2.0 ** -L * (L + _dist_extra(d)) for d, L in enumerate(dist) if L)
+    p = wm / w
+    bps = eb / w + p * ((ex / wm if wm > 0 else 0) + (ed / wd if wd > 0 else 0))
+    if bps <= 0:
+        return x
+    syms = 8 * nbytes / bps
+    x[6], x[7], x[8] = np.log2(1 + syms), p, np.log2(1 + syms * p)
+    x[9], x[10] = np.log2(1 + syms / 16384), bps / 16
+    return x
