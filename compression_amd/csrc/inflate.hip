@@ -69,6 +69,11 @@ constexpr uint32_t kIndirect = 0x400;                   // root entry pointing a
 constexpr uint32_t kSpanLog = 9, kSpan = 1u << kSpanLog; // input bits each lane takes per round
 constexpr uint32_t kReach = 3;                          // spans a hand-off may run ahead
 constexpr uint32_t kStageWords = 64 * kSpan / 32 + 8;   // one round's input, plus a symbol's overhang
+// Staged words and hand-off bits are stored with one padding word after every
+// 16: lane t's span starts at word 16t, so unpadded the 64 lanes' accesses
+// would fall on a few LDS banks; padded, word w sits at w + w / 16.
+MDZ_DEV uint32_t pad16(uint32_t w) { return w + (w >> 4); }
+constexpr uint32_t kStagePadded = kStageWords + kStageWords / 16 + 1;
 constexpr uint32_t kSymCap = 2048;                      // true-path symbols kept per round
 
 // Table entry: [3:0] code length (0: no entry, take the canonical path),
@@ -139,10 +144,10 @@ struct InflateLds {
     uint32_t lit[kLitCap];                     // literal/length table, 10-bit root
     uint32_t dist[kDistCap];                   // distance table, 8-bit root
     Code lit_c, dist_c;                        // canonical tables of the same codes
-    uint32_t in[kStageWords];                  // the round's input bits
+    uint32_t in[kStagePadded];                 // the round's input bits (padded: pad16)
     union {
         HdrScratch h;
-        uint32_t bm[64 * (kSpan / 32)];        // per lane: symbol starts met in its span
+        uint32_t bm[64 * (kSpan / 32) * 17 / 16];   // per lane: symbol starts met in its span (padded)
     } u;
     uint32_t sym[2][kSymCap];                  // rounds' symbols, in stream order (double buffer)
     Item item[2];
@@ -368,8 +373,8 @@ MDZ_DEV uint32_t lookup2(const uint32_t* T, uint32_t rb, const Code& c, uint32_t
 // 64 input bits from relative bit position p of the staged round input.
 MDZ_DEV uint64_t peek64(const uint32_t* in, uint32_t p) {
     const uint32_t w = p >> 5, sh = p & 31;
-    const uint64_t lo = (uint64_t)in[w] | ((uint64_t)in[w + 1] << 32);
-    const uint64_t hi = in[w + 2];
+    const uint64_t lo = (uint64_t)in[pad16(w)] | ((uint64_t)in[pad16(w + 1)] << 32);
+    const uint64_t hi = in[pad16(w + 2)];
     return sh ? (lo >> sh) | (hi << (64 - sh)) : lo;
 }
 
@@ -745,7 +750,7 @@ __global__ __launch_bounds__(128) void inflate_kernel(const mdz_inflate_in_t* __
                         const uint32_t rel = (uint32_t)(8 * (src_addr + byte - A) + (P & 7));
                         const uint64_t limb = 8 * (uint64_t)(src_addr + r.n - A);
                         const uint32_t lim = limb > 0xF0000000ull ? 0xF0000000u : (uint32_t)limb;
-                        for (uint32_t k = l; k < kStageWords; k += 64) S.in[k] = r.load_dword(A + 4 * k);
+                        for (uint32_t k = l; k < kStageWords; k += 64) S.in[pad16(k)] = r.load_dword(A + 4 * k);
                         wave_sync();
                         INF_PHASE(5);
 
@@ -774,7 +779,7 @@ __global__ __launch_bounds__(128) void inflate_kernel(const mdz_inflate_in_t* __
                         }
                         if (p < s_nx) p = s_nx;
 #pragma unroll
-                        for (uint32_t k = 0; k < kSpan / 32; ++k) S.u.bm[(kSpan / 32) * l + k] = bm[k];
+                        for (uint32_t k = 0; k < kSpan / 32; ++k) S.u.bm[pad16((kSpan / 32) * l + k)] = bm[k];
                         wave_sync();
                         INF_PHASE(6);
 
@@ -790,7 +795,7 @@ __global__ __launch_bounds__(128) void inflate_kernel(const mdz_inflate_in_t* __
                                 const uint32_t j = (h - rel) >> kSpanLog;
                                 if (j > l + kReach || j > 63) { lost = true; break; }
                                 const uint32_t k = h - (rel + kSpan * j);
-                                if ((S.u.bm[(kSpan / 32) * j + (k >> 5)] >> (k & 31)) & 1) { g = j; break; }
+                                if ((S.u.bm[pad16((kSpan / 32) * j + (k >> 5))] >> (k & 31)) & 1) { g = j; break; }
                                 const uint32_t rc = decode_sym(S, h, lim, &sym, &used);
                                 if (rc != kSymOk) { stop = true; break; }
                                 h += used;

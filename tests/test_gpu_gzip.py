@@ -147,11 +147,16 @@ def test_gzip_cost_features(gpu):
     datas = _datas(9)[:10]
     zs = [zlib.compress(d, (0, 1, 6)[i % 3]) for i, d in enumerate(datas)]
     gs = [gzip_member(d, (0, 1, 6)[i % 3], name=b"f" * i)[0] for i, d in enumerate(datas)]
+    F = sched.F
     xz = sched.features(_batch(zs, [len(d) for d in datas]), parse_hdr=ZLIB).download(
-        len(datas) * 24).view(np.float32).reshape(-1, 6)
+        len(datas) * 4 * F).view(np.float32).reshape(-1, F)
     xg = sched.features(_batch(gs, [len(d) for d in datas]), parse_hdr=GZIP).download(
-        len(datas) * 24).view(np.float32).reshape(-1, 6)
-    assert np.array_equal(xz[:, 3:], xg[:, 3:]) and (xz[:, 3:].sum(1) == 1).all()
+        len(datas) * 4 * F).view(np.float32).reshape(-1, F)
+    # the first block's type and what its code lengths imply (match share,
+    # bits per symbol) do not depend on the framing; the size-derived
+    # estimates do
+    cols = [3, 4, 5, 7, 10]
+    assert np.array_equal(xz[:, cols], xg[:, cols]) and (xz[:, 3:6].sum(1) == 1).all()
 
 
 @pytest.mark.parametrize("src_skew,dst_skew", [(0, 0), (4, 0), (1, 3), (5, 5), (0, 7)])
