@@ -197,7 +197,7 @@ struct LaneEnc {
         put(len & 0xFF);
         varint(lit(0));
         const int64_t d0 = n > 1 ? (int64_t)dl(0) : 0;
-        varint((uint64_t)(d0 << 1) ^ (uint64_t)(d0 >> 63));          // zigzag
+        varint(((uint64_t)d0 << 1) ^ (uint64_t)(d0 >> 63));          // zigzag
         if (!fixed_delta) write_aligned([&](uint32_t i) { return dl(i + 1); }, n - 2, bits);
         n = 0;
         var = 0;

@@ -10,7 +10,8 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB_PATH = os.path.join(_HERE, "libmdz_host.so")
+# MDZ_HOST_LIB names another build (the sanitizer build of oracle/asan.sh)
+_LIB_PATH = os.environ.get("MDZ_HOST_LIB") or os.path.join(_HERE, "libmdz_host.so")
 _lib = None
 
 u8p = C.POINTER(C.c_uint8)

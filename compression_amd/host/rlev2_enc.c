@@ -186,7 +186,7 @@ static void write_delta(enc_t* e) {
     put(e, (uint8_t)(len & 0xff));
     write_varint_u(e, e->lit[0]);
     const int64_t d0 = (int64_t)e->delta[0];
-    write_varint_u(e, (uint64_t)(d0 << 1) ^ (uint64_t)(d0 >> 63));       /* writeVslong */
+    write_varint_u(e, ((uint64_t)d0 << 1) ^ (uint64_t)(d0 >> 63));       /* writeVslong */
     if (!e->fixed_delta) write_aligned(e, e->delta + 1, e->n - 2, bits);
     for (uint32_t i = 0; i < e->n; ++i) e->delta[i] = 0;
     e->n = 0;

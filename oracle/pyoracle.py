@@ -23,7 +23,8 @@ from compression_amd.host import (BrleContainer, DeflateBatch, LzssContainer, Rl
 from compression_amd.host import lzss_lane_encode as _lzss_lane_encode
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB_PATH = os.path.join(_HERE, "libmdz_oracle.so")
+# MDZ_ORACLE_LIB names another build (the sanitizer build of oracle/asan.sh)
+_LIB_PATH = os.environ.get("MDZ_ORACLE_LIB") or os.path.join(_HERE, "libmdz_oracle.so")
 _lib = None
 
 u8p = C.POINTER(C.c_uint8)
