@@ -249,6 +249,51 @@ class LzssWorkload:
                           f"oracle C decoder, median of >=3 runs"}
 
 
+class LzssPlainWorkload:
+    """LZSS, plain per-chunk format: the same 1 GiB Markov+copy stream, each
+    64 KiB chunk one token stream (back-references up to 2 KiB kept whole);
+    one 512-thread workgroup per chunk."""
+    name = "lzss_plain_markov_1GiB_64KiB"
+    dtype = "u8"
+
+    def __init__(self, n=1 << 30, chunk=65536, seed=0x4D445A03, **_):
+        from compression_amd import host as H
+        import compression_amd as mdz
+        self.mdz = mdz
+        self.data = H.gen_lzss_markov(n, seed)
+        self.c = c = H.lzss_plain_encode(self.data, chunk)
+        self.chunk = chunk
+        self.d_data = mdz.DeviceBuffer.from_array(c.data, pad=16)
+        self.d_lens = mdz.DeviceBuffer.from_array(c.lens)
+        self.dst = mdz.DeviceBuffer(c.raw_bytes)
+        self.status = mdz.DeviceBuffer(4 * c.n_chunks)
+        self.out_bytes = c.raw_bytes
+        nc = c.n_chunks
+        self.alg_bytes = int(c.data.size) + 8 * nc + c.raw_bytes
+        self.config = {"workload": self.name, "codec": "lzss", "format": "plain", "out_bytes": c.raw_bytes,
+                       "payload_bytes": int(c.data.size), "n_chunks": nc, "chunk_bytes": chunk,
+                       "ratio": round(c.data.size / c.raw_bytes, 4)}
+
+    def step(self, stream):
+        from compression_amd import lzss
+        lzss.decompress_plain(self.d_data, self.c.data.size, self.d_lens, self.c.n_chunks, self.chunk,
+                              self.dst, self.out_bytes, self.status, stream=stream)
+
+    def verify(self):
+        got = self.dst.download(self.out_bytes)
+        st = self.status.download(dtype=np.int32)
+        return bool((st == 0).all() and np.array_equal(got, self.data))
+
+    def cpu(self, threads):
+        from oracle import pyoracle as O   # the CPU baseline leg
+        c = self.c
+        t = _cpu_time(lambda: O.lzss_plain_decode(c, threads=threads))
+        return {"value": round(c.raw_bytes / t / 1e9, 3), "unit": "GB/s", "cores": threads,
+                "kind": "port",
+                "sample": f"whole workload ({c.raw_bytes >> 20} MiB out, {c.n_chunks} chunks), "
+                          f"oracle C decoder (the reference's plain decompress_func loop), median of >=3 runs"}
+
+
 class Config2Workload:
     """RLE v2 + byte RLE: the 1 GiB int64 column and the 256 MiB byte-RLE
     stream (128 KiB chunks).  Batched (default): both containers' 10,240
@@ -760,7 +805,8 @@ class FileWorkload:
 WORKLOADS = {"config2": Config2Workload,
              "config2-plain": lambda **kw: Config2Workload(batched=False, **kw),
              "rlev1": RleV1Workload, "rlev2": RleV2Workload,
-             "brle": BrleWorkload, "lzss": LzssWorkload, "deflate": DeflateWorkload,
+             "brle": BrleWorkload, "lzss": LzssWorkload, "lzss-plain": LzssPlainWorkload,
+             "deflate": DeflateWorkload,
              "copy": CopyWorkload, "file": FileWorkload, "brle-encode": BrleEncodeWorkload,
              "rlev1-encode": RleV1EncodeWorkload, "rlev2-encode": RleV2EncodeWorkload,
              "lzss-encode": LzssEncodeWorkload, "inflate-verify": InflateVerifyWorkload,
@@ -908,7 +954,8 @@ def rlev1_line_sweep(args, stream, lines=(32, 128, 512)) -> list:
 
 ALL_CONFIGS = [("config1", "rlev1", "RLE v1, 64 MiB u32 column, plain"),
                ("config2", "config2", "RLE v2 1 GiB int64 + byte RLE 256 MiB, batched persistent"),
-               ("config3", "lzss", "LZSS, 1 GiB Markov+copy, 64 KiB chunks"),
+               ("config3", "lzss", "LZSS, 1 GiB Markov+copy, 64 KiB chunks, lane-transposed"),
+               ("config3-plain", "lzss-plain", "LZSS, 1 GiB Markov+copy, 64 KiB chunks, plain format"),
                ("config4", "deflate", "Deflate, 4,096 uneven zlib streams, scheduled persistent")]
 
 
@@ -956,7 +1003,8 @@ def run_all(args):
             "dtype": "integer (u8/u32/u64 decode)",
             "data": "synthetic (seeded generators, encoded by the host library)",
             "config": {"workload": "configs 1-4 decoded once each per step (RLE v1 64 MiB; RLE v2 1 GiB + "
-                                   "byte RLE 256 MiB batched; LZSS 1 GiB; Deflate 4,096 streams), "
+                                   "byte RLE 256 MiB batched; LZSS 1 GiB lane-transposed and 1 GiB plain; "
+                                   "Deflate 4,096 streams), "
                                    "config 5 serving reported beside",
                        "out_bytes": out_bytes, "parts": [k for k, _, _ in ALL_CONFIGS]},
             "roofline": {"bound": "hbm", "achieved": dom["GBps"], "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -51,6 +51,9 @@ def lib():
         L.mdz_lzss_encode.restype = C.c_int
         L.mdz_lzss_encode.argtypes = [C.c_void_p, C.c_uint64, C.c_uint64, C.POINTER(u8p), u64p,
                                       C.POINTER(u64p), C.POINTER(u64p), C.POINTER(u8p), u64p, C.c_int]
+        L.mdz_lzss_plain_encode.restype = C.c_int
+        L.mdz_lzss_plain_encode.argtypes = [C.c_void_p, C.c_uint64, C.c_uint64, C.POINTER(u8p),
+                                            C.POINTER(u64p), u64p, C.c_int]
         L.mdz_gen_rlev1_u32.argtypes = [C.c_void_p, C.c_uint64, C.c_uint64]
         L.mdz_gen_rlev2_i64.argtypes = [C.c_void_p, C.c_uint64, C.c_uint64]
         L.mdz_gen_brle_u8.argtypes = [C.c_void_p, C.c_uint64, C.c_uint64]
@@ -217,6 +220,29 @@ def lzss_encode(data: np.ndarray, chunk: int, threads: int | None = None) -> Lzs
     n = nc.value
     return LzssContainer(_take(pay, pb.value, np.uint8), _take(cl, 32 * n, np.uint64),
                          _take(bo, n + 1, np.uint64), _take(cm, 32 * n, np.uint8), n, chunk)
+
+class LzssPlainContainer:
+    """Plain LZSS (lzss_gpu2.h:1450-1566,718-792): chunk streams packed back to
+    back in `data`, lens[c] the inclusive prefix sum of their lengths."""
+
+    def __init__(self, data, lens, n_chunks, chunk, raw_bytes):
+        self.data, self.lens, self.n_chunks, self.chunk, self.raw_bytes = data, lens, n_chunks, chunk, raw_bytes
+
+    def stream(self, c: int) -> bytes:
+        b = int(self.lens[c - 1]) if c else 0
+        return self.data[b:int(self.lens[c])].tobytes()
+
+def lzss_plain_encode(data: np.ndarray, chunk: int, threads: int | None = None) -> LzssPlainContainer:
+    raw = np.ascontiguousarray(data).view(np.uint8).reshape(-1)
+    src = raw if raw.size else np.zeros(1, np.uint8)
+    d, ln, nc = u8p(), u64p(), C.c_uint64()
+    rc = lib().mdz_lzss_plain_encode(_ptr(src), raw.size, chunk, C.byref(d), C.byref(ln), C.byref(nc),
+                                     threads or default_threads())
+    if rc != 0:
+        raise ValueError(f"plain lzss encode rejected parameters (status {rc})")
+    n = nc.value
+    lens = _take(ln, n, np.uint64)
+    return LzssPlainContainer(_take(d, int(lens[-1]) if n else 0, np.uint8), lens, n, chunk, raw.size)
 
 def gen_lzss_markov(n: int, seed: int) -> np.ndarray:
     out = np.empty(max(n, 1), np.uint8)

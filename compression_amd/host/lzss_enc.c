@@ -186,3 +186,51 @@ int mdz_lzss_encode(const uint8_t* in, uint64_t in_bytes, uint64_t chunk, uint8_
     return 0;
 }
 
+
+/* ------------------------------------------------------ plain format -- */
+
+typedef struct {
+    const uint8_t* in;
+    uint64_t in_bytes, chunk;
+    uint8_t** streams;
+    uint64_t* len;
+} plain_ctx;
+
+static void plain_task(void* vctx, uint64_t c) {
+    plain_ctx* x = (plain_ctx*)vctx;
+    const uint64_t beg = c * x->chunk;
+    const uint64_t n = x->in_bytes - beg < x->chunk ? x->in_bytes - beg : x->chunk;
+    const uint64_t cap = n + n / 8 + 16;
+    uint8_t* s = (uint8_t*)malloc(cap);
+    x->len[c] = mdz_lzss_lane_encode(x->in + beg, n, s, cap);
+    x->streams[c] = s;
+}
+
+/* The plain (per-chunk) format of compress_func / kernel_shift_data
+ * (lzss_gpu2.h:1450-1566,1926-1948): each chunk of `chunk` input bytes (the
+ * last may be shorter) is one token stream with the lane grammar -- flag
+ * byte, 8 tokens, the next flag byte reserved after every 8th token -- and
+ * the streams are packed back to back; lens[c] is the inclusive prefix sum
+ * of their lengths (stream c is data[lens[c-1], lens[c])). */
+int mdz_lzss_plain_encode(const uint8_t* in, uint64_t in_bytes, uint64_t chunk, uint8_t** data,
+                          uint64_t** lens, uint64_t* n_chunks, int threads) {
+    if (chunk == 0) return -1;
+    const uint64_t n = (in_bytes + chunk - 1) / chunk;
+    plain_ctx x = {in, in_bytes, chunk, (uint8_t**)calloc(n + 1, sizeof(uint8_t*)),
+                   (uint64_t*)calloc(n + 1, sizeof(uint64_t))};
+    mdz_parallel_for(n, threads, plain_task, &x);
+    uint64_t* L = (uint64_t*)malloc((n + 1) * sizeof(uint64_t));
+    uint64_t tot = 0;
+    for (uint64_t c = 0; c < n; ++c) L[c] = (tot += x.len[c]);
+    uint8_t* d = (uint8_t*)malloc(tot + 1);
+    for (uint64_t c = 0; c < n; ++c) {
+        memcpy(d + (c ? L[c - 1] : 0), x.streams[c], x.len[c]);
+        free(x.streams[c]);
+    }
+    free(x.streams);
+    free(x.len);
+    *data = d;
+    *lens = L;
+    *n_chunks = n;
+    return 0;
+}

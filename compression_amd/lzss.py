@@ -5,6 +5,9 @@
 {chunk_size, blk_off[n+1], col_len[32n] (sorted), col_map[32n]} -> host
 bytes.  ``decompress`` is the device-resident, stream-ordered form, and
 ``split_meta`` reads a ``.meta`` side file image (src/lzss/lzss_gpu.cu:109-133).
+``decompress_plain`` is the plain per-chunk format's device decode
+(``lzss::kernel_decompress`` with ``lens``, include/lzss/lzss_gpu2.h:1908-1913,
+718-792).
 """
 from __future__ import annotations
 
@@ -25,6 +28,17 @@ def decompress(lanes: DeviceLanes, dst, dst_bytes: int | None = None, status=Non
     n = lanes.out_bytes if dst_bytes is None else int(dst_bytes)
     check(lib().mdz_lzss_decompress(C.byref(s), as_ptr(dst), n, as_ptr(status), as_ptr(stream)),
           "mdz_lzss_decompress")
+
+
+def decompress_plain(data, data_bytes: int, lens, n_chunks: int, chunk_bytes: int, dst, dst_bytes: int,
+                     status=None, stream=None) -> None:
+    """Enqueue the plain-format decode: chunk c's stream is
+    ``data[lens[c-1]:lens[c]]`` (device pointers; ``lens`` the inclusive
+    prefix sums) and decodes to ``dst[c*chunk_bytes:]``, clipped to
+    ``dst_bytes``."""
+    check(lib().mdz_lzss_plain_decompress(as_ptr(data), int(data_bytes), as_ptr(lens), int(n_chunks),
+                                          int(chunk_bytes), as_ptr(dst), int(dst_bytes), as_ptr(status),
+                                          as_ptr(stream)), "mdz_lzss_plain_decompress")
 
 
 def split_meta(meta: bytes) -> list[bytes]:

@@ -265,6 +265,24 @@ int mdz_lzss_decompress_host(const uint8_t* in, uint64_t in_n_bytes,
                              const uint8_t* const* in_metadata, const uint64_t* in_metadata_lens,
                              uint64_t in_metadata_num, uint8_t** out, uint64_t* out_n_bytes);
 
+/* Plain LZSS decode -- replaces lzss::kernel_decompress(in, out, in_n_bytes,
+ * out_n_bytes, out_chunk_size, n_chunks, lens) and the plain decompress_func
+ * (reference include/lzss/lzss_gpu2.h:1908-1913,718-792).  Chunk c's stream
+ * is data[lens[c-1], lens[c]) (lens = inclusive prefix sums of the stream
+ * lengths, as compress_func :1450-1566 and its scan leave them; lens[-1] = 0)
+ * with the same token grammar as the lane streams, and decodes to
+ * dst[c*chunk_bytes, ...) clipped to dst_bytes, so the last chunk may be
+ * short.  The reference strides its output by the compile-time CHUNK_SIZE
+ * (128 KiB) and stops when a chunk's input runs out; here a chunk stops when
+ * its output is complete, and status[c] (optional) gives the first problem in
+ * stream order: INPUT_OVERRUN (the stream ends first), BAD_DISTANCE (a match
+ * reaching before the chunk start), OUTPUT_OVERRUN (a match past the chunk's
+ * end), or BAD_HEADER (lens[c] < lens[c-1] or lens[c] > data_bytes).  Bytes
+ * of a failed chunk are unspecified.  chunk_bytes at most 256 MiB; any
+ * alignment (16-byte aligned output chunks take the vector-store path). */
+int mdz_lzss_plain_decompress(const void* data, uint64_t data_bytes, const uint64_t* lens, uint64_t n_chunks,
+                              uint64_t chunk_bytes, void* dst, uint64_t dst_bytes, int32_t* status, void* stream);
+
 /* LZSS encode -- replaces lzss::compress_gpu (reference include/lzss/
  * lzss_gpu2.h:1952-2123: find_match :70-109, lane encoder :332-494,511-707,
  * lane sort :1063-1085, interleave :497-509).  payload, col_len (sorted),

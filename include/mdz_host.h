@@ -67,6 +67,12 @@ uint64_t mdz_lzss_lane_encode(const uint8_t* lane, uint64_t n, uint8_t* out, uin
 int mdz_lzss_encode(const uint8_t* in, uint64_t in_bytes, uint64_t chunk, uint8_t** payload,
                     uint64_t* payload_bytes, uint64_t** col_len, uint64_t** blk_off,
                     uint8_t** col_map, uint64_t* n_chunks, int threads);
+/* Plain (per-chunk) LZSS, compress_func + kernel_shift_data
+ * (lzss_gpu2.h:1450-1566,1926-1948): chunk streams packed back to back,
+ * lens[c] = inclusive prefix sum of their lengths; the last chunk may be
+ * shorter than `chunk`. */
+int mdz_lzss_plain_encode(const uint8_t* in, uint64_t in_bytes, uint64_t chunk, uint8_t** data,
+                          uint64_t** lens, uint64_t* n_chunks, int threads);
 
 /* ----------------------------------------------------- synthetic data -- */
 /* Config 1: u32 column, runs 70% / literal segments 30% by count. */

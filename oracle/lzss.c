@@ -202,3 +202,30 @@ int mdz_ref_lzss_decode(const uint8_t* payload, uint64_t payload_bytes, const ui
     return 0;
 }
 
+
+/* Plain (per-chunk) decompress_func (lzss_gpu2.h:718-792): chunk c's stream
+ * is data[lens[c-1], lens[c]) and decodes to out[c*chunk, ...) (the last
+ * chunk holds what is left of out_bytes), with mdz_ref_lzss_lane_decode's
+ * bounds and statuses.  The reference strides its output by the compile-time
+ * CHUNK_SIZE and stops when its input is used up. */
+typedef struct {
+    const uint8_t* data; const uint64_t* lens; uint64_t chunk; uint8_t* out; uint64_t out_bytes;
+    int32_t* status;
+} plain_dec_ctx;
+
+static void plain_dec_task(void* vctx, uint64_t c) {
+    plain_dec_ctx* x = (plain_dec_ctx*)vctx;
+    const uint64_t b = c ? x->lens[c - 1] : 0, e = x->lens[c];
+    const uint64_t o = c * x->chunk;
+    const uint64_t n = o >= x->out_bytes ? 0 : (x->out_bytes - o < x->chunk ? x->out_bytes - o : x->chunk);
+    x->status[c] = e < b ? MDZ_ST_BAD_HEADER : mdz_ref_lzss_lane_decode(x->data + b, e - b, x->out + o, n);
+}
+
+int mdz_ref_lzss_plain_decode(const uint8_t* data, const uint64_t* lens, uint64_t n_chunks, uint64_t chunk,
+                              uint8_t* out, uint64_t out_bytes, int32_t* status, int threads) {
+    plain_dec_ctx x = {data, lens, chunk, out, out_bytes, status};
+    mdz_parallel_for(n_chunks, threads, plain_dec_task, &x);
+    for (uint64_t c = 0; c < n_chunks; ++c)
+        if (status[c] != MDZ_ST_OK) return 1;
+    return 0;
+}

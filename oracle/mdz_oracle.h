@@ -82,6 +82,11 @@ int mdz_ref_lzss_decode(const uint8_t* payload, uint64_t payload_bytes, const ui
                         const uint64_t* col_len, const uint8_t* col_map, uint64_t n_chunks,
                         uint64_t chunk, uint8_t* out, uint64_t out_bytes, int32_t* status,
                         int threads);
+/* Plain (per-chunk) decompress_func (lzss_gpu2.h:718-792) over packed
+ * streams with inclusive-prefix lens[]; per-chunk status as the lane
+ * decoder's. */
+int mdz_ref_lzss_plain_decode(const uint8_t* data, const uint64_t* lens, uint64_t n_chunks, uint64_t chunk,
+                              uint8_t* out, uint64_t out_bytes, int32_t* status, int threads);
 
 /* =========================================================== Deflate ===== *
  * zlib is the oracle (RFC 1950/1951/1952).  hdr = framing | verify:

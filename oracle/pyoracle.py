@@ -14,7 +14,8 @@ import subprocess
 
 import numpy as np
 
-from compression_amd.host import (BrleContainer, DeflateBatch, LzssContainer, RleV1Container,  # noqa: F401
+from compression_amd.host import (BrleContainer, DeflateBatch, LzssContainer, LzssPlainContainer,  # noqa: F401
+                                  RleV1Container, lzss_plain_encode,
                                   RleV2Container, _ptr, _take, brle_chunk_encode, brle_encode,
                                   default_threads, gen_brle_u8, gen_deflate_batch, gen_lzss_markov,
                                   gen_markov, gen_rlev1_u32, gen_rlev2_i64, host_cores, lzss_encode,
@@ -68,6 +69,9 @@ def lib():
         L.mdz_ref_lzss_lane_encode_naive.argtypes = [C.c_void_p, C.c_uint64, C.c_void_p, C.c_uint64]
         L.mdz_ref_lzss_lane_decode.restype = C.c_int
         L.mdz_ref_lzss_lane_decode.argtypes = [C.c_void_p, C.c_uint64, C.c_void_p, C.c_uint64]
+        L.mdz_ref_lzss_plain_decode.restype = C.c_int
+        L.mdz_ref_lzss_plain_decode.argtypes = [C.c_void_p, C.c_void_p, C.c_uint64, C.c_uint64, C.c_void_p,
+                                                C.c_uint64, C.c_void_p, C.c_int]
         L.mdz_ref_lzss_decode.restype = C.c_int
         L.mdz_ref_lzss_decode.argtypes = [C.c_void_p, C.c_uint64, C.c_void_p, C.c_void_p, C.c_void_p,
                                           C.c_uint64, C.c_uint64, C.c_void_p, C.c_uint64, C.c_void_p,
@@ -162,6 +166,18 @@ def lzss_decode(c: LzssContainer, out_bytes: int | None = None, threads: int | N
                                    c.n_chunks, c.chunk, _ptr(out), out_bytes, _ptr(status),
                                    threads or default_threads())
     return rc, out[:out_bytes], status[:c.n_chunks]
+
+def lzss_plain_decode(c, out_bytes: int | None = None, threads: int | None = None):
+    """Plain LZSS container (host.LzssPlainContainer) -> (rc, bytes, status[n])."""
+    out_bytes = c.raw_bytes if out_bytes is None else out_bytes
+    out = np.zeros(max(out_bytes, 1), np.uint8)
+    status = np.zeros(max(c.n_chunks, 1), np.int32)
+    data = c.data if c.data.size else np.zeros(1, np.uint8)
+    lens = c.lens if c.lens.size else np.zeros(1, np.uint64)
+    rc = lib().mdz_ref_lzss_plain_decode(_ptr(data), _ptr(lens), c.n_chunks, c.chunk, _ptr(out), out_bytes,
+                                         _ptr(status), threads or default_threads())
+    return rc, out[:out_bytes], status[:c.n_chunks]
+
 
 def zlib_version() -> str:
     return lib().mdz_zlib_version().decode()

@@ -158,3 +158,66 @@ def test_container_statuses():
     rc, out, st = O.lzss_decode(bad, threads=2)
     assert st[0] == 0 and st[1] == -11 and st[2] == -11 and st[3] == 0
     assert np.array_equal(out[:4096], x[:4096])
+
+
+# ------------------------------------------------------------ plain format --
+@pytest.mark.parametrize("size,chunk", [(0, 4096), (1, 4096), (4096, 4096), (10_000, 4096),
+                                        (3 * 65536 + 12_345, 65536)])
+def test_plain_container_layout_and_roundtrip(size, chunk):
+    """Plain LZSS (compress_func + kernel_shift_data, lzss_gpu2.h:1450-1566,
+    1926-1948): each chunk's stream is the chunk run through the lane grammar,
+    streams are packed back to back, lens is the inclusive prefix of their
+    lengths; the last chunk may be short.  Decoded by the independent Python
+    decoder and by the oracle."""
+    x = O.gen_lzss_markov(size, 0x51 + size)
+    c = O.lzss_plain_encode(x, chunk)
+    n = -(-size // chunk)
+    assert c.n_chunks == n and c.lens.size == n
+    pos = 0
+    for i in range(n):
+        part = x[i * chunk:(i + 1) * chunk].tobytes()
+        s = c.stream(i)
+        assert s == O.lzss_lane_encode(part)
+        assert int(c.lens[i]) == pos + len(s)
+        pos += len(s)
+        st, got = py_lane_decode(s, len(part))
+        assert st == 0 and got == part
+    rc, got, status = O.lzss_plain_decode(c)
+    assert rc == 0 and (status == 0).all() and got.tobytes() == x.tobytes()
+
+
+def test_plain_stream_equals_literal_scan():
+    """The plain chunk stream of the fast matcher equals the one from the
+    literal find_match scan (lzss_gpu2.h:70-109) on whole chunks."""
+    x = O.gen_lzss_markov(4096 * 3, 0x77)
+    c = O.lzss_plain_encode(x, 4096)
+    for i in range(3):
+        part = x[i * 4096:(i + 1) * 4096].tobytes()
+        assert c.stream(i) == O.lzss_lane_encode(part, naive=True)
+
+
+def test_plain_statuses():
+    """Damaged plain streams: cut short -> input overrun (2); a distance past
+    the chunk start -> bad distance (-9); statuses per chunk."""
+    x = O.gen_lzss_markov(3 * 4096, 0x78)
+    c = O.lzss_plain_encode(x, 4096)
+    d = c.data.copy()
+    lens = c.lens.copy()
+    lens[0] -= 5                                    # chunk 0 loses its tail; chunk 1 starts early
+    bad = O.LzssPlainContainer(d, lens, c.n_chunks, c.chunk, c.raw_bytes)
+    rc, _, st = O.lzss_plain_decode(bad)
+    assert st[0] == 2 and st[2] == 0
+    first = O.lzss_plain_encode(np.frombuffer(b"ab" * 100, np.uint8), 4096)
+    s = bytearray(first.stream(0))
+    # the first match token: distance field set past the output produced so far
+    flags, q = s[0], 1
+    for b in range(8):
+        if flags >> b & 1:
+            q += 1
+            continue
+        v = s[q] | s[q + 1] << 8
+        v = (v & 31) | (2000 << 5)
+        s[q], s[q + 1] = v & 255, v >> 8
+        break
+    c2 = O.LzssPlainContainer(np.frombuffer(bytes(s), np.uint8), np.array([len(s)], np.uint64), 1, 4096, 200)
+    assert O.lzss_plain_decode(c2)[2][0] == -9
