@@ -43,13 +43,14 @@ namespace {
 
 constexpr uint32_t kNT = 512;                   // threads per workgroup (8 waves)
 constexpr uint32_t kSeg = 64;                   // stream bytes per segment
-constexpr uint32_t kPiece = 16384;              // stream bytes whose groups one pass handles
+constexpr uint32_t kPiece = 8192;               // stream bytes whose groups one pass handles
 constexpr uint32_t kNSeg = kPiece / kSeg;
 constexpr uint32_t kStage = kPiece + 64;        // a group reaches at most 16 bytes past its start
-constexpr uint32_t kGCap = (kPiece - 1) / 9 + 2;  // groups that can start in one piece, + sentinel
+constexpr uint32_t kGCap = (kPiece - 1) / 9 + 1;  // groups that can start in one piece
 constexpr uint32_t kHist = 2048;                // distances are 11 bits
-constexpr uint32_t kWin = 16384;                // output bytes per window
+constexpr uint32_t kWin = 8192;                 // output bytes per window
 constexpr uint32_t kLit = 0x8000u;              // map entry holds a value
+constexpr uint32_t kWTab = kPiece / 2 * 34 / kWin + 2;  // window boundaries inside one piece's output
 constexpr uint32_t kNoErr = 0xffffffffu;
 constexpr uint64_t kMaxChunk = 1ull << 28;      // error keys are (offset << 2 | kind) in 32 bits
 
@@ -59,40 +60,32 @@ static_assert(kPiece / 2 * 34 < (1u << 20), "piece output must fit the 20-bit sc
 struct PlainLds {
     uint16_t map[kHist + kWin];  // [0, kHist): history; [kHist, ...): the window
     uint32_t gout[kGCap];        // output offset (chunk-relative) of each group of the piece
-    uint16_t gpos[kGCap];        // stream offset (piece-relative) of each group
+    uint32_t gpf[kGCap];         // stream offset (piece-relative) | flag byte << 16, per group
     uint32_t wtot[kNT / 64];
+    uint16_t wglo[kWTab];        // group covering each window boundary inside the piece's output
     uint8_t stage[kStage];
     uint8_t ex[kNSeg * 17];
     uint8_t entry[kNSeg];
     uint32_t err;                // min over (output offset << 2 | kind)
     uint32_t next_entry;         // first group's offset in the next piece
 };
-static_assert(sizeof(PlainLds) <= 80 * 1024, "two workgroups per CU");
+static_assert(sizeof(PlainLds) <= 160 * 1024 / 4, "four workgroups per CU");
 
 // Output bytes of the tokens of the group at stage[p] (flag f) that lie
-// wholly inside the first `avail` staged bytes of the stream.
+// wholly inside the first `avail` staged bytes of the stream.  Token t starts
+// at p + 1 + t + (matches before it); all eight first bytes are read.
 MDZ_DEV uint32_t group_out(const uint8_t* st, uint32_t p, uint32_t f, uint32_t avail) {
-    uint32_t ol = 0;
-    if (p + 17u - __popc(f) <= avail) {
-        ol = __popc(f);
-        const uint32_t nf = ~f & 0xffu;
-        for (uint32_t m = nf; m; m &= m - 1) {
-            const uint32_t b = __builtin_ctz(m);
-            ol += (st[p + 1 + b + __popc(nf & ((1u << b) - 1))] & 31u) + 3u;
-        }
-    } else {                                  // the stream ends inside this group
-        uint32_t q = p + 1;
-        for (uint32_t b = 0; b < 8; ++b) {
-            const uint32_t lit = (f >> b) & 1u, need = 2u - lit;
-            if (q + need > avail) break;
-            ol += lit ? 1u : (st[q] & 31u) + 3u;
-            q += need;
-        }
+    uint32_t ol = 0, q = p + 1;
+#pragma unroll
+    for (uint32_t t = 0; t < 8; ++t) {
+        const uint32_t lt = (f >> t) & 1u, c0 = st[q];
+        ol += q + 2 - lt <= avail ? (lt ? 1u : (c0 & 31u) + 3u) : 0u;
+        q += 2 - lt;
     }
     return ol;
 }
 
-__global__ __launch_bounds__(kNT) void lzss_plain_kernel(
+__global__ __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(8))) void lzss_plain_kernel(
     const uint8_t* __restrict__ data, uint64_t data_bytes, const uint64_t* __restrict__ lens,
     uint64_t n_chunks, uint64_t chunk, uint8_t* __restrict__ dst, uint64_t dst_bytes,
     int32_t* __restrict__ status) {
@@ -119,29 +112,41 @@ __global__ __launch_bounds__(kNT) void lzss_plain_kernel(
     uint32_t entry0 = 0;     // offset of the current piece's first group
     uint32_t win = 0;        // output offset of the window the map holds
     uint32_t my_err = kNoErr;
+    MDZ_PHASE_START();
     for (uint64_t pb = 0; pb < L && produced < n; pb += kPiece) {
         const uint32_t plen = (uint32_t)(L - pb < kPiece ? L - pb : kPiece);
         const uint32_t avail = (uint32_t)(L - pb < kStage ? L - pb : kStage);
         const uint32_t nseg = (plen + kSeg - 1) / kSeg;
 
-        // ---- stage stream bytes [pb, pb + avail) as words; loads are aligned
-        // words whose first byte lies inside the stream
+        // ---- stage stream bytes [pb, pb + avail) as words (bytes past avail
+        // in the last word zeroed; later words are never used); loads are
+        // aligned words whose first byte lies inside the stream, four words
+        // per thread in flight
         {
-            const uintptr_t ga = reinterpret_cast<uintptr_t>(src + pb);
-            const uint32_t mis = (uint32_t)(ga & 3u);
-            const uint32_t* aw = reinterpret_cast<const uint32_t*>(ga - mis);
-            const uint32_t nwords = (avail + mis + 3) / 4;
+            const uint32_t mis = (uint32_t)(reinterpret_cast<uintptr_t>(src + pb) & 3u);
+            const uint32_t* aw = reinterpret_cast<const uint32_t*>(src + pb - mis);
+            const uint32_t nwords = (avail + mis + 3) / 4, nw = (avail + 3) / 4;
             uint32_t* st32 = reinterpret_cast<uint32_t*>(S.stage);
-            for (uint32_t k = tid; k < kStage / 4; k += kNT) {
-                const uint32_t lo = k < nwords ? aw[k] : 0u;
-                const uint32_t hi = k + 1 < nwords ? aw[k + 1] : 0u;
-                uint32_t w = alignbyte(hi, lo, mis);
-                const uint32_t b0 = 4 * k;
-                if (b0 + 4 > avail) w = b0 >= avail ? 0u : w & ((1u << (8 * (avail - b0))) - 1u);
-                st32[k] = w;
+            for (uint32_t k0 = tid; k0 < nw; k0 += 4 * kNT) {
+                uint32_t lo[4], hi[4];
+#pragma unroll
+                for (uint32_t j = 0; j < 4; ++j) {
+                    const uint32_t k = k0 + j * kNT;
+                    lo[j] = k < nwords ? aw[k] : 0u;
+                    hi[j] = k + 1 < nwords ? aw[k + 1] : 0u;
+                }
+#pragma unroll
+                for (uint32_t j = 0; j < 4; ++j) {
+                    const uint32_t k = k0 + j * kNT, b0 = 4 * k;
+                    if (k < nw) {
+                        const uint32_t w = alignbyte(hi[j], lo[j], mis);
+                        st32[k] = b0 + 4 > avail ? w & ((1u << (8 * (avail - b0))) - 1u) : w;
+                    }
+                }
             }
         }
         __syncthreads();
+        MDZ_PHASE(0);
 
         // ---- 1. exit tables
         for (uint32_t w = tid; w < nseg * 17; w += kNT) {
@@ -152,101 +157,136 @@ __global__ __launch_bounds__(kNT) void lzss_plain_kernel(
             S.ex[w] = (uint8_t)(p - lim);
         }
         __syncthreads();
+        MDZ_PHASE(1);
 
-        // ---- 2. compose across segments (wave 0; the chain value is uniform)
+        // ---- 2. compose across segments (wave 0; the chain value is uniform).
+        // Rows are loaded one block ahead; lane j of the wave keeps the entry
+        // of segment (s0 + j), written out once per 64 segments.
         if (wv == 0) {
             uint32_t cur = __builtin_amdgcn_readfirstlane(entry0);
+            uint32_t ra[8], rb[8];
+#pragma unroll
+            for (uint32_t j = 0; j < 8; ++j) ra[j] = (j < nseg && ln < 17) ? S.ex[j * 17 + ln] : 0u;
+            uint32_t mine = 0;
             for (uint32_t s = 0; s < nseg; s += 8) {
-                uint32_t row[8];
 #pragma unroll
-                for (int j = 0; j < 8; ++j)
-                    row[j] = (s + j < nseg && ln < 17) ? S.ex[(s + j) * 17 + ln] : 0u;
+                for (uint32_t j = 0; j < 8; ++j)
+                    rb[j] = (s + 8 + j < nseg && ln < 17) ? S.ex[(s + 8 + j) * 17 + ln] : 0u;
 #pragma unroll
-                for (int j = 0; j < 8; ++j) {
-                    if (s + j < nseg) {
-                        if (ln == 0) S.entry[s + j] = (uint8_t)cur;
-                        cur = (uint32_t)__builtin_amdgcn_readlane((int)row[j], (int)cur);
-                    }
+                for (uint32_t j = 0; j < 8; ++j) {
+                    mine = ln == ((s + j) & 63u) ? cur : mine;
+                    const uint32_t nx = (uint32_t)__builtin_amdgcn_readlane((int)ra[j], (int)cur);
+                    cur = s + j < nseg ? nx : cur;
                 }
+                if (((s + 8) & 63u) == 0 || s + 8 >= nseg) {
+                    const uint32_t s0 = s & ~63u;
+                    if (s0 + ln < nseg && ln <= s + 7 - s0) S.entry[s0 + ln] = (uint8_t)mine;
+                }
+#pragma unroll
+                for (uint32_t j = 0; j < 8; ++j) ra[j] = rb[j];
             }
             if (ln == 0) S.next_entry = cur;
         }
         __syncthreads();
+        MDZ_PHASE(2);
         const uint32_t next_entry = S.next_entry;
 
-        // ---- 3. groups: per-segment counts and output bytes, scan, list
-        uint32_t v = 0;
+        // ---- 3. groups.  a: one thread per segment lists its group starts
+        // (count, scan, write); b: one thread per 4 consecutive groups sums
+        // their tokens' output bytes, a block scan gives output offsets, and
+        // each group covering a window boundary records itself there.
+        uint32_t cnt = 0;
         if (tid < nseg) {
             const uint32_t lim = min((tid + 1) * kSeg, plen);
-            uint32_t cnt = 0, sum = 0;
-            for (uint32_t p = tid * kSeg + S.entry[tid]; p < lim;) {
-                const uint32_t f = S.stage[p];
-                sum += group_out(S.stage, p, f, avail);
-                ++cnt;
-                p += 17u - __popc(f);
-            }
-            v = (cnt << 20) | sum;
+            for (uint32_t p = tid * kSeg + S.entry[tid]; p < lim; p += 17u - __popc(S.stage[p])) ++cnt;
         }
-        const uint32_t incl = wave_incl_scan_u32(v);
+        uint32_t incl = wave_incl_scan_u32(cnt);
         if (ln == 63) S.wtot[wv] = incl;
         __syncthreads();
-        uint32_t base = 0, total = 0;
+        uint32_t base = 0, ng = 0;
 #pragma unroll
         for (uint32_t k = 0; k < kNT / 64; ++k) {
             const uint32_t t = S.wtot[k];
             base += k < wv ? t : 0u;
-            total += t;
+            ng += t;
         }
         if (tid < nseg) {
-            const uint32_t ex = base + incl - v;
             const uint32_t lim = min((tid + 1) * kSeg, plen);
-            uint32_t gi = ex >> 20, o = produced + (ex & 0xfffffu);
-            for (uint32_t p = tid * kSeg + S.entry[tid]; p < lim;) {
+            uint32_t gi = base + incl - cnt;
+            for (uint32_t p = tid * kSeg + S.entry[tid]; p < lim; ++gi) {
                 const uint32_t f = S.stage[p];
-                S.gpos[gi] = (uint16_t)p;
-                S.gout[gi] = o;
-                o += group_out(S.stage, p, f, avail);
-                ++gi;
+                S.gpf[gi] = p | (f << 16);
                 p += 17u - __popc(f);
             }
         }
-        const uint32_t ng = total >> 20, pout = total & 0xfffffu;
-        if (tid == 0) S.gout[ng] = produced + pout;
         __syncthreads();
+        uint32_t gl[4], sum = 0;
+#pragma unroll
+        for (uint32_t j = 0; j < 4; ++j) {
+            const uint32_t g = 4 * tid + j;
+            gl[j] = 0;
+            if (g < ng) {
+                const uint32_t pf = S.gpf[g];
+                gl[j] = group_out(S.stage, pf & 0xffffu, pf >> 16, avail);
+            }
+            sum += gl[j];
+        }
+        incl = wave_incl_scan_u32(sum);
+        if (ln == 63) S.wtot[wv] = incl;
+        __syncthreads();
+        uint32_t pout = 0;
+        base = 0;
+#pragma unroll
+        for (uint32_t k = 0; k < kNT / 64; ++k) {
+            const uint32_t t = S.wtot[k];
+            base += k < wv ? t : 0u;
+            pout += t;
+        }
+        const uint32_t kb0 = produced / kWin + 1;   // first window boundary above `produced`
+        {
+            uint32_t o = produced + base + incl - sum;
+#pragma unroll
+            for (uint32_t j = 0; j < 4; ++j) {
+                const uint32_t g = 4 * tid + j;
+                if (g < ng) {
+                    S.gout[g] = o;
+                    if (gl[j])
+                        for (uint32_t b = max((o + kWin - 1) / kWin, kb0); b * kWin < o + gl[j]; ++b)
+                            S.wglo[b - kb0] = (uint16_t)g;
+                }
+                o += gl[j];
+            }
+        }
+        __syncthreads();
+        MDZ_PHASE(3);
 
         // ---- 4./5. windows
         const uint32_t pend = min(produced + pout, n);
         for (uint32_t x0 = produced; x0 < pend;) {
             const uint32_t wend = min(win + kWin, pend);
-            // groups [glo, ghi) overlap [x0, wend): glo = last with gout <= x0
-            uint32_t lo = 0, hi = ng;
-            while (hi - lo > 1) {
-                const uint32_t mid = (lo + hi) >> 1;
-                if (S.gout[mid] <= x0) lo = mid; else hi = mid;
-            }
-            const uint32_t glo = lo;
-            lo = glo; hi = ng;                     // ghi = first with gout >= wend
-            while (lo < hi) {
-                const uint32_t mid = (lo + hi) >> 1;
-                if (S.gout[mid] < wend) lo = mid + 1; else hi = mid;
-            }
-            const uint32_t ghi = lo;
+            // groups [glo, ghi) overlap [x0, wend): x0 is the piece start or a
+            // window boundary, wend the piece end or the next boundary
+            const uint32_t glo = x0 == produced ? 0u : S.wglo[x0 / kWin - kb0];
+            const uint32_t ghi = wend == pend ? ng : S.wglo[wend / kWin - kb0] + 1u;
             const uint32_t mb = kHist - win;       // map index of output offset x: x + mb
             for (uint32_t g = glo + (tid >> 3); g < ghi; g += kNT / 8) {
                 const uint32_t k = tid & 7u;
-                const uint32_t gp = S.gpos[g], go = S.gout[g];
-                const uint32_t f = S.stage[gp];
-                const uint32_t lit = (f >> k) & 1u;
-                const uint32_t q = gp + 1 + k + __popc(~f & ((1u << k) - 1u) & 0xffu);
-                const uint32_t b0 = S.stage[q], b1 = S.stage[q + 1];
-                const uint32_t tl = q + 2 - lit <= avail ? (lit ? 1u : (b0 & 31u) + 3u) : 0u;
-                uint32_t inc = tl;
+                const uint32_t pf = S.gpf[g];
+                uint32_t off = S.gout[g];
+                const uint32_t f = pf >> 16;
+                uint32_t q = (pf & 0xffffu) + 1, tl = 0, lit = 0, b0 = 0, b1 = 0;
 #pragma unroll
-                for (uint32_t o = 1; o < 8; o <<= 1) {
-                    const uint32_t y = __shfl_up(inc, o, 8);
-                    inc += k >= o ? y : 0u;
+                for (uint32_t t = 0; t < 8; ++t) {     // tokens 0..k: offset and bytes
+                    const uint32_t lt = (f >> t) & 1u;
+                    const uint32_t c0 = S.stage[q], c1 = S.stage[q + 1];
+                    const uint32_t len = q + 2 - lt <= avail ? (lt ? 1u : (c0 & 31u) + 3u) : 0u;
+                    off += t < k ? len : 0u;
+                    tl = t == k ? len : tl;
+                    lit = t == k ? lt : lit;
+                    b0 = t == k ? c0 : b0;
+                    b1 = t == k ? c1 : b1;
+                    q += 2 - lt;
                 }
-                const uint32_t off = go + inc - tl;
                 const uint32_t xa = max(off, x0), xb = min(off + tl, wend);
                 if (lit) {
                     if (xa < xb) S.map[off + mb] = (uint16_t)(kLit | b0);
@@ -259,7 +299,15 @@ __global__ __launch_bounds__(kNT) void lzss_plain_kernel(
                     }
                     if (bad) {
                         for (uint32_t x = xa; x < xb; ++x) S.map[x + mb] = (uint16_t)kLit;
-                    } else {
+                    } else if (d >= tl) {              // no self-overlap: entry i holds i - d
+                        uint32_t i = xa + mb;
+                        const uint32_t ie = xb + mb;
+                        if ((i & 1u) && i < ie) { S.map[i] = (uint16_t)(i - d); ++i; }
+                        uint32_t pk = (i - d) | ((i + 1 - d) << 16);
+                        for (; i + 2 <= ie; i += 2, pk += 0x20002u)
+                            *reinterpret_cast<uint32_t*>(&S.map[i]) = pk;
+                        if (i < ie) S.map[i] = (uint16_t)(i - d);
+                    } else {                           // first period of the match
                         uint32_t r = xa - off;
                         if (r >= d) r %= d;
                         const uint32_t sb = off - d + mb;
@@ -271,16 +319,29 @@ __global__ __launch_bounds__(kNT) void lzss_plain_kernel(
                 }
             }
             __syncthreads();
+            MDZ_PHASE(4);
 
-            // resolve [x0, wend)
-            for (uint32_t x = x0 + tid; x < wend; x += kNT) {
-                uint32_t e = S.map[x + mb];
-                if (!(e & kLit)) {
-                    do { e = S.map[e]; } while (!(e & kLit));
-                    S.map[x + mb] = (uint16_t)e;
+            // resolve [x0, wend) two entries per thread and step, both chains in
+            // flight; a resolved chain re-reads map[0] (a value) instead of
+            // branching.  The pair's low entry may precede x0 (already a
+            // value); its high entry past wend is left as it is.
+            for (uint32_t i = ((x0 + mb) & ~1u) + 2 * tid; i < wend + mb; i += 2 * kNT) {
+                const uint32_t w = *reinterpret_cast<const uint32_t*>(&S.map[i]);
+                const bool hi_in = i + 1 < wend + mb;
+                uint32_t e0 = w & 0xffffu, e1 = hi_in ? w >> 16 : kLit;
+                if ((e0 & e1 & kLit) == 0) {
+                    do {
+                        const uint32_t a0 = (e0 & kLit) ? 0u : e0, a1 = (e1 & kLit) ? 0u : e1;
+                        const uint32_t v0 = S.map[a0], v1 = S.map[a1];
+                        e0 = (e0 & kLit) ? e0 : v0;
+                        e1 = (e1 & kLit) ? e1 : v1;
+                    } while ((e0 & e1 & kLit) == 0);
+                    if (hi_in) *reinterpret_cast<uint32_t*>(&S.map[i]) = e0 | (e1 << 16);
+                    else S.map[i] = (uint16_t)e0;
                 }
             }
             __syncthreads();
+            MDZ_PHASE(5);
 
             const bool full = wend == win + kWin;
             if (full || wend == n) {
@@ -310,6 +371,7 @@ __global__ __launch_bounds__(kNT) void lzss_plain_kernel(
                     win += kWin;
                 }
                 __syncthreads();
+                MDZ_PHASE(6);
             }
             x0 = wend;
         }
