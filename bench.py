@@ -249,6 +249,50 @@ class LzssWorkload:
                           f"oracle C decoder, median of >=3 runs"}
 
 
+class BrleTransWorkload:
+    """Byte RLE, lane-transposed (brle_trans): the 256 MiB byte-RLE column at
+    128 KiB chunks in the LZSS lane container; a workgroup decodes 4 lanes of
+    a chunk, one wave per lane."""
+    name = "brle_trans_u8_256MiB"
+    dtype = "u8"
+
+    def __init__(self, n=256 << 20, chunk=131072, seed=0x4D445A02, **_):
+        from compression_amd import host as H
+        from compression_amd.lanes import DeviceLanes
+        import compression_amd as mdz
+        self.mdz = mdz
+        self.data = H.gen_brle_u8(n, seed)
+        self.c = c = H.brle_trans_encode(self.data, chunk)
+        self.lanes = DeviceLanes(c.payload, c.blk_off, c.col_len, c.n_chunks, chunk, 1, 4,
+                                 col_map=c.col_map)
+        self.dst = mdz.DeviceBuffer(c.raw_bytes)
+        self.status = mdz.DeviceBuffer(4 * c.n_chunks)
+        self.out_bytes = c.raw_bytes
+        nc = c.n_chunks
+        self.alg_bytes = c.payload.size + 256 * nc + 8 * (nc + 1) + 32 * nc + c.raw_bytes
+        self.config = {"workload": self.name, "codec": "brle_trans", "out_bytes": c.raw_bytes,
+                       "payload_bytes": int(c.payload.size), "n_chunks": nc, "chunk_bytes": chunk,
+                       "ratio": round(c.payload.size / c.raw_bytes, 4)}
+
+    def step(self, stream):
+        from compression_amd import brle
+        brle.decompress_trans(self.lanes, self.dst, self.out_bytes, self.status, stream=stream)
+
+    def verify(self):
+        got = self.dst.download(self.out_bytes)
+        st = self.status.download(dtype=np.int32)
+        return bool((st == 0).all() and np.array_equal(got, self.data))
+
+    def cpu(self, threads):
+        from oracle import pyoracle as O   # the CPU baseline leg
+        c = self.c
+        t = _cpu_time(lambda: O.brle_trans_decode(c, threads=threads))
+        return {"value": round(c.raw_bytes / t / 1e9, 3), "unit": "GB/s", "cores": threads,
+                "kind": "port",
+                "sample": f"whole workload ({c.raw_bytes >> 20} MiB out, {c.n_chunks} chunks), "
+                          f"oracle C decoder, median of >=3 runs"}
+
+
 class LzssPlainWorkload:
     """LZSS, plain per-chunk format: the same 1 GiB Markov+copy stream, each
     64 KiB chunk one token stream (back-references up to 2 KiB kept whole);
@@ -806,6 +850,7 @@ WORKLOADS = {"config2": Config2Workload,
              "config2-plain": lambda **kw: Config2Workload(batched=False, **kw),
              "rlev1": RleV1Workload, "rlev2": RleV2Workload,
              "brle": BrleWorkload, "lzss": LzssWorkload, "lzss-plain": LzssPlainWorkload,
+             "brle-trans": BrleTransWorkload,
              "deflate": DeflateWorkload,
              "copy": CopyWorkload, "file": FileWorkload, "brle-encode": BrleEncodeWorkload,
              "rlev1-encode": RleV1EncodeWorkload, "rlev2-encode": RleV2EncodeWorkload,

@@ -107,6 +107,7 @@ SIGNATURES = {
     "mdz_rlev2_compress_bound": (C.c_int, [u64, u64, C.c_uint32, C.c_uint32, C.POINTER(u64), C.POINTER(u64),
                                            C.POINTER(u64)]),
     "mdz_rlev2_compress": (C.c_int, [vp, u64, u64, C.c_uint32, C.c_uint32, vp, vp, vp, vp, vp, vp]),
+    "mdz_brle_trans_decompress": (C.c_int, [C.POINTER(LanesT), vp, u64, vp, vp]),
     "mdz_lzss_plain_decompress": (C.c_int, [vp, u64, vp, u64, u64, vp, u64, vp, vp]),
     "mdz_lzss_compress_bound": (C.c_int, [u64, u64, C.POINTER(u64), C.POINTER(u64), C.POINTER(u64)]),
     "mdz_lzss_compress": (C.c_int, [vp, u64, u64, vp, vp, vp, vp, vp, vp]),

@@ -3,6 +3,8 @@
 ``decompress_gpu`` takes the reference container
 [u32 n_ptr][u64 ptr[n_ptr]][u64 out_bytes][data] (include/brle/brle.h:305-342)
 and returns the decoded bytes; ``decompress`` is the device-resident form.
+``decompress_trans`` decodes the lane-transposed variant (``brle_trans``,
+include/brle/brle_trans.h:72-202), whose container is the LZSS lane container.
 """
 from __future__ import annotations
 
@@ -26,6 +28,17 @@ class DeviceBrle:
         self.max_chunk = int(np.diff(p).max()) if len(p) > 1 else 0
         self.chunk_sizes = np.diff(p) if len(p) > 1 else np.zeros(0, np.int64)
         self.data_bytes = data.nbytes
+
+
+def decompress_trans(lanes, dst, dst_bytes: int | None = None, status=None, stream=None) -> None:
+    """Enqueue the decode of a brle_trans container held as ``lanes.DeviceLanes``
+    (with ``col_map``)."""
+    if lanes.col_map is None:
+        raise ValueError("brle_trans containers carry col_map")
+    s = lanes.struct()
+    n = lanes.out_bytes if dst_bytes is None else int(dst_bytes)
+    check(lib().mdz_brle_trans_decompress(C.byref(s), as_ptr(dst), n, as_ptr(status), as_ptr(stream)),
+          "mdz_brle_trans_decompress")
 
 
 def decompress(b: DeviceBrle, dst, dst_bytes: int, status=None, stream=None, tuned: bool = True) -> None:

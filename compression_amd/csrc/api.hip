@@ -474,6 +474,11 @@ int mdz_lzss_decompress_host(const uint8_t* in, uint64_t in_n_bytes,
     return finish_host(d_out.p, raw, (const int32_t*)d_st.p, n, out, out_n_bytes);
 }
 
+int mdz_brle_trans_decompress(const mdz_lanes_t* in, void* dst, uint64_t dst_bytes, int32_t* status,
+                              void* stream) {
+    return mdz::brle_trans_decompress(in, dst, dst_bytes, status, S(stream));
+}
+
 int mdz_lzss_plain_decompress(const void* data, uint64_t data_bytes, const uint64_t* lens, uint64_t n_chunks,
                               uint64_t chunk_bytes, void* dst, uint64_t dst_bytes, int32_t* status, void* stream) {
     return mdz::lzss_plain_decompress(data, data_bytes, lens, n_chunks, chunk_bytes, dst, dst_bytes, status,

@@ -1,9 +1,13 @@
-// LZSS lane-transposed decoder for gfx950.
+// LZSS lane-transposed decoder for gfx950, and the lane-transposed byte RLE
+// that shares its container and flag-group grammar.
 //
 // Replaces lzss::kernel_decompress / decompress_func (reference
 // include/lzss/lzss_gpu2.h:1898-1907,114-325), which runs one thread per lane
 // stream with a 2 KiB history array in scratch memory and 4-byte strided
-// stores.
+// stores, and brle_trans::decompress_func (include/brle/brle_trans.h:72-202),
+// one thread per lane as well.  A brle_trans 2-byte token is a run (count
+// byte, value byte) instead of a match; the kernel is instantiated for each
+// meaning (kRle), and a run's map entries are values, so nothing is resolved.
 //
 // Container: chunk c holds 32 lane streams; sorted position s has
 // col_len[32c+s] bytes and decodes original lane t = col_map[32c+s]; lengths
@@ -72,7 +76,14 @@ MDZ_DEV void report(int32_t* status, uint64_t c, uint32_t lane, int code) {
     if (status) atomicMin(&status[c], (int32_t)((lane << 8) | ((uint32_t)code & 0xffu)));
 }
 
-template <int G>
+// Output bytes of a 2-byte token from its first byte: an LZSS match,
+// (b & 31) + 3; a byte-RLE run (brle_trans), the count byte itself.
+template <bool kRle>
+MDZ_DEV uint32_t tok_len(uint32_t b) {
+    return kRle ? b : (b & 31u) + 3u;
+}
+
+template <int G, bool kRle>
 __global__ __launch_bounds__(64 * G) void lzss_decode_kernel(
     const uint8_t* __restrict__ payload, uint64_t payload_bytes, const uint64_t* __restrict__ blk_off,
     const uint64_t* __restrict__ col_len, const uint8_t* __restrict__ col_map, uint64_t n_chunks,
@@ -187,7 +198,7 @@ __global__ __launch_bounds__(64 * G) void lzss_decode_kernel(
                 const uint32_t nf = ~f & 0xffu;
                 for (uint32_t m = nf; m; m &= m - 1) {
                     const uint32_t b = __builtin_ctz(m);
-                    ol += (d[p + 1 + b + __popc(nf & ((1u << b) - 1))] & 31u) + 3u;
+                    ol += tok_len<kRle>(d[p + 1 + b + __popc(nf & ((1u << b) - 1))]);
                 }
             } else {                               // stream ends inside this group
                 ol = 0;
@@ -195,7 +206,7 @@ __global__ __launch_bounds__(64 * G) void lzss_decode_kernel(
                 for (uint32_t b = 0; b < 8; ++b) {
                     const uint32_t lit = (f >> b) & 1u, need = 2u - lit;
                     if (q + need > L) break;
-                    ol += lit ? 1u : (d[q] & 31u) + 3u;
+                    ol += lit ? 1u : tok_len<kRle>(d[q]);
                     q += need;
                 }
             }
@@ -237,9 +248,13 @@ __global__ __launch_bounds__(64 * G) void lzss_decode_kernel(
                     }
                     const uint32_t v = (uint32_t)d[q] | ((uint32_t)d[q + 1] << 8);
                     q += 2;
-                    const uint32_t len = (v & 31u) + 3u, dist = v >> 5;
+                    const uint32_t len = tok_len<kRle>(v & 0xffu), dist = v >> 5;
                     const uint32_t e = min(o + len, lane_out);
-                    if (dist == 0 || dist > o) {
+                    if constexpr (kRle) {          // run: `len` copies of the value byte
+                        for (uint32_t i = o; i < e; ++i) mp[i] = (uint16_t)(0x8000u | (v >> 8));
+                        if (o + len > lane_out)
+                            err = min(err, (tok << 8) | ((uint32_t)MDZ_ST_OUTPUT_OVERRUN & 0xffu));
+                    } else if (dist == 0 || dist > o) {
                         err = min(err, (tok << 8) | ((uint32_t)MDZ_ST_BAD_DISTANCE & 0xffu));
                         for (uint32_t i = o; i < e; ++i) mp[i] = 0x8000u;
                     } else {
@@ -259,7 +274,8 @@ __global__ __launch_bounds__(64 * G) void lzss_decode_kernel(
         wave_sync();
 
         // resolve: follow sources down to a literal, store the value back
-        for (uint32_t i = 2 * ln; i < lane_out; i += 128) {
+        // (a run-length stream's map holds values only)
+        for (uint32_t i = 2 * ln; !kRle && i < lane_out; i += 128) {
             uint32_t* w = reinterpret_cast<uint32_t*>(mp + i);
             uint32_t e0 = *w & 0xffffu, e1 = *w >> 16;
             while (!(e0 & 0x8000u)) e0 = mp[e0];
@@ -291,24 +307,23 @@ __global__ __launch_bounds__(64 * G) void lzss_decode_kernel(
     }
 }
 
-template <int G>
+template <int G, bool kRle>
 int launch(const mdz_lanes_t* in, const LzssGeom& geo, void* dst, uint64_t dst_bytes,
            int32_t* status, hipStream_t s) {
     const size_t smem = sizeof(LzssHdr) + geo.r_bytes +
                         (size_t)G * (geo.cap_in + 2ull * geo.lane_out + 4ull * geo.gmax);
-    hipError_t e = set_dynamic_lds((const void*)lzss_decode_kernel<G>, smem);
+    hipError_t e = set_dynamic_lds((const void*)lzss_decode_kernel<G, kRle>, smem);
     if (e != hipSuccess) return record_hip(e, "lzss_decode_kernel LDS");
     const uint64_t blocks = (in->n_chunks + 7) / 8 * 8 * (32 / G);
-    hipLaunchKernelGGL(lzss_decode_kernel<G>, dim3((uint32_t)blocks), dim3(64 * G), smem, s,
+    hipLaunchKernelGGL((lzss_decode_kernel<G, kRle>), dim3((uint32_t)blocks), dim3(64 * G), smem, s,
                        (const uint8_t*)in->payload, in->payload_bytes, in->blk_off, in->col_len,
                        in->col_map, in->n_chunks, (uint8_t*)dst, dst_bytes, status, geo);
     return record_hip(hipGetLastError(), "lzss_decode_kernel launch");
 }
 
-}  // namespace
-
-int lzss_decompress(const mdz_lanes_t* in, void* dst, uint64_t dst_bytes, int32_t* status,
-                    hipStream_t s) {
+template <bool kRle>
+int lanes_flag_decompress(const mdz_lanes_t* in, void* dst, uint64_t dst_bytes, int32_t* status,
+                          hipStream_t s) {
     if (!in || !in->payload || !in->blk_off || !in->col_len || !in->col_map || (!dst && dst_bytes))
         return MDZ_ERR_INVALID;
     const uint64_t chunk = in->chunk_bytes;
@@ -330,13 +345,24 @@ int lzss_decompress(const mdz_lanes_t* in, void* dst, uint64_t dst_bytes, int32_
         int rc = record_hip(hipGetLastError(), "lzss status init");
         if (rc) return rc;
     }
-    int rc = geo.lane_out <= 1024 ? launch<8>(in, geo, dst, dst_bytes, status, s)
-           : geo.lane_out <= 2048 ? launch<4>(in, geo, dst, dst_bytes, status, s)
-                                  : launch<2>(in, geo, dst, dst_bytes, status, s);
+    int rc = geo.lane_out <= 1024 ? launch<8, kRle>(in, geo, dst, dst_bytes, status, s)
+           : geo.lane_out <= 2048 ? launch<4, kRle>(in, geo, dst, dst_bytes, status, s)
+                                  : launch<2, kRle>(in, geo, dst, dst_bytes, status, s);
     if (rc || !status) return rc;
     hipLaunchKernelGGL(status_finish_kernel, dim3((uint32_t)((n + 255) / 256 < 4096 ? (n + 255) / 256 : 4096)),
                        dim3(256), 0, s, status, n);
     return record_hip(hipGetLastError(), "lzss status finish");
+}
+}  // namespace
+
+int lzss_decompress(const mdz_lanes_t* in, void* dst, uint64_t dst_bytes, int32_t* status,
+                    hipStream_t s) {
+    return lanes_flag_decompress<false>(in, dst, dst_bytes, status, s);
+}
+
+int brle_trans_decompress(const mdz_lanes_t* in, void* dst, uint64_t dst_bytes, int32_t* status,
+                          hipStream_t s) {
+    return lanes_flag_decompress<true>(in, dst, dst_bytes, status, s);
 }
 
 }  // namespace mdz

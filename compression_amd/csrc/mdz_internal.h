@@ -70,6 +70,8 @@ int batch_submit(const mdz_batch_t* b, hipStream_t s);
 
 int lzss_decompress(const mdz_lanes_t* in, void* dst, uint64_t dst_bytes, int32_t* status,
                     hipStream_t s);
+int brle_trans_decompress(const mdz_lanes_t* in, void* dst, uint64_t dst_bytes, int32_t* status,
+                          hipStream_t s);
 int lzss_plain_decompress(const void* data, uint64_t data_bytes, const uint64_t* lens, uint64_t n_chunks,
                           uint64_t chunk, void* dst, uint64_t dst_bytes, int32_t* status, hipStream_t s);
 

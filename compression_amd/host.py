@@ -51,6 +51,11 @@ def lib():
         L.mdz_lzss_encode.restype = C.c_int
         L.mdz_lzss_encode.argtypes = [C.c_void_p, C.c_uint64, C.c_uint64, C.POINTER(u8p), u64p,
                                       C.POINTER(u64p), C.POINTER(u64p), C.POINTER(u8p), u64p, C.c_int]
+        L.mdz_brle_trans_lane_encode.restype = C.c_uint64
+        L.mdz_brle_trans_lane_encode.argtypes = [C.c_void_p, C.c_uint64, C.c_void_p, C.c_uint64]
+        L.mdz_brle_trans_encode.restype = C.c_int
+        L.mdz_brle_trans_encode.argtypes = [C.c_void_p, C.c_uint64, C.c_uint64, C.POINTER(u8p), u64p,
+                                            C.POINTER(u64p), C.POINTER(u64p), C.POINTER(u8p), u64p, C.c_int]
         L.mdz_lzss_plain_encode.restype = C.c_int
         L.mdz_lzss_plain_encode.argtypes = [C.c_void_p, C.c_uint64, C.c_uint64, C.POINTER(u8p),
                                             C.POINTER(u64p), u64p, C.c_int]
@@ -217,6 +222,28 @@ def lzss_encode(data: np.ndarray, chunk: int, threads: int | None = None) -> Lzs
                                    C.byref(bo), C.byref(cm), C.byref(nc), threads or default_threads())
     if rc != 0:
         raise ValueError(f"lzss encode rejected parameters (status {rc})")
+    n = nc.value
+    return LzssContainer(_take(pay, pb.value, np.uint8), _take(cl, 32 * n, np.uint64),
+                         _take(bo, n + 1, np.uint64), _take(cm, 32 * n, np.uint8), n, chunk)
+
+def brle_trans_lane_encode(lane: bytes) -> bytes:
+    """One lane of brle_trans::compress_func (include/brle/brle_trans.h:401-562)."""
+    a = np.frombuffer(bytes(lane), np.uint8) if len(lane) else np.zeros(1, np.uint8)
+    cap = len(lane) + len(lane) // 8 + 16
+    out = np.zeros(cap, np.uint8)
+    n = lib().mdz_brle_trans_lane_encode(_ptr(a), len(lane), _ptr(out), cap)
+    assert n <= cap
+    return out[:n].tobytes()
+
+def brle_trans_encode(data: np.ndarray, chunk: int, threads: int | None = None) -> LzssContainer:
+    """Lane-transposed byte RLE (brle_trans.h:565-669) in the LZSS lane container."""
+    raw = np.ascontiguousarray(data).view(np.uint8).reshape(-1)
+    src = raw if raw.size else np.zeros(1, np.uint8)
+    pay, pb, cl, bo, cm, nc = u8p(), C.c_uint64(), u64p(), u64p(), u8p(), C.c_uint64()
+    rc = lib().mdz_brle_trans_encode(_ptr(src), raw.size, chunk, C.byref(pay), C.byref(pb), C.byref(cl),
+                                     C.byref(bo), C.byref(cm), C.byref(nc), threads or default_threads())
+    if rc != 0:
+        raise ValueError(f"brle_trans encode rejected parameters (status {rc})")
     n = nc.value
     return LzssContainer(_take(pay, pb.value, np.uint8), _take(cl, 32 * n, np.uint64),
                          _take(bo, n + 1, np.uint64), _take(cm, 32 * n, np.uint8), n, chunk)

@@ -103,9 +103,12 @@ uint64_t mdz_lzss_lane_encode(const uint8_t* s, uint64_t n, uint8_t* out, uint64
 
 /* ------------------------------------------------------------- container -- */
 
+typedef uint64_t (*lane_encoder)(const uint8_t* lane, uint64_t n, uint8_t* out, uint64_t cap);
+
 typedef struct {
     const uint8_t* in;
     uint64_t chunk;
+    lane_encoder enc;
     uint8_t** streams;       /* [32n] lane streams */
     uint64_t* raw_len;       /* [32n] unsorted lane lengths */
 } enc_ctx;
@@ -118,8 +121,8 @@ static void enc_task(void* vctx, uint64_t c) {
     for (int t = 0; t < 32; ++t) {
         const uint8_t* src = x->in + c * x->chunk;
         for (uint64_t i = 0; i < lb; ++i) lane[i] = src[128 * (i / 4) + 4 * (uint64_t)t + (i & 3)];
-        uint8_t* s = (uint8_t*)malloc(cap);
-        x->raw_len[32 * c + (uint64_t)t] = mdz_lzss_lane_encode(lane, lb, s, cap);
+        uint8_t* s = (uint8_t*)calloc(cap, 1);
+        x->raw_len[32 * c + (uint64_t)t] = x->enc(lane, lb, s, cap);
         x->streams[32 * c + (uint64_t)t] = s;
     }
     free(lane);
@@ -137,12 +140,16 @@ static void sort_lanes(const uint64_t* len, uint8_t* map) {
     }
 }
 
-int mdz_lzss_encode(const uint8_t* in, uint64_t in_bytes, uint64_t chunk, uint8_t** payload,
-                        uint64_t* payload_bytes, uint64_t** col_len, uint64_t** blk_off,
+/* The lane container both lane-transposed flag-group codecs share
+ * (lzss_gpu2.h:332-509,1063-1085; brle_trans.h:204-224,238-398): each lane encoded
+ * alone, lanes sorted, rows of 4-byte words interleaved in sorted order,
+ * chunk extents rounded up to 128 bytes. */
+static int lanes_encode(lane_encoder enc, const uint8_t* in, uint64_t in_bytes, uint64_t chunk,
+                        uint8_t** payload, uint64_t* payload_bytes, uint64_t** col_len, uint64_t** blk_off,
                         uint8_t** col_map, uint64_t* n_chunks, int threads) {
     if (chunk == 0 || chunk % 128) return -1;
     const uint64_t n = in_bytes / chunk;               /* trailing partial chunk dropped */
-    enc_ctx x = {in, chunk, (uint8_t**)calloc(32 * n + 1, sizeof(uint8_t*)),
+    enc_ctx x = {in, chunk, enc, (uint8_t**)calloc(32 * n + 1, sizeof(uint8_t*)),
                  (uint64_t*)calloc(32 * n + 1, sizeof(uint64_t))};
     mdz_parallel_for(n, threads, enc_task, &x);
     uint64_t* cl = (uint64_t*)malloc((32 * n + 1) * sizeof(uint64_t));
@@ -184,6 +191,83 @@ int mdz_lzss_encode(const uint8_t* in, uint64_t in_bytes, uint64_t chunk, uint8_
     *col_map = cm;
     *n_chunks = n;
     return 0;
+}
+
+int mdz_lzss_encode(const uint8_t* in, uint64_t in_bytes, uint64_t chunk, uint8_t** payload,
+                    uint64_t* payload_bytes, uint64_t** col_len, uint64_t** blk_off,
+                    uint8_t** col_map, uint64_t* n_chunks, int threads) {
+    return lanes_encode(mdz_lzss_lane_encode, in, in_bytes, chunk, payload, payload_bytes, col_len, blk_off,
+                        col_map, n_chunks, threads);
+}
+
+/* ------------------------------------------- byte RLE, lane-transposed -- */
+
+/* brle_trans::compress_func (include/brle/brle_trans.h:401-562) for one lane,
+ * with compress_init_func's sizes (:238-379).  A run of the same byte
+ * counted in a uint8_t: at 3..255 it becomes a run token (count byte, value
+ * byte; flag bit 0), shorter ones become literals (flag bit 1).  The counter
+ * wraps at 256, so 256 bytes of a longer run vanish per wrap (the reference
+ * encoder's own loss, kept for byte identity).  A token that completes a
+ * group of 8 reserves the next flag byte, except the lane's final run,
+ * which never does.  A flag byte that no token rewrites stays 0 here (the
+ * reference leaves that device byte as it was). */
+uint64_t mdz_brle_trans_lane_encode(const uint8_t* s, uint64_t n, uint8_t* out, uint64_t cap) {
+    uint64_t o = 1, hpos = 0;
+    uint8_t hdr = 0, counter = 0, prev = 0;
+    uint32_t blocks = 0;
+    if (cap < n + n / 8 + 2) return 0;
+    out[0] = 0;
+#define MDZ_BRLE_T_NEXT_TOKEN()                                    \
+    do {                                                           \
+        if (++blocks == 8) {                                       \
+            out[hpos] = hdr;                                       \
+            hdr = 0;                                               \
+            hpos = o++;                                            \
+            out[hpos] = 0;                                         \
+            blocks = 0;                                            \
+        }                                                          \
+        if (blocks != 0) out[hpos] = hdr;                          \
+    } while (0)
+    for (uint64_t i = 0; i < n; ++i) {
+        const uint8_t v = s[i];
+        if (counter == 0) prev = v;
+        if (prev == v) {
+            ++counter;                                /* uint8_t: wraps at 256 */
+            continue;
+        }
+        if (counter >= 3) {
+            out[o++] = counter;
+            out[o++] = prev;
+            MDZ_BRLE_T_NEXT_TOKEN();
+        } else {
+            for (uint32_t j = 0; j < counter; ++j) {
+                hdr = (uint8_t)(hdr | (1u << blocks));
+                out[o++] = prev;
+                MDZ_BRLE_T_NEXT_TOKEN();
+            }
+        }
+        counter = 1;
+        prev = v;
+    }
+    if (counter >= 3) {                               /* final run: no flag bookkeeping */
+        out[o++] = counter;
+        out[o++] = prev;
+    } else {
+        for (uint32_t j = 0; j < counter; ++j) {
+            hdr = (uint8_t)(hdr | (1u << blocks));
+            out[o++] = prev;
+            MDZ_BRLE_T_NEXT_TOKEN();
+        }
+    }
+#undef MDZ_BRLE_T_NEXT_TOKEN
+    return o;
+}
+
+int mdz_brle_trans_encode(const uint8_t* in, uint64_t in_bytes, uint64_t chunk, uint8_t** payload,
+                          uint64_t* payload_bytes, uint64_t** col_len, uint64_t** blk_off,
+                          uint8_t** col_map, uint64_t* n_chunks, int threads) {
+    return lanes_encode(mdz_brle_trans_lane_encode, in, in_bytes, chunk, payload, payload_bytes, col_len,
+                        blk_off, col_map, n_chunks, threads);
 }
 
 

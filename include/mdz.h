@@ -265,6 +265,20 @@ int mdz_lzss_decompress_host(const uint8_t* in, uint64_t in_n_bytes,
                              const uint8_t* const* in_metadata, const uint64_t* in_metadata_lens,
                              uint64_t in_metadata_num, uint8_t** out, uint64_t* out_n_bytes);
 
+/* Lane-transposed byte RLE decode -- replaces brle_trans::decompress_func
+ * (reference include/brle/brle_trans.h:72-202) and the device part of
+ * brle_trans::decompress_gpu (:672-790).  The container is the LZSS one
+ * (in->col_len sorted, in->col_map, blk_off; lzss_gpu2.h:1063-1085 and
+ * brle_trans.h:204-224), with the same flag groups: bit b (LSB first) of a
+ * flag byte 1 = literal byte, 0 = run (count byte, value byte).  Lane t's
+ * byte i decodes to dst[c*chunk + 128*(i/4) + 4t + i%4], clipped to
+ * dst_bytes; chunk_bytes a multiple of 128, at most 128 KiB.  A lane stops
+ * when its chunk_bytes/32 bytes are complete; status[c] (optional) as for
+ * mdz_lzss_decompress (a run past the lane end is OUTPUT_OVERRUN; the
+ * reference reads col_len bytes and writes wherever they lead). */
+int mdz_brle_trans_decompress(const mdz_lanes_t* in, void* dst, uint64_t dst_bytes, int32_t* status,
+                              void* stream);
+
 /* Plain LZSS decode -- replaces lzss::kernel_decompress(in, out, in_n_bytes,
  * out_n_bytes, out_chunk_size, n_chunks, lens) and the plain decompress_func
  * (reference include/lzss/lzss_gpu2.h:1908-1913,718-792).  Chunk c's stream

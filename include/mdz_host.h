@@ -74,6 +74,16 @@ int mdz_lzss_encode(const uint8_t* in, uint64_t in_bytes, uint64_t chunk, uint8_
 int mdz_lzss_plain_encode(const uint8_t* in, uint64_t in_bytes, uint64_t chunk, uint8_t** data,
                           uint64_t** lens, uint64_t* n_chunks, int threads);
 
+/* ------------------------------------------ byte RLE, lane-transposed --
+ * include/brle/brle_trans.h (compress_init_func :238-379, compress_func
+ * :401-562, lane sort :204-224, compress_gpu :565-669): the LZSS lane
+ * container (the same sort, interleave and 128-byte chunk extents) holding
+ * flag-group streams whose 2-byte tokens are (count, value) runs. */
+uint64_t mdz_brle_trans_lane_encode(const uint8_t* lane, uint64_t n, uint8_t* out, uint64_t cap);
+int mdz_brle_trans_encode(const uint8_t* in, uint64_t in_bytes, uint64_t chunk, uint8_t** payload,
+                          uint64_t* payload_bytes, uint64_t** col_len, uint64_t** blk_off,
+                          uint8_t** col_map, uint64_t* n_chunks, int threads);
+
 /* ----------------------------------------------------- synthetic data -- */
 /* Config 1: u32 column, runs 70% / literal segments 30% by count. */
 void mdz_gen_rlev1_u32(uint32_t* out, uint64_t n, uint64_t seed);

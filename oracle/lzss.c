@@ -128,9 +128,12 @@ int mdz_ref_lzss_lane_decode(const uint8_t* in, uint64_t in_len, uint8_t* out, u
     return MDZ_ST_OK;
 }
 
+typedef int (*lane_decoder)(const uint8_t* in, uint64_t in_len, uint8_t* out, uint64_t out_len);
+
 typedef struct {
     const uint8_t* pay; const uint64_t* off; const uint64_t* cl; const uint8_t* cm;
     uint64_t chunk; uint64_t pay_bytes; uint8_t* out; uint64_t out_bytes; int32_t* status;
+    lane_decoder dec;
 } dec_ctx;
 
 /* Container checks shared with the HIP decoder (status BAD_HEADER): col_map
@@ -178,7 +181,7 @@ static void dec_task(void* vctx, uint64_t c) {
         int s = 0;
         while (cm[s] != t) ++s;
         memset(vals, 0, lb);
-        const int r = mdz_ref_lzss_lane_decode(lanes + base[s], L[s], vals, lb);
+        const int r = x->dec(lanes + base[s], L[s], vals, lb);
         if (r != MDZ_ST_OK && st == MDZ_ST_OK) st = r;
         for (uint64_t i = 0; i < lb; i += 4) {
             const uint64_t d = c * x->chunk + 128 * (i / 4) + 4 * (uint64_t)t;
@@ -195,13 +198,56 @@ int mdz_ref_lzss_decode(const uint8_t* payload, uint64_t payload_bytes, const ui
                         uint64_t chunk, uint8_t* out, uint64_t out_bytes, int32_t* status,
                         int threads) {
     if (chunk == 0 || chunk % 128) return MDZ_ST_BAD_HEADER;
-    dec_ctx x = {payload, blk_off, col_len, col_map, chunk, payload_bytes, out, out_bytes, status};
+    dec_ctx x = {payload, blk_off, col_len, col_map, chunk, payload_bytes, out, out_bytes, status,
+                 mdz_ref_lzss_lane_decode};
     mdz_parallel_for(n_chunks, threads, dec_task, &x);
     for (uint64_t c = 0; c < n_chunks; ++c)
         if (status[c] != MDZ_ST_OK) return status[c];
     return 0;
 }
 
+
+/* brle_trans::decompress_func (include/brle/brle_trans.h:72-202) for one
+ * lane: flag byte, then 8 tokens, bit b (LSB first) 1 = literal byte, 0 =
+ * run (count byte, value byte: `count` copies of value).  Output-driven with
+ * mdz_ref_lzss_lane_decode's bounds and statuses; the reference reads its
+ * col_len bytes and writes whatever they decode to. */
+int mdz_ref_brle_trans_lane_decode(const uint8_t* in, uint64_t in_len, uint8_t* out, uint64_t out_len) {
+    uint64_t p = 0, o = 0;
+    while (o < out_len) {
+        if (p >= in_len) return MDZ_ST_INPUT_OVERRUN;
+        const uint32_t flags = in[p++];
+        for (uint32_t b = 0; b < 8 && o < out_len; ++b) {
+            if (flags >> b & 1u) {
+                if (p >= in_len) return MDZ_ST_INPUT_OVERRUN;
+                out[o++] = in[p++];
+            } else {
+                if (p + 2 > in_len) return MDZ_ST_INPUT_OVERRUN;
+                const uint32_t cnt = in[p], v = in[p + 1];
+                p += 2;
+                for (uint32_t i = 0; i < cnt; ++i) {
+                    if (o >= out_len) return MDZ_ST_OUTPUT_OVERRUN;
+                    out[o++] = (uint8_t)v;
+                }
+            }
+        }
+    }
+    return MDZ_ST_OK;
+}
+
+/* The same lane container as mdz_ref_lzss_decode, with the checks above. */
+int mdz_ref_brle_trans_decode(const uint8_t* payload, uint64_t payload_bytes, const uint64_t* blk_off,
+                              const uint64_t* col_len, const uint8_t* col_map, uint64_t n_chunks,
+                              uint64_t chunk, uint8_t* out, uint64_t out_bytes, int32_t* status,
+                              int threads) {
+    if (chunk == 0 || chunk % 128) return MDZ_ST_BAD_HEADER;
+    dec_ctx x = {payload, blk_off, col_len, col_map, chunk, payload_bytes, out, out_bytes, status,
+                 mdz_ref_brle_trans_lane_decode};
+    mdz_parallel_for(n_chunks, threads, dec_task, &x);
+    for (uint64_t c = 0; c < n_chunks; ++c)
+        if (status[c] != MDZ_ST_OK) return status[c];
+    return 0;
+}
 
 /* Plain (per-chunk) decompress_func (lzss_gpu2.h:718-792): chunk c's stream
  * is data[lens[c-1], lens[c]) and decodes to out[c*chunk, ...) (the last

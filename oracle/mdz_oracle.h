@@ -82,6 +82,13 @@ int mdz_ref_lzss_decode(const uint8_t* payload, uint64_t payload_bytes, const ui
                         const uint64_t* col_len, const uint8_t* col_map, uint64_t n_chunks,
                         uint64_t chunk, uint8_t* out, uint64_t out_bytes, int32_t* status,
                         int threads);
+/* Lane-transposed byte RLE: brle_trans::decompress_func
+ * (include/brle/brle_trans.h:72-202) per lane, over the LZSS lane container. */
+int mdz_ref_brle_trans_lane_decode(const uint8_t* in, uint64_t in_len, uint8_t* out, uint64_t out_len);
+int mdz_ref_brle_trans_decode(const uint8_t* payload, uint64_t payload_bytes, const uint64_t* blk_off,
+                              const uint64_t* col_len, const uint8_t* col_map, uint64_t n_chunks,
+                              uint64_t chunk, uint8_t* out, uint64_t out_bytes, int32_t* status,
+                              int threads);
 /* Plain (per-chunk) decompress_func (lzss_gpu2.h:718-792) over packed
  * streams with inclusive-prefix lens[]; per-chunk status as the lane
  * decoder's. */

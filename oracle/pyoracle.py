@@ -15,7 +15,7 @@ import subprocess
 import numpy as np
 
 from compression_amd.host import (BrleContainer, DeflateBatch, LzssContainer, LzssPlainContainer,  # noqa: F401
-                                  RleV1Container, lzss_plain_encode,
+                                  RleV1Container, brle_trans_encode, brle_trans_lane_encode, lzss_plain_encode,
                                   RleV2Container, _ptr, _take, brle_chunk_encode, brle_encode,
                                   default_threads, gen_brle_u8, gen_deflate_batch, gen_lzss_markov,
                                   gen_markov, gen_rlev1_u32, gen_rlev2_i64, host_cores, lzss_encode,
@@ -72,6 +72,12 @@ def lib():
         L.mdz_ref_lzss_plain_decode.restype = C.c_int
         L.mdz_ref_lzss_plain_decode.argtypes = [C.c_void_p, C.c_void_p, C.c_uint64, C.c_uint64, C.c_void_p,
                                                 C.c_uint64, C.c_void_p, C.c_int]
+        L.mdz_ref_brle_trans_lane_decode.restype = C.c_int
+        L.mdz_ref_brle_trans_lane_decode.argtypes = [C.c_void_p, C.c_uint64, C.c_void_p, C.c_uint64]
+        L.mdz_ref_brle_trans_decode.restype = C.c_int
+        L.mdz_ref_brle_trans_decode.argtypes = [C.c_void_p, C.c_uint64, C.c_void_p, C.c_void_p, C.c_void_p,
+                                                C.c_uint64, C.c_uint64, C.c_void_p, C.c_uint64, C.c_void_p,
+                                                C.c_int]
         L.mdz_ref_lzss_decode.restype = C.c_int
         L.mdz_ref_lzss_decode.argtypes = [C.c_void_p, C.c_uint64, C.c_void_p, C.c_void_p, C.c_void_p,
                                           C.c_uint64, C.c_uint64, C.c_void_p, C.c_uint64, C.c_void_p,
@@ -165,6 +171,25 @@ def lzss_decode(c: LzssContainer, out_bytes: int | None = None, threads: int | N
     rc = lib().mdz_ref_lzss_decode(_ptr(pay), c.payload.size, _ptr(c.blk_off), _ptr(cl), _ptr(cm),
                                    c.n_chunks, c.chunk, _ptr(out), out_bytes, _ptr(status),
                                    threads or default_threads())
+    return rc, out[:out_bytes], status[:c.n_chunks]
+
+def brle_trans_lane_decode(stream: bytes, out_len: int) -> tuple[int, bytes]:
+    a = np.frombuffer(bytes(stream) + b"\0", np.uint8)
+    out = np.zeros(max(out_len, 1), np.uint8)
+    st = lib().mdz_ref_brle_trans_lane_decode(_ptr(a), len(stream), _ptr(out), out_len)
+    return st, out[:out_len].tobytes()
+
+def brle_trans_decode(c: LzssContainer, out_bytes: int | None = None, threads: int | None = None):
+    """Lane-transposed byte RLE container -> (rc, bytes, status[n])."""
+    out_bytes = c.raw_bytes if out_bytes is None else out_bytes
+    out = np.zeros(max(out_bytes, 1), np.uint8)
+    status = np.zeros(max(c.n_chunks, 1), np.int32)
+    pay = c.payload if c.payload.size else np.zeros(1, np.uint8)
+    cl = c.col_len if c.col_len.size else np.zeros(1, np.uint64)
+    cm = c.col_map if c.col_map.size else np.zeros(1, np.uint8)
+    rc = lib().mdz_ref_brle_trans_decode(_ptr(pay), c.payload.size, _ptr(c.blk_off), _ptr(cl), _ptr(cm),
+                                         c.n_chunks, c.chunk, _ptr(out), out_bytes, _ptr(status),
+                                         threads or default_threads())
     return rc, out[:out_bytes], status[:c.n_chunks]
 
 def lzss_plain_decode(c, out_bytes: int | None = None, threads: int | None = None):
