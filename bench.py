@@ -40,11 +40,14 @@ HBM_PEAK_GBS = 8000.0
 # FETCH_SIZE and WRITE_SIZE scaled by the copy-kernel calibration; the
 # summaries these come from are under profiles/.
 TRAFFIC: dict = {
-    "batch_kernel<8> (rle_v2+byte_rle)": 1792273658,   # profiles/config2_rlev2_brle_batched.md
-    "rlev2_i64_1GiB_plain": 1430079078,
-    "brle_u8_256MiB_plain": 547122419,
-    "rlev1_u32_64MiB_plain": 105750425,                # profiles/rlev1_u32_64MiB_plain.md
-    "lzss_markov_1GiB_64KiB": 1861163514,              # profiles/lzss_markov_1GiB_64KiB.md
+    "batch_kernel<8> (rle_v2+byte_rle)": 1790353203,   # profiles/config2_rlev2_brle_batched.md
+    "rlev2_i64_1GiB_plain": 1422589952,                # profiles/rlev2_i64_1GiB_plain.md
+    "brle_u8_256MiB_plain": 536279348,                 # profiles/brle_u8_256MiB_plain.md
+    "rlev1_u32_64MiB_plain": 105728819,                # profiles/rlev1_u32_64MiB_plain.md
+    "lzss_markov_1GiB_64KiB": 1861173555,              # profiles/lzss_markov_1GiB_64KiB.md
+    "lzss_plain_markov_1GiB_64KiB": 1199976756,        # profiles/lzss_plain_markov_1GiB_64KiB.md
+    "brle_trans_u8_256MiB": 436797235,                 # profiles/brle_trans_u8_256MiB.md
+    "deflate_4096_streams_uneven": 2818533785,         # profiles/inflate_config4_4096_streams.md
 }
 
 
