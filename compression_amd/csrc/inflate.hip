@@ -1,29 +1,32 @@
-// Deflate / zlib inflate for gfx950: one wavefront per stream.
+// Deflate / zlib inflate for gfx950: two wavefronts per stream.
 //
 // Replaces the reference's deflate::decompress_gpu inflate kernels (reference
 // include/zlib/zlib.h:832-962; reader, decoder and writer warps of one lane
 // each doing useful work) and cuDF's gpuinflate (include/cudf_deflate.cuh:
 // 1229-1345,1399-1407) behind the same batched descriptor/status interface.
 //
-// Per stream, one 64-lane wave:
-//   * bit reader: a 256-byte input window lives in one VGPR (lane l holds
-//     bytes 4l..4l+3); the 64-bit bit buffer is refilled from it with
-//     readlane, and the next window is prefetched when one is entered;
-//   * Huffman decode: 10-bit (literal/length) and 8-bit (distance) lookup
-//     tables hold, per bit pattern, the code length, the symbol's base value
-//     and extra-bit count; longer codes take a canonical decode
-//     (count/symbol tables), which also rejects codes an incomplete set does
-//     not define.  Tables are built in LDS by the whole wave from the code
-//     lengths (ballot-ranked canonical codes, strided table fill), then
-//     copied to 20 registers per lane so that each lookup is an indexed
-//     register move and a readlane;
-//   * LZ77: symbols are decoded in batches of up to 64 (one record per lane,
-//     at most ~4 KiB of output per batch); the wave then writes the batch's
-//     literals and copies its matches, in order, within a 36.5 KiB LDS ring
-//     that keeps the last 32 KiB of output; byte j of a match reads ring
-//     position start - dist + (j mod dist), so overlapping matches need no
-//     serial step;
-//   * finished output is flushed to HBM in dword stores after each batch.
+// Per stream, one 128-thread workgroup: a producer wave and a consumer wave
+// meeting at one workgroup barrier per step.
+//   * producer: parses block headers, builds each block's tables in LDS
+//     (literal/length: 10-bit root plus subtables; distance: 8-bit root plus
+//     subtables; codes longer than a subtable holds take a canonical
+//     count/symbol decode, which also rejects codes an incomplete set does
+//     not define), and decodes symbols in rounds.  A round stages 64 spans
+//     of 512 input bits in LDS; lane t decodes its span as if a symbol
+//     started at its first bit and marks each start it meets.  Huffman codes
+//     resynchronise, so the true symbol sequence, followed past the end of
+//     lane t's span, soon meets a start a later lane marked (at most 3 spans
+//     ahead); the lanes it passes through form a chain, each lists its piece
+//     of the sequence, and the pieces are placed by popcounts into one of two
+//     LDS symbol lists.  An end-of-block, an error or a lost hand-off ends the
+//     round there and the next round starts from it;
+//   * consumer: turns the previous round's list into output in a 34.5 KiB
+//     LDS ring holding the last 32 KiB: batches of up to 64 symbols (about
+//     2 KiB of output), literals stored by their lanes, then matches copied
+//     in order, one byte per lane (byte j of a match reads start - dist +
+//     (j mod dist), so overlapping matches need no serial step); stored
+//     blocks are copied 32 bytes per lane; finished output is stored to HBM
+//     with dword stores at dword-aligned addresses.
 // Errors follow zlib's checks and order (invalid block type, stored length,
 // code-set completeness, repeat codes, missing end-of-block, invalid codes,
 // distance too far back) with cuDF's status codes.  parse_hdr selects the

@@ -4,7 +4,8 @@ orders it produces.
 The decoder logs a cost sample per stream (mdz_inflate_cost_t: cycles and
 what the stream held).  ``CostModel`` is trained on those samples with the
 library's HIP forward/backward/Adam kernels and predicts log2(cycles) from
-features known before decoding (stream sizes and first block type).
+features read before decoding: stream sizes, and from the first block header
+its type and the symbol, match and block counts its code lengths imply.
 ``lpt_order`` sorts streams longest-predicted-first for the persistent
 work-queue launch; the fallback orders by compressed size.
 """
