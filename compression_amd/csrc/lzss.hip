@@ -76,6 +76,19 @@ MDZ_DEV void report(int32_t* status, uint64_t c, uint32_t lane, int code) {
     if (status) atomicMin(&status[c], (int32_t)((lane << 8) | ((uint32_t)code & 0xffu)));
 }
 
+// Stores v, v + inc, v + 2*inc, ... into m[i, e) as u16 entries, two per
+// 32-bit store (m is 4-byte aligned; entries stay below 0x10000).
+MDZ_DEV void fill_u16(uint16_t* m, uint32_t i, uint32_t e, uint32_t v, uint32_t inc) {
+    if ((i & 1u) && i < e) {
+        m[i++] = (uint16_t)v;
+        v += inc;
+    }
+    uint32_t pk = (v & 0xffffu) | ((v + inc) << 16);
+    const uint32_t step = 2 * inc * 0x10001u;
+    for (; i + 2 <= e; i += 2, pk += step) *reinterpret_cast<uint32_t*>(m + i) = pk;
+    if (i < e) m[i] = (uint16_t)pk;
+}
+
 // Output bytes of a 2-byte token from its first byte: an LZSS match,
 // (b & 31) + 3; a byte-RLE run (brle_trans), the count byte itself.
 template <bool kRle>
@@ -251,14 +264,14 @@ __global__ __launch_bounds__(64 * G) void lzss_decode_kernel(
                     const uint32_t len = tok_len<kRle>(v & 0xffu), dist = v >> 5;
                     const uint32_t e = min(o + len, lane_out);
                     if constexpr (kRle) {          // run: `len` copies of the value byte
-                        for (uint32_t i = o; i < e; ++i) mp[i] = (uint16_t)(0x8000u | (v >> 8));
+                        fill_u16(mp, o, e, 0x8000u | (v >> 8), 0);
                         if (o + len > lane_out)
                             err = min(err, (tok << 8) | ((uint32_t)MDZ_ST_OUTPUT_OVERRUN & 0xffu));
                     } else if (dist == 0 || dist > o) {
                         err = min(err, (tok << 8) | ((uint32_t)MDZ_ST_BAD_DISTANCE & 0xffu));
-                        for (uint32_t i = o; i < e; ++i) mp[i] = 0x8000u;
+                        fill_u16(mp, o, e, 0x8000u, 0);
                     } else {
-                        for (uint32_t i = o; i < e; ++i) mp[i] = (uint16_t)(i - dist);
+                        fill_u16(mp, o, e, o - dist, 1);   // entry i holds i - dist
                         if (o + len > lane_out)
                             err = min(err, (tok << 8) | ((uint32_t)MDZ_ST_OUTPUT_OVERRUN & 0xffu));
                     }
@@ -275,12 +288,19 @@ __global__ __launch_bounds__(64 * G) void lzss_decode_kernel(
 
         // resolve: follow sources down to a literal, store the value back
         // (a run-length stream's map holds values only)
+        // (both chains of a pair in flight; a resolved chain re-reads mp[0]
+        // instead of branching)
         for (uint32_t i = 2 * ln; !kRle && i < lane_out; i += 128) {
             uint32_t* w = reinterpret_cast<uint32_t*>(mp + i);
             uint32_t e0 = *w & 0xffffu, e1 = *w >> 16;
-            while (!(e0 & 0x8000u)) e0 = mp[e0];
-            while (!(e1 & 0x8000u)) e1 = mp[e1];
-            *w = e0 | (e1 << 16);
+            if ((e0 & e1 & 0x8000u) == 0) {
+                do {
+                    const uint32_t v0 = mp[(e0 & 0x8000u) ? 0u : e0], v1 = mp[(e1 & 0x8000u) ? 0u : e1];
+                    e0 = (e0 & 0x8000u) ? e0 : v0;
+                    e1 = (e1 & 0x8000u) ? e1 : v1;
+                } while ((e0 & e1 & 0x8000u) == 0);
+                *w = e0 | (e1 << 16);
+            }
         }
         const uint32_t lane_err = wave_min_u32(err);
         if (ln == 0 && lane_err != kNoErr) report(status, c, t, (int)(int8_t)(lane_err & 0xffu));
