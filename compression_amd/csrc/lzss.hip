@@ -171,8 +171,25 @@ __global__ __launch_bounds__(64 * G) void lzss_decode_kernel(
         const uint32_t sj = h.inv[part * G + j], Wj = h.W[sj];
         const uint32_t* src = reinterpret_cast<const uint32_t*>(payload + beg);
         uint32_t* d32 = reinterpret_cast<uint32_t*>(inb + j * geo.cap_in);
-        for (uint32_t r = tid / G; r < mw; r += NT / G)
-            if (r < Wj) d32[r] = src[R[r] + sj];
+        // eight rows per thread in flight: row indices, then the loads, then
+        // the LDS stores (one store per iteration would wait out a global
+        // load before the next index read)
+        constexpr uint32_t kStep = NT / G;
+        for (uint32_t r0 = tid / G; r0 < mw; r0 += 8 * kStep) {
+            uint32_t at[8], w[8];
+#pragma unroll
+            for (uint32_t k = 0; k < 8; ++k) {
+                const uint32_t r = r0 + k * kStep;
+                at[k] = r < Wj ? R[r] + sj : 0xffffffffu;
+            }
+#pragma unroll
+            for (uint32_t k = 0; k < 8; ++k) w[k] = at[k] != 0xffffffffu ? src[at[k]] : 0u;
+#pragma unroll
+            for (uint32_t k = 0; k < 8; ++k) {
+                const uint32_t r = r0 + k * kStep;
+                if (r < Wj) d32[r] = w[k];
+            }
+        }
     }
     __syncthreads();
 
