@@ -342,13 +342,18 @@ MDZ_DEV uint64_t wave_incl_scan_u64(uint64_t v) {
     }
     return v;
 }
+// Inclusive prefix sum over the 64 lanes, in registers with DPP (call with
+// every lane active): row_shr 1, 2, 4, 8 sum within each row of 16 lanes
+// (sources outside the row read 0), then row_bcast:15 adds row 0's total to
+// row 1 and row 2's to row 3, and row_bcast:31 adds lane 31 (rows 0 and 1)
+// to rows 2 and 3.
 MDZ_DEV uint32_t wave_incl_scan_u32(uint32_t v) {
-    const uint32_t l = lane_id();
-#pragma unroll
-    for (int o = 1; o < kWave; o <<= 1) {
-        const uint32_t y = __shfl_up(v, o, kWave);
-        if (l >= (uint32_t)o) v += y;
-    }
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xf, 0xf, true);   // row_shr:1
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xf, 0xf, true);   // row_shr:2
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xf, 0xf, true);   // row_shr:4
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xf, 0xf, true);   // row_shr:8
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xa, 0xf, false);  // row_bcast:15
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xc, 0xf, false);  // row_bcast:31
     return v;
 }
 MDZ_DEV uint64_t shfl_u64(uint64_t v, int src) {
