@@ -169,7 +169,7 @@ __global__ __launch_bounds__(kWave) void brle_encode_kernel(const uint8_t* __res
                     for (uint32_t i = 0; i < n; ++i) o[1 + i] = src[at + i];
                 }
             }
-            written += __shfl(incl, kWave - 1, kWave);
+            written += (uint32_t)__builtin_amdgcn_readlane((int)incl, kWave - 1);
         }
         wave_sync();
     }

@@ -132,7 +132,7 @@ __global__ __launch_bounds__(64 * G) void lzss_decode_kernel(
         for (int o = 1; o < 32; o <<= 1) seen |= __shfl_xor(seen, o, 64);
         const uint32_t Ws = bad ? 0u : (uint32_t)((Ls + 3) / 4);
         const uint32_t incl = wave_incl_scan_u32(ln < 32 ? Ws : 0u);
-        const uint32_t total = __shfl(incl, 31, 64);
+        const uint32_t total = (uint32_t)__builtin_amdgcn_readlane((int)incl, 31);
         const uint64_t end = blk_off[c + 1];
         bad = bad || seen != 0xffffffffu || end < beg || end > payload_bytes || (beg & 3u) ||
               4ull * total > end - beg;
@@ -244,7 +244,7 @@ __global__ __launch_bounds__(64 * G) void lzss_decode_kernel(
             sum += ol;
         }
         const uint32_t incl = wave_incl_scan_u32(sum);
-        const uint32_t T = __shfl(incl, 63, 64);
+        const uint32_t T = (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
         uint32_t run = incl - sum;
         for (uint32_t g = g0; g < g1; ++g) {
             const uint32_t ol = oj[g];

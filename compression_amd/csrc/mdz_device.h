@@ -40,19 +40,34 @@ constexpr int kLanes = 32;   // sub-streams per chunk, fixed by the container fo
 
 MDZ_DEV uint32_t lane_id() { return __lane_id(); }
 
-MDZ_DEV uint32_t wave_min_u32(uint32_t v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-        uint32_t w = __shfl_xor(v, o, kWave);
-        v = w < v ? w : v;
-    }
+// v of the lane the DPP control names (row_shr:n = 0x110 + n within each row
+// of 16 lanes, row_bcast:15 = 0x142, row_bcast:31 = 0x143); kFill where the
+// control names no lane or the row is masked off.  The scans and reductions
+// below are built from these, in registers, with every lane active.
+template <int kCtrl, int kRowMask, uint32_t kFill>
+MDZ_DEV uint32_t dpp_src(uint32_t v) {
+    return (uint32_t)__builtin_amdgcn_update_dpp((int)kFill, (int)v, kCtrl, kRowMask, 0xf, false);
+}
+
+// Inclusive min-scan; lane 63 holds the wave minimum.
+MDZ_DEV uint32_t wave_incl_min_u32(uint32_t v) {
+    v = min(v, dpp_src<0x111, 0xf, 0xffffffffu>(v));
+    v = min(v, dpp_src<0x112, 0xf, 0xffffffffu>(v));
+    v = min(v, dpp_src<0x114, 0xf, 0xffffffffu>(v));
+    v = min(v, dpp_src<0x118, 0xf, 0xffffffffu>(v));
+    v = min(v, dpp_src<0x142, 0xa, 0xffffffffu>(v));
+    v = min(v, dpp_src<0x143, 0xc, 0xffffffffu>(v));
     return v;
 }
 
+MDZ_DEV uint32_t wave_incl_scan_u32(uint32_t v);
+
+MDZ_DEV uint32_t wave_min_u32(uint32_t v) {
+    return (uint32_t)__builtin_amdgcn_readlane((int)wave_incl_min_u32(v), kWave - 1);
+}
+
 MDZ_DEV uint32_t wave_sum_u32(uint32_t v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
-    return v;
+    return (uint32_t)__builtin_amdgcn_readlane((int)wave_incl_scan_u32(v), kWave - 1);
 }
 
 MDZ_DEV uint32_t alignbyte(uint32_t hi, uint32_t lo, uint32_t k) {
@@ -332,14 +347,18 @@ MDZ_DEV void stage_chunk_lanes(const LaneTable& T, const uint32_t* lbase, uint32
 }
 
 // Wave-wide inclusive prefix sums.
-MDZ_DEV uint64_t wave_incl_scan_u64(uint64_t v) {
-    const uint32_t l = lane_id();
-#pragma unroll
-    for (int o = 1; o < kWave; o <<= 1) {
-        const uint32_t ylo = __shfl_up((uint32_t)v, o, kWave);
-        const uint32_t yhi = __shfl_up((uint32_t)(v >> 32), o, kWave);
-        if (l >= (uint32_t)o) v += ((uint64_t)yhi << 32) | ylo;
-    }
+template <int kCtrl, int kRowMask>
+MDZ_DEV uint64_t dpp_src_u64(uint64_t v) {
+    return ((uint64_t)dpp_src<kCtrl, kRowMask, 0u>((uint32_t)(v >> 32)) << 32) |
+           dpp_src<kCtrl, kRowMask, 0u>((uint32_t)v);
+}
+MDZ_DEV uint64_t wave_incl_scan_u64(uint64_t v) {       // as wave_incl_scan_u32, in 64 bits
+    v += dpp_src_u64<0x111, 0xf>(v);
+    v += dpp_src_u64<0x112, 0xf>(v);
+    v += dpp_src_u64<0x114, 0xf>(v);
+    v += dpp_src_u64<0x118, 0xf>(v);
+    v += dpp_src_u64<0x142, 0xa>(v);
+    v += dpp_src_u64<0x143, 0xc>(v);
     return v;
 }
 // Inclusive prefix sum over the 64 lanes, in registers with DPP (call with
@@ -348,13 +367,18 @@ MDZ_DEV uint64_t wave_incl_scan_u64(uint64_t v) {
 // row 1 and row 2's to row 3, and row_bcast:31 adds lane 31 (rows 0 and 1)
 // to rows 2 and 3.
 MDZ_DEV uint32_t wave_incl_scan_u32(uint32_t v) {
-    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xf, 0xf, true);   // row_shr:1
-    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xf, 0xf, true);   // row_shr:2
-    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xf, 0xf, true);   // row_shr:4
-    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xf, 0xf, true);   // row_shr:8
-    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xa, 0xf, false);  // row_bcast:15
-    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xc, 0xf, false);  // row_bcast:31
+    v += dpp_src<0x111, 0xf, 0u>(v);      // row_shr:1
+    v += dpp_src<0x112, 0xf, 0u>(v);      // row_shr:2
+    v += dpp_src<0x114, 0xf, 0u>(v);      // row_shr:4
+    v += dpp_src<0x118, 0xf, 0u>(v);      // row_shr:8
+    v += dpp_src<0x142, 0xa, 0u>(v);      // row_bcast:15
+    v += dpp_src<0x143, 0xc, 0u>(v);      // row_bcast:31
     return v;
+}
+// v of lane 63 (wave-uniform).
+MDZ_DEV uint64_t last_lane_u64(uint64_t v) {
+    return ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), kWave - 1) << 32) |
+           (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, kWave - 1);
 }
 MDZ_DEV uint64_t shfl_u64(uint64_t v, int src) {
     const uint32_t lo = __shfl((uint32_t)v, src, kWave), hi = __shfl((uint32_t)(v >> 32), src, kWave);

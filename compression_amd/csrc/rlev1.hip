@@ -202,7 +202,7 @@ MDZ_DEV int32_t rlev1_windowed(const u32x4* __restrict__ src, LaneTable& T, uint
     }
     uint64_t bad = __ballot(err != MDZ_ST_OK);
     if (!bad) return MDZ_ST_OK;
-    return __shfl(err, (int)__builtin_ctzll(bad), kWave);
+    return __builtin_amdgcn_readlane(err, (int)__builtin_ctzll(bad));
 }
 
 

@@ -396,7 +396,7 @@ MDZ_DEV void run_values(const Run& rr, uint64_t a, uint64_t b, const uint8_t* da
                 if (neg) d = 0 - d;
                 const uint64_t incl = wave_incl_scan_u64(d);
                 if (k < nd) put(2 + k, (carry + incl) & vmask);
-                carry += shfl_u64(incl, kWave - 1);
+                carry += last_lane_u64(incl);
             }
         }
     } else {                                                    // PATCHED_BASE
