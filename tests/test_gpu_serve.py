@@ -38,3 +38,27 @@ def test_budget_admission_limits_inflight(gpu):
     res = srv.run(reqs, time_limit=120)
     assert res["completed"] == 12 and res["bit_exact"]
     assert srv.in_use == 0
+
+
+def test_full_request_range_and_largest_alone(gpu):
+    """Config 5's whole size range (16 KiB..64 MiB, log-uniform) under its
+    64 GB budget, plus one 64 MiB request of every codec: every result is
+    bit-exact, and the 64 MiB ones equal their decode alone."""
+    from compression_amd import serve
+    n = 12
+    sizes = serve.request_sizes(n, 0x5E, 16 << 10, 64 << 20)
+    rng = np.random.default_rng(0x5E)
+    codecs = [serve.CODECS[i] for i in rng.integers(0, 4, n)]
+    arr = serve.poisson_arrivals(n + 4, 200.0, 0x5F)
+    reqs = [serve.make_request(i, codecs[i], int(sizes[i]), 2000 + i, float(arr[i])) for i in range(n)]
+    big = [serve.make_request(n + k, c, 64 << 20, 3000 + k, float(arr[n + k]))
+           for k, c in enumerate(serve.CODECS)]
+    srv = serve.Server(hbm_budget=64 << 30, n_streams=4, verify=True)
+    res = srv.run(reqs + big, time_limit=300)
+    assert res["completed"] == n + 4 and res["bit_exact"]
+    assert srv.in_use == 0
+    for r in big:
+        alone = serve.Server(hbm_budget=64 << 30, n_streams=1, verify=True)
+        out = alone.run([serve.make_request(r.rid, r.codec, r.out_bytes, 3000 + r.rid - n, 0.0)],
+                        time_limit=120)
+        assert out["bit_exact"] and np.array_equal(alone.outputs[r.rid], srv.outputs[r.rid])
