@@ -1089,7 +1089,9 @@ def serve_record(args) -> dict:
     codecs = [serve.CODECS[i] for i in np.random.default_rng(0x4D445A05).integers(0, 4, n)]
     pool = [serve.make_request(i, codecs[i], int(sizes[i]), 0x4D445A05 + i) for i in range(n)]
     total = sum(r.out_bytes for r in pool)
-    srv = serve.Server(hbm_budget=64 << 30, n_streams=4, verify=False)
+    mode = getattr(args, "serve_deflate", "live")
+    ns = getattr(args, "serve_streams", 0) or (2 if mode == "live" else 4)
+    srv = serve.Server(hbm_budget=64 << 30, n_streams=ns, verify=False, deflate_mode=mode)
     srv.run(pool[: min(8, n)], time_limit=60)            # warm-up: allocations, code objects
     for r in pool:
         r.arrival = 0.0
@@ -1104,7 +1106,7 @@ def serve_record(args) -> dict:
         sweep.append({"load": frac, "p50_ms": round(res["p50_ms"], 3), "p99_ms": round(res["p99_ms"], 3),
                       "GBps": round(res["GBps"], 2), "completed": res["completed"]})
     # bit-exactness of the whole pool, once, with verification on
-    chk = serve.Server(hbm_budget=64 << 30, n_streams=4, verify=True)
+    chk = serve.Server(hbm_budget=64 << 30, n_streams=ns, verify=True, deflate_mode=mode)
     for r in pool:
         r.arrival = 0.0
     ok = chk.run(pool, time_limit=180)["bit_exact"]
@@ -1112,7 +1114,7 @@ def serve_record(args) -> dict:
     return {"value": at80["GBps"], "unit": "GB/s", "dtype": "mixed",
             "config": {"workload": "serve_mixed_codecs", "requests": n, "out_bytes": total,
                        "codecs": {c: codecs.count(c) for c in serve.CODECS}, "hbm_budget_GiB": 64,
-                       "streams": 4},
+                       "streams": ns, "deflate": mode},
             "saturation_GBps": round(sat["GBps"], 2), "p50_ms_at_80": at80["p50_ms"],
             "p99_ms_at_80": at80["p99_ms"], "sweep": sweep, "bit_exact": ok}
 
@@ -1142,6 +1144,12 @@ def main():
     ap.add_argument("--no-serve", action="store_true", help="all: skip config 5")
     ap.add_argument("--no-sweep", action="store_true", help="all: skip the RLE v1 line sweep")
     ap.add_argument("--requests", type=int, default=64, help="request pool size for serving")
+    ap.add_argument("--serve-streams", type=int, default=0,
+                    help="serving: HIP streams for the per-request launches (0: 2 with the live "
+                         "inflate queue, 4 without)")
+    ap.add_argument("--serve-deflate", default="live", choices=["live", "launch"],
+                    help="serving: Deflate requests join one running inflate kernel (live) "
+                         "or a new scheduled launch per tick (launch)")
     ap.add_argument("--no-tune", action="store_true",
                     help="config2: size order and default shape instead of the autotuner's choice")
     ap.add_argument("--no-graph", action="store_true",

@@ -99,6 +99,11 @@ SIGNATURES = {
     "mdz_inflate_batch_sched": (C.c_int, [vp, vp, C.c_int, C.c_int, vp, u32, vp, vp, vp]),
     "mdz_inflate_batch_sched_noalloc": (C.c_int, [vp, vp, C.c_int, C.c_int, vp, u32, vp, vp]),
     "mdz_inflate_slots": (C.c_int, [C.POINTER(u32)]),
+    "mdz_inflate_live_open": (C.c_int, [u32, C.c_int, u32, u32, C.POINTER(vp)]),
+    "mdz_inflate_live_submit": (C.c_int, [vp, vp, u32, C.POINTER(u32)]),
+    "mdz_inflate_live_statuses": (vp, [vp]),
+    "mdz_inflate_live_expired": (C.c_int, [vp]),
+    "mdz_inflate_live_close": (C.c_int, [vp, vp]),
     "mdz_copy_uncompressed": (C.c_int, [vp, C.c_int, vp]),
     "mdz_rlev1_compress_bound": (C.c_int, [u64, u64, C.c_uint32, C.c_uint32, C.POINTER(u64), C.POINTER(u64),
                                            C.POINTER(u64)]),

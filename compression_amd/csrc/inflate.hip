@@ -34,6 +34,7 @@
 // rules, applied by a prepass kernel; the 8-byte footer is excluded from the
 // Deflate data); with MDZ_HDR_VERIFY a kernel after the decode checks each
 // trailer (Adler-32, or CRC-32 and ISIZE) against the output written.
+#include <cstring>
 #include <mutex>
 
 #include "deflate_hdr_dev.h"
@@ -141,6 +142,7 @@ struct Ctrl {
     uint32_t done;        // the consumer has finished the stream
     uint32_t n_stored, n_fixed, n_dyn;
     uint64_t consumed;    // Deflate input bits through the last item
+    uint64_t desc[4];     // a live queue's descriptor, read once by thread 0
 };
 
 struct InflateLds {
@@ -567,6 +569,28 @@ MDZ_DEV uint32_t stream_checksum(const uint8_t* p, uint64_t n, bool crc) {
     return crc ? ~R : (B << 16) | A;
 }
 
+// Live work queue (mdz_inflate_live_*): live[0] is the number of published
+// items (the host raises it after writing their descriptors), live[1] is set
+// once no more will come, live[2] is set here when a worker gives up idle,
+// live[3] is the number of polls (about 15 us apart) before it does.  All
+// live in host memory the host writes while the kernel runs.
+constexpr uint32_t kLivePollsPerMs = 64;
+
+// Waits until item q is published and returns q, or returns ~0u once the
+// queue is closed without it, or after live[3] polls (live[2] records that:
+// the host then knows items past this point will not be decoded).
+MDZ_DEV uint32_t live_wait(uint32_t* live, uint32_t q) {
+    const uint32_t polls = __hip_atomic_load(&live[3], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    for (uint32_t it = 0; it < polls; ++it) {
+        if (q < __hip_atomic_load(&live[0], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM)) return q;
+        if (__hip_atomic_load(&live[1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM))   // closed: the count is final
+            return q < __hip_atomic_load(&live[0], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM) ? q : ~0u;
+        for (int k = 0; k < 4; ++k) __builtin_amdgcn_s_sleep(127);   // ~8K cycles each
+    }
+    __hip_atomic_store(&live[2], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    return ~0u;
+}
+
 // Two waves per stream.  Wave 1, the producer, parses block headers, builds
 // the tables and decodes the symbols of one round at a time into one of two
 // LDS lists; wave 0, the consumer, turns the previous round's list (or a
@@ -578,7 +602,8 @@ __global__ __launch_bounds__(128) void inflate_kernel(const mdz_inflate_in_t* __
                                                       const uint32_t* __restrict__ order,
                                                       uint32_t count, int parse_hdr,
                                                       uint32_t* __restrict__ cursor,
-                                                      mdz_inflate_cost_t* __restrict__ cost) {
+                                                      mdz_inflate_cost_t* __restrict__ cost,
+                                                      uint32_t* live) {
     extern __shared__ __align__(16) uint8_t smem[];
     InflateLds& S = *reinterpret_cast<InflateLds*>(smem);
     HdrScratch& H = S.u.h;
@@ -588,7 +613,18 @@ __global__ __launch_bounds__(128) void inflate_kernel(const mdz_inflate_in_t* __
     for (;;) {
         // next stream: one per block (plain) or from the shared cursor (persistent)
         if (threadIdx.x == 0) {
-            S.ctrl.qi = cursor ? atomicAdd(cursor, 1u) : blockIdx.x;
+            const uint32_t q = cursor ? atomicAdd(cursor, 1u) : blockIdx.x;
+            const uint32_t lq = live && q < count ? live_wait(live, q) : q;
+            if (live && lq < count) {
+                // the host wrote this descriptor after the kernel started: read
+                // it past every cache with system-scope loads, after the acquire
+                // that saw it published (a plain uniform load could come from
+                // the scalar cache, filled earlier from the same line)
+                const uint64_t* d = reinterpret_cast<const uint64_t*>(ins + lq);
+                for (int k = 0; k < 4; ++k)
+                    S.ctrl.desc[k] = __hip_atomic_load(d + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+            }
+            S.ctrl.qi = lq;
             S.ctrl.halt = 0;
             S.ctrl.done = 0;
         }
@@ -601,7 +637,15 @@ __global__ __launch_bounds__(128) void inflate_kernel(const mdz_inflate_in_t* __
             __syncthreads();
             continue;
         }
-        const mdz_inflate_in_t in = ins[si];
+        mdz_inflate_in_t in;
+        if (live) {
+            in.src = reinterpret_cast<const void*>(S.ctrl.desc[0]);
+            in.src_bytes = S.ctrl.desc[1];
+            in.dst = reinterpret_cast<void*>(S.ctrl.desc[2]);
+            in.dst_bytes = S.ctrl.desc[3];
+        } else {
+            in = ins[si];
+        }
         const int hdr = parse_hdr & 3;
 
         if (producer) {
@@ -1117,6 +1161,9 @@ __global__ __launch_bounds__(128) void inflate_kernel(const mdz_inflate_in_t* __
                 if (S.ctrl.done) break;
             }
             flush(pos, true);
+            // a live queue's host reads the status while the kernel runs: the
+            // output (and the status fields) must reach memory first
+            if (live) __threadfence_system();
             if (l == 0) {
                 mdz_inflate_status_t o;
                 o.bytes_written = pos;
@@ -1135,7 +1182,14 @@ __global__ __launch_bounds__(128) void inflate_kernel(const mdz_inflate_in_t* __
                 // Deflate input consumed through the last block (from the start
                 // of the Deflate data), for the trailer check; saturates
                 o.reserved = (uint32_t)min<uint64_t>((S.ctrl.consumed + 7) / 8, 0xFFFFFFFFull);
-                outs[si] = o;
+                if (live) {
+                    outs[si].bytes_written = o.bytes_written;
+                    outs[si].reserved = o.reserved;
+                    __threadfence_system();
+                    __hip_atomic_store(&outs[si].status, o.status, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+                } else {
+                    outs[si] = o;
+                }
             }
         }
         __syncthreads();
@@ -1254,6 +1308,18 @@ int inflate_slots(uint32_t* slots) {
     return MDZ_OK;
 }
 
+// Launches the persistent kernel on a live queue (descriptors, statuses and
+// the control words in host memory the host writes while it runs).
+int inflate_live_launch(const mdz_inflate_in_t* in, mdz_inflate_status_t* out, uint32_t cap, int parse_hdr,
+                        uint32_t blocks, uint32_t* cursor, uint32_t* live, hipStream_t s) {
+    const size_t smem = sizeof(InflateLds);
+    hipError_t e = set_dynamic_lds((const void*)inflate_kernel, smem);
+    if (e != hipSuccess) return record_hip(e, "inflate_kernel LDS");
+    hipLaunchKernelGGL(inflate_kernel, dim3(blocks), dim3(128), smem, s, in, out, nullptr, cap, parse_hdr,
+                       cursor, nullptr, live);
+    return record_hip(hipGetLastError(), "inflate_kernel live launch");
+}
+
 int inflate_batch(const mdz_inflate_in_t* in, mdz_inflate_status_t* out, uint32_t count,
                   int parse_hdr, const uint32_t* order, uint32_t persistent_blocks,
                   uint32_t* cursor, mdz_inflate_cost_t* cost, hipStream_t s) {
@@ -1280,7 +1346,7 @@ int inflate_batch(const mdz_inflate_in_t* in, mdz_inflate_status_t* out, uint32_
         if ((e = hipGetLastError()) != hipSuccess) return record_hip(e, "gzip_prepass_kernel launch");
     }
     hipLaunchKernelGGL(inflate_kernel, dim3(grid), dim3(128), smem, s, in, out, order, count,
-                       parse_hdr, persistent ? cursor : nullptr, cost);
+                       parse_hdr, persistent ? cursor : nullptr, cost, nullptr);
     if ((e = hipGetLastError()) != hipSuccess) return record_hip(e, "inflate_kernel launch");
     if ((parse_hdr & MDZ_HDR_VERIFY) && hdr) {
         hipLaunchKernelGGL(trailer_check_kernel, dim3(count), dim3(64), 0, s, in, out, hdr);
@@ -1336,4 +1402,121 @@ extern "C" int mdz_inflate_batch_sched_noalloc(const mdz_inflate_in_t* in, mdz_i
     // the cursor is cleared by inflate_batch's reset kernel before the launch
     return mdz::inflate_batch(in, out, (uint32_t)count, parse_hdr, order, blocks, cursor, cost,
                               reinterpret_cast<hipStream_t>(stream));
+}
+
+// ------------------------------------------------------------ live queue --
+// mdz_inflate_live_*: one persistent inflate launch that decodes streams the
+// host publishes while it runs (descriptors, statuses and the control words
+// {published, closed, expired, idle polls} in pinned host memory the GPU
+// reads and writes coherently).  The kernel's workers claim items in order
+// and wait for them; each ends when the queue is closed and drained, or
+// after idle_ms without its item (8 s by default).
+struct mdz_inflate_live {
+    mdz_inflate_in_t* ins = nullptr;
+    mdz_inflate_status_t* outs = nullptr;
+    uint32_t* ctrl = nullptr;
+    uint32_t* cursor = nullptr;
+    uint32_t cap = 0, published = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t done = nullptr;
+    bool launched = false;
+};
+
+static void live_free(mdz_inflate_live* q) {
+    if (q->done) (void)hipEventDestroy(q->done);
+    if (q->stream) (void)hipStreamDestroy(q->stream);
+    if (q->cursor) (void)hipFree(q->cursor);
+    if (q->ins) (void)hipHostFree(q->ins);
+    if (q->outs) (void)hipHostFree(q->outs);
+    if (q->ctrl) (void)hipHostFree(q->ctrl);
+    delete q;
+}
+
+extern "C" int mdz_inflate_live_open(uint32_t capacity, int parse_hdr, uint32_t blocks, uint32_t idle_ms,
+                                     mdz_inflate_live_t** out) {
+    if (!out || capacity == 0 || capacity > 0x7fffffffu)
+        return mdz::set_error(MDZ_ERR_INVALID, "inflate live: capacity %u", capacity);
+    *out = nullptr;
+    if (parse_hdr != MDZ_HDR_RAW && parse_hdr != MDZ_HDR_ZLIB)
+        return mdz::set_error(MDZ_ERR_UNSUPPORTED, "inflate live: raw or zlib framing only (parse_hdr %d)",
+                              parse_hdr);
+    mdz_inflate_live* q = new mdz_inflate_live();
+    q->cap = capacity;
+    hipError_t e;
+    // coherent (fine-grained) pinned memory: each side sees the other's
+    // writes while the kernel runs
+    const unsigned fl = hipHostMallocCoherent;
+    if ((e = hipHostMalloc((void**)&q->ins, sizeof(mdz_inflate_in_t) * capacity, fl)) != hipSuccess ||
+        (e = hipHostMalloc((void**)&q->outs, sizeof(mdz_inflate_status_t) * capacity, fl)) != hipSuccess ||
+        (e = hipHostMalloc((void**)&q->ctrl, 64, fl)) != hipSuccess ||
+        (e = hipMalloc((void**)&q->cursor, 4)) != hipSuccess) {
+        live_free(q);
+        return mdz::set_error(MDZ_ERR_NOMEM, "inflate live: allocation failed (%s)", hipGetErrorString(e));
+    }
+    memset(q->ins, 0, sizeof(mdz_inflate_in_t) * capacity);
+    for (uint32_t i = 0; i < capacity; ++i) {
+        q->outs[i].bytes_written = 0;
+        q->outs[i].status = (uint32_t)MDZ_ST_LIVE_PENDING;
+        q->outs[i].reserved = 0;
+    }
+    memset(q->ctrl, 0, 64);
+    const uint64_t polls = (uint64_t)(idle_ms ? idle_ms : 8000u) * mdz::kLivePollsPerMs;
+    q->ctrl[3] = polls > 0xffffffffull ? 0xffffffffu : (uint32_t)polls;
+    uint32_t grid = blocks;
+    if (!grid) {                                   // a quarter of the co-resident slots
+        int rc = mdz::inflate_slots(&grid);
+        if (rc) { live_free(q); return rc; }
+        grid = grid / 4 ? grid / 4 : 1;
+    }
+    if ((e = hipMemset(q->cursor, 0, 4)) != hipSuccess ||
+        (e = hipStreamCreateWithFlags(&q->stream, hipStreamNonBlocking)) != hipSuccess ||
+        (e = hipEventCreateWithFlags(&q->done, hipEventDisableTiming)) != hipSuccess) {
+        live_free(q);
+        return mdz::record_hip(e, "inflate live setup");
+    }
+    int rc = mdz::inflate_live_launch(q->ins, q->outs, capacity, parse_hdr, grid, q->cursor, q->ctrl, q->stream);
+    if (rc) { live_free(q); return rc; }
+    q->launched = true;
+    if ((e = hipEventRecord(q->done, q->stream)) != hipSuccess) {
+        __atomic_store_n(&q->ctrl[1], 1u, __ATOMIC_RELEASE);   // let the workers end before freeing
+        (void)hipStreamSynchronize(q->stream);
+        live_free(q);
+        return mdz::record_hip(e, "inflate live event");
+    }
+    *out = q;
+    return MDZ_OK;
+}
+
+extern "C" int mdz_inflate_live_submit(mdz_inflate_live_t* q, const mdz_inflate_in_t* in, uint32_t n,
+                                       uint32_t* first) {
+    if (!q || (!in && n)) return mdz::set_error(MDZ_ERR_INVALID, "inflate live submit: null argument");
+    if (__atomic_load_n(&q->ctrl[1], __ATOMIC_ACQUIRE))
+        return mdz::set_error(MDZ_ERR_INVALID, "inflate live submit: queue closed");
+    if (n > q->cap - q->published)
+        return mdz::set_error(MDZ_ERR_INVALID, "inflate live submit: %u items past capacity %u", n, q->cap);
+    if (n) memcpy(q->ins + q->published, in, sizeof(mdz_inflate_in_t) * n);
+    if (first) *first = q->published;
+    q->published += n;
+    __atomic_store_n(&q->ctrl[0], q->published, __ATOMIC_RELEASE);   // descriptors before the count
+    return MDZ_OK;
+}
+
+extern "C" const mdz_inflate_status_t* mdz_inflate_live_statuses(const mdz_inflate_live_t* q) {
+    return q ? q->outs : nullptr;
+}
+
+extern "C" int mdz_inflate_live_expired(mdz_inflate_live_t* q) {
+    if (!q) return mdz::set_error(MDZ_ERR_INVALID, "inflate live: null queue");
+    if (__atomic_load_n(&q->ctrl[2], __ATOMIC_ACQUIRE)) return 1;
+    return hipEventQuery(q->done) == hipSuccess ? 1 : 0;          // ended before the close
+}
+
+extern "C" int mdz_inflate_live_close(mdz_inflate_live_t* q, mdz_inflate_status_t* statuses) {
+    if (!q) return mdz::set_error(MDZ_ERR_INVALID, "inflate live close: null queue");
+    __atomic_store_n(&q->ctrl[1], 1u, __ATOMIC_RELEASE);
+    const hipError_t e = hipEventSynchronize(q->done);
+    if (statuses && q->published) memcpy(statuses, q->outs, sizeof(mdz_inflate_status_t) * q->published);
+    live_free(q);
+    if (e != hipSuccess) return mdz::record_hip(e, "inflate live close");
+    return MDZ_OK;
 }

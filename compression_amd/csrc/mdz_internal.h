@@ -52,6 +52,8 @@ int inflate_batch(const mdz_inflate_in_t* in, mdz_inflate_status_t* out, uint32_
                   int parse_hdr, const uint32_t* order, uint32_t persistent_blocks,
                   uint32_t* cursor, mdz_inflate_cost_t* cost, hipStream_t s);
 int inflate_slots(uint32_t* slots);
+int inflate_live_launch(const mdz_inflate_in_t* in, mdz_inflate_status_t* out, uint32_t cap, int parse_hdr,
+                        uint32_t blocks, uint32_t* cursor, uint32_t* live, hipStream_t s);
 int copy_uncompressed(const mdz_inflate_in_t* in, uint32_t count, hipStream_t s);
 
 // parse_hdr: MDZ_HDR_RAW / ZLIB / GZIP, optionally | MDZ_HDR_VERIFY (framed streams only)

@@ -4,7 +4,9 @@
 
 ``DeviceBatch`` keeps a batch of streams and their output slots resident in
 HBM with the descriptor array the kernels read; ``inflate`` is the plain
-per-stream launch and ``inflate_sched`` the persistent work-queue launch.
+per-stream launch and ``inflate_sched`` the persistent work-queue launch;
+``LiveInflate`` keeps one persistent launch running and takes streams while
+it runs.
 """
 from __future__ import annotations
 
@@ -41,6 +43,7 @@ class DeviceBatch:
         d["src_bytes"] = np.asarray(src_len, np.uint64)
         d["dst"] = self.dst.ptr + self.dst_off
         d["dst_bytes"] = caps
+        self.host_desc = d
         self.desc = DeviceBuffer.from_array(d, stream=stream)
         self.status = DeviceBuffer(max(self.count, 1) * STATUS.itemsize)
         self.cursor = DeviceBuffer(4)
@@ -85,6 +88,87 @@ def inflate_sched(b: DeviceBatch, parse_hdr: int = HDR_ZLIB, blocks: int = 0, st
                                         b.order.ptr if b.order is not None else None, blocks,
                                         b.cursor.ptr, b.cost.ptr if b.cost is not None else None,
                                         as_ptr(stream)), "mdz_inflate_batch_sched")
+
+
+ST_LIVE_PENDING = 0x7FFFFFFF
+
+
+class LiveInflate:
+    """A persistent inflate launch that takes streams while it runs -- no
+    counterpart in the reference; for serving loops that fold Deflate
+    requests into a decode already on the GPU.
+
+    ``submit`` publishes descriptors (``DESC`` records of device pointers) and
+    returns the first one's index; ``statuses`` reads the live status array,
+    where an item shows ``ST_LIVE_PENDING`` until its output is in memory;
+    ``close`` ends the queue and returns the final statuses.  Raw or zlib
+    framing.  The workers end on their own after ``idle_ms`` (0: 8 s) without
+    their next item; once one has, ``expired`` is True: items already taken
+    still finish, later ones may stay pending.
+    While a queue is open, hipFree and hipDeviceSynchronize on this device
+    wait for its running kernel: allocate first, free after ``close``."""
+
+    def __init__(self, capacity: int, parse_hdr: int = HDR_ZLIB, blocks: int = 0, idle_ms: int = 0):
+        q = C.c_void_p()
+        check(lib().mdz_inflate_live_open(int(capacity), int(parse_hdr), int(blocks), int(idle_ms),
+                                          C.byref(q)), "mdz_inflate_live_open")
+        self._q = q
+        self.capacity = int(capacity)
+        self.published = 0
+        ptr = lib().mdz_inflate_live_statuses(q)
+        raw = (C.c_uint8 * (self.capacity * STATUS.itemsize)).from_address(ptr)
+        self._st = np.frombuffer(raw, np.uint8).view(STATUS)
+        self._keep = []                   # buffers the submitted descriptors point into
+
+    def submit(self, desc: np.ndarray, keep=None) -> int:
+        d = np.ascontiguousarray(desc, DESC)
+        first = C.c_uint32()
+        check(lib().mdz_inflate_live_submit(self._q, d.ctypes.data, len(d), C.byref(first)),
+              "mdz_inflate_live_submit")
+        self.published += len(d)
+        if keep is not None:
+            self._keep.append(keep)
+        return first.value
+
+    def statuses(self) -> np.ndarray:
+        """Snapshot of the first ``published`` statuses: each status is read
+        before its byte count, so a finished item's count is final."""
+        n = self.published
+        st = self._st["status"][:n].copy()
+        out = np.zeros(n, STATUS)
+        out["status"] = st
+        out["bytes_written"] = self._st["bytes_written"][:n]
+        out["reserved"] = self._st["reserved"][:n]
+        return out
+
+    @property
+    def expired(self) -> bool:
+        return bool(lib().mdz_inflate_live_expired(self._q))
+
+    def close(self) -> np.ndarray:
+        if self._q is None:
+            raise MdzError(-1, "mdz_inflate_live_close: queue already closed")
+        out = np.zeros(max(self.published, 1), STATUS)
+        q, self._q = self._q, None
+        self._st = None
+        rc = lib().mdz_inflate_live_close(q, out.ctypes.data)
+        self._keep = []
+        check(rc, "mdz_inflate_live_close")
+        return out[:self.published]
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        if self._q is not None:
+            self.close()
+
+    def __del__(self):
+        if getattr(self, "_q", None) is not None:
+            try:
+                self.close()
+            except Exception:
+                pass
 
 
 def copy_uncompressed(b: DeviceBatch, stream=None) -> None:

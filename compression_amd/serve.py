@@ -3,20 +3,26 @@
 Requests (RLE v1, RLE v2, LZSS, Deflate) arrive open-loop.  The loop:
   * admits waiting requests in arrival order while the HBM budget allows
     (compressed input + output + metadata of every request on the device);
-  * at each tick folds every admitted, not yet launched request into launches:
-    all pending Deflate chunk streams go into ONE scheduled inflate launch
-    (persistent work queue, longest stream first), lane-transposed requests
-    get one launch each; launches rotate over several HIP streams so new work
-    overlaps work already on the GPU;
-  * records each request's completion with a HIP event; latency is arrival ->
-    output ready on the device; budget and buffers are released then.
-Device buffers come from a caching pool (size-binned free lists), so the loop
-never calls hipFree while kernels run.  Decoding is the library's: the loop
-only decides when and with what work is launched.
+  * Deflate requests join ONE inflate kernel that runs for the whole serving
+    run (the library's live queue): once a request's chunk streams are in
+    HBM, its descriptors are published to the running kernel, longest stream
+    first, and its completion is read from the queue's status words;
+  * lane-transposed requests (RLE v1, RLE v2, LZSS) get one launch each;
+    launches rotate over several HIP streams so new work overlaps work
+    already on the GPU, and each completion is recorded with a HIP event;
+  * latency is arrival -> output ready on the device; budget and buffers are
+    released then.
+``deflate_mode="launch"`` instead folds each tick's Deflate requests into a
+new scheduled inflate launch.  Device buffers come from a caching pool
+(size-binned free lists), so the loop never calls hipFree while kernels run
+(with the live queue's kernel running, a hipFree would wait for it).
+Decoding is the library's: the loop only decides when and with what work is
+launched.
 """
 from __future__ import annotations
 
 import ctypes as C
+import gc
 import time
 from dataclasses import dataclass, field
 
@@ -61,6 +67,7 @@ class Request:
     out: DeviceBuffer | None = None
     status: DeviceBuffer | None = None
     done_event: Event | None = None
+    live: dict = field(default_factory=dict)   # live queue: descriptors, (queue, first, n), final statuses
     launched: float = -1.0
     finished: float = -1.0
 
@@ -115,7 +122,14 @@ def make_request(rid: int, codec: str, size: int, seed: int, arrival: float = 0.
 
 
 class Server:
-    def __init__(self, hbm_budget: int = 64 << 30, n_streams: int = 4, verify: bool = False):
+    def __init__(self, hbm_budget: int = 64 << 30, n_streams: int = 4, verify: bool = False,
+                 deflate_mode: str = "live", live_blocks: int = 0, live_idle_ms: int = 0):
+        if deflate_mode not in ("live", "launch"):
+            raise ValueError(deflate_mode)
+        self.deflate_mode = deflate_mode
+        self.live_blocks = int(live_blocks)
+        self.live_idle_ms = int(live_idle_ms)
+        self.live = None
         self.budget = int(hbm_budget)
         self.in_use = 0
         self.pool = Pool()
@@ -189,6 +203,66 @@ class Server:
             first += k
         reqs[-1].bufs.extend([desc, st, od, cur])     # released with the batch's last request
 
+    def _stage_deflate(self, r: Request) -> None:
+        """Uploads a Deflate request's streams; it joins the live queue once
+        the upload has finished (``_publish``)."""
+        s = self._stream()
+        src = self._up(r, r.payload, s)
+        r.out = self.pool.get(r.out_bytes)
+        m = r.meta
+        r.live.clear()
+        k = len(m["src_off"])
+        d = np.zeros(k, _deflate.DESC)
+        d0 = np.arange(k, dtype=np.uint64) * np.uint64(m["chunk"])
+        d["src"] = np.uint64(src.ptr) + m["src_off"].astype(np.uint64)
+        d["src_bytes"] = m["src_len"]
+        d["dst"] = np.uint64(r.out.ptr) + d0
+        d["dst_bytes"] = np.minimum(np.uint64(m["chunk"]), np.uint64(r.out_bytes) - d0)
+        r.live["desc"] = d[np.lexsort((np.arange(k), -m["src_len"].astype(np.int64)))]   # longest first
+        r.done_event = Event()
+        r.done_event.record(s)
+
+    def _publish(self, r: Request) -> None:
+        d = r.live["desc"]
+        if self.live is not None and (self.live.expired or self.live.capacity - self.live.published < len(d)):
+            self._retire_live()
+        if self.live is None:
+            self.live = _deflate.LiveInflate(max(self.live_cap, len(d)), _deflate.HDR_ZLIB, self.live_blocks,
+                                             self.live_idle_ms)
+        r.live["q"] = (self.live, self.live.submit(d), len(d))
+
+    def _retire_live(self, republish: bool = True) -> None:
+        """Ends the current queue.  Its workers give up after a long idle
+        spell: the requests with items it never took are published again to
+        a new queue (or, at the end of a run, left unfinished)."""
+        q, self.live = self.live, None
+        final = q.close()
+        for r in self.live_reqs:
+            lq, first, k = r.live["q"]
+            if lq is not q:
+                continue
+            st = final[first:first + k]
+            if not (st["status"] == _deflate.ST_LIVE_PENDING).any():
+                r.live["final"] = st
+            elif republish:
+                self._publish(r)
+
+    def _live_done(self, reqs: list[Request]) -> list[Request]:
+        snaps = {}
+        done = []
+        for r in reqs:
+            if "final" in r.live:
+                done.append(r)
+                continue
+            q, first, k = r.live["q"]
+            if id(q) not in snaps:
+                snaps[id(q)] = q.statuses()
+            st = snaps[id(q)][first:first + k]
+            if (st["status"] != _deflate.ST_LIVE_PENDING).all():
+                r.live["final"] = st
+                done.append(r)
+        return done
+
     # -- main loop --------------------------------------------------------
     def run(self, requests: list[Request], time_limit: float = 120.0) -> dict:
         pending = sorted(requests, key=lambda r: r.arrival)
@@ -197,9 +271,26 @@ class Server:
         done: list[Request] = []
         self.outputs: dict[int, np.ndarray] = {}
         self.all_ok = True
+        live = self.deflate_mode == "live"
+        self.live_cap = sum(len(r.meta["src_off"]) for r in requests if r.codec == "deflate")
+        self.live_reqs: list[Request] = []      # published to the live queue, not yet finished
+        staged: list[Request] = []              # uploading, not yet published
+        gc_was = gc.isenabled()
+        if live:
+            gc.disable()                        # no collector-driven hipFree while the queue runs
+        try:
+            return self._loop(pending, waiting, inflight, done, staged, live, time_limit)
+        finally:
+            if self.live is not None:
+                self._retire_live(republish=False)
+            if gc_was:
+                gc.enable()
+
+    def _loop(self, pending, waiting, inflight, done, staged, live, time_limit) -> dict:
         t0 = time.perf_counter()
         i = 0
-        while (i < len(pending) or waiting or inflight) and time.perf_counter() - t0 < time_limit:
+        while ((i < len(pending) or waiting or inflight or staged or self.live_reqs)
+               and time.perf_counter() - t0 < time_limit):
             now = time.perf_counter() - t0
             while i < len(pending) and pending[i].arrival <= now:
                 waiting.append(pending[i])
@@ -210,7 +301,7 @@ class Server:
                 r = waiting.pop(0)
                 self.in_use += r.hbm_bytes()
                 admitted.append(r)
-            if not admitted and waiting and not inflight:
+            if not admitted and waiting and not (inflight or staged or self.live_reqs):
                 r = waiting.pop(0)                 # larger than the whole budget: alone
                 self.in_use += r.hbm_bytes()
                 admitted.append(r)
@@ -220,15 +311,36 @@ class Server:
                 r.launched = now
                 if r.codec != "deflate":
                     self._launch_lanes(r)
+                elif live:
+                    self._stage_deflate(r)
+                    staged.append(r)
+            inflight.extend(r for r in admitted if r.codec != "deflate")
             defl = [r for r in admitted if r.codec == "deflate"]
-            if defl:
+            if defl and not live:
                 self._launch_deflate(defl)
-            inflight.extend(admitted)
+                inflight.extend(defl)
+            # uploads finished: publish to the running inflate kernel
+            still = []
+            for r in staged:
+                if r.done_event.query():
+                    self._publish(r)
+                    self.live_reqs.append(r)
+                else:
+                    still.append(r)
+            staged = still
             # completions: verify (optional), then release budget and buffers
             finished, still = [], []
             for r in inflight:
                 (finished if r.done_event.query() else still).append(r)
             inflight = still
+            if self.live_reqs:
+                ld = self._live_done(self.live_reqs)
+                if ld:
+                    ids = {id(r) for r in ld}
+                    self.live_reqs = [r for r in self.live_reqs if id(r) not in ids]
+                    finished.extend(ld)
+                elif self.live is not None and self.live.expired:
+                    self._retire_live()              # republishes what it never took
             t_now = time.perf_counter() - t0
             for r in finished:
                 r.finished = t_now
@@ -239,12 +351,13 @@ class Server:
             done.extend(finished)
             if not admitted and not finished:
                 nxt = pending[i].arrival - (time.perf_counter() - t0) if i < len(pending) else 0.0
-                time.sleep(min(max(nxt, 0.0), 0.0002) if inflight or nxt > 0 else 0.0)
+                busy = inflight or staged or self.live_reqs
+                time.sleep(min(max(nxt, 0.0), 0.0002) if busy or nxt > 0 else 0.0)
         for s in self.streams:
             s.sync()
         lat = np.array([r.finished - r.arrival for r in done]) if done else np.zeros(1)
         span = (max(r.finished for r in done) - min(r.arrival for r in done)) if done else 0.0
-        return {"completed": len(done), "submitted": len(requests),
+        return {"completed": len(done), "submitted": len(pending),
                 "bit_exact": self.all_ok if self.verify else None,
                 "p50_ms": float(np.percentile(lat, 50) * 1e3),
                 "p99_ms": float(np.percentile(lat, 99) * 1e3),
@@ -254,7 +367,9 @@ class Server:
         if not self.verify:
             return
         got = r.out.download(r.out_bytes)
-        if r.codec == "deflate":
+        if r.codec == "deflate" and "final" in r.live:
+            st = r.live["final"]["status"]
+        elif r.codec == "deflate":
             st_buf, first, k = r.meta["_batch_status"]
             allst = st_buf.download((first + k) * _deflate.STATUS.itemsize).view(_deflate.STATUS)
             st = allst["status"][first:first + k]

@@ -411,6 +411,34 @@ int mdz_inflate_batch_sched_noalloc(const mdz_inflate_in_t* in, mdz_inflate_stat
                                     int count, int parse_hdr, const uint32_t* order,
                                     uint32_t blocks, mdz_inflate_cost_t* cost, void* stream);
 
+/* Live work queue -- no counterpart in the reference; for serving loops that
+ * fold Deflate streams into a decode already running on the GPU.  open
+ * launches one persistent inflate kernel (blocks workgroups; 0 = a quarter
+ * of the co-resident slots, leaving the rest to other kernels) on its own
+ * non-blocking stream, over `capacity` items whose descriptors and statuses
+ * live in pinned host memory.  submit appends n descriptors (device src/dst
+ * pointers, as for mdz_inflate_batch; the source bytes must be in device
+ * memory before the call) and publishes them: the running kernel starts on
+ * them at once; *first is the first one's index.  statuses() points at the
+ * status array: an item's status reads MDZ_ST_LIVE_PENDING until its output
+ * and byte count are in memory.  Framing: MDZ_HDR_RAW or MDZ_HDR_ZLIB.  The
+ * workers end when the queue is closed and drained, or after idle_ms (0 =
+ * 8000) without their next item: once one has, expired() returns 1, items
+ * already taken still finish, and items published later may stay pending
+ * (close the queue and publish them to a new one).  close ends the
+ * queue, waits for the kernel, copies the first `published` statuses to
+ * `statuses` (optional) and frees everything.  While a queue is open, calls
+ * that wait for the whole device (hipDeviceSynchronize, hipFree) wait for
+ * its kernel too.  One thread drives a queue. */
+#define MDZ_ST_LIVE_PENDING 0x7fffffff
+typedef struct mdz_inflate_live mdz_inflate_live_t;
+int mdz_inflate_live_open(uint32_t capacity, int parse_hdr, uint32_t blocks, uint32_t idle_ms,
+                          mdz_inflate_live_t** q);
+int mdz_inflate_live_submit(mdz_inflate_live_t* q, const mdz_inflate_in_t* in, uint32_t n, uint32_t* first);
+const mdz_inflate_status_t* mdz_inflate_live_statuses(const mdz_inflate_live_t* q);
+int mdz_inflate_live_expired(mdz_inflate_live_t* q);
+int mdz_inflate_live_close(mdz_inflate_live_t* q, mdz_inflate_status_t* statuses);
+
 /* Compute units of the current device (sizes persistent grids). */
 int mdz_device_cus(int* cus);
 /* Workgroups of the persistent inflate kernel that fit on the device at once. */

@@ -62,3 +62,23 @@ def test_full_request_range_and_largest_alone(gpu):
         out = alone.run([serve.make_request(r.rid, r.codec, r.out_bytes, 3000 + r.rid - n, 0.0)],
                         time_limit=120)
         assert out["bit_exact"] and np.array_equal(alone.outputs[r.rid], srv.outputs[r.rid])
+
+
+def test_live_queue_expiry_republishes_and_matches_launch_mode(gpu):
+    """Deflate requests arriving farther apart than the live queue's idle
+    limit: expired queues are replaced and their requests published again;
+    every output equals the per-tick launch mode's."""
+    from compression_amd import serve
+    n = 10
+    sizes = serve.request_sizes(n, 0x61, 16 << 10, 1 << 20)
+    codecs = ["deflate" if i % 2 == 0 else serve.CODECS[i % 3] for i in range(n)]
+    arr = np.arange(n) * 0.08
+    mk = lambda: [serve.make_request(i, codecs[i], int(sizes[i]), 4000 + i, float(arr[i])) for i in range(n)]
+    live = serve.Server(hbm_budget=1 << 30, n_streams=2, verify=True, live_idle_ms=20)
+    res = live.run(mk(), time_limit=120)
+    assert res["completed"] == n and res["bit_exact"]
+    launch = serve.Server(hbm_budget=1 << 30, n_streams=2, verify=True, deflate_mode="launch")
+    res2 = launch.run(mk(), time_limit=120)
+    assert res2["completed"] == n and res2["bit_exact"]
+    for i in range(n):
+        assert np.array_equal(live.outputs[i], launch.outputs[i])
