@@ -38,6 +38,11 @@ def lib():
                                        C.POINTER(u8p), u64p, C.POINTER(u64p), C.POINTER(u64p), u64p, C.c_int]
         L.mdz_rlev2_lane_encode.restype = C.c_uint64
         L.mdz_rlev2_lane_encode.argtypes = [C.c_void_p, C.c_uint64, C.c_void_p, C.c_uint64]
+        L.mdz_rlev2_plain_chunk_encode.restype = C.c_uint64
+        L.mdz_rlev2_plain_chunk_encode.argtypes = [C.c_void_p, C.c_uint64, C.c_void_p, C.c_uint64]
+        L.mdz_rlev2_plain_encode.restype = C.c_int
+        L.mdz_rlev2_plain_encode.argtypes = [C.c_void_p, C.c_uint64, C.c_uint64, C.POINTER(u8p),
+                                             C.POINTER(C.c_uint64), C.c_int]
         L.mdz_rlev2_encode.restype = C.c_int
         L.mdz_rlev2_encode.argtypes = [C.c_void_p, C.c_uint64, C.c_uint64, C.c_int, C.c_uint32,
                                        C.POINTER(u8p), u64p, C.POINTER(u64p), C.POINTER(u64p), u64p, C.c_int]
@@ -156,6 +161,38 @@ def rlev2_encode(data: np.ndarray, chunk: int, read_unit: int, threads: int | No
     return RleV2Container(_take(pay, pb.value, np.uint8), _take(col, 32 * n, np.uint64),
                           _take(off, n + 1, np.uint64), n, chunk, data.dtype.itemsize, read_unit,
                           raw.size, pb.value)
+
+def rlev2_plain_chunk_encode(vals) -> bytes:
+    """One chunk's plain RLE v2 run stream (int64 values in [0, 2^63))."""
+    v = np.ascontiguousarray(np.asarray(vals, dtype=np.int64))
+    cap = len(v) * 10 + 64
+    out = np.zeros(cap, np.uint8)
+    n = lib().mdz_rlev2_plain_chunk_encode(_ptr(v) if v.size else None, len(v), _ptr(out), cap)
+    return out[:n].tobytes()
+
+class RleV2PlainContainer:
+    """Plain RLE v2 container bytes [u32 n+1][u64 ptr[n+1]][u64 raw_bytes][streams],
+    with its parts parsed out: ptr (n+1 offsets into data), data (the
+    streams), raw_bytes."""
+
+    def __init__(self, blob: np.ndarray, chunk: int):
+        self.blob = np.ascontiguousarray(blob, np.uint8)
+        self.chunk = chunk
+        np_ = int(self.blob[:4].view(np.uint32)[0])
+        self.n_chunks = np_ - 1
+        self.ptr = self.blob[4:4 + 8 * np_].view(np.uint64).copy()
+        self.raw_bytes = int(self.blob[4 + 8 * np_:12 + 8 * np_].view(np.uint64)[0])
+        self.data_off = 12 + 8 * np_
+        self.data = self.blob[self.data_off:]
+
+def rlev2_plain_encode(data: np.ndarray, chunk: int = 32768, threads: int | None = None) -> RleV2PlainContainer:
+    raw = np.ascontiguousarray(data).view(np.uint8).reshape(-1)
+    out, ob = u8p(), C.c_uint64()
+    rc = lib().mdz_rlev2_plain_encode(_ptr(raw), raw.size, chunk, C.byref(out), C.byref(ob),
+                                      threads or default_threads())
+    if rc != 0:
+        raise ValueError(f"plain rlev2 encode rejected its input (status {rc})")
+    return RleV2PlainContainer(_take(out, ob.value, np.uint8), chunk)
 
 def gen_rlev2_i64(n: int, seed: int) -> np.ndarray:
     out = np.empty(n, np.uint64)

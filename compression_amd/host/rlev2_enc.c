@@ -1,5 +1,6 @@
-/* RLE v2 lane-transposed encoder (host side).  Restates the "theirs" side
- * of the reference's unresolved merge:
+/* RLE v2 encoders (host side): the lane-transposed format, and the plain
+ * per-chunk format at the end of the file.  The lane encoder restates the
+ * "theirs" side of the reference's unresolved merge:
  *   include/rlev2/utils.h:6-69       width maps, constants (INPUT_T = u64)
  *   include/rlev2/encoder.h:83-785   determineEncoding and the four writers
  *   include/rlev2/encoder_trans.h:14-289  per-lane block_encode loop
@@ -70,6 +71,7 @@ typedef struct {
     uint32_t n, fix, var;
     int fixed_delta;
     uint8_t delta_bits;
+    int head;       /* plain format: the HEAD side's int64 determineEncoding */
 } enc_t;
 
 static void put(enc_t* e, uint8_t b) {
@@ -247,13 +249,20 @@ static void determine(enc_t* e) {
         inc &= l0 <= l1;
         dec &= l0 >= l1;
         e->fixed_delta &= cd == d_init;
-        e->delta[i - 1] = cd;
-        if (cd > max_delta) max_delta = cd;
+        /* HEAD (encoder.h:414): deltas[i-1] = abs(currDelta) over int64; the
+         * plain encoder takes values in [0, 2^63), where int64 order is the
+         * u64 order and every difference is exact */
+        const uint64_t ad = e->head && (int64_t)cd < 0 ? 0 - cd : cd;
+        e->delta[i - 1] = ad;
+        if (ad > max_delta) max_delta = ad;
     }
     e->delta[0] = e->lit[1] - e->lit[0];
     /* is_safe_subtract(max, min) with INT8_MIN: false only when min == 0
-     * and max <= 128 (encoder.h:93-97, B5) */
-    if (!(mn > 0 || mx > mn + 128)) { write_direct(e); return; }
+     * and max <= 128 (encoder.h:93-97, B5).  The HEAD side subtracts
+     * INT64_MIN (encoder.h:85-89): with min == 0 that wraps to INT64_MIN in
+     * the device arithmetic the encoder ran with, so values in [0, 2^63)
+     * are always safe. */
+    if (!e->head && !(mn > 0 || mx > mn + 128)) { write_direct(e); return; }
     if (mn == mx || e->fixed_delta) { write_delta(e); return; }
     if (d_init != 0) {
         e->delta_bits = closest_num_bits(max_delta);
@@ -274,9 +283,9 @@ static void determine(enc_t* e) {
 
 /* block_encode main loop (encoder_trans.h:82-196, theirs) over one lane's
  * values. */
-uint64_t mdz_rlev2_lane_encode(const uint64_t* vals, uint64_t nv, uint8_t* out, uint64_t cap) {
+static uint64_t run_encode(const uint64_t* vals, uint64_t nv, uint8_t* out, uint64_t cap, int head) {
     enc_t* e = (enc_t*)calloc(1, sizeof(enc_t));
-    e->out = out; e->cap = cap;
+    e->out = out; e->cap = cap; e->head = head;
     uint64_t prev_delta = 0;
     for (uint64_t k = 0; k < nv; ++k) {
         const uint64_t val = vals[k];
@@ -353,6 +362,79 @@ uint64_t mdz_rlev2_lane_encode(const uint64_t* vals, uint64_t nv, uint8_t* out, 
     uint64_t len = e->pos;
     free(e);
     return len;
+}
+
+uint64_t mdz_rlev2_lane_encode(const uint64_t* vals, uint64_t nv, uint8_t* out, uint64_t cap) {
+    return run_encode(vals, nv, out, cap, 0);
+}
+
+/* ------------------------------------------------------ plain format -- */
+/* block_encode (encoder.h:808-958, HEAD side: int64 values, CHUNK_SIZE
+ * bytes per chunk) over one chunk's values: the same state machine as the
+ * lane encoder, with the HEAD determineEncoding (absolute deltas, INT64_MIN
+ * safety test).  Two HEAD slips are not kept: a long fixed run written after
+ * a variable run reuses that run's stale deltas[0] (theirs resets it, as the
+ * lane encoder does), and writePatchedBasedValues prints a line. */
+uint64_t mdz_rlev2_plain_chunk_encode(const int64_t* vals, uint64_t nv, uint8_t* out, uint64_t cap) {
+    return run_encode((const uint64_t*)vals, nv, out, cap, 1);
+}
+
+typedef struct {
+    const int64_t* in; uint64_t in_bytes, chunk;
+    uint8_t** streams; uint64_t* len; int bad;
+} plain2_ctx;
+
+static void plain2_task(void* vctx, uint64_t c) {
+    plain2_ctx* x = (plain2_ctx*)vctx;
+    const uint64_t start = c * x->chunk / 8;
+    const uint64_t stop = ((c + 1) * x->chunk < x->in_bytes ? (c + 1) * x->chunk : x->in_bytes) / 8;
+    const uint64_t nv = stop > start ? stop - start : 0;
+    /* worst case per 128 values: a 2-byte DIRECT header and 64-bit values */
+    const uint64_t cap = nv * 8 + (nv / 128 + 2) * 4 + 64;
+    uint8_t* buf = (uint8_t*)malloc(cap);
+    const uint64_t n = mdz_rlev2_plain_chunk_encode(x->in + start, nv, buf, cap);
+    if (n > cap) x->bad = 1;
+    x->streams[c] = buf;
+    x->len[c] = n;
+}
+
+/* compress_gpu (encoder.h:1003-1058): [u32 n+1][u64 ptr[n+1]][u64 in_bytes]
+ * [chunk streams back to back], ptr the prefix sums with ptr[0] = 0.  The
+ * last in_bytes % 8 bytes are not encoded (each chunk ends at its last whole
+ * int64). */
+int mdz_rlev2_plain_encode(const uint8_t* in, uint64_t in_bytes, uint64_t chunk, uint8_t** out,
+                           uint64_t* out_bytes, int threads) {
+    if (chunk == 0 || chunk % 8 || in_bytes == 0 || !out || !out_bytes) return -1;
+    const uint64_t n = (in_bytes - 1) / chunk + 1;
+    if (n + 1 > 0xffffffffull) return -1;
+    for (uint64_t i = 0; i < in_bytes / 8; ++i)
+        if (((const int64_t*)in)[i] < 0) return -2;       /* values in [0, 2^63) only */
+    plain2_ctx x = {(const int64_t*)in, in_bytes, chunk, (uint8_t**)calloc(n, sizeof(uint8_t*)),
+                    (uint64_t*)calloc(n, sizeof(uint64_t)), 0};
+    mdz_parallel_for(n, threads, plain2_task, &x);
+    uint64_t tot = 0;
+    for (uint64_t c = 0; c < n; ++c) tot += x.len[c];
+    const uint64_t hdr = 4 + 8 * (n + 1) + 8;
+    uint8_t* o = x.bad ? NULL : (uint8_t*)malloc(hdr + tot);
+    if (o) {
+        const uint32_t np = (uint32_t)(n + 1);
+        memcpy(o, &np, 4);
+        uint64_t acc = 0;
+        memcpy(o + 4, &acc, 8);
+        for (uint64_t c = 0; c < n; ++c) {
+            memcpy(o + hdr + acc, x.streams[c], x.len[c]);
+            acc += x.len[c];
+            memcpy(o + 4 + 8 * (c + 1), &acc, 8);
+        }
+        memcpy(o + 4 + 8 * (n + 1), &in_bytes, 8);
+    }
+    for (uint64_t c = 0; c < n; ++c) free(x.streams[c]);
+    free(x.streams);
+    free(x.len);
+    if (!o) return -3;
+    *out = o;
+    *out_bytes = hdr + tot;
+    return 0;
 }
 
 /* ---------------------------------------------------------- container -- */

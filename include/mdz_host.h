@@ -52,6 +52,15 @@ int mdz_rlev2_encode(const uint8_t* in, uint64_t in_bytes, uint64_t chunk, int e
                      uint32_t read_unit, uint8_t** payload, uint64_t* payload_bytes,
                      uint64_t** col_len, uint64_t** blk_off, uint64_t* n_chunks, int threads);
 
+/* Plain RLE v2 (one run stream per chunk of int64 values): one chunk's
+ * stream (block_encode, include/rlev2/encoder.h:808-958, HEAD side), and the
+ * whole container as compress_gpu writes it (encoder.h:1003-1058):
+ * [u32 n+1][u64 ptr[n+1]][u64 in_bytes][streams].  Values must lie in
+ * [0, 2^63) (returns -2 otherwise); *out is freed with mdz_free. */
+uint64_t mdz_rlev2_plain_chunk_encode(const int64_t* vals, uint64_t nv, uint8_t* out, uint64_t cap);
+int mdz_rlev2_plain_encode(const uint8_t* in, uint64_t in_bytes, uint64_t chunk, uint8_t** out,
+                           uint64_t* out_bytes, int threads);
+
 /* ----------------------------------------------------------- byte RLE --
  * include/brle/brle.h (encode :129-230, compress_gpu :251-303). */
 uint64_t mdz_brle_chunk_encode(const uint8_t* in, uint64_t n, uint8_t* out);

@@ -63,6 +63,14 @@ int mdz_ref_rlev2_decode(const uint8_t* payload, const uint64_t* blk_off, const 
                          uint64_t n_chunks, uint64_t chunk, int elem_bytes, uint32_t read_unit,
                          uint8_t* out, uint64_t out_bytes, int32_t* status, int threads);
 
+/* Plain RLE v2 (include/rlev2/decoder.h:190-262): one chunk's run stream
+ * into at most cap int64 values (*n_out decoded), and a whole container
+ * [u32 n+1][u64 ptr[n+1]][u64 raw_bytes][streams] into out. */
+int mdz_ref_rlev2_plain_chunk_decode(const uint8_t* in, uint64_t in_len, uint64_t* vals, uint64_t cap,
+                                     uint64_t* n_out);
+int mdz_ref_rlev2_plain_decode(const uint8_t* in, uint64_t in_bytes, uint64_t chunk, uint8_t* out,
+                               uint64_t out_bytes, int32_t* status, uint64_t n_status, int threads);
+
 /* ========================================================== byte RLE ===== *
  * brle::decode (include/brle/brle.h:76-97) with bounds checks. */
 int mdz_ref_brle_chunk_decode(const uint8_t* in, uint64_t in_len, uint8_t* out,

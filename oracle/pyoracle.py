@@ -20,6 +20,7 @@ from compression_amd.host import (BrleContainer, DeflateBatch, LzssContainer, Lz
                                   default_threads, gen_brle_u8, gen_deflate_batch, gen_lzss_markov,
                                   gen_markov, gen_rlev1_u32, gen_rlev2_i64, host_cores, lzss_encode,
                                   rlev1_encode, rlev1_lane_encode, rlev2_encode, rlev2_lane_encode,
+                                  rlev2_plain_chunk_encode, rlev2_plain_encode,
                                   rng_stream)
 from compression_amd.host import lzss_lane_encode as _lzss_lane_encode
 
@@ -62,6 +63,11 @@ def lib():
                                           C.c_uint64, C.c_void_p, C.c_int]
         L.mdz_ref_rlev2_lane_decode.restype = C.c_int
         L.mdz_ref_rlev2_lane_decode.argtypes = [C.c_void_p, C.c_uint64, C.c_int, C.c_void_p, C.c_uint64]
+        L.mdz_ref_rlev2_plain_chunk_decode.restype = C.c_int
+        L.mdz_ref_rlev2_plain_chunk_decode.argtypes = [C.c_void_p, C.c_uint64, C.c_void_p, C.c_uint64, u64p]
+        L.mdz_ref_rlev2_plain_decode.restype = C.c_int
+        L.mdz_ref_rlev2_plain_decode.argtypes = [C.c_void_p, C.c_uint64, C.c_uint64, C.c_void_p, C.c_uint64,
+                                                 C.c_void_p, C.c_uint64, C.c_int]
         L.mdz_ref_rlev2_decode.restype = C.c_int
         L.mdz_ref_rlev2_decode.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64, C.c_uint64,
                                            C.c_int, C.c_uint32, C.c_void_p, C.c_uint64, C.c_void_p, C.c_int]
@@ -138,6 +144,26 @@ def rlev2_decode(c: RleV2Container, out_bytes: int | None = None, threads: int |
                                     c.elem_bytes, c.read_unit, _ptr(out), out_bytes, _ptr(status),
                                     threads or default_threads())
     return rc, out[:out_bytes], status[:c.n_chunks]
+
+def rlev2_plain_chunk_decode(stream: bytes, cap: int) -> tuple[int, np.ndarray]:
+    """One plain RLE v2 chunk stream -> (status, int64 values decoded)."""
+    src = np.frombuffer(bytes(stream) + b"\0", np.uint8)
+    out = np.zeros(max(cap, 1), np.uint64)
+    got = C.c_uint64()
+    st = lib().mdz_ref_rlev2_plain_chunk_decode(_ptr(src), len(stream), _ptr(out), cap, C.byref(got))
+    return st, out[:got.value].view(np.int64)
+
+def rlev2_plain_decode(blob: np.ndarray, chunk: int, out_bytes: int | None = None, threads: int | None = None):
+    """A plain RLE v2 container -> (rc, output bytes, per-chunk statuses)."""
+    blob = np.ascontiguousarray(blob, np.uint8)
+    n = int(blob[:4].view(np.uint32)[0]) - 1 if blob.size >= 4 else 0
+    if out_bytes is None:
+        out_bytes = int(blob[4 + 8 * (n + 1):12 + 8 * (n + 1)].view(np.uint64)[0])
+    out = np.zeros(max(out_bytes, 1), np.uint8)
+    status = np.zeros(max(n, 1), np.int32)
+    rc = lib().mdz_ref_rlev2_plain_decode(_ptr(blob), blob.size, chunk, _ptr(out), out_bytes, _ptr(status),
+                                          max(n, 0), threads or default_threads())
+    return rc, out[:out_bytes], status[:max(n, 0)]
 
 def brle_chunk_decode(stream: bytes, cap: int) -> tuple[int, bytes]:
     src = np.frombuffer(stream, np.uint8).copy() if stream else np.zeros(1, np.uint8)

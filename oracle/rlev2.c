@@ -234,3 +234,166 @@ int mdz_ref_rlev2_decode(const uint8_t* payload, const uint64_t* blk_off, const 
         if (status[c] != MDZ_ST_OK) return status[c];
     return 0;
 }
+
+/* ------------------------------------------------------ plain format -- *
+ * block_decode (include/rlev2/decoder.h:190-222) and its four run decoders
+ * (:28-188): one run stream per chunk of int64 values, read until the
+ * chunk's bytes are consumed.  Each run is checked in this order before any
+ * of its values is written: header bytes present (input overrun), DELTA
+ * length >= 2 and varints of at most 10 bytes (malformed symbol), values fit
+ * the chunk (output overrun), data and patch list present (input overrun),
+ * patch positions inside the run (malformed symbol).  The reference checks
+ * none of these.  Three places follow the intended values rather than the
+ * reference's arithmetic (the plain encoder never relies on them):
+ *   - PATCHED_BASE patches are applied with a full pw-bit mask; decoder.h:176
+ *     builds it in 16 bits, which drops patch bits at positions >= 16;
+ *   - a packed-delta run writes its len values only; decoder.h:117-124 runs
+ *     its prefix loop over len entries from the third value on, so it
+ *     rewrites the two values after the run (later runs overwrite them; after
+ *     a chunk's last run they land in the next chunk);
+ *   - a patch entry shifted past 63 bits contributes nothing.
+ * DELTA runs ascend when the zigzag delta base is positive and descend
+ * otherwise (decoder.h:115), with absolute packed deltas. */
+static int rd_plain_varint(rd_t* r, uint64_t* out) {
+    uint64_t v = 0;
+    for (uint32_t i = 0;; ++i) {
+        uint8_t b;
+        if (i == 10) return MDZ_ST_BAD_SYMBOL;
+        if (r->p >= r->len) return MDZ_ST_INPUT_OVERRUN;
+        b = r->in[r->p++];
+        v |= (uint64_t)(b & 0x7f) << (7 * i);           /* a 10th byte keeps bit 0 */
+        if (!(b & 0x80)) break;
+    }
+    *out = v;
+    return MDZ_ST_OK;
+}
+
+int mdz_ref_rlev2_plain_chunk_decode(const uint8_t* in, uint64_t in_len, uint64_t* vals, uint64_t cap,
+                                     uint64_t* n_out) {
+    rd_t r = {in, in_len, 0};
+    uint64_t k = 0;
+    int st = MDZ_ST_OK;
+    while (r.p < in_len) {
+        const uint64_t p0 = r.p;
+        const uint8_t first = in[r.p++];
+        const uint32_t kind = first >> 6;
+        if (kind == 0) {                                   /* SHORT_REPEAT :28-40 */
+            const uint32_t nb = ((first >> 3) & 7) + 1, cnt = (first & 7) + MIN_REPEAT;
+            if (p0 + 1 + nb > in_len) { st = MDZ_ST_INPUT_OVERRUN; break; }
+            if (cnt > cap - k) { st = MDZ_ST_OUTPUT_OVERRUN; break; }
+            uint64_t v = 0;
+            for (uint32_t i = 0; i < nb; ++i) v = (v << 8) | in[r.p++];
+            for (uint32_t i = 0; i < cnt; ++i) vals[k++] = v;
+            continue;
+        }
+        if (p0 + 2 > in_len) { st = MDZ_ST_INPUT_OVERRUN; break; }
+        const uint32_t w = DEC_W[(first >> 1) & 0x1f], code = (first >> 1) & 0x1f;
+        const uint32_t len = (((uint32_t)first & 1) << 8 | in[r.p++]) + 1;
+        if (kind == 1) {                                   /* DIRECT :134-141 */
+            if (len > cap - k) { st = MDZ_ST_OUTPUT_OVERRUN; break; }
+            if (r.p + ((uint64_t)len * w + 7) / 8 > in_len) { st = MDZ_ST_INPUT_OVERRUN; break; }
+            bits_t bs = {0, 0};
+            for (uint32_t i = 0; i < len; ++i) {
+                uint64_t v;
+                rd_bits(&r, &bs, w, &v);
+                vals[k++] = v;
+            }
+        } else if (kind == 3) {                            /* DELTA :94-132 */
+            if (len < 2) { st = MDZ_ST_BAD_SYMBOL; break; }
+            uint64_t base, zz;
+            if ((st = rd_plain_varint(&r, &base))) break;
+            if ((st = rd_plain_varint(&r, &zz))) break;
+            if (len > cap - k) { st = MDZ_ST_OUTPUT_OVERRUN; break; }
+            const uint64_t nd = code ? len - 2 : 0;
+            if (r.p + (nd * w + 7) / 8 > in_len) { st = MDZ_ST_INPUT_OVERRUN; break; }
+            const uint64_t bd = (zz >> 1) ^ (0 - (zz & 1));
+            uint64_t cur = base;
+            vals[k++] = cur;
+            cur += bd;
+            vals[k++] = cur;
+            const int up = (int64_t)bd > 0;
+            bits_t bs = {0, 0};
+            for (uint32_t i = 2; i < len; ++i) {
+                if (code) {
+                    uint64_t d;
+                    rd_bits(&r, &bs, w, &d);
+                    cur = up ? cur + d : cur - d;
+                } else {
+                    cur += bd;
+                }
+                vals[k++] = cur;
+            }
+        } else {                                           /* PATCHED_BASE :143-188 */
+            if (p0 + 4 > in_len) { st = MDZ_ST_INPUT_OVERRUN; break; }
+            const uint8_t third = in[r.p++], fourth = in[r.p++];
+            const uint32_t bw = ((third >> 5) & 7) + 1, pw = DEC_W[third & 0x1f];
+            const uint32_t pgw = ((fourth >> 5) & 7) + 1, pll = fourth & 0x1f;
+            const uint32_t cfb = closest_bit(pw + pgw);
+            if (len > cap - k) { st = MDZ_ST_OUTPUT_OVERRUN; break; }
+            const uint64_t need = bw + ((uint64_t)len * w + 7) / 8 + ((uint64_t)pll * cfb + 7) / 8;
+            if (r.p + need > in_len) { st = MDZ_ST_INPUT_OVERRUN; break; }
+            uint64_t base = 0;
+            for (uint32_t i = 0; i < bw; ++i) base = (base << 8) | in[r.p++];
+            uint64_t tmp[512];
+            bits_t bs = {0, 0};
+            for (uint32_t i = 0; i < len; ++i) rd_bits(&r, &bs, w, &tmp[i]);
+            bits_t ps = {0, 0};
+            uint64_t gap = 0;
+            const uint64_t pmask = pw >= 64 ? ~0ull : ((1ull << pw) - 1);
+            for (uint32_t i = 0; i < pll && st == MDZ_ST_OK; ++i) {
+                uint64_t e;
+                rd_bits(&r, &ps, cfb, &e);
+                gap += pw >= 64 ? 0 : e >> pw;
+                if (gap >= len) { st = MDZ_ST_BAD_SYMBOL; break; }
+                tmp[gap] |= w >= 64 ? 0 : (e & pmask) << w;
+            }
+            if (st) break;
+            for (uint32_t i = 0; i < len; ++i) vals[k++] = tmp[i] + base;
+        }
+    }
+    *n_out = k;
+    return st;
+}
+
+typedef struct {
+    const uint8_t* data; const uint64_t* ptr; uint64_t data_bytes, chunk;
+    uint8_t* out; uint64_t out_bytes; int32_t* status;
+} plain_dec_ctx;
+
+static void plain_dec_task(void* vctx, uint64_t c) {
+    plain_dec_ctx* x = (plain_dec_ctx*)vctx;
+    const uint64_t a = x->ptr[c], b = x->ptr[c + 1];
+    if (a > b || b > x->data_bytes) { x->status[c] = MDZ_ST_BAD_HEADER; return; }
+    const uint64_t cap = x->chunk / 8;
+    uint64_t* v = (uint64_t*)malloc((cap ? cap : 1) * 8);
+    uint64_t n = 0;
+    x->status[c] = mdz_ref_rlev2_plain_chunk_decode(x->data + a, b - a, v, cap, &n);
+    for (uint64_t i = 0; i < n; ++i) {
+        const uint64_t o = c * x->chunk + 8 * i;
+        if (o + 8 <= x->out_bytes) memcpy(x->out + o, &v[i], 8);
+        else if (o < x->out_bytes) memcpy(x->out + o, &v[i], (size_t)(x->out_bytes - o));
+    }
+    free(v);
+}
+
+/* decompress_gpu (decoder.h:227-262) over a container in memory:
+ * [u32 n+1][u64 ptr[n+1]][u64 raw_bytes][streams]; chunk c's values go to
+ * byte c*chunk of out (out_bytes = raw_bytes normally).  Returns -1 for a
+ * container whose header does not fit, else the first failing chunk's
+ * status (0 when all are good). */
+int mdz_ref_rlev2_plain_decode(const uint8_t* in, uint64_t in_bytes, uint64_t chunk, uint8_t* out,
+                               uint64_t out_bytes, int32_t* status, uint64_t n_status, int threads) {
+    if (in_bytes < 4 || chunk == 0 || chunk % 8) return -1;
+    uint32_t np;
+    memcpy(&np, in, 4);
+    if (np == 0 || 4 + 8 * (uint64_t)np + 8 > in_bytes || (uint64_t)np - 1 > n_status) return -1;
+    const uint64_t n = np - 1, hdr = 4 + 8 * (uint64_t)np + 8;
+    uint64_t* ptr = (uint64_t*)malloc(8 * (uint64_t)np);
+    memcpy(ptr, in + 4, 8 * (uint64_t)np);
+    plain_dec_ctx x = {in + hdr, ptr, in_bytes - hdr, chunk, out, out_bytes, status};
+    mdz_parallel_for(n, threads, plain_dec_task, &x);
+    free(ptr);
+    for (uint64_t c = 0; c < n; ++c)
+        if (status[c] != MDZ_ST_OK) return status[c];
+    return 0;
+}
