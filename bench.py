@@ -166,6 +166,53 @@ class RleV2Workload:
                           f"oracle C decoder, median of >=3 runs"}
 
 
+class RleV2PlainWorkload:
+    """RLE v2, plain per-chunk format (include/rlev2/decoder.h:190-262): the
+    1 GiB int64 column of the RLE v2 workload with its values masked to
+    [0, 2^63) (the plain format's encoder and decoder agree only on those),
+    one run stream per 32 KiB chunk (the reference's CHUNK_SIZE); one wave
+    per chunk, four per 256-thread workgroup."""
+    name = "rlev2_plain_i64_1GiB_32KiB"
+    dtype = "int64"
+
+    def __init__(self, n_vals=1 << 27, chunk=32768, seed=0x4D445A02, **_):
+        from compression_amd import host as H
+        import compression_amd as mdz
+        self.mdz = mdz
+        self.data = H.gen_rlev2_i64(n_vals, seed).view(np.int64) & np.int64(0x7FFFFFFFFFFFFFFF)
+        self.c = c = H.rlev2_plain_encode(self.data, chunk)
+        self.chunk = chunk
+        self.d_data = mdz.DeviceBuffer.from_array(c.data, pad=16)
+        self.d_ptr = mdz.DeviceBuffer.from_array(c.ptr)
+        self.dst = mdz.DeviceBuffer(c.raw_bytes)
+        self.status = mdz.DeviceBuffer(4 * c.n_chunks)
+        self.out_bytes = c.raw_bytes
+        nc = c.n_chunks
+        self.alg_bytes = int(c.data.size) + 8 * (nc + 1) + c.raw_bytes
+        self.config = {"workload": self.name, "codec": "rle_v2", "format": "plain", "elem": "int64",
+                       "out_bytes": c.raw_bytes, "payload_bytes": int(c.data.size), "n_chunks": nc,
+                       "chunk_bytes": chunk, "ratio": round(c.data.size / c.raw_bytes, 4)}
+
+    def step(self, stream):
+        from compression_amd import rlev2
+        rlev2.decompress_plain(self.d_data, self.c.data.size, self.d_ptr, self.c.n_chunks, self.chunk,
+                               self.dst, self.out_bytes, self.status, stream=stream)
+
+    def verify(self):
+        got = self.dst.download(self.out_bytes)
+        st = self.status.download(dtype=np.int32)
+        return bool((st == 0).all() and np.array_equal(got, self.data.view(np.uint8)))
+
+    def cpu(self, threads):
+        from oracle import pyoracle as O   # the CPU baseline leg
+        c = self.c
+        t = _cpu_time(lambda: O.rlev2_plain_decode(c.blob, c.chunk, threads=threads))
+        return {"value": round(c.raw_bytes / t / 1e9, 3), "unit": "GB/s", "cores": threads,
+                "kind": "port",
+                "sample": f"whole workload ({c.raw_bytes >> 20} MiB out, {c.n_chunks} chunks), "
+                          f"oracle C decoder (the reference's plain block_decode), median of >=3 runs"}
+
+
 class BrleWorkload:
     """Byte RLE, plain method: 256 MiB u8 stream of runs (fresh byte, length
     Geom(1/12)), 128 KiB chunks; one 256-thread workgroup per chunk."""
@@ -853,7 +900,7 @@ WORKLOADS = {"config2": Config2Workload,
              "config2-plain": lambda **kw: Config2Workload(batched=False, **kw),
              "rlev1": RleV1Workload, "rlev2": RleV2Workload,
              "brle": BrleWorkload, "lzss": LzssWorkload, "lzss-plain": LzssPlainWorkload,
-             "brle-trans": BrleTransWorkload,
+             "brle-trans": BrleTransWorkload, "rlev2-plain": RleV2PlainWorkload,
              "deflate": DeflateWorkload,
              "copy": CopyWorkload, "file": FileWorkload, "brle-encode": BrleEncodeWorkload,
              "rlev1-encode": RleV1EncodeWorkload, "rlev2-encode": RleV2EncodeWorkload,
@@ -1003,6 +1050,8 @@ def rlev1_line_sweep(args, stream, lines=(32, 128, 512)) -> list:
 ALL_CONFIGS = [("config1", "rlev1", "RLE v1, 64 MiB u32 column, plain"),
                ("config2", "config2", "RLE v2 1 GiB int64 + byte RLE 256 MiB, batched persistent"),
                ("config3", "lzss", "LZSS, 1 GiB Markov+copy, 64 KiB chunks, lane-transposed"),
+               ("config2-rlev2-plain", "rlev2-plain", "RLE v2 1 GiB int64, 32 KiB chunks, plain format"),
+               ("config2-brle-trans", "brle-trans", "byte RLE 256 MiB, 128 KiB chunks, lane-transposed"),
                ("config3-plain", "lzss-plain", "LZSS, 1 GiB Markov+copy, 64 KiB chunks, plain format"),
                ("config4", "deflate", "Deflate, 4,096 uneven zlib streams, scheduled persistent")]
 

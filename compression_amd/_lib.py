@@ -95,6 +95,9 @@ SIGNATURES = {
     "mdz_lzss_decompress_host": (C.c_int, [vp, u64, C.POINTER(vp), C.POINTER(u64), u64,
                                            C.POINTER(vp), C.POINTER(u64)]),
     "mdz_meta_split": (C.c_int, [vp, u64, C.POINTER(vp), C.POINTER(u64), C.POINTER(u64)]),
+    "mdz_rlev2_plain_decompress": (C.c_int, [vp, u64, vp, u64, u64, vp, u64, vp, vp]),
+    "mdz_rlev2_plain_decompress_host": (C.c_int, [vp, u64, u64, C.POINTER(vp), C.POINTER(u64)]),
+    "mdz_rlev2_plain_decompress_gpu": (C.c_int, [vp, u64, C.POINTER(vp), C.POINTER(u64)]),
     "mdz_inflate_batch": (C.c_int, [vp, vp, C.c_int, C.c_int, vp]),
     "mdz_inflate_batch_sched": (C.c_int, [vp, vp, C.c_int, C.c_int, vp, u32, vp, vp, vp]),
     "mdz_inflate_batch_sched_noalloc": (C.c_int, [vp, vp, C.c_int, C.c_int, vp, u32, vp, vp]),

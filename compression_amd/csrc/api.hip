@@ -485,6 +485,55 @@ int mdz_lzss_plain_decompress(const void* data, uint64_t data_bytes, const uint6
                                       S(stream));
 }
 
+int mdz_rlev2_plain_decompress(const void* data, uint64_t data_bytes, const uint64_t* ptr, uint64_t n_chunks,
+                               uint64_t chunk_bytes, void* dst, uint64_t dst_bytes, int32_t* status,
+                               void* stream) {
+    return mdz::rlev2_plain_decompress(data, data_bytes, ptr, n_chunks, chunk_bytes, dst, dst_bytes, status,
+                                       S(stream));
+}
+
+int mdz_rlev2_plain_decompress_host(const uint8_t* in, uint64_t in_n_bytes, uint64_t chunk_bytes, uint8_t** out,
+                                    uint64_t* out_n_bytes) {
+    if (!in || !out || !out_n_bytes || in_n_bytes < 4 || chunk_bytes == 0 || chunk_bytes % 8)
+        return MDZ_ERR_INVALID;
+    uint32_t n_ptr;
+    memcpy(&n_ptr, in, 4);
+    if (n_ptr < 2) return MDZ_ERR_INVALID;
+    const uint64_t n = n_ptr - 1, hdr = 4 + 8ull * n_ptr;
+    if (in_n_bytes < hdr + 8) return MDZ_ERR_INVALID;
+    uint64_t* ptr = (uint64_t*)malloc(8ull * n_ptr);
+    if (!ptr) return MDZ_ERR_NOMEM;
+    memcpy(ptr, in + 4, 8ull * n_ptr);
+    uint64_t raw;
+    memcpy(&raw, in + hdr, 8);
+    const uint8_t* data = in + hdr + 8;
+    const uint64_t data_bytes = in_n_bytes - hdr - 8;
+    int rc = MDZ_OK;
+    for (uint64_t c = 0; c < n && rc == MDZ_OK; ++c)
+        if (ptr[c + 1] < ptr[c] || ptr[c + 1] > data_bytes) rc = MDZ_ERR_INVALID;
+    if (rc == MDZ_OK && raw > n * chunk_bytes) rc = MDZ_ERR_INVALID;
+    DevBuf d_ptr, d_data, d_out, d_st;
+    if (rc == MDZ_OK &&
+        ((rc = d_ptr.alloc(8ull * n_ptr)) || (rc = d_data.alloc(data_bytes + 16)) ||
+         (rc = d_out.alloc(raw + 8)) || (rc = d_st.alloc(4 * n)))) {
+    }
+    if (rc == MDZ_OK &&
+        (hipMemcpy(d_ptr.p, ptr, 8ull * n_ptr, hipMemcpyHostToDevice) != hipSuccess ||
+         hipMemcpy(d_data.p, data, data_bytes, hipMemcpyHostToDevice) != hipSuccess))
+        rc = MDZ_ERR_HIP;
+    free(ptr);
+    if (rc) return rc;
+    rc = mdz::rlev2_plain_decompress(d_data.p, data_bytes, (const uint64_t*)d_ptr.p, n, chunk_bytes, d_out.p, raw,
+                                     (int32_t*)d_st.p, nullptr);
+    if (rc) return rc;
+    return finish_host(d_out.p, raw, (const int32_t*)d_st.p, n, out, out_n_bytes);
+}
+
+int mdz_rlev2_plain_decompress_gpu(const uint8_t* in, uint64_t in_n_bytes, int64_t** out, uint64_t* out_n_bytes) {
+    return mdz_rlev2_plain_decompress_host(in, in_n_bytes, 32 * 1024, reinterpret_cast<uint8_t**>(out),
+                                           out_n_bytes);
+}
+
 int mdz_lzss_compress_bound(uint64_t in_bytes, uint64_t chunk_bytes, uint64_t* n_chunks, uint64_t* work_bytes,
                             uint64_t* payload_cap) {
     return mdz::lzss_compress_bound(in_bytes, chunk_bytes, n_chunks, work_bytes, payload_cap);

@@ -76,5 +76,7 @@ int brle_trans_decompress(const mdz_lanes_t* in, void* dst, uint64_t dst_bytes, 
                           hipStream_t s);
 int lzss_plain_decompress(const void* data, uint64_t data_bytes, const uint64_t* lens, uint64_t n_chunks,
                           uint64_t chunk, void* dst, uint64_t dst_bytes, int32_t* status, hipStream_t s);
+int rlev2_plain_decompress(const void* data, uint64_t data_bytes, const uint64_t* ptr, uint64_t n_chunks,
+                           uint64_t chunk, void* dst, uint64_t dst_bytes, int32_t* status, hipStream_t s);
 
 }  // namespace mdz

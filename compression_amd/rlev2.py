@@ -3,7 +3,9 @@
 ``decompress_gpu`` keeps the shape of ``rlev2::decompress_gpu``
 (include/rlev2/decoder_trans.h:615-681): payload, n_chunks, blk_off (whose
 last entry is the uncompressed size), col_len -> host bytes.  ``decompress``
-is the device-resident, stream-ordered form.
+is the device-resident, stream-ordered form.  ``decompress_plain`` and
+``decompress_plain_gpu`` are the same two forms for the plain format (one
+run stream per chunk, include/rlev2/decoder.h:190-262).
 """
 from __future__ import annotations
 
@@ -53,6 +55,34 @@ def decompress_gpu(payload: bytes, n_chunks: int, blk_off: np.ndarray, col_len: 
     lib().mdz_host_free(out)
     if rc != 0:
         raise MdzError(rc, "mdz_rlev2_decompress_host (corrupt chunk)")
+    return data
+
+
+def decompress_plain(data, data_bytes: int, ptr, n_chunks: int, chunk_bytes: int, dst, dst_bytes: int,
+                     status=None, stream=None) -> None:
+    """Enqueue the plain-format decode (one run stream per chunk of int64
+    values, include/rlev2/decoder.h:190-226): chunk c's stream is
+    ``data[ptr[c]:ptr[c+1]]`` (device pointers; ``ptr`` the container's
+    offset table) and decodes to ``dst[c*chunk_bytes:]``, clipped to
+    ``dst_bytes``."""
+    check(lib().mdz_rlev2_plain_decompress(as_ptr(data), int(data_bytes), as_ptr(ptr), int(n_chunks),
+                                           int(chunk_bytes), as_ptr(dst), int(dst_bytes), as_ptr(status),
+                                           as_ptr(stream)), "mdz_rlev2_plain_decompress")
+
+
+def decompress_plain_gpu(container: bytes, chunk_size: int = 32 * 1024) -> bytes:
+    """``rlev2::decompress_gpu`` of the plain container (decoder.h:227-262):
+    [u32 n+1][u64 ptr[n+1]][u64 raw_bytes][streams] -> raw bytes."""
+    src = np.frombuffer(container, np.uint8)
+    out = C.c_void_p()
+    n = C.c_uint64()
+    rc = lib().mdz_rlev2_plain_decompress_host(src.ctypes.data, src.size, chunk_size, C.byref(out), C.byref(n))
+    if rc not in (0, -5):
+        raise MdzError(rc, "mdz_rlev2_plain_decompress_host")
+    data = C.string_at(out.value, n.value) if n.value else b""
+    lib().mdz_host_free(out)
+    if rc != 0:
+        raise MdzError(rc, "mdz_rlev2_plain_decompress_host (corrupt chunk)")
     return data
 
 

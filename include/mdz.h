@@ -188,6 +188,31 @@ int mdz_rlev2_decompress_host(const uint8_t* in, uint64_t in_n_bytes, uint64_t n
                               uint64_t* out_n_bytes, uint64_t chunk_size, uint32_t read_unit,
                               uint32_t elem_bytes);
 
+/* Plain RLE v2 decode -- replaces rlev2::kernel_decompress / block_decode
+ * (reference include/rlev2/decoder.h:190-226): one run stream per chunk of
+ * int64 values.  Chunk c's stream is data[ptr[c], ptr[c+1]) (device u64
+ * ptr[n_chunks+1], the container's offset table) and decodes to
+ * dst[c*chunk_bytes, ...) clipped to dst_bytes (8-byte aligned).  A chunk
+ * ends where its stream does.  status[c] (optional) is the first failing
+ * run's problem: INPUT_OVERRUN (the stream ends inside a run),
+ * OUTPUT_OVERRUN (more than chunk_bytes/8 values), BAD_SYMBOL (DELTA length
+ * 1, a varint over 10 bytes, a patch past its run) or BAD_HEADER (ptr out of
+ * order or past data_bytes); values of the runs before it are written.
+ * PATCHED_BASE uses the full patch width as its mask (decoder.h:176 builds
+ * it in 16 bits), and a packed-delta run writes only its own values
+ * (decoder.h:117-124 rewrites the two after it). */
+int mdz_rlev2_plain_decompress(const void* data, uint64_t data_bytes, const uint64_t* ptr, uint64_t n_chunks,
+                               uint64_t chunk_bytes, void* dst, uint64_t dst_bytes, int32_t* status,
+                               void* stream);
+/* rlev2::decompress_gpu for the plain container (decoder.h:227-262):
+ * in = [u32 n+1][u64 ptr[n+1]][u64 raw_bytes][streams]; *out (raw_bytes) is
+ * allocated here, free with mdz_host_free.  chunk_bytes is the reference's
+ * compile-time CHUNK_SIZE (32 KiB on the HEAD side of utils.h:62-66, which
+ * mdz_rlev2_plain_decompress_gpu uses).  MDZ_ERR_CORRUPT if a chunk fails. */
+int mdz_rlev2_plain_decompress_host(const uint8_t* in, uint64_t in_n_bytes, uint64_t chunk_bytes, uint8_t** out,
+                                    uint64_t* out_n_bytes);
+int mdz_rlev2_plain_decompress_gpu(const uint8_t* in, uint64_t in_n_bytes, int64_t** out, uint64_t* out_n_bytes);
+
 /* RLE v2 encode -- replaces rlev2::compress_gpu_transpose (reference
  * include/rlev2/encoder_trans.h:428-537; block_encode :14-289; writers
  * include/rlev2/encoder.h:124-785).  payload, col_len and blk_off equal that
