@@ -47,6 +47,7 @@ TRAFFIC: dict = {
     "lzss_markov_1GiB_64KiB": 1861173555,              # profiles/lzss_markov_1GiB_64KiB.md
     "lzss_plain_markov_1GiB_64KiB": 1199976756,        # profiles/lzss_plain_markov_1GiB_64KiB.md
     "brle_trans_u8_256MiB": 436797235,                 # profiles/brle_trans_u8_256MiB.md
+    "rlev2_plain_i64_1GiB_32KiB": 1616674816,          # profiles/rlev2_plain_i64_1GiB_32KiB.md
     "deflate_4096_streams_uneven": 2818533785,         # profiles/inflate_config4_4096_streams.md
 }
 

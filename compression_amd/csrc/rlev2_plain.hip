@@ -6,13 +6,17 @@
 // [u64 raw_bytes][streams]; chunk c's stream is data[ptr[c], ptr[c+1]) and
 // its values are the int64s at byte c * chunk of the output.
 //
-// One wave per chunk, four chunks per 256-thread workgroup:
-//   * the wave stages its stream in 8 KiB windows of LDS with 16-byte loads
+// One 64-thread workgroup (one wave) per chunk:
+//   * the wave stages its stream in 5 KiB windows of LDS with 16-byte loads
 //     (a window is restaged at the run that would cross its end; a run is at
-//     most 4.4 KB);
+//     most 4.4 KB), small enough for seven waves per SIMD: the walk is bound
+//     by LDS and memory latency, which more resident chunks hide (8 KiB
+//     windows and four waves per SIMD took 1.15 ms on the 1 GiB column,
+//     5 KiB and seven 0.84 ms; one chunk per workgroup frees each chunk's
+//     LDS as soon as it ends);
 //   * the header walk is wave-uniform: every lane reads the same header
-//     bytes (an LDS broadcast) and the run's fields live in scalar
-//     registers;
+//     bytes (an LDS broadcast), which readfirstlane moves to scalar
+//     registers, so the run's fields and the control flow are scalar;
 //   * each run is checked as the oracle's mdz_ref_rlev2_plain_chunk_decode
 //     checks it, in the same order, before any of its values is written; the
 //     first failing run ends the chunk with its status;
@@ -33,8 +37,8 @@
 namespace mdz {
 namespace {
 
-constexpr uint32_t kPlainWaves = 4;              // chunks per workgroup
-constexpr uint32_t kPlainWin = 8192;             // staged stream bytes per wave
+constexpr uint32_t kPlainWaves = 1;              // chunks (waves) per workgroup
+constexpr uint32_t kPlainWin = 5120;             // staged stream bytes per wave (a multiple of 1 KiB)
 constexpr uint32_t kPlainSlack = 64;             // room for the field readers' over-reads
 constexpr uint32_t kPlainHdr = 40;               // bytes a header may span (two varints)
 
@@ -42,6 +46,10 @@ struct PlainV2Lds {
     uint8_t win[kPlainWaves][kPlainWin + kPlainSlack];
 };
 static_assert((kPlainWin + kPlainSlack) % 16 == 0, "16-byte aligned windows");
+static_assert(kPlainWin % (16 * kWave) == 0, "whole 16-byte loads per lane");
+// the longest run (PATCHED_BASE: 4 header + 8 base bytes, 512 64-bit values,
+// 31 64-bit patch entries) fits a window staged 15 bytes before it
+static_assert(kPlainWin >= 15 + 4 + 8 + 512 * 8 + 31 * 8, "a run fits a window");
 
 // Copies the 16-byte-aligned absolute range [gb, gb + kPlainWin) of the
 // stream buffer [lo, hi) into the wave's window (bytes outside it read 0).
@@ -77,7 +85,7 @@ MDZ_DEV void plain_put(uint8_t* out, uint64_t limit, uint32_t q, uint64_t v) {
     }
 }
 
-__global__ __launch_bounds__(kPlainWaves * kWave) void rlev2_plain_kernel(
+__global__ __launch_bounds__(kPlainWaves * kWave) __attribute__((amdgpu_waves_per_eu(7))) void rlev2_plain_kernel(
     const uint8_t* __restrict__ data, uint64_t data_bytes, const uint64_t* __restrict__ ptr, uint64_t n_chunks,
     uint64_t chunk, uint8_t* __restrict__ dst, uint64_t dst_bytes, int32_t* __restrict__ status) {
     __shared__ __align__(16) PlainV2Lds S;
@@ -115,7 +123,9 @@ __global__ __launch_bounds__(kPlainWaves * kWave) void rlev2_plain_kernel(
         const uint32_t avail = slen - p;
         if (!covers(p, min(kPlainHdr, avail))) restage(p);
         uint32_t lp = (uint32_t)(s0 + p - gb);                // p's LDS offset
-        const uint32_t first = win[lp];
+        // header bytes are wave-uniform: keep them (and all that derives
+        // from them) in scalar registers
+        const uint32_t first = __builtin_amdgcn_readfirstlane(win[lp]);
         const uint32_t kind = first >> 6;
         uint32_t n = 0, size = 0, w = 0, code = 0;
         uint32_t nb = 0, n1 = 0, n2 = 0, bw = 0, pw = 0, pll = 0, cfb = 0;
@@ -131,7 +141,7 @@ __global__ __launch_bounds__(kPlainWaves * kWave) void rlev2_plain_kernel(
         } else {
             code = (first >> 1) & 31;
             w = decode_width(code);
-            n = (((first & 1) << 8) | win[lp + 1]) + 1;
+            n = (((first & 1) << 8) | __builtin_amdgcn_readfirstlane(win[lp + 1])) + 1;
             if (kind == 1) {                                // DIRECT
                 size = 2 + (n * w + 7) / 8;
                 if (n > cap - k) st = MDZ_ST_OUTPUT_OVERRUN;
@@ -141,10 +151,10 @@ __global__ __launch_bounds__(kPlainWaves * kWave) void rlev2_plain_kernel(
                 int32_t e = MDZ_ST_OK;
                 if (n < 2) {
                     st = MDZ_ST_BAD_SYMBOL;
-                } else if (!(n1 = uvarint(win, lp + 2, hend, va, e))) {
-                    st = e;
-                } else if (!(n2 = uvarint(win, lp + 2 + n1, hend, vb, e))) {
-                    st = e;
+                } else if (!(n1 = __builtin_amdgcn_readfirstlane(uvarint(win, lp + 2, hend, va, e)))) {
+                    st = __builtin_amdgcn_readfirstlane(e);
+                } else if (!(n2 = __builtin_amdgcn_readfirstlane(uvarint(win, lp + 2 + n1, hend, vb, e)))) {
+                    st = __builtin_amdgcn_readfirstlane(e);
                 } else {
                     const uint32_t nd = code ? n - 2 : 0;
                     size = 2 + n1 + n2 + (nd * w + 7) / 8;
@@ -155,7 +165,8 @@ __global__ __launch_bounds__(kPlainWaves * kWave) void rlev2_plain_kernel(
                 if (avail < 4) {
                     st = MDZ_ST_INPUT_OVERRUN;
                 } else {
-                    const uint32_t third = win[lp + 2], fourth = win[lp + 3];
+                    const uint32_t third = __builtin_amdgcn_readfirstlane(win[lp + 2]),
+                                   fourth = __builtin_amdgcn_readfirstlane(win[lp + 3]);
                     bw = ((third >> 5) & 7) + 1;
                     pw = decode_width(third & 31);
                     pll = fourth & 31;
