@@ -29,6 +29,14 @@ import statistics
 import sys
 import time
 
+# HIP maps streams onto GPU_MAX_HW_QUEUES hardware queues (4 by default).
+# Serving keeps a persistent inflate kernel running on its own stream (the
+# live queue); a stream that shared its hardware queue would wait behind it.
+# The workloads before it leave a few streams alive, so the process asks for
+# more queues before anything starts HIP.
+if int(os.environ.get("GPU_MAX_HW_QUEUES", "0") or 0) < 16:
+    os.environ["GPU_MAX_HW_QUEUES"] = "16"
+
 import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))

@@ -129,6 +129,7 @@ class Server:
         self.deflate_mode = deflate_mode
         self.live_blocks = int(live_blocks)
         self.live_idle_ms = int(live_idle_ms)
+        self.stall_s = 0.05          # upload wait that ends an idle live kernel
         self.live = None
         self.budget = int(hbm_budget)
         self.in_use = 0
@@ -169,6 +170,7 @@ class Server:
         check(fn(C.byref(L), req.out.ptr, req.out_bytes, req.status.ptr, s.ptr), req.codec)
         req.done_event = Event()
         req.done_event.record(s)
+        req.live["t_launch"] = time.perf_counter()
 
     def _launch_deflate(self, reqs: list[Request]) -> None:
         """All pending Deflate requests' chunk streams in one scheduled launch."""
@@ -219,6 +221,7 @@ class Server:
         d["dst"] = np.uint64(r.out.ptr) + d0
         d["dst_bytes"] = np.minimum(np.uint64(m["chunk"]), np.uint64(r.out_bytes) - d0)
         r.live["desc"] = d[np.lexsort((np.arange(k), -m["src_len"].astype(np.int64)))]   # longest first
+        r.live["staged_at"] = time.perf_counter()
         r.done_event = Event()
         r.done_event.record(s)
 
@@ -328,6 +331,15 @@ class Server:
                 else:
                     still.append(r)
             staged = still
+            # work that does not finish while the live kernel has nothing to do
+            # may sit behind it on a shared hardware queue (more streams than
+            # GPU_MAX_HW_QUEUES): end the idle kernel; the next publish starts
+            # a new one
+            if self.live is not None and not self.live_reqs and (staged or inflight):
+                now_abs = time.perf_counter()
+                since = [r.live["staged_at"] for r in staged] + [r.live.get("t_launch", now_abs) for r in inflight]
+                if now_abs - min(since) > self.stall_s:
+                    self._retire_live()
             # completions: verify (optional), then release budget and buffers
             finished, still = [], []
             for r in inflight:

@@ -3,6 +3,12 @@ import sys
 
 import pytest
 
+# the live inflate queue's persistent kernel needs a hardware queue of its
+# own: HIP maps streams onto GPU_MAX_HW_QUEUES of them (4 by default), and
+# the test session creates more streams than that; set before HIP starts
+if int(os.environ.get("GPU_MAX_HW_QUEUES", "0") or 0) < 16:
+    os.environ["GPU_MAX_HW_QUEUES"] = "16"
+
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
