@@ -60,6 +60,8 @@ SIGNATURES = {
     "mdz_device_cus": (C.c_int, [C.POINTER(C.c_int)]),
     "mdz_dev_malloc": (C.c_int, [C.POINTER(vp), u64]),
     "mdz_dev_free": (C.c_int, [vp]),
+    "mdz_pinned_alloc": (C.c_int, [C.POINTER(vp), u64]),
+    "mdz_pinned_free": (C.c_int, [vp]),
     "mdz_memcpy_htod": (C.c_int, [vp, vp, u64, vp]),
     "mdz_memcpy_dtoh": (C.c_int, [vp, vp, u64, vp]),
     "mdz_memset_d8": (C.c_int, [vp, C.c_int, u64, vp]),
@@ -215,6 +217,30 @@ class DeviceBuffer:
             self.free()
         except Exception:
             pass
+
+
+class _PinnedOwner:
+    def __init__(self, ptr):
+        self.ptr = ptr
+
+    def __del__(self):
+        try:
+            lib().mdz_pinned_free(self.ptr)
+        except Exception:
+            pass
+
+
+def pinned_copy(a: np.ndarray) -> np.ndarray:
+    """A copy of `a` in pinned host memory (freed with the returned array):
+    host-to-device copies from it are asynchronous DMA."""
+    a = np.ascontiguousarray(a)
+    p = vp()
+    check(lib().mdz_pinned_alloc(C.byref(p), a.nbytes), "mdz_pinned_alloc")
+    raw = (C.c_uint8 * max(a.nbytes, 1)).from_address(p.value)
+    raw.owner = _PinnedOwner(p.value)          # frees the memory with the last view
+    out = np.frombuffer(raw, np.uint8, count=a.nbytes).view(a.dtype).reshape(a.shape)
+    out[...] = a
+    return out
 
 
 class Stream:

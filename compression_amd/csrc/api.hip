@@ -119,6 +119,10 @@ int mdz_dev_malloc(void** ptr, uint64_t bytes) {
     return hipMalloc(ptr, bytes ? bytes : 1) == hipSuccess ? MDZ_OK : MDZ_ERR_NOMEM;
 }
 int mdz_dev_free(void* ptr) { return herr(hipFree(ptr)); }
+int mdz_pinned_alloc(void** ptr, uint64_t bytes) {
+    return hipHostMalloc(ptr, bytes ? bytes : 1, hipHostMallocDefault) == hipSuccess ? MDZ_OK : MDZ_ERR_NOMEM;
+}
+int mdz_pinned_free(void* ptr) { return herr(hipHostFree(ptr)); }
 int mdz_memcpy_htod(void* dst, const void* src, uint64_t bytes, void* stream) {
     return herr(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, S(stream)));
 }

@@ -29,7 +29,7 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from . import deflate as _deflate
-from ._lib import DeviceBuffer, Event, LanesT, Stream, check, lib
+from ._lib import DeviceBuffer, Event, LanesT, Stream, check, lib, pinned_copy
 
 CODECS = ("rlev1", "rlev2", "lzss", "deflate")
 
@@ -123,12 +123,14 @@ def make_request(rid: int, codec: str, size: int, seed: int, arrival: float = 0.
 
 class Server:
     def __init__(self, hbm_budget: int = 64 << 30, n_streams: int = 4, verify: bool = False,
-                 deflate_mode: str = "live", live_blocks: int = 0, live_idle_ms: int = 0):
+                 deflate_mode: str = "live", live_blocks: int = 0, live_idle_ms: int = 0,
+                 pinned: bool = True):
         if deflate_mode not in ("live", "launch"):
             raise ValueError(deflate_mode)
         self.deflate_mode = deflate_mode
         self.live_blocks = int(live_blocks)
         self.live_idle_ms = int(live_idle_ms)
+        self.pinned = pinned
         self.stall_s = 0.05          # upload wait that ends an idle live kernel
         self.live = None
         self.budget = int(hbm_budget)
@@ -212,7 +214,8 @@ class Server:
         src = self._up(r, r.payload, s)
         r.out = self.pool.get(r.out_bytes)
         m = r.meta
-        r.live.clear()
+        r.live.pop("final", None)
+        r.live.pop("q", None)
         k = len(m["src_off"])
         d = np.zeros(k, _deflate.DESC)
         d0 = np.arange(k, dtype=np.uint64) * np.uint64(m["chunk"])
@@ -275,6 +278,13 @@ class Server:
         self.outputs: dict[int, np.ndarray] = {}
         self.all_ok = True
         live = self.deflate_mode == "live"
+        if self.pinned:
+            # requests arrive in pinned receive buffers: their uploads are DMA
+            # copies that return at once (moved before the run starts)
+            for r in requests:
+                if not r.live.get("pinned"):
+                    r.payload = pinned_copy(r.payload)
+                    r.live["pinned"] = True
         self.live_cap = sum(len(r.meta["src_off"]) for r in requests if r.codec == "deflate")
         self.live_reqs: list[Request] = []      # published to the live queue, not yet finished
         staged: list[Request] = []              # uploading, not yet published

@@ -60,6 +60,10 @@ const char* mdz_last_error(void);
 int mdz_device_count(int* count);
 int mdz_dev_malloc(void** ptr, uint64_t bytes);
 int mdz_dev_free(void* ptr);
+/* Pinned (page-locked) host memory: copies from it to the device are DMA
+ * transfers that return at once (hipHostMalloc / hipHostFree). */
+int mdz_pinned_alloc(void** ptr, uint64_t bytes);
+int mdz_pinned_free(void* ptr);
 int mdz_memcpy_htod(void* dst, const void* src, uint64_t bytes, void* stream);
 int mdz_memcpy_dtoh(void* dst, const void* src, uint64_t bytes, void* stream);
 int mdz_memset_d8(void* dst, int value, uint64_t bytes, void* stream);
