@@ -1148,8 +1148,9 @@ def serve_record(args) -> dict:
     pool = [serve.make_request(i, codecs[i], int(sizes[i]), 0x4D445A05 + i) for i in range(n)]
     total = sum(r.out_bytes for r in pool)
     mode = getattr(args, "serve_deflate", "live")
+    native = getattr(args, "serve_loop", "python") == "native" and mode == "live"
     ns = getattr(args, "serve_streams", 0) or (2 if mode == "live" else 4)
-    srv = serve.Server(hbm_budget=64 << 30, n_streams=ns, verify=False, deflate_mode=mode)
+    srv = serve.Server(hbm_budget=64 << 30, n_streams=ns, verify=False, deflate_mode=mode, native=native)
     srv.run(pool[: min(8, n)], time_limit=60)            # warm-up: allocations, code objects
     for r in pool:
         r.arrival = 0.0
@@ -1164,7 +1165,7 @@ def serve_record(args) -> dict:
         sweep.append({"load": frac, "p50_ms": round(res["p50_ms"], 3), "p99_ms": round(res["p99_ms"], 3),
                       "GBps": round(res["GBps"], 2), "completed": res["completed"]})
     # bit-exactness of the whole pool, once, with verification on
-    chk = serve.Server(hbm_budget=64 << 30, n_streams=ns, verify=True, deflate_mode=mode)
+    chk = serve.Server(hbm_budget=64 << 30, n_streams=ns, verify=True, deflate_mode=mode, native=native)
     for r in pool:
         r.arrival = 0.0
     ok = chk.run(pool, time_limit=180)["bit_exact"]
@@ -1172,7 +1173,7 @@ def serve_record(args) -> dict:
     return {"value": at80["GBps"], "unit": "GB/s", "dtype": "mixed",
             "config": {"workload": "serve_mixed_codecs", "requests": n, "out_bytes": total,
                        "codecs": {c: codecs.count(c) for c in serve.CODECS}, "hbm_budget_GiB": 64,
-                       "streams": ns, "deflate": mode},
+                       "streams": ns, "deflate": mode, "loop": "native" if native else "python"},
             "saturation_GBps": round(sat["GBps"], 2), "p50_ms_at_80": at80["p50_ms"],
             "p99_ms_at_80": at80["p99_ms"], "sweep": sweep, "bit_exact": ok}
 
@@ -1205,6 +1206,9 @@ def main():
     ap.add_argument("--serve-streams", type=int, default=0,
                     help="serving: HIP streams for the per-request launches (0: 2 with the live "
                          "inflate queue, 4 without)")
+    ap.add_argument("--serve-loop", default="python", choices=["native", "python"],
+                    help="serving: the Python loop (default: measured faster) or the library's "
+                         "engine (mdz_serve_*)")
     ap.add_argument("--serve-deflate", default="live", choices=["live", "launch"],
                     help="serving: Deflate requests join one running inflate kernel (live) "
                          "or a new scheduled launch per tick (launch)")

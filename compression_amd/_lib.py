@@ -109,6 +109,12 @@ SIGNATURES = {
     "mdz_inflate_live_statuses": (vp, [vp]),
     "mdz_inflate_live_expired": (C.c_int, [vp]),
     "mdz_inflate_live_close": (C.c_int, [vp, vp]),
+    "mdz_serve_open": (C.c_int, [u64, u32, u32, u32, C.POINTER(vp)]),
+    "mdz_serve_submit": (C.c_int, [vp, vp]),
+    "mdz_serve_poll": (C.c_int, [vp, vp, u32, C.POINTER(u32)]),
+    "mdz_serve_release": (C.c_int, [vp, u64]),
+    "mdz_serve_pending": (C.c_int, [vp, C.POINTER(u64), C.POINTER(u64)]),
+    "mdz_serve_close": (C.c_int, [vp]),
     "mdz_copy_uncompressed": (C.c_int, [vp, C.c_int, vp]),
     "mdz_rlev1_compress_bound": (C.c_int, [u64, u64, C.c_uint32, C.c_uint32, C.POINTER(u64), C.POINTER(u64),
                                            C.POINTER(u64)]),

@@ -32,6 +32,23 @@ from . import deflate as _deflate
 from ._lib import DeviceBuffer, Event, LanesT, Stream, check, lib, pinned_copy
 
 CODECS = ("rlev1", "rlev2", "lzss", "deflate")
+_NATIVE_CODEC = {"rlev1": 1, "rlev2": 2, "lzss": 3, "deflate": 4}    # MDZ_SERVE_*
+
+
+class ServeRequestT(C.Structure):
+    """mdz_serve_request_t (include/mdz.h)."""
+    _fields_ = [("codec", C.c_int), ("elem_bytes", C.c_uint32), ("lane_line", C.c_uint32),
+                ("reserved", C.c_uint32), ("payload", C.c_void_p), ("payload_bytes", C.c_uint64),
+                ("blk_off", C.c_void_p), ("col_len", C.c_void_p), ("col_map", C.c_void_p),
+                ("n_chunks", C.c_uint64), ("chunk_bytes", C.c_uint64), ("src_off", C.c_void_p),
+                ("src_len", C.c_void_p), ("n_streams", C.c_uint64), ("out_bytes", C.c_uint64),
+                ("tag", C.c_uint64)]
+
+
+class ServeDoneT(C.Structure):
+    """mdz_serve_done_t (include/mdz.h)."""
+    _fields_ = [("tag", C.c_uint64), ("status", C.c_int32), ("reserved", C.c_uint32),
+                ("d_out", C.c_void_p), ("out_bytes", C.c_uint64)]
 
 
 class Pool:
@@ -124,13 +141,15 @@ def make_request(rid: int, codec: str, size: int, seed: int, arrival: float = 0.
 class Server:
     def __init__(self, hbm_budget: int = 64 << 30, n_streams: int = 4, verify: bool = False,
                  deflate_mode: str = "live", live_blocks: int = 0, live_idle_ms: int = 0,
-                 pinned: bool = True):
+                 pinned: bool = True, native: bool = False):
         if deflate_mode not in ("live", "launch"):
             raise ValueError(deflate_mode)
         self.deflate_mode = deflate_mode
         self.live_blocks = int(live_blocks)
         self.live_idle_ms = int(live_idle_ms)
         self.pinned = pinned
+        self.native = native
+        self.n_streams = n_streams
         self.stall_s = 0.05          # upload wait that ends an idle live kernel
         self.live = None
         self.budget = int(hbm_budget)
@@ -285,6 +304,14 @@ class Server:
                 if not r.live.get("pinned"):
                     r.payload = pinned_copy(r.payload)
                     r.live["pinned"] = True
+        if self.native:
+            gc_was = gc.isenabled()
+            gc.disable()                        # no collector-driven hipFree while its live kernel runs
+            try:
+                return self._run_native(pending, time_limit)
+            finally:
+                if gc_was:
+                    gc.enable()
         self.live_cap = sum(len(r.meta["src_off"]) for r in requests if r.codec == "deflate")
         self.live_reqs: list[Request] = []      # published to the live queue, not yet finished
         staged: list[Request] = []              # uploading, not yet published
@@ -377,6 +404,84 @@ class Server:
                 time.sleep(min(max(nxt, 0.0), 0.0002) if busy or nxt > 0 else 0.0)
         for s in self.streams:
             s.sync()
+        lat = np.array([r.finished - r.arrival for r in done]) if done else np.zeros(1)
+        span = (max(r.finished for r in done) - min(r.arrival for r in done)) if done else 0.0
+        return {"completed": len(done), "submitted": len(pending),
+                "bit_exact": self.all_ok if self.verify else None,
+                "p50_ms": float(np.percentile(lat, 50) * 1e3),
+                "p99_ms": float(np.percentile(lat, 99) * 1e3),
+                "GBps": float(sum(r.out_bytes for r in done) / span / 1e9) if span > 0 else 0.0}
+
+    # -- native engine (mdz_serve_*) ---------------------------------------
+    def _native_request(self, r: Request, tag: int) -> ServeRequestT:
+        m = r.meta
+        q = ServeRequestT()
+        q.codec = _NATIVE_CODEC[r.codec]
+        q.payload = r.payload.ctypes.data if r.payload.size else None
+        q.payload_bytes = int(m.get("payload_bytes", r.payload.nbytes)) if r.codec == "rlev2" else r.payload.nbytes
+        q.out_bytes = r.out_bytes
+        q.tag = tag
+        if r.codec == "deflate":
+            q.chunk_bytes = m["chunk"]
+            q.src_off = m["src_off"].ctypes.data
+            q.src_len = m["src_len"].ctypes.data
+            q.n_streams = len(m["src_off"])
+        else:
+            q.elem_bytes = m.get("eb", 1)
+            q.lane_line = m.get("line", 4)
+            q.blk_off = m["blk_off"].ctypes.data
+            q.col_len = m["col_len"].ctypes.data
+            q.col_map = m["col_map"].ctypes.data if "col_map" in m else None
+            q.n_chunks = m["n"]
+            q.chunk_bytes = m["chunk"]
+        return q
+
+    def _run_native(self, pending: list[Request], time_limit: float) -> dict:
+        """The same loop with admission, launches, live-queue publishing and
+        completion in the library (mdz_serve_poll); Python only submits
+        requests as they arrive and takes the finished ones."""
+        L = lib()
+        srv = C.c_void_p()
+        check(L.mdz_serve_open(self.budget, self.n_streams, self.live_blocks, self.live_idle_ms, C.byref(srv)),
+              "mdz_serve_open")
+        cap = 64
+        dbuf = (ServeDoneT * cap)()
+        nd = C.c_uint32()
+        by_tag = {}
+        done: list[Request] = []
+        i = 0
+        try:
+            t0 = time.perf_counter()
+            while len(done) < len(pending) and time.perf_counter() - t0 < time_limit:
+                now = time.perf_counter() - t0
+                while i < len(pending) and pending[i].arrival <= now:
+                    r = pending[i]
+                    r.launched = now
+                    by_tag[i] = r
+                    check(L.mdz_serve_submit(srv, C.byref(self._native_request(r, i))), "mdz_serve_submit")
+                    i += 1
+                check(L.mdz_serve_poll(srv, dbuf, cap, C.byref(nd)), "mdz_serve_poll")
+                t_now = time.perf_counter() - t0
+                for k in range(nd.value):
+                    d = dbuf[k]
+                    r = by_tag.pop(d.tag)
+                    r.finished = t_now
+                    if self.verify:
+                        got = np.empty(max(d.out_bytes, 1), np.uint8)
+                        if d.out_bytes:
+                            check(L.mdz_memcpy_dtoh(got.ctypes.data, d.d_out, d.out_bytes, None), "dtoh")
+                        check(L.mdz_stream_sync(None), "sync")
+                        got = got[:d.out_bytes]
+                        good = d.status == 0 and np.array_equal(got, r.raw)
+                        self.all_ok = self.all_ok and bool(good)
+                        self.outputs[r.rid] = got
+                    check(L.mdz_serve_release(srv, d.tag), "mdz_serve_release")
+                    done.append(r)
+                if nd.value == 0:
+                    nxt = pending[i].arrival - (time.perf_counter() - t0) if i < len(pending) else 0.0
+                    time.sleep(min(max(nxt, 0.0), 0.0002) if (by_tag or nxt > 0) else 0.0)
+        finally:
+            check(L.mdz_serve_close(srv), "mdz_serve_close")
         lat = np.array([r.finished - r.arrival for r in done]) if done else np.zeros(1)
         span = (max(r.finished for r in done) - min(r.arrival for r in done)) if done else 0.0
         return {"completed": len(done), "submitted": len(pending),

@@ -468,6 +468,60 @@ const mdz_inflate_status_t* mdz_inflate_live_statuses(const mdz_inflate_live_t* 
 int mdz_inflate_live_expired(mdz_inflate_live_t* q);
 int mdz_inflate_live_close(mdz_inflate_live_t* q, mdz_inflate_status_t* statuses);
 
+/* Serving engine -- no counterpart in the reference; the admission and
+ * launch loop of a decode server, native.  A request is one container in
+ * host memory (keep it alive until the request is released; pinned memory,
+ * mdz_pinned_alloc, makes its upload asynchronous): the lane-transposed
+ * codecs give payload, blk_off, col_len (and col_map for LZSS), n_chunks,
+ * chunk_bytes, elem_bytes, lane_line as for mdz_lanes_t; Deflate gives
+ * n_streams zlib streams at payload + src_off[i] (src_len[i] bytes), stream
+ * i decoding to output byte i*chunk_bytes.  mdz_serve_poll admits waiting
+ * requests in submission order under hbm_budget (payload + output +
+ * metadata bytes), uploads and launches them on n_streams streams, publishes
+ * Deflate streams to a live inflate queue (mdz_inflate_live_*, live_blocks
+ * and idle_ms as there) and reports up to cap finished requests: tag, the
+ * first failing chunk or stream status, and the device output, which stays
+ * valid until mdz_serve_release(tag).  The live queue's kernel needs a
+ * hardware queue of its own (GPU_MAX_HW_QUEUES above the process's stream
+ * count).  One thread drives a server; close waits for all its work. */
+#define MDZ_SERVE_RLEV1   1
+#define MDZ_SERVE_RLEV2   2
+#define MDZ_SERVE_LZSS    3
+#define MDZ_SERVE_DEFLATE 4
+typedef struct {
+    int             codec;         /* MDZ_SERVE_* */
+    uint32_t        elem_bytes;    /* lane codecs */
+    uint32_t        lane_line;     /* lane codecs */
+    uint32_t        reserved;
+    const void*     payload;
+    uint64_t        payload_bytes;
+    const uint64_t* blk_off;       /* lane codecs: n_chunks + 1 */
+    const uint64_t* col_len;       /* lane codecs: 32 * n_chunks */
+    const uint8_t*  col_map;       /* LZSS: 32 * n_chunks */
+    uint64_t        n_chunks;
+    uint64_t        chunk_bytes;   /* lane codecs: chunk size; Deflate: output bytes per stream */
+    const uint64_t* src_off;       /* Deflate: n_streams */
+    const uint64_t* src_len;
+    uint64_t        n_streams;
+    uint64_t        out_bytes;
+    uint64_t        tag;           /* caller's request id, unique among unreleased requests */
+} mdz_serve_request_t;
+typedef struct {
+    uint64_t    tag;
+    int32_t     status;            /* first failing chunk's or stream's status, 0 if none */
+    uint32_t    reserved;
+    void*       d_out;             /* device output, out_bytes */
+    uint64_t    out_bytes;
+} mdz_serve_done_t;
+typedef struct mdz_server mdz_server_t;
+int mdz_serve_open(uint64_t hbm_budget, uint32_t n_streams, uint32_t live_blocks, uint32_t idle_ms,
+                   mdz_server_t** s);
+int mdz_serve_submit(mdz_server_t* s, const mdz_serve_request_t* r);
+int mdz_serve_poll(mdz_server_t* s, mdz_serve_done_t* done, uint32_t cap, uint32_t* n_done);
+int mdz_serve_release(mdz_server_t* s, uint64_t tag);
+int mdz_serve_pending(const mdz_server_t* s, uint64_t* waiting, uint64_t* active);
+int mdz_serve_close(mdz_server_t* s);
+
 /* Compute units of the current device (sizes persistent grids). */
 int mdz_device_cus(int* cus);
 /* Workgroups of the persistent inflate kernel that fit on the device at once. */

@@ -82,3 +82,47 @@ def test_live_queue_expiry_republishes_and_matches_launch_mode(gpu):
     assert res2["completed"] == n and res2["bit_exact"]
     for i in range(n):
         assert np.array_equal(live.outputs[i], launch.outputs[i])
+
+
+def test_native_engine_matches_python_loop(gpu):
+    """The native serving engine (mdz_serve_*): every codec over the full
+    request range, bit-exact, each output equal to the Python loop's, the
+    budget released; a second run reuses its pools."""
+    from compression_amd import serve
+    n = 12
+    sizes = serve.request_sizes(n, 0x71, 16 << 10, 64 << 20)
+    codecs = [serve.CODECS[i % 4] for i in range(n)]
+    arr = serve.poisson_arrivals(n, 300.0, 0x72)
+    mk = lambda: [serve.make_request(i, codecs[i], int(sizes[i]), 5000 + i, float(arr[i])) for i in range(n)]
+    nat = serve.Server(hbm_budget=64 << 30, n_streams=2, verify=True, native=True)
+    reqs = mk()
+    res = nat.run(reqs, time_limit=120)
+    assert res["completed"] == n and res["bit_exact"]
+    for r in reqs:
+        r.arrival = 0.0
+    res2 = nat.run(reqs, time_limit=120)                 # all at once
+    assert res2["completed"] == n and res2["bit_exact"]
+    py = serve.Server(hbm_budget=64 << 30, n_streams=2, verify=True)
+    res3 = py.run(mk(), time_limit=120)
+    assert res3["completed"] == n and res3["bit_exact"]
+    for i in range(n):
+        assert np.array_equal(nat.outputs[i], py.outputs[i])
+
+
+def test_native_engine_budget_and_damaged_request(gpu):
+    """A budget of about one request admits them one by one; a request with
+    damaged input reports its failing status and the others stay exact."""
+    from compression_amd import serve
+    reqs = _requests(10, 13)
+    budget = max(r.hbm_bytes() for r in reqs)
+    bad = next(r for r in reqs if r.codec == "deflate")
+    pay = np.array(bad.payload, copy=True)
+    o = int(bad.meta["src_off"][0])
+    pay[o + 2:o + 40] ^= 0x5A                               # inside the first zlib stream
+    bad.payload = pay
+    srv = serve.Server(hbm_budget=budget, n_streams=2, verify=True, native=True)
+    res = srv.run(reqs, time_limit=120)
+    assert res["completed"] == 10 and res["bit_exact"] is False
+    for r in reqs:
+        if r is not bad:
+            assert np.array_equal(srv.outputs[r.rid], r.raw)
