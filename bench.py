@@ -1150,7 +1150,9 @@ def serve_record(args) -> dict:
     mode = getattr(args, "serve_deflate", "live")
     native = getattr(args, "serve_loop", "python") == "native" and mode == "live"
     ns = getattr(args, "serve_streams", 0) or (2 if mode == "live" else 4)
-    srv = serve.Server(hbm_budget=64 << 30, n_streams=ns, verify=False, deflate_mode=mode, native=native)
+    lb = getattr(args, "serve_live_blocks", 0)
+    srv = serve.Server(hbm_budget=64 << 30, n_streams=ns, verify=False, deflate_mode=mode, native=native,
+                       live_blocks=lb)
     srv.run(pool[: min(8, n)], time_limit=60)            # warm-up: allocations, code objects
     for r in pool:
         r.arrival = 0.0
@@ -1165,7 +1167,8 @@ def serve_record(args) -> dict:
         sweep.append({"load": frac, "p50_ms": round(res["p50_ms"], 3), "p99_ms": round(res["p99_ms"], 3),
                       "GBps": round(res["GBps"], 2), "completed": res["completed"]})
     # bit-exactness of the whole pool, once, with verification on
-    chk = serve.Server(hbm_budget=64 << 30, n_streams=ns, verify=True, deflate_mode=mode, native=native)
+    chk = serve.Server(hbm_budget=64 << 30, n_streams=ns, verify=True, deflate_mode=mode, native=native,
+                       live_blocks=lb)
     for r in pool:
         r.arrival = 0.0
     ok = chk.run(pool, time_limit=180)["bit_exact"]
@@ -1173,7 +1176,8 @@ def serve_record(args) -> dict:
     return {"value": at80["GBps"], "unit": "GB/s", "dtype": "mixed",
             "config": {"workload": "serve_mixed_codecs", "requests": n, "out_bytes": total,
                        "codecs": {c: codecs.count(c) for c in serve.CODECS}, "hbm_budget_GiB": 64,
-                       "streams": ns, "deflate": mode, "loop": "native" if native else "python"},
+                       "streams": ns, "deflate": mode, "loop": "native" if native else "python",
+                       "live_blocks": lb or "quarter of the inflate slots"},
             "saturation_GBps": round(sat["GBps"], 2), "p50_ms_at_80": at80["p50_ms"],
             "p99_ms_at_80": at80["p99_ms"], "sweep": sweep, "bit_exact": ok}
 
@@ -1206,6 +1210,8 @@ def main():
     ap.add_argument("--serve-streams", type=int, default=0,
                     help="serving: HIP streams for the per-request launches (0: 2 with the live "
                          "inflate queue, 4 without)")
+    ap.add_argument("--serve-live-blocks", type=int, default=0,
+                    help="serving: workgroups of the live inflate kernel (0: a quarter of the slots)")
     ap.add_argument("--serve-loop", default="python", choices=["native", "python"],
                     help="serving: the Python loop (default: measured faster) or the library's "
                          "engine (mdz_serve_*)")
