@@ -797,7 +797,25 @@ __global__ __launch_bounds__(128) void inflate_kernel(const mdz_inflate_in_t* __
                         const uint32_t rel = (uint32_t)(8 * (src_addr + byte - A) + (P & 7));
                         const uint64_t limb = 8 * (uint64_t)(src_addr + r.n - A);
                         const uint32_t lim = limb > 0xF0000000ull ? 0xF0000000u : (uint32_t)limb;
-                        for (uint32_t k = l; k < kStageWords; k += 64) S.in[pad16(k)] = r.load_dword(A + 4 * k);
+                        const uint64_t s_lo = (uint64_t)(uintptr_t)r.src;
+                        if (A >= s_lo && A + 4ull * kStageWords <= s_lo + r.n) {
+                            // the whole round inside the stream: every load issued
+                            // before the first LDS store (one memory latency a round)
+                            constexpr uint32_t kPer = (kStageWords + 63) / 64;
+                            uint32_t v[kPer];
+#pragma unroll
+                            for (uint32_t i = 0; i < kPer; ++i) {
+                                const uint32_t k = l + 64 * i;
+                                v[i] = k < kStageWords ? *reinterpret_cast<const uint32_t*>(A + 4ull * k) : 0u;
+                            }
+#pragma unroll
+                            for (uint32_t i = 0; i < kPer; ++i) {
+                                const uint32_t k = l + 64 * i;
+                                if (k < kStageWords) S.in[pad16(k)] = v[i];
+                            }
+                        } else {
+                            for (uint32_t k = l; k < kStageWords; k += 64) S.in[pad16(k)] = r.load_dword(A + 4 * k);
+                        }
                         wave_sync();
                         INF_PHASE(5);
 
