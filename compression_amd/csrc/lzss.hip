@@ -7,7 +7,9 @@
 // stores, and brle_trans::decompress_func (include/brle/brle_trans.h:72-202),
 // one thread per lane as well.  A brle_trans 2-byte token is a run (count
 // byte, value byte) instead of a match; the kernel is instantiated for each
-// meaning (kRle), and a run's map entries are values, so nothing is resolved.
+// meaning (kRle), and a run-length stream's map holds output bytes directly
+// (one byte per entry: half the LDS of the 16-bit source map, so more
+// workgroups fit a CU), so nothing is resolved.
 //
 // Container: chunk c holds 32 lane streams; sorted position s has
 // col_len[32c+s] bytes and decodes original lane t = col_map[32c+s]; lengths
@@ -31,7 +33,7 @@
 //      output length, prefix-sum them across the wave, then write a u16
 //      source map per output byte: 0x8000|byte for literals, the source
 //      position for match bytes (byte-serial semantics, so overlapping
-//      matches repeat their source);
+//      matches repeat their source); for run tokens, a byte map of values;
 //   5. each map entry is resolved by following source positions down to a
 //      literal (positions strictly decrease) and replaced by its value, which
 //      shortens later walks;
@@ -41,6 +43,8 @@
 // (the first failing token in that lane's stream order), as the oracle
 // reports it.  Parts combine by atomicMin on (lane << 8 | code) after an init
 // fill; a finishing pass turns the key back into the status code.
+#include <type_traits>
+
 #include "mdz_device.h"
 #include "mdz_internal.h"
 #include "status_dev.h"
@@ -89,6 +93,19 @@ MDZ_DEV void fill_u16(uint16_t* m, uint32_t i, uint32_t e, uint32_t v, uint32_t 
     if (i < e) m[i] = (uint16_t)pk;
 }
 
+// Byte-map fill: m[i..e) = v (aligned dword stores in the middle).
+MDZ_DEV void fill_u8(uint8_t* m, uint32_t i, uint32_t e, uint32_t v) {
+    for (; (i & 3u) && i < e; ++i) m[i] = (uint8_t)v;
+    const uint32_t pk = (v & 0xffu) * 0x01010101u;
+    for (; i + 4 <= e; i += 4) *reinterpret_cast<uint32_t*>(m + i) = pk;
+    for (; i < e; ++i) m[i] = (uint8_t)v;
+}
+
+// A run-length stream's map holds output bytes (values only, one byte
+// each); an LZSS stream's holds 16-bit sources (a literal's byte with bit 15
+// set, or an earlier output position) until resolution.
+template <bool kRle> using MapT = typename std::conditional<kRle, uint8_t, uint16_t>::type;
+
 // Output bytes of a 2-byte token from its first byte: an LZSS match,
 // (b & 31) + 3; a byte-RLE run (brle_trans), the count byte itself.
 template <bool kRle>
@@ -106,8 +123,8 @@ __global__ __launch_bounds__(64 * G) void lzss_decode_kernel(
     LzssHdr& h = *reinterpret_cast<LzssHdr*>(smem);
     uint16_t* const R = reinterpret_cast<uint16_t*>(smem + sizeof(LzssHdr));
     uint8_t* const inb = smem + sizeof(LzssHdr) + geo.r_bytes;
-    uint16_t* const map = reinterpret_cast<uint16_t*>(inb + G * geo.cap_in);
-    uint16_t* const gs = map + G * geo.lane_out;
+    MapT<kRle>* const map = reinterpret_cast<MapT<kRle>*>(inb + G * geo.cap_in);
+    uint16_t* const gs = reinterpret_cast<uint16_t*>(map + G * geo.lane_out);
     uint16_t* const go = gs + G * geo.gmax;
 
     // XCD-aware placement: block ids congruent mod 8 share an XCD; the P parts
@@ -212,7 +229,7 @@ __global__ __launch_bounds__(64 * G) void lzss_decode_kernel(
         const uint32_t j = wv, t = part * G + j;
         const uint32_t L = h.L[h.inv[t]];
         const uint8_t* d = inb + j * geo.cap_in;
-        uint16_t* mp = map + j * lane_out;
+        MapT<kRle>* mp = map + j * lane_out;
         const uint16_t* gj = gs + j * geo.gmax;
         uint16_t* oj = go + j * geo.gmax;
         const uint32_t ng = h.ng[j];
@@ -269,7 +286,8 @@ __global__ __launch_bounds__(64 * G) void lzss_decode_kernel(
                         stop = true;
                         break;
                     }
-                    mp[o++] = (uint16_t)(0x8000u | d[q++]);
+                    if constexpr (kRle) mp[o++] = d[q++];
+                    else mp[o++] = (uint16_t)(0x8000u | d[q++]);
                 } else {
                     if (q + 2 > L) {
                         err = min(err, (tok << 8) | ((uint32_t)MDZ_ST_INPUT_OVERRUN & 0xffu));
@@ -281,7 +299,7 @@ __global__ __launch_bounds__(64 * G) void lzss_decode_kernel(
                     const uint32_t len = tok_len<kRle>(v & 0xffu), dist = v >> 5;
                     const uint32_t e = min(o + len, lane_out);
                     if constexpr (kRle) {          // run: `len` copies of the value byte
-                        fill_u16(mp, o, e, 0x8000u | (v >> 8), 0);
+                        fill_u8(mp, o, e, v >> 8);
                         if (o + len > lane_out)
                             err = min(err, (tok << 8) | ((uint32_t)MDZ_ST_OUTPUT_OVERRUN & 0xffu));
                     } else if (dist == 0 || dist > o) {
@@ -298,7 +316,7 @@ __global__ __launch_bounds__(64 * G) void lzss_decode_kernel(
             if (stop) break;
         }
         if (T < lane_out) {                        // stream ended early: fill the rest
-            for (uint32_t i = T + ln; i < lane_out; i += 64) mp[i] = 0x8000u;
+            for (uint32_t i = T + ln; i < lane_out; i += 64) mp[i] = kRle ? 0u : 0x8000u;
             err = min(err, ((ng * 8) << 8) | ((uint32_t)MDZ_ST_INPUT_OVERRUN & 0xffu));
         }
         wave_sync();
@@ -308,7 +326,7 @@ __global__ __launch_bounds__(64 * G) void lzss_decode_kernel(
         // (both chains of a pair in flight; a resolved chain re-reads mp[0]
         // instead of branching)
         for (uint32_t i = 2 * ln; !kRle && i < lane_out; i += 128) {
-            uint32_t* w = reinterpret_cast<uint32_t*>(mp + i);
+            uint32_t* w = reinterpret_cast<uint32_t*>(reinterpret_cast<uint16_t*>(mp) + i);
             uint32_t e0 = *w & 0xffffu, e1 = *w >> 16;
             if ((e0 & e1 & 0x8000u) == 0) {
                 do {
@@ -330,9 +348,14 @@ __global__ __launch_bounds__(64 * G) void lzss_decode_kernel(
         const uint32_t nrows = lane_out / 4;
         for (uint32_t idx = tid; idx < nrows * G; idx += NT) {
             const uint32_t j = idx & (G - 1), r = idx / G;
-            const uint64_t e = *reinterpret_cast<const uint64_t*>(map + j * lane_out + 4 * r);
-            const uint32_t word = (uint32_t)(e & 0xff) | (uint32_t)((e >> 8) & 0xff00) |
-                                  (uint32_t)((e >> 16) & 0xff0000) | (uint32_t)((e >> 24) & 0xff000000u);
+            uint32_t word;
+            if constexpr (kRle) {
+                word = *reinterpret_cast<const uint32_t*>(map + j * lane_out + 4 * r);
+            } else {
+                const uint64_t e = *reinterpret_cast<const uint64_t*>(map + j * lane_out + 4 * r);
+                word = (uint32_t)(e & 0xff) | (uint32_t)((e >> 8) & 0xff00) | (uint32_t)((e >> 16) & 0xff0000) |
+                       (uint32_t)((e >> 24) & 0xff000000u);
+            }
             const uint64_t at = out0 + 128ull * r + 4ull * (part * G + j);
             if (at + 4 <= dst_bytes) {
                 *reinterpret_cast<uint32_t*>(dst + at) = word;
@@ -348,7 +371,7 @@ template <int G, bool kRle>
 int launch(const mdz_lanes_t* in, const LzssGeom& geo, void* dst, uint64_t dst_bytes,
            int32_t* status, hipStream_t s) {
     const size_t smem = sizeof(LzssHdr) + geo.r_bytes +
-                        (size_t)G * (geo.cap_in + 2ull * geo.lane_out + 4ull * geo.gmax);
+                        (size_t)G * (geo.cap_in + sizeof(MapT<kRle>) * geo.lane_out + 4ull * geo.gmax);
     hipError_t e = set_dynamic_lds((const void*)lzss_decode_kernel<G, kRle>, smem);
     if (e != hipSuccess) return record_hip(e, "lzss_decode_kernel LDS");
     const uint64_t blocks = (in->n_chunks + 7) / 8 * 8 * (32 / G);
