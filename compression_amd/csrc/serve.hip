@@ -26,6 +26,7 @@
 #include <deque>
 #include <map>
 #include <unordered_map>
+#include <unordered_set>
 #include <vector>
 
 #include "mdz_internal.h"
@@ -98,6 +99,7 @@ struct mdz_server {
     std::deque<Req*> waiting;
     std::vector<Req*> active;
     std::unordered_map<uint64_t, Req*> done;           // reported, not released
+    std::unordered_set<uint64_t> tags;                  // every submitted, unreleased tag
     std::vector<hipEvent_t> spare_events;
     mdz_inflate_live_t* live = nullptr;
     uint32_t live_cap = 1u << 16, live_used = 0;
@@ -277,7 +279,8 @@ extern "C" int mdz_serve_submit(mdz_server_t* s, const mdz_serve_request_t* r) {
         (!lanes && r->n_streams && (!r->src_off || !r->src_len)))
         return mdz::set_error(MDZ_ERR_INVALID, "serve submit: request %llu incomplete",
                               (unsigned long long)r->tag);
-    if (s->done.count(r->tag)) return mdz::set_error(MDZ_ERR_INVALID, "serve submit: tag in use");
+    if (s->tags.count(r->tag)) return mdz::set_error(MDZ_ERR_INVALID, "serve submit: tag in use");
+    s->tags.insert(r->tag);
     Req* q = new Req();
     q->r = *r;
     q->hbm = hbm_bytes(*r);
@@ -378,6 +381,7 @@ extern "C" int mdz_serve_release(mdz_server_t* s, uint64_t tag) {
     if (it == s->done.end()) return mdz::set_error(MDZ_ERR_INVALID, "serve release: tag not done");
     Req* q = it->second;
     s->done.erase(it);
+    s->tags.erase(tag);
     for (auto& b : q->dbufs) s->dev.put(b.first, b.second);
     if (q->h_status) s->host.put(q->h_status, q->h_status_bin);
     if (q->ev) s->spare_events.push_back(q->ev);
