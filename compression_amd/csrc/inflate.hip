@@ -6,7 +6,9 @@
 // 1229-1345,1399-1407) behind the same batched descriptor/status interface.
 //
 // Per stream, one 128-thread workgroup: a producer wave and a consumer wave
-// meeting at one workgroup barrier per step.
+// joined by a queue of three items (symbol lists) in LDS, with a published
+// and a finished count, so either wave can run up to two items ahead of the
+// other (a barrier only at the start and end of each stream).
 //   * producer: parses block headers, builds each block's tables in LDS
 //     (literal/length: 10-bit root plus subtables; distance: 8-bit root plus
 //     subtables; codes longer than a subtable holds take a canonical
@@ -17,10 +19,10 @@
 //     resynchronise, so the true symbol sequence, followed past the end of
 //     lane t's span, soon meets a start a later lane marked (at most 3 spans
 //     ahead); the lanes it passes through form a chain, each lists its piece
-//     of the sequence, and the pieces are placed by popcounts into one of two
-//     LDS symbol lists.  An end-of-block, an error or a lost hand-off ends the
+//     of the sequence, and the pieces are placed by popcounts into one of the
+//     queue's LDS symbol lists.  An end-of-block, an error or a lost hand-off ends the
 //     round there and the next round starts from it;
-//   * consumer: turns the previous round's list into output in a 34.5 KiB
+//   * consumer: turns each published round's list into output in a 34.5 KiB
 //     LDS ring holding the last 32 KiB: batches of up to 64 symbols (about
 //     2 KiB of output), literals stored by their lanes, then matches copied
 //     in order, one byte per lane (byte j of a match reads start - dist +
@@ -79,6 +81,11 @@ constexpr uint32_t kStageWords = 64 * kSpan / 32 + 8;   // one round's input, pl
 MDZ_DEV uint32_t pad16(uint32_t w) { return w + (w >> 4); }
 constexpr uint32_t kStagePadded = kStageWords + kStageWords / 16 + 1;
 constexpr uint32_t kSymCap = 2048;                      // true-path symbols kept per round
+constexpr uint32_t kSlots = 3;                          // queue depth between the two waves
+// Polls (each after an s_sleep of about 64 cycles) before a wave waiting on
+// the other gives up on the stream with MDZ_ST_INTERNAL: far above any
+// item's decode or output time, so only a broken queue reaches it.
+constexpr uint32_t kWaitPolls = 1u << 24;
 
 // Table entry: [3:0] code length (0: no entry, take the canonical path),
 // [7:4] extra bits, [9:8] kind (0 literal, 1 end of block, 2 length/distance,
@@ -138,8 +145,9 @@ struct Item {
 
 struct Ctrl {
     uint32_t qi;          // queue position of the stream in hand
-    uint32_t halt;        // the consumer has stopped on an error
-    uint32_t done;        // the consumer has finished the stream
+    uint32_t halt;        // a wave has stopped the stream (consumer finished or failed, or a wait ran out)
+    uint32_t n_pub;       // items the producer has published
+    uint32_t n_fin;       // items the consumer has finished with
     uint32_t n_stored, n_fixed, n_dyn;
     uint64_t consumed;    // Deflate input bits through the last item
     uint64_t desc[4];     // a live queue's descriptor, read once by thread 0
@@ -154,8 +162,8 @@ struct InflateLds {
         HdrScratch h;
         uint32_t bm[64 * (kSpan / 32) * 17 / 16];   // per lane: symbol starts met in its span (padded)
     } u;
-    uint32_t sym[2][kSymCap];                  // rounds' symbols, in stream order (double buffer)
-    Item item[2];
+    uint32_t sym[kSlots][kSymCap];             // rounds' symbols, in stream order (item k in slot k % kSlots)
+    Item item[kSlots];
     Ctrl ctrl;
     uint8_t ring[kRing];                       // the consumer's output window
     uint32_t sink[64];                         // per-lane target of stores past a match's end
@@ -489,6 +497,37 @@ struct Reader {
 
 MDZ_DEV uint32_t ring_at(uint64_t p) { return (uint32_t)(p % kRing); }
 
+// The item queue between a stream's two waves.  A wave publishes a count
+// (items published, or items finished with) after a workgroup-scope release
+// that orders every lane's LDS reads and writes of the item before it; the
+// other wave reads the count with an acquire, so once it sees the count it
+// sees the item (or may overwrite the item's slot).
+MDZ_DEV void q_publish(uint32_t* ctr, uint32_t v) {
+    wave_sync();
+    if (lane_id() == 0) __hip_atomic_store(ctr, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    wave_sync();
+}
+MDZ_DEV uint32_t q_read(uint32_t* ctr) {
+    return __builtin_amdgcn_readfirstlane(__hip_atomic_load(ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP));
+}
+// Waits until *ctr >= want.  False if the stream is halted first, or after
+// kWaitPolls polls (the wave then halts the stream itself).
+MDZ_DEV bool q_wait(uint32_t* ctr, uint32_t want, uint32_t* halt) {
+    for (uint32_t n = 0;; ++n) {
+        const uint32_t v = q_read(ctr);
+        if (v >= want) break;
+        if (q_read(halt)) return false;
+        if (n >= kWaitPolls) {
+            if (lane_id() == 0) __hip_atomic_store(halt, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            wave_sync();
+            return false;
+        }
+        __builtin_amdgcn_s_sleep(1);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    return true;
+}
+
 // Trailer checksums over the output, 1 KiB blocks per wave step: lane l holds
 // bytes 16l..16l+15 of a block.  A final block of t < 1024 bytes is read as
 // 1024 - t zero bytes followed by its t bytes; leading zeros change neither
@@ -592,11 +631,12 @@ MDZ_DEV uint32_t live_wait(uint32_t* live, uint32_t q) {
 }
 
 // Two waves per stream.  Wave 1, the producer, parses block headers, builds
-// the tables and decodes the symbols of one round at a time into one of two
-// LDS lists; wave 0, the consumer, turns the previous round's list (or a
-// stored block) into output in the LDS window and stores it to HBM.  Both
-// meet at one workgroup barrier per step, so round k+1 decodes while round k
-// is written out.
+// the tables and decodes the symbols of one round at a time into the LDS
+// item queue; wave 0, the consumer, turns each published item (a round's
+// list, or a stored block) into output in the LDS window and stores it to
+// HBM.  The queue holds kSlots items, so the producer may decode up to two
+// rounds ahead of the one being written out; the waves meet at a workgroup
+// barrier only when a stream starts and when it ends.
 __global__ __launch_bounds__(128) void inflate_kernel(const mdz_inflate_in_t* __restrict__ ins,
                                                       mdz_inflate_status_t* __restrict__ outs,
                                                       const uint32_t* __restrict__ order,
@@ -626,7 +666,8 @@ __global__ __launch_bounds__(128) void inflate_kernel(const mdz_inflate_in_t* __
             }
             S.ctrl.qi = lq;
             S.ctrl.halt = 0;
-            S.ctrl.done = 0;
+            S.ctrl.n_pub = 0;
+            S.ctrl.n_fin = 0;
         }
         __syncthreads();
         const uint32_t qi = S.ctrl.qi;
@@ -680,10 +721,17 @@ __global__ __launch_bounds__(128) void inflate_kernel(const mdz_inflate_in_t* __
             bool need_hdr = true, final = false, pdone = false;
             uint32_t fixed_built = 0, n_stored = 0, n_fixed = 0, n_dyn = 0;
             const uintptr_t src_addr = (uintptr_t)r.src;
-            for (uint32_t step = 0;; ++step) {
-                if (!pdone && !S.ctrl.halt) {
+            for (uint32_t ik = 0; !pdone; ++ik) {
+                // item ik goes to slot ik % kSlots once the consumer has finished
+                // with item ik - kSlots; a halted stream gets no more items
+                if (q_read(&S.ctrl.halt)) break;
+                INF_PHASE(11);
+                const bool free = q_wait(&S.ctrl.n_fin, ik >= kSlots ? ik + 1 - kSlots : 0, &S.ctrl.halt);
+                INF_PHASE(9);                      // producer waiting for the consumer
+                if (!free) break;
+                {
                     Item it = {kItemNone, 0, MDZ_ST_OK, 0, 0, 0};
-                    const uint32_t b = step & 1;
+                    const uint32_t b = ik % kSlots;
                     if (st != MDZ_ST_OK) {
                         it.status = st;
                         it.last = 1;
@@ -983,22 +1031,19 @@ __global__ __launch_bounds__(128) void inflate_kernel(const mdz_inflate_in_t* __
                         INF_PHASE(1);
                     }
                     if (l == 0) S.item[b] = it;
-                    if (it.last) {
-                        pdone = true;
-                        if (l == 0) {
-                            S.ctrl.n_stored = n_stored;
-                            S.ctrl.n_fixed = n_fixed;
-                            S.ctrl.n_dyn = n_dyn;
-                            S.ctrl.consumed = P;
-                        }
-                    }
-                    wave_sync();
+                    if (it.last) pdone = true;
+                    q_publish(&S.ctrl.n_pub, ik + 1);
                 }
-                INF_PHASE(11);
-                __syncthreads();
-                INF_PHASE(9);                      // producer waiting for the consumer
-                if (S.ctrl.done) break;
             }
+            // block counts and Deflate input consumed, for the cost log and the
+            // trailer check; the consumer reads them after the closing barrier
+            if (l == 0) {
+                S.ctrl.n_stored = n_stored;
+                S.ctrl.n_fixed = n_fixed;
+                S.ctrl.n_dyn = n_dyn;
+                S.ctrl.consumed = P;
+            }
+            __syncthreads();                       // the stream's closing barrier
         } else {
             // ------------------------------------------------------ consumer
             uint8_t* const dst = reinterpret_cast<uint8_t*>(in.dst);
@@ -1050,10 +1095,14 @@ __global__ __launch_bounds__(128) void inflate_kernel(const mdz_inflate_in_t* __
             const uint8_t* src = reinterpret_cast<const uint8_t*>(in.src);
             if (hdr == MDZ_HDR_GZIP && outs[si].status == MDZ_ST_OK) src += outs[si].bytes_written;
             INF_PHASE_START();
-            bool cdone = false;
-            for (uint32_t step = 0;; ++step) {
-                if (step > 0 && !cdone) {
-                    const uint32_t b = (step - 1) & 1;
+            for (uint32_t ik = 0;; ++ik) {
+                INF_PHASE(12);
+                const bool have = q_wait(&S.ctrl.n_pub, ik + 1, &S.ctrl.halt);
+                INF_PHASE(10);                     // consumer waiting for the producer
+                // the producer halts a stream only when a wait of its own ran out
+                if (!have) { st = MDZ_ST_INTERNAL; break; }
+                {
+                    const uint32_t b = ik % kSlots;
                     const Item it = S.item[b];
                     if (it.kind == kItemStored) {
                         const uint64_t q = it.src, take = it.take;
@@ -1165,20 +1214,17 @@ __global__ __launch_bounds__(128) void inflate_kernel(const mdz_inflate_in_t* __
                     }
                     if (st == MDZ_ST_OK && it.status != MDZ_ST_OK) st = it.status;
                     if (st != MDZ_ST_OK || it.last) {
-                        cdone = true;
-                        if (l == 0) {
-                            S.ctrl.halt = 1;
-                            S.ctrl.done = 1;
-                        }
+                        // the stream is finished: a producer still decoding, or
+                        // waiting for a free slot, stops
+                        if (l == 0) __hip_atomic_store(&S.ctrl.halt, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                        wave_sync();
+                        break;
                     }
-                    wave_sync();
+                    q_publish(&S.ctrl.n_fin, ik + 1);
                 }
-                INF_PHASE(12);
-                __syncthreads();
-                INF_PHASE(10);                     // consumer waiting for the producer
-                if (S.ctrl.done) break;
             }
             flush(pos, true);
+            __syncthreads();                       // the stream's closing barrier
             // a live queue's host reads the status while the kernel runs: the
             // output (and the status fields) must reach memory first
             if (live) __threadfence_system();
@@ -1210,7 +1256,6 @@ __global__ __launch_bounds__(128) void inflate_kernel(const mdz_inflate_in_t* __
                 }
             }
         }
-        __syncthreads();
         if (!cursor) return;
     }
 }

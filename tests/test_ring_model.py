@@ -1,15 +1,18 @@
 """CPU models of the two cross-wave protocols in the HIP decoders, run over
 many schedules (no GPU).
 
-1. Inflate's producer/consumer waves (compression_amd/csrc/inflate.hip): one
-   workgroup barrier per step; at step s the producer writes item s into
-   buffer s % 2 unless it has produced the stream's last item or the consumer
-   has halted, and the consumer processes item s - 1 from buffer (s - 1) % 2;
-   after the barrier both leave when the consumer has set `done`.  Checked:
-   the consumer sees every item in order exactly once up to the first that
-   ends the stream (last item, or an error of its own), never reads a buffer
-   the producer is writing in the same step, and both waves pass the same
-   number of barriers.
+1. Inflate's producer/consumer waves (compression_amd/csrc/inflate.hip): an
+   item queue of KSLOTS slots in LDS with two counts.  The producer writes
+   item k into slot k % KSLOTS once the consumer has finished with item
+   k - KSLOTS, then publishes n_pub = k + 1; the consumer waits for
+   n_pub > k, processes item k, and publishes n_fin = k + 1.  The consumer
+   sets `halt` when it stops (the stream's last item, or an error); the
+   producer checks `halt` before each item and in its wait, and a wave whose
+   wait runs out sets `halt` itself.  Checked over random interleavings: the
+   consumer sees every item in order exactly once up to the one that ends the
+   stream, no slot is written while the consumer reads it, the producer runs
+   at most KSLOTS items past the consumer's stop, every wait ends, and a
+   producer that gives up makes the consumer stop with an internal error.
 
 2. RLE v1's token walk and unit decoders (compression_amd/csrc/rlev1.hip):
    the walker appends tokens to a lane's list, then publishes the list length
@@ -23,53 +26,90 @@ import random
 import pytest
 
 
-def run_stream(items, consumer_error_at=None, rng=None):
-    """items: list of (kind, last, status) the producer would emit in order."""
-    buf = [None, None]
-    halt = done = False
-    pdone = False
-    pi = 0
+KSLOTS = 3
+
+
+def run_stream(items, consumer_error_at=None, rng=None, producer_gives_up_at=None):
+    """items: list of (kind, last, status) the producer would emit in order.
+    Returns (seen, produced, consumer_status)."""
+    st = {"n_pub": 0, "n_fin": 0, "halt": False}
+    slot = [None] * KSLOTS
+    writing = [False] * KSLOTS
+    reading = [False] * KSLOTS
     seen = []
-    barriers = [0, 0]
-    writing = None
-    for step in range(len(items) + 3):
-        # both roles act within the step; their order inside it is arbitrary
-        order = [0, 1]
-        if rng:
-            rng.shuffle(order)
-        for role in order:
-            if role == 0:                        # producer
-                if not pdone and not halt:
-                    b = step & 1
-                    writing = b
-                    buf[b] = (pi, items[pi])
-                    if items[pi][1]:
-                        pdone = True
-                    pi += 1
-            else:                                # consumer
-                if step > 0 and not done:
-                    b = (step - 1) & 1
-                    assert writing != b or role_wrote_other(step, b)
-                    idx, (kind, last, status) = buf[b]
-                    assert idx == step - 1       # the item produced last step
-                    seen.append(idx)
-                    err = consumer_error_at is not None and idx == consumer_error_at
-                    if err or status or last:
-                        halt = done = True
-        barriers[0] += 1
-        barriers[1] += 1
-        writing = None
-        if done:
+    out = {"produced": 0, "cstatus": 0}
+
+    def producer():
+        ik = 0
+        while True:
+            if st["halt"]:
+                return
+            want = ik + 1 - KSLOTS if ik >= KSLOTS else 0
+            polls = 0
+            while st["n_fin"] < want:
+                if st["halt"]:
+                    return
+                if producer_gives_up_at is not None and polls >= producer_gives_up_at:
+                    st["halt"] = True            # the wait ran out
+                    return
+                polls += 1
+                yield
+            b = ik % KSLOTS
+            assert not reading[b]
+            writing[b] = True
+            yield
+            assert not reading[b]
+            slot[b] = (ik, items[ik])
+            writing[b] = False
+            out["produced"] = ik + 1
+            yield
+            st["n_pub"] = ik + 1
+            yield
+            if items[ik][1]:
+                return
+            ik += 1
+
+    def consumer():
+        ik = 0
+        while True:
+            while st["n_pub"] < ik + 1:
+                if st["halt"]:
+                    out["cstatus"] = "internal"
+                    return
+                yield
+            b = ik % KSLOTS
+            assert not writing[b]
+            reading[b] = True
+            idx, (kind, last, status) = slot[b]
+            assert idx == ik
+            yield
+            assert not writing[b]
+            reading[b] = False
+            seen.append(idx)
+            err = consumer_error_at is not None and idx == consumer_error_at
+            if err or status or last:
+                out["cstatus"] = status or ("error" if err else 0)
+                st["halt"] = True
+                return
+            st["n_fin"] = ik + 1
+            yield
+            ik += 1
+
+    live = [producer(), consumer()]
+    for n in range(100_000):
+        if not live:
             break
-    return seen, barriers, pi
+        g = rng.choice(live) if rng else live[n % len(live)]
+        try:
+            next(g)
+        except StopIteration:
+            live.remove(g)
+    assert not live, "a wait never ended"
+    return seen, out["produced"], out["cstatus"]
 
 
-def role_wrote_other(step, b):
-    return (step & 1) != b
-
-
-@pytest.mark.parametrize("seed", range(200))
-def test_inflate_step_protocol(seed):
+@pytest.mark.parametrize("seed", range(300))
+def test_inflate_queue_protocol(seed):
     rng = random.Random(seed)
     n = rng.randint(1, 12)
     items = [("round", False, 0) for _ in range(n)]
@@ -78,11 +118,23 @@ def test_inflate_step_protocol(seed):
     # an item that carries an error status is also the stream's last
     items[end] = ("round", True, -5 if kind == "error" else 0)
     cerr = rng.choice([None, rng.randrange(end + 1)])
-    seen, barriers, produced = run_stream(items[:end + 1], cerr, rng)
+    seen, produced, cstatus = run_stream(items[:end + 1], cerr, rng)
     stop = end if cerr is None else min(end, cerr)
     assert seen == list(range(stop + 1))            # in order, once each, up to the end
-    assert barriers[0] == barriers[1]               # both waves pass the same barriers
-    assert produced <= stop + 2                     # at most one item decoded past the end
+    assert produced <= stop + 1 + KSLOTS            # bounded run-ahead past the stop
+    assert cstatus != "internal"
+
+
+@pytest.mark.parametrize("seed", range(100))
+def test_inflate_queue_producer_gives_up(seed):
+    rng = random.Random(1000 + seed)
+    n = rng.randint(KSLOTS + 1, 12)
+    items = [("round", False, 0) for _ in range(n)]
+    items[-1] = ("round", True, 0)
+    seen, produced, cstatus = run_stream(items, None, rng, producer_gives_up_at=rng.randint(0, 3))
+    assert seen == list(range(len(seen)))
+    # the consumer ends: with the last item, or with an internal error
+    assert cstatus == "internal" or seen[-1] == n - 1
 
 
 def walker(lst, pub, tokens, covers):
