@@ -6,9 +6,12 @@
 // 1229-1345,1399-1407) behind the same batched descriptor/status interface.
 //
 // Per stream, one 128-thread workgroup: a producer wave and a consumer wave
-// joined by a queue of three items (symbol lists) in LDS, with a published
-// and a finished count, so either wave can run up to two items ahead of the
-// other (a barrier only at the start and end of each stream).
+// joined by a queue of three items in LDS (a round's symbol list, or a
+// stored block), with a published and a finished count, so either wave can
+// run up to two items ahead of the other (a barrier only at the start and
+// end of each stream).  The rounds' lists share one ring of 6,144 symbols:
+// a round keeps up to 4,096, and the producer lists a round once the
+// consumer has freed room for it.
 //   * producer: parses block headers, builds each block's tables in LDS
 //     (literal/length: 10-bit root plus subtables; distance: 8-bit root plus
 //     subtables; codes longer than a subtable holds take a canonical
@@ -19,9 +22,9 @@
 //     resynchronise, so the true symbol sequence, followed past the end of
 //     lane t's span, soon meets a start a later lane marked (at most 3 spans
 //     ahead); the lanes it passes through form a chain, each lists its piece
-//     of the sequence, and the pieces are placed by popcounts into one of the
-//     queue's LDS symbol lists.  An end-of-block, an error or a lost hand-off ends the
-//     round there and the next round starts from it;
+//     of the sequence, and the pieces are placed by popcounts in the symbol
+//     ring.  An end-of-block, an error, a lost hand-off or a full list ends
+//     the round there and the next round starts from it;
 //   * consumer: turns each published round's list into output in a 34.5 KiB
 //     LDS ring holding the last 32 KiB: batches of up to 64 symbols (about
 //     2 KiB of output), literals stored by their lanes, then matches copied
@@ -80,8 +83,10 @@ constexpr uint32_t kStageWords = 64 * kSpan / 32 + 8;   // one round's input, pl
 // would fall on a few LDS banks; padded, word w sits at w + w / 16.
 MDZ_DEV uint32_t pad16(uint32_t w) { return w + (w >> 4); }
 constexpr uint32_t kStagePadded = kStageWords + kStageWords / 16 + 1;
-constexpr uint32_t kSymCap = 2048;                      // true-path symbols kept per round
-constexpr uint32_t kSlots = 3;                          // queue depth between the two waves
+constexpr uint32_t kSymCap = 4096;                      // true-path symbols kept per round
+constexpr uint32_t kSymRing = 6144;                     // symbol ring the rounds' lists share
+constexpr uint32_t kSlots = 3;                          // queue depth (items) between the two waves
+MDZ_DEV uint32_t sym_wrap(uint32_t x) { return x >= kSymRing ? x - kSymRing : x; }   // x < 2 * kSymRing
 // Polls (each after an s_sleep of about 64 cycles) before a wave waiting on
 // the other gives up on the stream with MDZ_ST_INTERNAL: far above any
 // item's decode or output time, so only a broken queue reaches it.
@@ -141,6 +146,7 @@ struct Item {
     uint32_t last;        // the stream ends with this item
     uint64_t src;         // stored: stream byte offset of the data
     uint64_t take;        // stored: bytes present (take < LEN ends in an input overrun)
+    uint32_t sym0;        // round: symbol ring index of its first symbol
 };
 
 struct Ctrl {
@@ -148,6 +154,7 @@ struct Ctrl {
     uint32_t halt;        // a wave has stopped the stream (consumer finished or failed, or a wait ran out)
     uint32_t n_pub;       // items the producer has published
     uint32_t n_fin;       // items the consumer has finished with
+    uint32_t sym_fin;     // symbols of finished items (a running count, mod 2^32)
     uint32_t n_stored, n_fixed, n_dyn;
     uint64_t consumed;    // Deflate input bits through the last item
     uint64_t desc[4];     // a live queue's descriptor, read once by thread 0
@@ -162,7 +169,7 @@ struct InflateLds {
         HdrScratch h;
         uint32_t bm[64 * (kSpan / 32) * 17 / 16];   // per lane: symbol starts met in its span (padded)
     } u;
-    uint32_t sym[kSlots][kSymCap];             // rounds' symbols, in stream order (item k in slot k % kSlots)
+    uint32_t sym[kSymRing];                    // rounds' symbols, in stream order, as a ring
     Item item[kSlots];
     Ctrl ctrl;
     uint8_t ring[kRing];                       // the consumer's output window
@@ -510,12 +517,13 @@ MDZ_DEV void q_publish(uint32_t* ctr, uint32_t v) {
 MDZ_DEV uint32_t q_read(uint32_t* ctr) {
     return __builtin_amdgcn_readfirstlane(__hip_atomic_load(ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP));
 }
-// Waits until *ctr >= want.  False if the stream is halted first, or after
-// kWaitPolls polls (the wave then halts the stream itself).
+// Waits until *ctr >= want (mod 2^32: the two lie within 2^31).  False if
+// the stream is halted first, or after kWaitPolls polls (the wave then halts
+// the stream itself).
 MDZ_DEV bool q_wait(uint32_t* ctr, uint32_t want, uint32_t* halt) {
     for (uint32_t n = 0;; ++n) {
         const uint32_t v = q_read(ctr);
-        if (v >= want) break;
+        if ((int32_t)(v - want) >= 0) break;             // counts run mod 2^32
         if (q_read(halt)) return false;
         if (n >= kWaitPolls) {
             if (lane_id() == 0) __hip_atomic_store(halt, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
@@ -668,6 +676,7 @@ __global__ __launch_bounds__(128) void inflate_kernel(const mdz_inflate_in_t* __
             S.ctrl.halt = 0;
             S.ctrl.n_pub = 0;
             S.ctrl.n_fin = 0;
+            S.ctrl.sym_fin = 0;
         }
         __syncthreads();
         const uint32_t qi = S.ctrl.qi;
@@ -720,6 +729,7 @@ __global__ __launch_bounds__(128) void inflate_kernel(const mdz_inflate_in_t* __
             INF_PHASE_START();
             bool need_hdr = true, final = false, pdone = false;
             uint32_t fixed_built = 0, n_stored = 0, n_fixed = 0, n_dyn = 0;
+            uint32_t sym_pos = 0, sym_at = 0;      // symbols listed (running count) and their ring end
             const uintptr_t src_addr = (uintptr_t)r.src;
             for (uint32_t ik = 0; !pdone; ++ik) {
                 // item ik goes to slot ik % kSlots once the consumer has finished
@@ -996,14 +1006,18 @@ __global__ __launch_bounds__(128) void inflate_kernel(const mdz_inflate_in_t* __
                         const uint32_t r_h = __builtin_amdgcn_readlane(h, last);
                         INF_PHASE(8);
 
-                        // phase 4: list the true symbols in order
+                        // phase 4: list the true symbols in order, once the ring has
+                        // room for them (the consumer frees a finished item's)
+                        const bool room = q_wait(&S.ctrl.sym_fin, sym_pos + nsym - kSymRing, &S.ctrl.halt);
+                        INF_PHASE(9);
                         bool astray = false;
-                        if (member && l <= last) {
-                            uint32_t o = incl - cnt;
+                        if (room && member && l <= last) {
+                            uint32_t o = sym_wrap(sym_at + incl - cnt);
                             p = a;
                             for (uint32_t i = 0; i < cnt; ++i) {
                                 decode_sym(S, p, lim, &sym, &used);
-                                S.sym[b][o++] = sym;
+                                S.sym[o] = sym;
+                                o = o + 1 == kSymRing ? 0 : o + 1;
                                 p += used;
                             }
                             // the listed symbols must end where the piece does
@@ -1011,7 +1025,13 @@ __global__ __launch_bounds__(128) void inflate_kernel(const mdz_inflate_in_t* __
                         }
                         it.kind = kItemRound;
                         it.nsym = nsym;
-                        if (__ballot(astray)) {                 // never expected
+                        it.sym0 = sym_at;
+                        if (!room) {                            // halted, or the wait ran out
+                            it.nsym = 0;
+                            st = MDZ_ST_INTERNAL;
+                            it.status = st;
+                            it.last = 1;
+                        } else if (__ballot(astray)) {          // never expected
                             it.nsym = 0;
                             st = MDZ_ST_INTERNAL;
                             it.status = st;
@@ -1028,6 +1048,8 @@ __global__ __launch_bounds__(128) void inflate_kernel(const mdz_inflate_in_t* __
                         } else {
                             P += r_h - rel;
                         }
+                        sym_pos += it.nsym;
+                        sym_at = sym_wrap(sym_at + it.nsym);
                         INF_PHASE(1);
                     }
                     if (l == 0) S.item[b] = it;
@@ -1050,6 +1072,7 @@ __global__ __launch_bounds__(128) void inflate_kernel(const mdz_inflate_in_t* __
             const uint64_t cap = in.dst_bytes;
             uint64_t pos = 0, flushed = 0;             // output produced / flushed to dst
             uint32_t n_sym = 0, n_match = 0;
+            uint32_t sym_done = 0;                     // symbols of the items finished (mod 2^32)
             int st = MDZ_ST_OK;
             const uint64_t t0 = clock64();
 
@@ -1146,7 +1169,7 @@ __global__ __launch_bounds__(128) void inflate_kernel(const mdz_inflate_in_t* __
                         const uint32_t nsym = it.nsym;
                         for (uint32_t b0 = 0; b0 < nsym && st == MDZ_ST_OK;) {
                             const uint32_t i = b0 + l;
-                            const uint32_t x = i < nsym ? S.sym[b][i] : 0;
+                            const uint32_t x = i < nsym ? S.sym[sym_wrap(it.sym0 + i)] : 0;
                             const bool have = i < nsym, ref = have && (x >> 31);
                             const uint32_t rec_len = !have ? 0 : ref ? ((x >> 15) & 255) + 3 : 1;
                             const uint32_t rec_val = ref ? (x & 32767) + 1 : (x & 255);
@@ -1220,7 +1243,14 @@ __global__ __launch_bounds__(128) void inflate_kernel(const mdz_inflate_in_t* __
                         wave_sync();
                         break;
                     }
-                    q_publish(&S.ctrl.n_fin, ik + 1);
+                    // the item's symbols are free for the producer's next lists
+                    sym_done += it.nsym;
+                    wave_sync();
+                    if (l == 0) {
+                        __hip_atomic_store(&S.ctrl.sym_fin, sym_done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                        __hip_atomic_store(&S.ctrl.n_fin, ik + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                    }
+                    wave_sync();
                 }
             }
             flush(pos, true);

@@ -7,10 +7,14 @@ many schedules (no GPU).
    k - KSLOTS, then publishes n_pub = k + 1; the consumer waits for
    n_pub > k, processes item k, and publishes n_fin = k + 1.  The consumer
    sets `halt` when it stops (the stream's last item, or an error); the
-   producer checks `halt` before each item and in its wait, and a wave whose
-   wait runs out sets `halt` itself.  Checked over random interleavings: the
+   producer checks `halt` before each item and in its waits, and a wave whose
+   wait runs out sets `halt` itself.  A round's symbols go to a ring of RING
+   entries shared by the items in flight: the producer lists a round of n
+   symbols once the symbols listed minus those of finished items (sym_fin,
+   published by the consumer) leave room for n.  Checked over random interleavings: the
    consumer sees every item in order exactly once up to the one that ends the
-   stream, no slot is written while the consumer reads it, the producer runs
+   stream, no slot is written while the consumer reads it, no ring entry is
+   reused before the consumer has finished with its item, the producer runs
    at most KSLOTS items past the consumer's stop, every wait ends, and a
    producer that gives up makes the consumer stop with an internal error.
 
@@ -27,17 +31,24 @@ import pytest
 
 
 KSLOTS = 3
+RING = 6144                      # symbol ring entries; a round lists at most CAP
+CAP = 4096
 
 
 def run_stream(items, consumer_error_at=None, rng=None, producer_gives_up_at=None):
     """items: list of (kind, last, status) the producer would emit in order.
     Returns (seen, produced, consumer_status)."""
-    st = {"n_pub": 0, "n_fin": 0, "halt": False}
+    st = {"n_pub": 0, "n_fin": 0, "sym_fin": 0, "halt": False}
     slot = [None] * KSLOTS
+    ring_owner = [None] * RING       # item whose symbol occupies each ring entry
+    ring_reading = set()             # items the consumer is reading
     writing = [False] * KSLOTS
     reading = [False] * KSLOTS
     seen = []
     out = {"produced": 0, "cstatus": 0}
+    sym_pos = [0, 0]                 # producer: symbols listed, ring end
+    sym_done = [0]                   # consumer: symbols of finished items
+    finished = {}
 
     def producer():
         ik = 0
@@ -58,8 +69,25 @@ def run_stream(items, consumer_error_at=None, rng=None, producer_gives_up_at=Non
             assert not reading[b]
             writing[b] = True
             yield
+            # the round's symbols go to the ring once the consumer freed room
+            nsym = items[ik][3] if len(items[ik]) > 3 else 0
+            polls = 0
+            while sym_pos[0] + nsym - st["sym_fin"] > RING:
+                if st["halt"]:
+                    writing[b] = False
+                    return
+                polls += 1
+                yield
+            for j in range(nsym):
+                e = (sym_pos[1] + j) % RING
+                # an entry is reused only after the consumer finished its item
+                assert ring_owner[e] is None or finished.get(ring_owner[e], False)
+                ring_owner[e] = ik
+            sym0 = sym_pos[1]
+            sym_pos[0] += nsym
+            sym_pos[1] = (sym_pos[1] + nsym) % RING
             assert not reading[b]
-            slot[b] = (ik, items[ik])
+            slot[b] = (ik, items[ik], sym0, nsym)
             writing[b] = False
             out["produced"] = ik + 1
             yield
@@ -80,10 +108,16 @@ def run_stream(items, consumer_error_at=None, rng=None, producer_gives_up_at=Non
             b = ik % KSLOTS
             assert not writing[b]
             reading[b] = True
-            idx, (kind, last, status) = slot[b]
+            idx, item, sym0, nsym = slot[b]
+            kind, last, status = item[:3]
             assert idx == ik
+            ring_reading.add(ik)
+            assert all(ring_owner[(sym0 + j) % RING] == ik for j in range(nsym))
             yield
             assert not writing[b]
+            assert all(ring_owner[(sym0 + j) % RING] == ik for j in range(nsym))
+            ring_reading.discard(ik)
+            finished[ik] = True
             reading[b] = False
             seen.append(idx)
             err = consumer_error_at is not None and idx == consumer_error_at
@@ -91,6 +125,8 @@ def run_stream(items, consumer_error_at=None, rng=None, producer_gives_up_at=Non
                 out["cstatus"] = status or ("error" if err else 0)
                 st["halt"] = True
                 return
+            sym_done[0] += nsym
+            st["sym_fin"] = sym_done[0]
             st["n_fin"] = ik + 1
             yield
             ik += 1
@@ -112,11 +148,11 @@ def run_stream(items, consumer_error_at=None, rng=None, producer_gives_up_at=Non
 def test_inflate_queue_protocol(seed):
     rng = random.Random(seed)
     n = rng.randint(1, 12)
-    items = [("round", False, 0) for _ in range(n)]
+    items = [("round", False, 0, rng.choice([0, 1, 2048, CAP, rng.randint(0, CAP)])) for _ in range(n)]
     end = rng.randrange(n)
     kind = rng.choice(["last", "error"])
     # an item that carries an error status is also the stream's last
-    items[end] = ("round", True, -5 if kind == "error" else 0)
+    items[end] = ("round", True, -5 if kind == "error" else 0, rng.randint(0, CAP))
     cerr = rng.choice([None, rng.randrange(end + 1)])
     seen, produced, cstatus = run_stream(items[:end + 1], cerr, rng)
     stop = end if cerr is None else min(end, cerr)
@@ -129,8 +165,8 @@ def test_inflate_queue_protocol(seed):
 def test_inflate_queue_producer_gives_up(seed):
     rng = random.Random(1000 + seed)
     n = rng.randint(KSLOTS + 1, 12)
-    items = [("round", False, 0) for _ in range(n)]
-    items[-1] = ("round", True, 0)
+    items = [("round", False, 0, rng.randint(0, CAP)) for _ in range(n)]
+    items[-1] = ("round", True, 0, rng.randint(0, CAP))
     seen, produced, cstatus = run_stream(items, None, rng, producer_gives_up_at=rng.randint(0, 3))
     assert seen == list(range(len(seen)))
     # the consumer ends: with the last item, or with an internal error
