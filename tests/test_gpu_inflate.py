@@ -67,6 +67,27 @@ def test_generated_batch(gpu, sched):
         assert outs[i] == b.raw_stream(i), i
 
 
+@pytest.mark.parametrize("sched", [False, True])
+def test_symbol_ring_full_rounds(gpu, sched):
+    """Literal codes of 1-3 bits: a round's 64 spans hold far more than the
+    4,096 symbols a round keeps, so every round is cut at the cap and the
+    rounds' lists wrap the 6,144-entry symbol ring hundreds of times per
+    stream (Huffman-only, run-length, default and fixed-code streams)."""
+    rng = np.random.default_rng(41)
+    datas = [rng.choice(np.frombuffer(b"abcd", np.uint8), size=1 << 20, p=[0.7, 0.15, 0.1, 0.05]).tobytes(),
+             rng.choice(np.frombuffer(b"xy", np.uint8), size=700_001, p=[0.95, 0.05]).tobytes()]
+    streams, raws = [], []
+    for d in datas:
+        for strat in (zlib.Z_HUFFMAN_ONLY, zlib.Z_RLE, zlib.Z_DEFAULT_STRATEGY, zlib.Z_FIXED):
+            c = zlib.compressobj(6, zlib.DEFLATED, 15, 9, strat)
+            streams.append(c.compress(d) + c.flush())
+            raws.append(d)
+    outs, st = _run(streams, [len(r) for r in raws], sched=sched)
+    assert (st["status"] == 0).all()
+    for i, (o, r) in enumerate(zip(outs, raws)):
+        assert o == r, i
+
+
 def test_edge_streams(gpu):
     rng = np.random.default_rng(5)
     datas = [b"", b"x", b"\0" * (1 << 20), rng.integers(0, 256, 300_000).astype(np.uint8).tobytes(),
