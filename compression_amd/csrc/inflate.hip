@@ -39,6 +39,7 @@
 // rules, applied by a prepass kernel; the 8-byte footer is excluded from the
 // Deflate data); with MDZ_HDR_VERIFY a kernel after the decode checks each
 // trailer (Adler-32, or CRC-32 and ISIZE) against the output written.
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 
@@ -89,7 +90,9 @@ constexpr uint32_t kSlots = 3;                          // queue depth (items) b
 MDZ_DEV uint32_t sym_wrap(uint32_t x) { return x >= kSymRing ? x - kSymRing : x; }   // x < 2 * kSymRing
 // Polls (each after an s_sleep of about 64 cycles) before a wave waiting on
 // the other gives up on the stream with MDZ_ST_INTERNAL: far above any
-// item's decode or output time, so only a broken queue reaches it.
+// item's decode or output time, so only a broken queue reaches it.  The
+// launch passes the limit; a test lowers it (MDZ_INFLATE_WAIT_POLLS) to
+// drive the give-up path from the host.
 constexpr uint32_t kWaitPolls = 1u << 24;
 
 // Table entry: [3:0] code length (0: no entry, take the canonical path),
@@ -518,14 +521,14 @@ MDZ_DEV uint32_t q_read(uint32_t* ctr) {
     return __builtin_amdgcn_readfirstlane(__hip_atomic_load(ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP));
 }
 // Waits until *ctr >= want (mod 2^32: the two lie within 2^31).  False if
-// the stream is halted first, or after kWaitPolls polls (the wave then halts
+// the stream is halted first, or after `limit` polls (the wave then halts
 // the stream itself).
-MDZ_DEV bool q_wait(uint32_t* ctr, uint32_t want, uint32_t* halt) {
+MDZ_DEV bool q_wait(uint32_t* ctr, uint32_t want, uint32_t* halt, uint32_t limit) {
     for (uint32_t n = 0;; ++n) {
         const uint32_t v = q_read(ctr);
         if ((int32_t)(v - want) >= 0) break;             // counts run mod 2^32
         if (q_read(halt)) return false;
-        if (n >= kWaitPolls) {
+        if (n >= limit) {
             if (lane_id() == 0) __hip_atomic_store(halt, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
             wave_sync();
             return false;
@@ -651,7 +654,7 @@ __global__ __launch_bounds__(128) void inflate_kernel(const mdz_inflate_in_t* __
                                                       uint32_t count, int parse_hdr,
                                                       uint32_t* __restrict__ cursor,
                                                       mdz_inflate_cost_t* __restrict__ cost,
-                                                      uint32_t* live) {
+                                                      uint32_t* live, uint32_t wait_polls) {
     extern __shared__ __align__(16) uint8_t smem[];
     InflateLds& S = *reinterpret_cast<InflateLds*>(smem);
     HdrScratch& H = S.u.h;
@@ -736,7 +739,7 @@ __global__ __launch_bounds__(128) void inflate_kernel(const mdz_inflate_in_t* __
                 // with item ik - kSlots; a halted stream gets no more items
                 if (q_read(&S.ctrl.halt)) break;
                 INF_PHASE(11);
-                const bool free = q_wait(&S.ctrl.n_fin, ik >= kSlots ? ik + 1 - kSlots : 0, &S.ctrl.halt);
+                const bool free = q_wait(&S.ctrl.n_fin, ik >= kSlots ? ik + 1 - kSlots : 0, &S.ctrl.halt, wait_polls);
                 INF_PHASE(9);                      // producer waiting for the consumer
                 if (!free) break;
                 {
@@ -1008,7 +1011,7 @@ __global__ __launch_bounds__(128) void inflate_kernel(const mdz_inflate_in_t* __
 
                         // phase 4: list the true symbols in order, once the ring has
                         // room for them (the consumer frees a finished item's)
-                        const bool room = q_wait(&S.ctrl.sym_fin, sym_pos + nsym - kSymRing, &S.ctrl.halt);
+                        const bool room = q_wait(&S.ctrl.sym_fin, sym_pos + nsym - kSymRing, &S.ctrl.halt, wait_polls);
                         INF_PHASE(9);
                         bool astray = false;
                         if (room && member && l <= last) {
@@ -1120,7 +1123,7 @@ __global__ __launch_bounds__(128) void inflate_kernel(const mdz_inflate_in_t* __
             INF_PHASE_START();
             for (uint32_t ik = 0;; ++ik) {
                 INF_PHASE(12);
-                const bool have = q_wait(&S.ctrl.n_pub, ik + 1, &S.ctrl.halt);
+                const bool have = q_wait(&S.ctrl.n_pub, ik + 1, &S.ctrl.halt, wait_polls);
                 INF_PHASE(10);                     // consumer waiting for the producer
                 // the producer halts a stream only when a wait of its own ran out
                 if (!have) { st = MDZ_ST_INTERNAL; break; }
@@ -1385,6 +1388,17 @@ __global__ __launch_bounds__(kCopyThreads) void copy_uncompressed_kernel(const m
         copy_one(ins[si], slice);
 }
 
+// The queue's poll limit for a launch: kWaitPolls, or a lower value from
+// MDZ_INFLATE_WAIT_POLLS (read at each launch; a test sets it to make the
+// waves give up, and it can never raise the limit).
+uint32_t wait_polls() {
+    const char* v = std::getenv("MDZ_INFLATE_WAIT_POLLS");
+    if (!v || !*v) return kWaitPolls;
+    char* end = nullptr;
+    const unsigned long x = std::strtoul(v, &end, 10);
+    return end && *end == 0 && x < kWaitPolls ? (uint32_t)x : kWaitPolls;
+}
+
 }  // namespace
 
 int inflate_slots(uint32_t* slots) {
@@ -1409,7 +1423,7 @@ int inflate_live_launch(const mdz_inflate_in_t* in, mdz_inflate_status_t* out, u
     hipError_t e = set_dynamic_lds((const void*)inflate_kernel, smem);
     if (e != hipSuccess) return record_hip(e, "inflate_kernel LDS");
     hipLaunchKernelGGL(inflate_kernel, dim3(blocks), dim3(128), smem, s, in, out, nullptr, cap, parse_hdr,
-                       cursor, nullptr, live);
+                       cursor, nullptr, live, wait_polls());
     return record_hip(hipGetLastError(), "inflate_kernel live launch");
 }
 
@@ -1439,7 +1453,7 @@ int inflate_batch(const mdz_inflate_in_t* in, mdz_inflate_status_t* out, uint32_
         if ((e = hipGetLastError()) != hipSuccess) return record_hip(e, "gzip_prepass_kernel launch");
     }
     hipLaunchKernelGGL(inflate_kernel, dim3(grid), dim3(128), smem, s, in, out, order, count,
-                       parse_hdr, persistent ? cursor : nullptr, cost, nullptr);
+                       parse_hdr, persistent ? cursor : nullptr, cost, nullptr, wait_polls());
     if ((e = hipGetLastError()) != hipSuccess) return record_hip(e, "inflate_kernel launch");
     if ((parse_hdr & MDZ_HDR_VERIFY) && hdr) {
         hipLaunchKernelGGL(trailer_check_kernel, dim3(count), dim3(64), 0, s, in, out, hdr);

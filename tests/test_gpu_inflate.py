@@ -88,6 +88,28 @@ def test_symbol_ring_full_rounds(gpu, sched):
         assert o == r, i
 
 
+def test_queue_wait_gives_up(gpu, monkeypatch):
+    """The producer/consumer queue's bounded wait, driven from the host: with
+    the poll limit forced to 0 a wave that would wait gives up at once and
+    halts its stream.  The launch still ends, every stream reports either
+    OK with zlib's bytes or MDZ_ST_INTERNAL (-12), and the next launch at
+    the default limit decodes every stream."""
+    b = O.gen_deflate_batch(64, 0x4D445A44, 4096, 1 << 20)
+    streams = [b.stream(i) for i in range(b.count)]
+    monkeypatch.setenv("MDZ_INFLATE_WAIT_POLLS", "0")
+    outs, st = _run(streams, b.raw_len, sched=True)
+    assert set(np.unique(st["status"]).tolist()) <= {0, -12}
+    assert (st["status"] != 0).any()
+    for i in range(b.count):
+        if st["status"][i] == 0:
+            assert outs[i] == b.raw_stream(i), i
+    monkeypatch.delenv("MDZ_INFLATE_WAIT_POLLS")
+    outs, st = _run(streams, b.raw_len, sched=True)
+    assert (st["status"] == 0).all()
+    for i in range(b.count):
+        assert outs[i] == b.raw_stream(i), i
+
+
 def test_edge_streams(gpu):
     rng = np.random.default_rng(5)
     datas = [b"", b"x", b"\0" * (1 << 20), rng.integers(0, 256, 300_000).astype(np.uint8).tobytes(),
