@@ -887,7 +887,7 @@ __global__ __launch_bounds__(128) void inflate_kernel(const mdz_inflate_in_t* __
 #pragma unroll
                         for (uint32_t k = 0; k < kSpan / 32; ++k) bm[k] = 0;
                         uint32_t p = s_me, sym = 0, used = 0;
-                        uint32_t ev1 = ~0u, ev2 = ~0u, nev = 0;   // first two events (restarts)
+                        uint32_t ev1 = ~0u, ev2 = ~0u, ev3 = ~0u, ev4 = ~0u, nev = 0;   // first four events (restarts)
                         bool at_end = false;
                         while (p < s_nx) {
                             if (p >= lim) { at_end = true; break; }
@@ -899,6 +899,8 @@ __global__ __launch_bounds__(128) void inflate_kernel(const mdz_inflate_in_t* __
                             if (rc != kSymOk) {
                                 if (nev == 0) ev1 = p;
                                 else if (nev == 1) ev2 = p;
+                                else if (nev == 2) ev3 = p;
+                                else if (nev == 3) ev4 = p;
                                 ++nev;
                             }
                             p += rc == kSymOk ? used : 1;
@@ -947,14 +949,16 @@ __global__ __launch_bounds__(128) void inflate_kernel(const mdz_inflate_in_t* __
                         // that ends it, if any.  From a (one of its noted starts) the
                         // lane's own sequence is the true one, so up to the first event
                         // at or after a its noted starts are the piece's symbols, and
-                        // past its span the hand-off steps are.  A lane that met more
-                        // than two events, or the stream's end, decodes its piece to
-                        // count it.
+                        // past its span the hand-off steps are.  Events come in
+                        // position order, so the first tracked one at or after a is
+                        // that event; a lane whose four tracked events all lie before
+                        // a (and that met more), or that met the stream's end, decodes
+                        // its piece to count it.
                         uint32_t cnt = 0, ev = kSymOk, evst = 0, evpos = 0, evlen = 0;
                         if (member) {
-                            const uint32_t x = ev1 >= a ? ev1 : ev2 >= a ? ev2 : ~0u;
+                            const uint32_t x = ev1 >= a ? ev1 : ev2 >= a ? ev2 : ev3 >= a ? ev3 : ev4 >= a ? ev4 : ~0u;
                             bool need_ev = false;
-                            if (nev > 2 || at_end) {
+                            if (at_end || (nev > 4 && x == ~0u)) {
                                 p = a;
                                 while (p < h) {
                                     const uint32_t rc = decode_sym(S, p, lim, &sym, &used);
