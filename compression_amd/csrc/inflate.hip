@@ -78,6 +78,7 @@ constexpr uint32_t kDistRoot = 8, kDistCap = 256 + 384;
 constexpr uint32_t kIndirect = 0x400;                   // root entry pointing at a subtable
 constexpr uint32_t kSpanLog = 9, kSpan = 1u << kSpanLog; // input bits each lane takes per round
 constexpr uint32_t kReach = 3;                          // spans a hand-off may run ahead
+constexpr uint32_t kEvents = 8;                         // span-decode events a lane keeps
 constexpr uint32_t kStageWords = 64 * kSpan / 32 + 8;   // one round's input, plus a symbol's overhang
 // Staged words and hand-off bits are stored with one padding word after every
 // 16: lane t's span starts at word 16t, so unpadded the 64 lanes' accesses
@@ -887,7 +888,9 @@ __global__ __launch_bounds__(128) void inflate_kernel(const mdz_inflate_in_t* __
 #pragma unroll
                         for (uint32_t k = 0; k < kSpan / 32; ++k) bm[k] = 0;
                         uint32_t p = s_me, sym = 0, used = 0;
-                        uint32_t ev1 = ~0u, ev2 = ~0u, ev3 = ~0u, ev4 = ~0u, nev = 0;   // first four events (restarts)
+                        uint32_t evp[kEvents], nev = 0;  // positions of the first kEvents events (restarts)
+#pragma unroll
+                        for (uint32_t k = 0; k < kEvents; ++k) evp[k] = ~0u;
                         bool at_end = false;
                         while (p < s_nx) {
                             if (p >= lim) { at_end = true; break; }
@@ -897,10 +900,9 @@ __global__ __launch_bounds__(128) void inflate_kernel(const mdz_inflate_in_t* __
                                 if ((k >> 5) == w) bm[w] |= 1u << (k & 31);
                             const uint32_t rc = decode_sym(S, p, lim, &sym, &used);
                             if (rc != kSymOk) {
-                                if (nev == 0) ev1 = p;
-                                else if (nev == 1) ev2 = p;
-                                else if (nev == 2) ev3 = p;
-                                else if (nev == 3) ev4 = p;
+#pragma unroll
+                                for (uint32_t k = 0; k < kEvents; ++k)
+                                    if (nev == k) evp[k] = p;
                                 ++nev;
                             }
                             p += rc == kSymOk ? used : 1;
@@ -951,14 +953,17 @@ __global__ __launch_bounds__(128) void inflate_kernel(const mdz_inflate_in_t* __
                         // at or after a its noted starts are the piece's symbols, and
                         // past its span the hand-off steps are.  Events come in
                         // position order, so the first tracked one at or after a is
-                        // that event; a lane whose four tracked events all lie before
-                        // a (and that met more), or that met the stream's end, decodes
-                        // its piece to count it.
+                        // that event; a lane whose kEvents tracked events all lie
+                        // before a (and that met more), or that met the stream's end,
+                        // decodes its piece to count it.
                         uint32_t cnt = 0, ev = kSymOk, evst = 0, evpos = 0, evlen = 0;
                         if (member) {
-                            const uint32_t x = ev1 >= a ? ev1 : ev2 >= a ? ev2 : ev3 >= a ? ev3 : ev4 >= a ? ev4 : ~0u;
+                            uint32_t x = ~0u;
+#pragma unroll
+                            for (uint32_t k = kEvents; k-- > 0;)
+                                if (evp[k] >= a) x = evp[k];
                             bool need_ev = false;
-                            if (at_end || (nev > 4 && x == ~0u)) {
+                            if (at_end || (nev > kEvents && x == ~0u)) {
                                 p = a;
                                 while (p < h) {
                                     const uint32_t rc = decode_sym(S, p, lim, &sym, &used);
