@@ -78,7 +78,7 @@ constexpr uint32_t kDistRoot = 8, kDistCap = 256 + 384;
 constexpr uint32_t kIndirect = 0x400;                   // root entry pointing at a subtable
 constexpr uint32_t kSpanLog = 9, kSpan = 1u << kSpanLog; // input bits each lane takes per round
 constexpr uint32_t kReach = 3;                          // spans a hand-off may run ahead
-constexpr uint32_t kEvents = 8;                         // span-decode events a lane keeps
+constexpr uint32_t kEvents = 16;                        // span-decode events a lane keeps
 constexpr uint32_t kStageWords = 64 * kSpan / 32 + 8;   // one round's input, plus a symbol's overhang
 // Staged words and hand-off bits are stored with one padding word after every
 // 16: lane t's span starts at word 16t, so unpadded the 64 lanes' accesses
