@@ -1207,17 +1207,21 @@ __global__ __launch_bounds__(128) void inflate_kernel(const mdz_inflate_in_t* __
                             const uint64_t room = pos < cap ? cap - pos : 0;
                             const uint32_t rb = ring_at(pos);          // ring index of the batch start
                             auto wrap = [](uint32_t v) { return v >= kRing ? v - kRing : v; };
-                            if (l < n && !ref) S.ring[wrap(rb + rec_off)] = (uint8_t)rec_val;
+                            // each lane's ring destination, and for a match its source
+                            // start (dist <= 32768 < kRing), computed for all symbols at
+                            // once so the serial match loop only reads them
+                            const uint32_t my_d0 = wrap(rb + rec_off);
+                            const uint32_t my_s0 = wrap(my_d0 + kRing - (ref ? rec_val : 1u));
+                            if (l < n && !ref) S.ring[my_d0] = (uint8_t)rec_val;
                             __builtin_amdgcn_wave_barrier();
                             uint64_t mm = __ballot(l < n && ref);
                             while (mm) {
                                 const uint32_t m = __builtin_ctzll(mm);
                                 mm &= mm - 1;
-                                const uint32_t off = __builtin_amdgcn_readlane(rec_off, m);
                                 const uint32_t len = __builtin_amdgcn_readlane(rec_len, m);
                                 const uint32_t dist = __builtin_amdgcn_readlane(rec_val, m);
-                                const uint32_t d0 = wrap(rb + off);                  // destination start
-                                const uint32_t s0 = wrap(d0 + kRing - dist);         // source start (dist <= 32768)
+                                const uint32_t d0 = __builtin_amdgcn_readlane(my_d0, m);   // destination start
+                                const uint32_t s0 = __builtin_amdgcn_readlane(my_s0, m);   // source start
                                 if (dist >= len) {
                                     for (uint32_t j = l; j < len; j += 64) S.ring[wrap(d0 + j)] = S.ring[wrap(s0 + j)];
                                 } else {
